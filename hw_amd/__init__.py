@@ -1,0 +1,383 @@
+"""hw_amd -- MI355X (gfx950) trace-analysis path of the smrt-basecaller.
+
+Python view of libhwpa.so's C-ABI (include/hwpa.h).  The library does all
+analysis work on the GPU; this module only loads it, mirrors its structures
+and moves buffers through the C-ABI's own device helpers.  Importing it when the
+library has not been built raises immediately: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import types as T
+from .types import (BASELINER_STATS, LANE_HIST, LANE_MODEL, PULSE, STAT_ACC, TRACEGEN_STATE, Config, FilterParams,
+                    TracegenParams, Transition, half_to_float)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhwpa.so")
+
+
+class HwpaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("hwpa error %d (%s): %s" % (code, T.STATUS_NAMES.get(code, "?"), msg))
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libhwpa.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.hwpa_last_error.restype = ctypes.c_char_p
+    lib.hwpa_edge_state_bytes.restype = ctypes.c_size_t
+    return lib
+
+
+lib = _load()
+_vp = ctypes.c_void_p
+
+
+def check(rc):
+    if rc != T.OK:
+        raise HwpaError(rc, lib.hwpa_last_error().decode())
+    return rc
+
+
+def _sizes_ok():
+    n = lib.hwpa_layout_sizes(None, 0)
+    arr = (ctypes.c_int64 * n)()
+    lib.hwpa_layout_sizes(arr, n)
+    for (name, mine), theirs in zip(T.LAYOUT_ORDER, arr):
+        if mine != theirs:
+            raise ImportError("layout mismatch for %s: python %d vs C %d" % (name, mine, theirs))
+
+
+_sizes_ok()
+
+
+def default_config(**overrides):
+    cfg = Config()
+    check(lib.hwpa_config_default(ctypes.byref(cfg)))
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    check(lib.hwpa_config_validate(ctypes.byref(cfg)))
+    return cfg
+
+
+def load_json(cfg, text):
+    check(lib.hwpa_config_load_json(ctypes.byref(cfg), text.encode()))
+    return cfg
+
+
+def bench_config(frames=512):
+    """The every-chunk-estimates configuration used for measurement: sigma
+    EMA off (baseliner start-up 1024 frames) and one estimate per chunk."""
+    cfg = default_config()
+    cfg.sigma_ema_scale_strides = 0.0
+    cfg.min_frames_for_estimate = frames
+    cfg.num_frames_pre_accum_stats = frames
+    check(lib.hwpa_config_validate(ctypes.byref(cfg)))
+    return cfg
+
+
+def filter_params(ftype):
+    fp = FilterParams()
+    check(lib.hwpa_filter_params_get(int(ftype), ctypes.byref(fp)))
+    return fp
+
+
+def transition_matrix(cfg):
+    tr = Transition()
+    check(lib.hwpa_transition_matrix(ctypes.byref(cfg), ctypes.byref(tr)))
+    return tr
+
+
+def tracegen_default(**overrides):
+    p = TracegenParams()
+    check(lib.hwpa_tracegen_default(ctypes.byref(p)))
+    for k, v in overrides.items():
+        if k == "amplitude":
+            for i, a in enumerate(v):
+                p.amplitude[i] = a
+        else:
+            setattr(p, k, v)
+    return p
+
+
+# ---- device buffers ----------------------------------------------------------
+
+def device_count():
+    n = ctypes.c_int32(0)
+    check(lib.hwpa_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class DevBuf:
+    """A device allocation owned by Python (hipMalloc through the C-ABI)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = max(int(nbytes), 1)
+        self.p = _vp()
+        check(lib.hwpa_dev_alloc(ctypes.byref(self.p), ctypes.c_size_t(self.nbytes)))
+
+    def data_ptr(self):
+        return self.p.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise HwpaError(1, "upload larger than the buffer")
+        if a.nbytes:
+            check(lib.hwpa_memcpy_htod(self.p, a.ctypes.data_as(_vp), ctypes.c_size_t(a.nbytes)))
+        return self
+
+    def download(self, dtype, shape=None, nbytes=None):
+        n = self.nbytes if nbytes is None else nbytes
+        if shape is not None:
+            n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        out = _dtoh(self.p.value, n).view(dtype)
+        return out.reshape(shape) if shape is not None else out
+
+    def __del__(self):
+        if getattr(self, "p", None) is not None and self.p.value:
+            lib.hwpa_dev_free(self.p)
+            self.p = None
+
+
+def dev_bytes(nbytes):
+    return DevBuf(nbytes)
+
+
+def dev_from_numpy(a):
+    a = np.ascontiguousarray(a)
+    return DevBuf(a.nbytes).upload(a)
+
+
+def to_numpy(buf, dtype, shape=None):
+    sync()
+    return buf.download(dtype, shape)
+
+
+def ptr(buf):
+    return _vp(buf.data_ptr())
+
+
+def sync():
+    check(lib.hwpa_device_sync())
+
+
+# ---- stage wrappers ------------------------------------------------------------
+
+class Baseliner:
+    """HostMultiScaleBaseliner/DeviceMultiScaleBaseliner counterpart: one
+    persistent filter state per lane, one call per chunk."""
+
+    def __init__(self, cfg, lanes):
+        self.h = _vp()
+        self.lanes = lanes
+        check(lib.hwpa_baseliner_create(ctypes.byref(cfg), ctypes.c_uint32(lanes), ctypes.byref(self.h)))
+
+    def run(self, d_raw, frames, d_out, d_stats, stream=None):
+        check(lib.hwpa_baseliner_run(self.h, ptr(d_raw), ctypes.c_uint32(frames), ptr(d_out), ptr(d_stats),
+                                     _vp(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.hwpa_baseliner_destroy(self.h)
+            self.h = None
+
+
+def stats_reset(d_agg, lanes, stream=None):
+    check(lib.hwpa_stats_reset(ptr(d_agg), ctypes.c_uint32(lanes), _vp(stream)))
+
+
+def stats_merge(d_agg, d_chunk, lanes, stream=None):
+    check(lib.hwpa_stats_merge(ptr(d_agg), ptr(d_chunk), ctypes.c_uint32(lanes), _vp(stream)))
+
+
+def edge_state_bytes():
+    return lib.hwpa_edge_state_bytes()
+
+
+def edge_state_init(d_edge, lanes, stream=None):
+    check(lib.hwpa_edge_state_init(ptr(d_edge), ctypes.c_uint32(lanes), _vp(stream)))
+
+
+def hist_reset(cfg, d_hist, d_stats, lanes, stream=None):
+    check(lib.hwpa_hist_reset(ctypes.byref(cfg), ptr(d_hist), ptr(d_stats), ctypes.c_uint32(lanes), _vp(stream)))
+
+
+def hist_add(d_hist, d_edge, d_traces, frames, d_models, lanes, stream=None):
+    check(lib.hwpa_hist_add(ptr(d_hist), ptr(d_edge), ptr(d_traces), ctypes.c_uint32(frames), ptr(d_models),
+                            ctypes.c_uint32(lanes), _vp(stream)))
+
+
+def dme_init_models(cfg, d_stats, d_models, lanes, stream=None):
+    check(lib.hwpa_dme_init_models(ctypes.byref(cfg), ptr(d_stats), ptr(d_models), ctypes.c_uint32(lanes),
+                                   _vp(stream)))
+
+
+def dme_estimate(cfg, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters=None, stream=None):
+    mfi = (ctypes.c_int32 * 2)(*model_fi)
+    efi = (ctypes.c_int32 * 2)(*est_fi)
+    check(lib.hwpa_dme_estimate(ctypes.byref(cfg), ptr(d_hist), ptr(d_stats), ptr(d_models), mfi, efi,
+                                ctypes.c_uint32(lanes), ptr(d_iters) if d_iters is not None else None, _vp(stream)))
+
+
+class Labeler:
+    """FrameLabelerHost/FrameLabelerDevice counterpart."""
+
+    def __init__(self, cfg, lanes, max_frames):
+        self.h = _vp()
+        check(lib.hwpa_labeler_create(ctypes.byref(cfg), ctypes.c_uint32(lanes), ctypes.c_uint32(max_frames),
+                                      ctypes.byref(self.h)))
+
+    def run(self, d_traces, frames, d_models, d_labels, d_latent, d_score, stream=None):
+        check(lib.hwpa_labeler_run(self.h, ptr(d_traces), ctypes.c_uint32(frames), ptr(d_models), ptr(d_labels),
+                                   ptr(d_latent), ptr(d_score), _vp(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.hwpa_labeler_destroy(self.h)
+            self.h = None
+
+
+class Pulser:
+    """HostPulseAccumulator/DevicePulseAccumulator counterpart."""
+
+    def __init__(self, cfg, lanes):
+        self.h = _vp()
+        check(lib.hwpa_pulser_create(ctypes.byref(cfg), ctypes.c_uint32(lanes), ctypes.byref(self.h)))
+
+    def run(self, d_labels, d_latent, d_traces, frames, first_frame, d_models, d_pulses, d_counts, d_ovf,
+            d_blstats, stream=None):
+        check(lib.hwpa_pulser_run(self.h, ptr(d_labels), ptr(d_latent), ptr(d_traces), ctypes.c_uint32(frames),
+                                  ctypes.c_int32(first_frame), ptr(d_models), ptr(d_pulses), ptr(d_counts),
+                                  ptr(d_ovf), ptr(d_blstats), _vp(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.hwpa_pulser_destroy(self.h)
+            self.h = None
+
+
+def tracegen(params, lanes, frames, first_frame, d_state, init, d_out, lane_offset=0, stream=None):
+    check(lib.hwpa_tracegen_run(ctypes.byref(params), ctypes.c_uint32(lanes), ctypes.c_uint32(lane_offset),
+                                ctypes.c_uint32(frames), ctypes.c_uint32(first_frame), ptr(d_state),
+                                ctypes.c_int32(1 if init else 0), ptr(d_out), _vp(stream)))
+
+
+class BatchAnalyzer:
+    """Per-pool analyzer with the reference's BatchAnalyzer shape
+    (analyzer/BatchAnalyzer.h:51-106): construct once per pool, call once per
+    chunk in frame order; results stay on the device until fetched."""
+
+    def __init__(self, pool_id, lanes, frames_per_chunk, cfg):
+        self.h = _vp()
+        self.pool_id = pool_id
+        self.lanes = lanes
+        self.frames = frames_per_chunk
+        self.cfg = cfg
+        check(lib.hwpa_analyzer_create(ctypes.c_uint32(pool_id), ctypes.c_uint32(lanes),
+                                       ctypes.c_uint32(frames_per_chunk), ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def pool_id_(self):
+        return self.pool_id
+
+    def __call__(self, traces, first_frame, pool_id=None, on_device=None):
+        """traces: numpy int16 [lanes, frames, 64] (host) or a device buffer
+        (anything with data_ptr()) already in HBM."""
+        pid = self.pool_id if pool_id is None else pool_id
+        if isinstance(traces, np.ndarray):
+            a = np.ascontiguousarray(traces, dtype=np.int16)
+            if a.size != self.lanes * self.frames * T.LANE:
+                raise HwpaError(1, "trace batch shape does not match the analyzer")
+            rc = lib.hwpa_analyzer_process(self.h, a.ctypes.data_as(_vp), ctypes.c_int32(0), ctypes.c_uint32(pid),
+                                           ctypes.c_int32(first_frame))
+        else:
+            rc = lib.hwpa_analyzer_process(self.h, ptr(traces), ctypes.c_int32(1), ctypes.c_uint32(pid),
+                                           ctypes.c_int32(first_frame))
+        check(rc)
+        return self
+
+    def sync(self):
+        check(lib.hwpa_analyzer_sync(self.h))
+
+    def stream(self):
+        s = _vp()
+        check(lib.hwpa_analyzer_stream(self.h, ctypes.byref(s)))
+        return s.value
+
+    def flags(self):
+        f = (ctypes.c_int32 * 4)()
+        check(lib.hwpa_result_flags(self.h, f))
+        return list(f)
+
+    def _fetch(self, getter, nbytes, dtype, shape):
+        p = _vp()
+        check(getter(self.h, ctypes.byref(p)))
+        self.sync()
+        return _dtoh(p.value, nbytes).view(dtype).reshape(shape)
+
+    def labels(self):
+        return self._fetch(lib.hwpa_result_labels, self.lanes * self.frames * T.LANE * 2, np.int16,
+                           (self.lanes, self.frames, T.LANE))
+
+    def baselined(self):
+        return self._fetch(lib.hwpa_result_baselined, self.lanes * self.frames * T.LANE * 2, np.int16,
+                           (self.lanes, self.frames, T.LANE))
+
+    def models(self):
+        return self._fetch(lib.hwpa_result_models, self.lanes * LANE_MODEL.itemsize, LANE_MODEL, (self.lanes,))
+
+    def baseliner_stats(self):
+        return self._fetch(lib.hwpa_result_baseliner_stats, self.lanes * BASELINER_STATS.itemsize,
+                           BASELINER_STATS, (self.lanes,))
+
+    def viterbi_score(self):
+        return self._fetch(lib.hwpa_result_viterbi_score, self.lanes * T.LANE * 4, np.float32,
+                           (self.lanes, T.LANE))
+
+    def pulse_baseline_stats(self):
+        return self._fetch(lib.hwpa_result_pulse_baseline_stats, self.lanes * STAT_ACC.itemsize, STAT_ACC,
+                           (self.lanes,))
+
+    def pulses(self):
+        """Returns (pulses [lanes*64, max_calls], counts [lanes*64], overflows [lanes*64])."""
+        pp, pc, po = _vp(), _vp(), _vp()
+        mc = ctypes.c_uint32()
+        check(lib.hwpa_result_pulses(self.h, ctypes.byref(pp), ctypes.byref(pc), ctypes.byref(po),
+                                     ctypes.byref(mc)))
+        n = self.lanes * T.LANE
+        mc = mc.value
+        self.sync()
+        res = []
+        for p, nbytes, dt, shape in ((pp, n * mc * PULSE.itemsize, PULSE, (n, mc)), (pc, n * 4, np.uint32, (n,)),
+                                     (po, n * 4, np.uint32, (n,))):
+            res.append(_dtoh(p.value, nbytes).view(dt).reshape(shape))
+        return tuple(res)
+
+    def set_profiling(self, on=True):
+        check(lib.hwpa_analyzer_set_profiling(self.h, ctypes.c_int32(1 if on else 0)))
+
+    def stage_ms(self):
+        ms = (ctypes.c_float * 8)()
+        check(lib.hwpa_analyzer_stage_ms(self.h, ms))
+        return list(ms)[:6]
+
+    def em_iterations(self):
+        it = ctypes.c_int32()
+        check(lib.hwpa_analyzer_em_iterations(self.h, ctypes.byref(it)))
+        return it.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.hwpa_analyzer_destroy(self.h)
+            self.h = None
+
+
+def _dtoh(src, nbytes):
+    out = np.empty(nbytes, dtype=np.uint8)
+    if nbytes:
+        check(lib.hwpa_memcpy_dtoh(out.ctypes.data_as(_vp), _vp(src), ctypes.c_size_t(nbytes)))
+    return out

@@ -1,0 +1,408 @@
+// analyzer.hip -- the per-pool analyzer behind the C-ABI boundary
+// (DynamicEstimateBatchAnalyzer, analyzer/BatchAnalyzer.cpp:100-118,149-170,288-373;
+// DetectionModelEstimator::AddBatch, traceAnalysis/DetectionModelEstimator.cpp:93-160).
+//
+// The host side only sequences stages and tracks frame intervals; every
+// stage runs on the GPU on the analyzer's own stream, with all per-pool
+// state resident in HBM between chunks.  Chunk order and pool identity are
+// checked before any work is enqueued.
+
+#include "internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace hwpa {
+
+__global__ void ZeroCountsKernel(uint32_t* counts, uint32_t* ovf, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        counts[i] = 0;
+        ovf[i] = 0;
+    }
+}
+
+struct Interval {
+    int32_t lo = 0, hi = 0;
+    bool Empty() const { return lo == hi; }
+    void Hull(int32_t a, int32_t b)
+    {
+        if (Empty()) {
+            lo = a;
+            hi = b;
+        } else {
+            lo = std::min(lo, a);
+            hi = std::max(hi, b);
+        }
+    }
+};
+
+enum { kHistInit = 0, kDmeInit = 1, kSequencing = 2 };
+enum { kStBaseline = 0, kStAggregate, kStHistogram, kStDme, kStLabels, kStPulses, kNumStages };
+
+}  // namespace hwpa
+
+using namespace hwpa;
+
+struct hwpa_analyzer {
+    hwpa_config cfg;
+    uint32_t poolId, lanes, frames;
+    int32_t nextFrame = 0;
+    uint32_t baselinerLatency = 0, dmeDelay = 0;
+    bool fullEstimation = false;
+    int status = kHistInit;
+    int32_t framesRemaining = 0;
+    Interval aggFi, histFi, modelFi;
+    int32_t flags[4] = {0, 0, 0, 0};
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    hipEvent_t ev[kNumStages + 1] = {};
+    bool stageRan[kNumStages] = {};
+
+    BaselinerDev* bl = nullptr;
+    LabelerDev* lab = nullptr;
+    PulserDev* pul = nullptr;
+    int16_t* dTraces = nullptr;
+    int16_t* dBaselined = nullptr;
+    hwpa_baseliner_stats* dStats = nullptr;
+    hwpa_baseliner_stats* dAgg = nullptr;
+    hwpa_lane_hist* dHist = nullptr;
+    void* dEdge = nullptr;
+    hwpa_lane_model* dModels = nullptr;
+    int16_t* dLabels = nullptr;
+    int16_t* dLatent = nullptr;
+    float* dScore = nullptr;
+    hwpa_pulse* dPulses = nullptr;
+    uint32_t* dCounts = nullptr;
+    uint32_t* dOvf = nullptr;
+    hwpa_stat_acc* dPulseBl = nullptr;
+    float* dEmScratch = nullptr;
+    int32_t* dIters = nullptr;
+    bool ranEstimate = false;
+};
+
+static void FreeAnalyzer(hwpa_analyzer* a)
+{
+    if (!a) return;
+    if (a->stream) (void)hipStreamSynchronize(a->stream);
+    BaselinerDestroy(a->bl);
+    LabelerDestroy(a->lab);
+    PulserDestroy(a->pul);
+    void* bufs[] = {a->dTraces, a->dBaselined, a->dStats,  a->dAgg,    a->dHist,      a->dEdge,
+                    a->dModels, a->dLabels,    a->dLatent, a->dScore,  a->dPulses,    a->dCounts,
+                    a->dOvf,    a->dPulseBl,   a->dEmScratch, a->dIters};
+    for (void* p : bufs) DeviceFree(p);
+    for (auto& e : a->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (a->stream) (void)hipStreamDestroy(a->stream);
+    delete a;
+}
+
+extern "C" {
+
+int32_t hwpa_analyzer_create(uint32_t pool_id, uint32_t lanes, uint32_t frames, const hwpa_config* cfg,
+                             hwpa_analyzer** out)
+{
+    if (!cfg || !out || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "analyzer_create: bad argument");
+    *out = nullptr;
+    int32_t rc = hwpa_config_validate(cfg);
+    if (rc != HWPA_OK) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return Fail(HWPA_ERR_NO_DEVICE, "no HIP device");
+    const hwpa_filter_params fp = FilterParams(cfg->filter_type);
+    if (frames < (uint32_t)NL || frames % (uint32_t)fp.aggregate_stride != 0)
+        return Fail(HWPA_ERR_INVALID_ARG, "frames_per_chunk must be >= 24 and a multiple of the filter's aggregate stride");
+    auto* a = new hwpa_analyzer;
+    a->cfg = *cfg;
+    a->poolId = pool_id;
+    a->lanes = lanes;
+    a->frames = frames;
+    a->baselinerLatency = BaselinerStartupLatency(*cfg);
+    const uint32_t cpe = (std::max(cfg->min_frames_for_estimate, 1u) + frames - 1) / frames;
+    a->dmeDelay = (pool_id % cpe) * frames;
+    a->framesRemaining = (int32_t)cfg->num_frames_pre_accum_stats;
+    if (hipStreamCreate(&a->stream) != hipSuccess) {
+        delete a;
+        return Fail(HWPA_ERR_HIP, "hipStreamCreate failed");
+    }
+    for (auto& e : a->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            FreeAnalyzer(a);
+            return Fail(HWPA_ERR_HIP, "hipEventCreate failed");
+        }
+    }
+    const size_t trace = (size_t)lanes * frames * LS;
+    const uint32_t mc = cfg->max_calls_per_zmw;
+    struct A {
+        void** p;
+        size_t bytes;
+        const char* what;
+    } allocs[] = {
+        {(void**)&a->dTraces, trace * 2, "traces"},
+        {(void**)&a->dBaselined, trace * 2, "baselined"},
+        {(void**)&a->dStats, sizeof(hwpa_baseliner_stats) * lanes, "stats"},
+        {(void**)&a->dAgg, sizeof(hwpa_baseliner_stats) * lanes, "aggregate"},
+        {(void**)&a->dHist, sizeof(hwpa_lane_hist) * lanes, "histograms"},
+        {(void**)&a->dEdge, EdgeStateBytes() * lanes, "edge state"},
+        {(void**)&a->dModels, sizeof(hwpa_lane_model) * lanes, "models"},
+        {(void**)&a->dLabels, trace * 2, "labels"},
+        {(void**)&a->dLatent, (size_t)lanes * NL * LS * 2, "latent"},
+        {(void**)&a->dScore, (size_t)lanes * LS * 4, "score"},
+        {(void**)&a->dPulses, (size_t)lanes * LS * mc * sizeof(hwpa_pulse), "pulses"},
+        {(void**)&a->dCounts, (size_t)lanes * LS * 4, "counts"},
+        {(void**)&a->dOvf, (size_t)lanes * LS * 4, "overflows"},
+        {(void**)&a->dPulseBl, sizeof(hwpa_stat_acc) * lanes, "pulse stats"},
+        {(void**)&a->dEmScratch, DmeScratchBytes(lanes), "EM scratch"},
+        {(void**)&a->dIters, sizeof(int32_t) * lanes, "EM iterations"},
+    };
+    for (auto& x : allocs) {
+        rc = DeviceAlloc(x.p, x.bytes, x.what);
+        if (rc != HWPA_OK) {
+            FreeAnalyzer(a);
+            return rc;
+        }
+    }
+    if ((rc = BaselinerCreate(*cfg, lanes, &a->bl)) != HWPA_OK || (rc = LabelerCreate(*cfg, lanes, frames, &a->lab)) != HWPA_OK ||
+        (rc = PulserCreate(*cfg, lanes, &a->pul)) != HWPA_OK) {
+        FreeAnalyzer(a);
+        return rc;
+    }
+    if ((rc = StatsReset(a->dAgg, lanes, a->stream)) != HWPA_OK || (rc = EdgeStateInit(a->dEdge, lanes, a->stream)) != HWPA_OK ||
+        (rc = Fill32(a->dModels, 0, sizeof(hwpa_lane_model) * lanes / 4, a->stream)) != HWPA_OK ||
+        (rc = Fill32(a->dCounts, 0, (size_t)lanes * LS, a->stream)) != HWPA_OK ||
+        (rc = Fill32(a->dOvf, 0, (size_t)lanes * LS, a->stream)) != HWPA_OK ||
+        (rc = Fill32(a->dIters, 0, lanes, a->stream)) != HWPA_OK) {
+        FreeAnalyzer(a);
+        return rc;
+    }
+    if (hipStreamSynchronize(a->stream) != hipSuccess) {
+        FreeAnalyzer(a);
+        return Fail(HWPA_ERR_HIP, "analyzer init failed");
+    }
+    *out = a;
+    return HWPA_OK;
+}
+
+int32_t hwpa_analyzer_destroy(hwpa_analyzer* a)
+{
+    FreeAnalyzer(a);
+    return HWPA_OK;
+}
+
+#define HWPA_STAGE_BEGIN(a, k)                                          \
+    do {                                                                \
+        if ((a)->profiling) HWPA_HIP(hipEventRecord((a)->ev[k], (a)->stream)); \
+        (a)->stageRan[k] = true;                                        \
+    } while (0)
+
+int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t on_device, uint32_t pool_id,
+                              int32_t first_frame)
+{
+    if (!a || !traces) return Fail(HWPA_ERR_INVALID_ARG, "analyzer_process: null argument");
+    // BatchAnalyzer::operator() checks (BatchAnalyzer.cpp:151-152)
+    if (pool_id != a->poolId) return Fail(HWPA_ERR_BAD_POOL, "Bad pool ID.");
+    if (first_frame != a->nextFrame) return Fail(HWPA_ERR_BAD_FRAME, "Bad frame ID.");
+    const uint32_t L = a->lanes, F = a->frames;
+    const size_t traceBytes = (size_t)L * F * LS * 2;
+    hipStream_t s = a->stream;
+    int32_t rc;
+    std::memset(a->flags, 0, sizeof(a->flags));
+    std::fill(std::begin(a->stageRan), std::end(a->stageRan), false);
+    a->ranEstimate = false;
+
+    const int16_t* src = traces;
+    if (!on_device) {
+        HWPA_HIP(hipMemcpyAsync(a->dTraces, traces, traceBytes, hipMemcpyHostToDevice, s));
+        src = a->dTraces;
+    }
+    HWPA_STAGE_BEGIN(a, kStBaseline);
+    if ((rc = BaselinerRun(a->bl, src, F, a->dBaselined, a->dStats, s)) != HWPA_OK) return rc;
+    a->nextFrame = first_frame + (int32_t)F;
+
+    auto finish = [&](int32_t r) -> int32_t {
+        if (a->profiling) {
+            const hipError_t e = hipEventRecord(a->ev[kNumStages], s);
+            if (e != hipSuccess) return Fail(HWPA_ERR_HIP, "hipEventRecord failed");
+        }
+        return r;
+    };
+    const size_t nz = (size_t)L * LS;
+    auto zeroCounts = [&]() -> int32_t {
+        hipLaunchKernelGGL(ZeroCountsKernel, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, a->dCounts, a->dOvf, nz);
+        return CheckLaunch("ZeroCountsKernel");
+    };
+    // Early empty result during the baseliner start-up and the pool stagger (BatchAnalyzer.cpp:331-337)
+    if (first_frame < (int32_t)(a->baselinerLatency + a->dmeDelay)) {
+        if ((rc = zeroCounts()) != HWPA_OK) return rc;
+        return finish(HWPA_OK);
+    }
+
+    // DetectionModelEstimator::AddBatch
+    HWPA_STAGE_BEGIN(a, kStAggregate);
+    if ((rc = StatsMerge(a->dAgg, a->dStats, L, s)) != HWPA_OK) return rc;
+    a->aggFi.Hull(first_frame, first_frame + (int32_t)F);
+    if (a->status != kHistInit) {
+        HWPA_STAGE_BEGIN(a, kStHistogram);
+        if ((rc = HistAdd(a->dHist, a->dEdge, a->dBaselined, F, a->dModels, L, s)) != HWPA_OK) return rc;
+        a->histFi.Hull(first_frame, first_frame + (int32_t)F);
+    }
+    HWPA_STAGE_BEGIN(a, kStDme);
+    if (a->status != kSequencing) {
+        if ((rc = DmeInitModels(a->cfg, a->dAgg, a->dModels, L, s)) != HWPA_OK) return rc;
+        a->modelFi = a->aggFi;
+    }
+    a->framesRemaining -= (int32_t)F;
+    if (a->framesRemaining <= 0) {
+        if (a->status == kDmeInit) a->status = kSequencing;
+        if (a->status == kSequencing) {
+            const int32_t mfi[2] = {a->modelFi.lo, a->modelFi.hi};
+            const int32_t efi[2] = {a->histFi.lo, a->histFi.hi};
+            if ((rc = DmeEstimate(a->cfg, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, a->dEmScratch, s)) != HWPA_OK)
+                return rc;
+            a->modelFi = a->histFi;
+            a->flags[1] = 1;
+            a->ranEstimate = true;
+            a->fullEstimation = true;
+        }
+        if ((rc = HistReset(a->cfg, a->dHist, a->dAgg, L, s)) != HWPA_OK) return rc;
+        a->histFi = Interval{};
+        if ((rc = StatsReset(a->dAgg, L, s)) != HWPA_OK) return rc;
+        a->aggFi = Interval{};
+        if (a->status == kHistInit) a->status = kDmeInit;
+        a->framesRemaining = (int32_t)a->cfg.min_frames_for_estimate;
+    }
+
+    HWPA_STAGE_BEGIN(a, kStLabels);
+    if ((rc = LabelerRun(a->lab, a->dBaselined, F, a->dModels, a->dLabels, a->dLatent, a->dScore, s)) != HWPA_OK)
+        return rc;
+    a->flags[2] = 1;
+    HWPA_STAGE_BEGIN(a, kStPulses);
+    if ((rc = PulserRun(a->pul, a->dLabels, a->dLatent, a->dBaselined, F, first_frame - NL, a->dModels, a->dPulses,
+                        a->dCounts, a->dOvf, a->dPulseBl, s)) != HWPA_OK)
+        return rc;
+    if (a->fullEstimation) {
+        a->flags[0] = 1;
+    } else if ((rc = zeroCounts()) != HWPA_OK) {
+        return rc;
+    }
+    return finish(HWPA_OK);
+}
+
+int32_t hwpa_analyzer_sync(hwpa_analyzer* a)
+{
+    if (!a) return Fail(HWPA_ERR_INVALID_ARG, "null analyzer");
+    HWPA_HIP(hipStreamSynchronize(a->stream));
+    return HWPA_OK;
+}
+
+int32_t hwpa_result_flags(const hwpa_analyzer* a, int32_t flags[4])
+{
+    if (!a || !flags) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    std::memcpy(flags, a->flags, sizeof(a->flags));
+    return HWPA_OK;
+}
+
+int32_t hwpa_result_pulses(const hwpa_analyzer* a, const hwpa_pulse** d_pulses, const uint32_t** d_counts,
+                           const uint32_t** d_overflows, uint32_t* max_calls)
+{
+    if (!a) return Fail(HWPA_ERR_INVALID_ARG, "null analyzer");
+    if (d_pulses) *d_pulses = a->dPulses;
+    if (d_counts) *d_counts = a->dCounts;
+    if (d_overflows) *d_overflows = a->dOvf;
+    if (max_calls) *max_calls = a->cfg.max_calls_per_zmw;
+    return HWPA_OK;
+}
+
+int32_t hwpa_result_labels(const hwpa_analyzer* a, const int16_t** d)
+{
+    if (!a || !d) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *d = a->dLabels;
+    return HWPA_OK;
+}
+
+int32_t hwpa_result_baselined(const hwpa_analyzer* a, const int16_t** d)
+{
+    if (!a || !d) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *d = a->dBaselined;
+    return HWPA_OK;
+}
+
+int32_t hwpa_result_models(const hwpa_analyzer* a, const hwpa_lane_model** d)
+{
+    if (!a || !d) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *d = a->dModels;
+    return HWPA_OK;
+}
+
+int32_t hwpa_result_baseliner_stats(const hwpa_analyzer* a, const hwpa_baseliner_stats** d)
+{
+    if (!a || !d) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *d = a->dStats;
+    return HWPA_OK;
+}
+
+int32_t hwpa_result_pulse_baseline_stats(const hwpa_analyzer* a, const hwpa_stat_acc** d)
+{
+    if (!a || !d) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *d = a->dPulseBl;
+    return HWPA_OK;
+}
+
+int32_t hwpa_result_viterbi_score(const hwpa_analyzer* a, const float** d)
+{
+    if (!a || !d) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *d = a->dScore;
+    return HWPA_OK;
+}
+
+int32_t hwpa_analyzer_set_profiling(hwpa_analyzer* a, int32_t enable)
+{
+    if (!a) return Fail(HWPA_ERR_INVALID_ARG, "null analyzer");
+    a->profiling = enable != 0;
+    return HWPA_OK;
+}
+
+int32_t hwpa_analyzer_stage_ms(const hwpa_analyzer* a, float ms[8])
+{
+    if (!a || !ms) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    for (int i = 0; i < 8; ++i) ms[i] = 0.0f;
+    if (!a->profiling) return Fail(HWPA_ERR_INVALID_ARG, "profiling is not enabled");
+    HWPA_HIP(hipEventSynchronize(a->ev[kNumStages]));
+    // Each stage runs from its begin event to the next recorded event.
+    for (int k = 0; k < kNumStages; ++k) {
+        if (!a->stageRan[k]) continue;
+        int next = kNumStages;
+        for (int j = k + 1; j < kNumStages; ++j)
+            if (a->stageRan[j]) {
+                next = j;
+                break;
+            }
+        float t = 0.0f;
+        HWPA_HIP(hipEventElapsedTime(&t, a->ev[k], a->ev[next]));
+        ms[k] = t;
+    }
+    return HWPA_OK;
+}
+
+int32_t hwpa_analyzer_em_iterations(const hwpa_analyzer* a, int32_t* iters)
+{
+    if (!a || !iters) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *iters = 0;
+    if (!a->ranEstimate) return HWPA_OK;
+    std::vector<int32_t> h(a->lanes);
+    HWPA_HIP(hipStreamSynchronize(a->stream));
+    HWPA_HIP(hipMemcpy(h.data(), a->dIters, sizeof(int32_t) * a->lanes, hipMemcpyDeviceToHost));
+    for (int32_t v : h) *iters = std::max(*iters, v);
+    return HWPA_OK;
+}
+
+int32_t hwpa_analyzer_stream(const hwpa_analyzer* a, void** stream)
+{
+    if (!a || !stream) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *stream = (void*)a->stream;
+    return HWPA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,598 @@
+// dme.hip -- detection-model estimation: model initialisation from baseline
+// statistics and the grouped-data EM fit of the 5-component mixture
+// (DmeEmHost, traceAnalysis/DmeEmHost.cpp:131-925; DetectionModelHost
+// update/export, traceAnalysis/DetectionModelHost.cpp:69-285).
+//
+// Mapping: one wave64 per lane, one thread per ZMW.  The only coupling
+// between ZMWs is the EM early exit, which the reference takes when every ZMW
+// of the lane has converged or failed (DmeEmHost.cpp:485); here that is one
+// wave vote per iteration.  Boundary densities and baseline responsibilities
+// of the E-step are kept in a per-lane HBM scratch ([boundary][2][zmw], so
+// every access is a coalesced 256-byte row) for the M-step's variance pass.
+// All arithmetic keeps the host path's operation order so results match the
+// CPU restatement bit for bit.
+
+#include "internal.h"
+
+#include <cfloat>
+
+namespace hwpa {
+
+struct DmeDevCfg {
+    float blVarMin, blVarMax, blVarNominal, signalScaler, shotVarCoeff;
+    float refSnr;
+    float relAmp[NA], cv[NA];
+    float mixThresh0, mixThresh1, scaleSnrConfTol, pulseAmpReg, snrDrop, snrThresh0, snrThresh1, successThresh;
+    float hl0, hl1;
+    float log2pi;
+    int32_t iterLimit;
+    int32_t iterToLimit;
+    int32_t updateMethod;
+};
+
+static DmeDevCfg MakeDevCfg(const hwpa_config& c)
+{
+    DmeDevCfg d{};
+    const DmeConsts k = MakeDmeConsts(c);
+    d.blVarMin = k.blVarMin;
+    d.blVarMax = k.blVarMax;
+    d.blVarNominal = k.blVarNominal;
+    d.signalScaler = k.signalScaler;
+    d.shotVarCoeff = k.shotVarCoeff;
+    d.refSnr = c.movie.ref_snr;
+    for (int a = 0; a < NA; ++a) {
+        d.relAmp[a] = c.movie.analogs[a].rel_amplitude;
+        d.cv[a] = c.movie.analogs[a].excess_noise_cv;
+    }
+    d.mixThresh0 = c.analog_mix_fraction_threshold[0];
+    d.mixThresh1 = c.analog_mix_fraction_threshold[1];
+    d.scaleSnrConfTol = c.scale_snr_conf_tol;
+    d.pulseAmpReg = c.pulse_amp_regularization;
+    d.snrDrop = c.snr_drop_thresh;
+    d.snrThresh0 = c.min_analog_snr_thresh0;
+    d.snrThresh1 = c.min_analog_snr_thresh1;
+    d.successThresh = c.success_confidence_thresh;
+    d.hl0 = c.confidence_half_life[0];
+    d.hl1 = c.confidence_half_life[1];
+    d.log2pi = hw_logf(2.0f * 3.1415926536f);
+    d.iterLimit = (int32_t)c.em_iteration_limit;
+    d.iterToLimit = c.iterate_to_limit;
+    d.updateMethod = (int32_t)c.model_update_method;
+    return d;
+}
+
+struct Mode {
+    float w, mean, var;
+};
+struct Model {
+    Mode bl;
+    Mode an[NA];
+    float conf;
+};
+
+__device__ __forceinline__ float Covar(const DmeDevCfg& d, float cv2, float mean, float blVar)
+{
+    float r = blVar;
+    r += d.shotVarCoeff * mean;
+    r += cv2 * (mean * mean);
+    return r;
+}
+
+__device__ __forceinline__ float Variance(float m0, float m1, float m2)
+{
+    float var = m1 * m1 / m0;
+    var = (m2 - var) / (m0 - 1.0f);
+    var = hw_maxf(var, 0.0f);
+    return (m0 > 1.0f) ? var : hw_u2f(0x7fc00000u);
+}
+
+__device__ Model LoadModel(const hwpa_lane_model& m, int z)
+{
+    Model r;
+    r.bl.w = hw_h2f(m.baseline.weights[z]);
+    r.bl.mean = hw_h2f(m.baseline.means[z]);
+    r.bl.var = hw_h2f(m.baseline.vars[z]);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        r.an[a].w = hw_h2f(m.analogs[a].weights[z]);
+        r.an[a].mean = hw_h2f(m.analogs[a].means[z]);
+        r.an[a].var = hw_h2f(m.analogs[a].vars[z]);
+    }
+    r.conf = hw_h2f(m.confidence[z]);
+    return r;
+}
+
+__device__ void StoreModel(const Model& r, hwpa_lane_model& m, int z)
+{
+    m.baseline.weights[z] = hw_f2h(r.bl.w);
+    m.baseline.means[z] = hw_f2h(r.bl.mean);
+    m.baseline.vars[z] = hw_f2h(hw_minf(r.bl.var, 65000.0f));
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        m.analogs[a].weights[z] = hw_f2h(r.an[a].w);
+        m.analogs[a].means[z] = hw_f2h(r.an[a].mean);
+        m.analogs[a].vars[z] = hw_f2h(hw_minf(r.an[a].var, 65000.0f));
+    }
+    m.confidence[z] = hw_f2h(r.conf);
+}
+
+// DetectionModelHost::Update0/1/2 + Update(other) (DetectionModelHost.cpp:69-205)
+__device__ void UpdateConf(const DmeDevCfg& d, Model& t, const Model& o)
+{
+    const float confSum = t.conf + o.conf;
+    const float f = (confSum > 0.0f) ? o.conf / confSum : 0.0f;
+    const float a = 1.0f - f;
+    const float b = f;
+    if (d.updateMethod == 0) {
+        t.bl.mean = a * t.bl.mean + b * o.bl.mean;
+        t.bl.var = a * t.bl.var + b * o.bl.var;
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            t.an[i].mean = a * t.an[i].mean + b * o.an[i].mean;
+            t.an[i].var = a * t.an[i].var + b * o.an[i].var;
+        }
+    } else {
+        const float prevBlVar = t.bl.var;
+        const float bw = a * t.bl.w + b * o.bl.w;
+        t.bl.w = bw;
+        t.bl.mean = a * t.bl.mean + b * o.bl.mean;
+        const float bv = hw_powf(t.bl.var, a) * hw_powf(o.bl.var, b);
+        t.bl.var = bv;
+        const float aw = 0.25f * (1.0f - bw);
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            t.an[i].w = aw;
+            float cv2;
+            if (d.updateMethod == 2) {
+                float tx = t.an[i].var - prevBlVar;
+                tx -= t.an[i].mean * d.shotVarCoeff;
+                tx /= (t.an[i].mean * t.an[i].mean);
+                float ox = o.an[i].var - o.bl.var;
+                ox -= o.an[i].mean * d.shotVarCoeff;
+                ox /= (o.an[i].mean * o.an[i].mean);
+                cv2 = a * tx + b * ox;
+            } else {
+                cv2 = d.cv[i] * d.cv[i];
+            }
+            const float am = hw_powf(t.an[i].mean, a) * hw_powf(o.an[i].mean, b);
+            t.an[i].mean = am;
+            t.an[i].var = Covar(d, cv2, am, bv);
+        }
+    }
+    t.conf = confSum;
+}
+
+// DmeEmHost::InitLaneDetModel (DmeEmHost.cpp:871-925)
+__global__ __launch_bounds__(64) void DmeInitKernel(const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
+                                                    DmeDevCfg d)
+{
+    const hwpa_baseliner_stats& s = stats[blockIdx.x];
+    hwpa_lane_model& m = models[blockIdx.x];
+    const int z = threadIdx.x;
+    const float m0 = s.baseline.m0[z], m1 = s.baseline.m1[z], m2 = s.baseline.m2[z];
+    const float blWeight = m0 / s.full.basic.m0[z];
+    float blMean = m1 / m0 + s.baseline.offset[z];
+    float blVar = Variance(m0, m1, m2);
+    if (hw_isnan(blMean)) blMean = 0.0f;
+    if (hw_isnan(blVar)) blVar = 100.0f;
+    blVar = hw_clampf(blVar, d.blVarMin, d.blVarMax);
+    m.confidence[z] = hw_f2h(0.1f);
+    m.baseline.weights[z] = hw_f2h(blWeight);
+    m.baseline.means[z] = hw_f2h(blMean);
+    m.baseline.vars[z] = hw_f2h(blVar);
+    const float amw = (1.0f - blWeight) / (float)NA;
+    const float refSignal = d.refSnr * hw_sqrtf(blVar);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        m.analogs[a].weights[z] = hw_f2h(amw);
+        const float aMean = hw_maxf(0.0f, blMean + d.relAmp[a] * refSignal);
+        m.analogs[a].means[z] = hw_f2h(aMean);
+        m.analogs[a].vars[z] = hw_f2h(Covar(d, d.cv[a] * d.cv[a], aMean, blVar));
+    }
+}
+
+// Histogram view of one ZMW (UHistogramSimd(const LaneHistogram&), UHistogramSimd.cpp:100-118).
+struct HistView {
+    const hwpa_lane_hist* h;
+    int z;
+    float lb, bs;
+    __device__ float Start(int b) const { return lb + (float)b * bs; }
+    // Index NB is the always-empty overflow bin of UHistogramSimd (UHistogramSimd.cpp:66).
+    __device__ uint32_t Cnt(int b) const { return (b < NB) ? (uint32_t)h->bin_count[b][z] : 0u; }
+};
+
+__device__ uint32_t InRange(const HistView& h)
+{
+    uint32_t s = 0;
+    for (int i = 0; i < NB; ++i) s = (s + h.Cnt(i)) & 0xffffu;
+    return s;
+}
+
+// UHistogramSimd::CumulativeCount (UHistogramSimd.cpp:199-238)
+__device__ float CumulativeCount(const HistView& h, uint32_t lowOut, uint32_t inRange, float x)
+{
+    const bool mNan = hw_isnan(x);
+    const bool mLow = x < h.Start(0);
+    const bool mHigh = x >= h.Start(NB);
+    const bool mIn = !(mNan || mLow || mHigh);
+    int xbin = hw_floor_i32((x - h.Start(0)) / h.bs);
+    if (!mIn) xbin = 0;
+    const float xrem = x - h.Start(xbin);
+    const float xbinCount = (float)h.Cnt(xbin);
+    float cc = (float)lowOut;
+    uint32_t below = 0;
+    for (int i = 0; i < xbin; ++i) below = (below + h.Cnt(i)) & 0xffffu;
+    cc += (float)below;
+    cc += xbinCount * xrem / h.bs;
+    if (mLow) cc = (float)lowOut;
+    if (mHigh) cc = (float)((lowOut + inRange) & 0xffffu);
+    if (mNan) cc = hw_u2f(0x7fc00000u);
+    return cc;
+}
+
+// UHistogramSimd::Fractile (UHistogramSimd.cpp:140-197)
+__device__ float Fractile(const HistView& h, uint32_t lowOut, uint32_t total, float frac)
+{
+    const float nfz = frac * (float)total;
+    uint32_t n = lowOut;
+    if (n > 0 && (float)n >= nfz) return hw_u2f(0xff800000u);
+    int i = 0;
+    while ((n == 0 || (float)n < nfz) && i < NB) n = (n + h.Cnt(i++)) & 0xffffu;
+    if ((float)n < nfz) return hw_u2f(0x7f800000u);
+    i -= 1;
+    const float x0 = h.Start(i);
+    const uint32_t niz = h.Cnt(i);
+    const int m = (int)n - (int)niz;
+    return x0 + h.bs * (nfz - (float)m) / (float)(niz + 1);
+}
+
+__device__ __forceinline__ float Sum5Tree(const float* v) { return (v[0] + v[1]) + (v[2] + (v[3] + v[4])); }
+
+// DmeEmHost::EstimateLaneDetModel (DmeEmHost.cpp:216-615)
+__global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __restrict__ hists,
+                                                        const hwpa_baseliner_stats* __restrict__ stats,
+                                                        hwpa_lane_model* models, float* __restrict__ scratch,
+                                                        int32_t* iters, DmeDevCfg d, float evolve)
+{
+    const int lane = blockIdx.x;
+    const int z = threadIdx.x;
+    const hwpa_lane_hist& lh = hists[lane];
+    const hwpa_baseliner_stats& bs = stats[lane];
+    hwpa_lane_model& lm = models[lane];
+    // scratch rows: [b][0][z] = boundary density, [b][1][z] = baseline responsibility
+    float* sc = scratch + (size_t)lane * (NB + 1) * 2 * LS + z;
+
+    HistView h{&lh, z, lh.low_bound[z], lh.bin_size[z]};
+    const uint32_t lowOut = lh.outlier_low[z], highOut = lh.outlier_high[z];
+    const uint32_t inRange = InRange(h);
+    const uint32_t numFrames = (inRange + ((lowOut + highOut) & 0xffffu)) & 0xffffu;
+
+    const float rpa[NA] = {d.relAmp[0], d.relAmp[1], d.relAmp[2], d.relAmp[3]};
+    const float rpaSqSum = (rpa[0] * rpa[0] + rpa[2] * rpa[2]) + (rpa[1] * rpa[1] + rpa[3] * rpa[3]);
+    const float rpaMin = hw_minf(hw_minf(rpa[0], rpa[1]), hw_minf(rpa[2], rpa[3]));
+
+    Model det = LoadModel(lm, z);
+    det.conf *= evolve;
+
+    // PrelimEstimate (DmeEmHost.cpp:161-214)
+    {
+        Model w = det;
+        const float nBl = bs.baseline.m0[z];
+        const float total = bs.full.basic.m0[z];
+        const float blWeight = hw_maxf(nBl / total, 0.01f);
+        const bool mask = nBl >= 3.0f;
+        float blVar = mask ? Variance(bs.baseline.m0[z], bs.baseline.m1[z], bs.baseline.m2[z]) : w.bl.var;
+        blVar = hw_clampf(blVar, d.blVarMin, d.blVarMax);
+        const float blMean = mask ? (bs.baseline.m1[z] / bs.baseline.m0[z] + bs.baseline.offset[z]) : w.bl.mean;
+        const float scale = hw_sqrtf(blVar / w.bl.var);
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const float mean = w.an[a].mean * scale;
+            const float var = Covar(d, d.cv[a] * d.cv[a], mean, blVar);
+            w.an[a].mean = mean;
+            w.an[a].var = var;
+            w.an[a].w = 0.25f * (1.0f - blWeight);
+        }
+        w.bl.w = blWeight;
+        w.bl.mean = blMean;
+        w.bl.var = blVar;
+        float conf = 0.1f * hw_satlin(0.0f, 500.0f, nBl - 3.0f);
+        conf *= hw_satlin(d.blVarMax, 0.5f * (d.blVarNominal + d.blVarMax), blVar);
+        w.conf = conf;
+        UpdateConf(d, det, w);
+    }
+    Model work = det;
+
+    // PrelimScaleFactor (DmeEmHost.cpp:618-654) and ScaleModelSnr (:837-852)
+    float scaleFactor;
+    {
+        const float bgSigma = hw_sqrtf(work.bl.var);
+        const float thresh = 2.5f * bgSigma + work.bl.mean;
+        const float bgCount = CumulativeCount(h, lowOut, inRange, thresh);
+        const float totalCount = (float)((lowOut + inRange) & 0xffffu);
+        const float fractile = 0.5f * (bgCount + totalCount) / totalCount;
+        float avg = 0.0f;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) avg += work.an[a].mean;
+        avg /= (float)NA;
+        const float w = hw_satlin(0.0f, d.scaleSnrConfTol, work.conf);
+        float sf = Fractile(h, lowOut, numFrames, fractile) / avg;
+        sf = (1.0f - w) * sf + w;
+        scaleFactor = hw_minf(hw_maxf(sf, 0.1f), 10.0f);
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        work.an[a].mean = scaleFactor * work.an[a].mean;
+        work.an[a].var = Covar(d, d.cv[a] * d.cv[a], work.an[a].mean, work.bl.var);
+    }
+
+    float rho[5], mu[5], var[5], rhoEst[5], muEst[5], varEst[5];
+    const float rhoMin = 0.001f;
+    rho[0] = hw_maxf(work.bl.w, rhoMin);
+    mu[0] = work.bl.mean;
+    var[0] = work.bl.var;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        rho[a + 1] = hw_maxf(work.an[a].w, rhoMin);
+        mu[a + 1] = work.an[a].mean;
+        var[a + 1] = work.an[a].var;
+    }
+    const float varQuant = (h.bs * h.bs) / 12.0f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) var[k] += varQuant;
+    var[0] = hw_minf(var[0], d.blVarMax);
+    {
+        const float rs = Sum5Tree(rho);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rho[k] = rho[k] / rs;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        rhoEst[k] = rho[k];
+        muEst[k] = mu[k];
+        varEst[k] = var[k];
+    }
+    float s = ((mu[1] * rpa[0] + mu[2] * rpa[1]) + (mu[3] * rpa[2] + mu[4] * rpa[3])) / rpaSqSum;
+    const float sExpect = s / scaleFactor;
+    const float sExpectWeight = det.conf * d.pulseAmpReg;
+    float logLike = -FLT_MAX;
+    const float binProbMin = hw_maxf((float)numFrames * h.bs, 1.0f) * FLT_MIN;
+    int status = (numFrames < 160u) ? HWPA_ZS_INSUF_DATA : HWPA_ZS_OK;
+    bool converged = false;
+    float mlLogLike = hw_u2f(0xff800000u);
+    float njSum = 0.0f;
+    for (int b = 0; b < NB; ++b) njSum += (float)h.Cnt(b);
+    const float halfBin = 0.5f * h.bs;
+
+    int it = 0;
+    for (; it < d.iterLimit; ++it) {
+        // ---- E-step
+        float varinv[5], pref[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            varinv[k] = 1.0f / var[k];
+            pref[k] = (hw_logf(rho[k]) - 0.5f * hw_logf(var[k])) - 0.5f * d.log2pi;
+        }
+        float probSum = 0.0f, wps = 0.0f;
+        float ci[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, mom1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+        float bpPrev, tauPrev[5];
+        // corner b: boundary density and responsibilities at h.Start(b)
+        auto corner = [&](int b, float& bpOut, float* tauOut) {
+            const float x = h.Start(b);
+            float t[5];
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                const float y = x - mu[i];
+                t[i] = pref[i] - 0.5f * y * y * varinv[i];
+            }
+            float tmax = t[0];
+#pragma unroll
+            for (int i = 1; i < 5; ++i) tmax = hw_maxf(tmax, t[i]);
+            float es = hw_expf(t[0] - tmax);
+#pragma unroll
+            for (int i = 1; i < 5; ++i) es = es + hw_expf(t[i] - tmax);
+            const float llb = tmax + hw_logf(es);
+#pragma unroll
+            for (int i = 0; i < 5; ++i) tauOut[i] = hw_expf(t[i] - llb);
+            bpOut = hw_expf(llb);
+            sc[(size_t)(b * 2 + 0) * LS] = bpOut;
+            sc[(size_t)(b * 2 + 1) * LS] = tauOut[0];
+        };
+        corner(0, bpPrev, tauPrev);
+        float x0 = h.Start(0);
+        for (int b = 0; b < NB; ++b) {
+            float bpNext, tauNext[5];
+            corner(b + 1, bpNext, tauNext);
+            const float binp = hw_maxf(halfBin * (bpPrev + bpNext), binProbMin);
+            probSum += binp;
+            const float nj = (float)h.Cnt(b);
+            wps += nj * hw_logf(binp);
+            const float factor = nj / binp;
+            const float x1 = h.Start(b + 1);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const float tmp1 = bpPrev * tauPrev[m];
+                const float tmp2 = bpNext * tauNext[m];
+                ci[m] += (tmp1 + tmp2) * factor;
+                mom1[m] += (tmp1 * x0 + tmp2 * x1) * factor;
+            }
+            bpPrev = bpNext;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) tauPrev[m] = tauNext[m];
+            x0 = x1;
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            ci[m] *= halfBin;
+            mom1[m] *= halfBin;
+        }
+        const float logLikePrev = logLike;
+        logLike = wps - njSum * hw_logf(probSum);
+        const float ds = s - sExpect;
+        logLike -= 0.5f * (ds * ds) * sExpectWeight;
+        const float delta = logLike - logLikePrev;
+        const float convTol = 1e-4f * hw_fabsf(logLike);
+        bool conv = (delta >= 0.0f) && (delta < convTol);
+        conv = conv && (logLike >= mlLogLike);
+        if (conv) {
+            mlLogLike = logLike;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                rhoEst[k] = rho[k];
+                muEst[k] = mu[k];
+                varEst[k] = var[k];
+            }
+        }
+        converged = converged || conv;
+        // Lane-wide early exit (DmeEmHost.cpp:485)
+        const bool done = converged || (status != HWPA_ZS_OK);
+        if (!d.iterToLimit && __all(done)) break;
+
+        // ---- M-step
+        float c[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) c[m] = hw_maxf(ci[m], 0.1f);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) rho[m] = c[m] / njSum;
+        {
+            const float rs = Sum5Tree(rho);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) rho[m] = rho[m] / rs;
+        }
+        mu[0] = mom1[0] / c[0];
+        float numer = 0.0f, denom = 0.0f;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const int i = a + 1;
+            numer += rpa[a] * varinv[i] * mom1[i];
+            denom += (rpa[a] * rpa[a]) * varinv[i] * c[i];
+        }
+        numer += sExpect * sExpectWeight;
+        denom += sExpectWeight;
+        s = numer / denom;
+        const float minPulseMean = hw_maxf(1.0f, d.signalScaler);
+        const float sMin = hw_maxf(0.5f * hw_sqrtf(var[0]), minPulseMean) / rpaMin;
+        if (s < sMin) {
+            s = sMin;
+            status |= HWPA_ZS_VLOW_SIGNAL;
+        }
+#pragma unroll
+        for (int a = 0; a < NA; ++a) mu[a + 1] = s * rpa[a];
+        float mom2 = 0.0f;
+        float bpa = sc[0], taua = sc[LS];
+        for (int j = 0; j < NB; ++j) {
+            const float bpb = sc[(size_t)((j + 1) * 2 + 0) * LS];
+            const float taub = sc[(size_t)((j + 1) * 2 + 1) * LS];
+            const float xa = h.Start(j) - mu[0];
+            const float xb = h.Start(j + 1) - mu[0];
+            float tmp = bpa * xa * xa * taua;
+            tmp += bpb * xb * xb * taub;
+            const float binp = hw_maxf(halfBin * (bpa + bpb), binProbMin);
+            tmp *= (float)h.Cnt(j) / binp;
+            mom2 += tmp;
+            bpa = bpb;
+            taua = taub;
+        }
+        mom2 *= 0.5f * h.bs;
+        var[0] = mom2 / c[0] + varQuant;
+        var[0] = hw_clampf(var[0], d.blVarMin, d.blVarMax);
+#pragma unroll
+        for (int a = 0; a < NA; ++a) var[a + 1] = Covar(d, d.cv[a] * d.cv[a], mu[a + 1], var[0]);
+    }
+    if (!converged) status |= HWPA_ZS_NO_CONVERGE;
+    if (z == 0 && iters) iters[lane] = it;
+
+    work.bl.w = rhoEst[0];
+    work.bl.mean = muEst[0];
+    work.bl.var = varEst[0];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        work.an[a].w = rhoEst[a + 1];
+        work.an[a].mean = muEst[a + 1];
+        work.an[a].var = varEst[a + 1];
+    }
+
+    // ComputeConfidence (DmeEmHost.cpp:717-803); the G-test factor stays 1.
+    float conf = 1.0f;
+    {
+        float cf[8];
+        cf[0] = converged ? 1.0f : 0.0f;
+        cf[1] = hw_satlin(0.05f, 0.15f, work.bl.w);
+        float x = (work.bl.mean * work.bl.mean) / work.bl.var;
+        cf[2] = hw_expf(-x / (2.0f * (1.0f * 1.0f)));
+        x = hw_log2f(work.bl.var / det.bl.var);
+        const float tol = 1.5f / (0.5f + det.conf);
+        cf[3] = hw_expf(-x * x / (2.0f * (tol * tol)));
+        x = 1.0f;
+        if (d.mixThresh1 > 0.0f) {
+            x *= hw_satlin(d.mixThresh0, d.mixThresh1, work.an[0].w);
+            x *= hw_satlin(d.mixThresh0, d.mixThresh1, work.an[NA - 1].w);
+        }
+        cf[4] = x;
+        const float bgSigma = hw_sqrtf(work.bl.var);
+        x = work.an[NA - 1].mean;
+        x /= bgSigma;
+        cf[5] = hw_satlin(d.snrThresh0, d.snrThresh1, x);
+        if (d.snrDrop < 0.0f) {
+            cf[6] = 1.0f;
+        } else {
+            const float snrEst = work.an[0].mean / bgSigma;
+            const float refBgSigma = hw_sqrtf(det.bl.var);
+            const float refSnr0 = det.an[0].mean / refBgSigma;
+            float refSnr1 = det.an[1].mean / refBgSigma;
+            refSnr1 *= d.snrDrop;
+            refSnr1 *= hw_minf(det.conf, 1.0f);
+            cf[6] = hw_satlin(refSnr1, hw_sqrtf(refSnr0 * refSnr1), snrEst);
+        }
+        cf[7] = 1.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) conf *= cf[k];
+    }
+    conf = hw_minf(hw_maxf(0.0f, conf), 1.0f);
+    conf = (conf >= d.successThresh) ? conf : 0.0f;
+    work.conf = conf;
+    UpdateConf(d, det, work);
+    StoreModel(det, lm, z);
+}
+
+size_t DmeScratchBytes(uint32_t lanes) { return (size_t)lanes * (NB + 1) * 2 * LS * sizeof(float); }
+
+int32_t DmeInitModels(const hwpa_config& cfg, const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
+                      uint32_t lanes, hipStream_t s)
+{
+    if (!stats || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "dme_init_models: bad argument");
+    hipLaunchKernelGGL(DmeInitKernel, dim3(lanes), dim3(LS), 0, s, stats, models, MakeDevCfg(cfg));
+    return CheckLaunch("DmeInitKernel");
+}
+
+int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
+                    hwpa_lane_model* models, const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
+                    int32_t* d_iters, float* scratch, hipStream_t s)
+{
+    if (!hist || !stats || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "dme_estimate: bad argument");
+    const DmeDevCfg d = MakeDevCfg(cfg);
+    // DmeEmHost::EvolveModel (DmeEmHost.cpp:806-834), DetectionModelHost::EvolveConfidence
+    const int32_t tModel = model_fi[0] + (int32_t)((uint32_t)(model_fi[1] - model_fi[0]) / 2u);
+    const int32_t tEst = est_fi[0] + (int32_t)((uint32_t)(est_fi[1] - est_fi[0]) / 2u);
+    const float thl = 0.5f * (float)(tModel + tEst);
+    const float hl = d.hl0 + (d.hl1 - d.hl0) * hw_satlin(56160.0f, 168480.0f, thl);
+    const float tDiff = (float)(tEst - tModel);
+    const float evolve = hw_exp2f(-hw_fabsf(tDiff) / hl);
+    float* tmp = nullptr;
+    if (!scratch) {
+        // Stand-alone call: private scratch for this call only.
+        const int32_t rc = DeviceAlloc((void**)&tmp, DmeScratchBytes(lanes), "EM scratch");
+        if (rc != HWPA_OK) return rc;
+        scratch = tmp;
+    }
+    hipLaunchKernelGGL(DmeEstimateKernel, dim3(lanes), dim3(LS), 0, s, hist, stats, models, scratch, d_iters, d,
+                       evolve);
+    int32_t rc = CheckLaunch("DmeEstimateKernel");
+    if (tmp) {
+        if (hipStreamSynchronize(s) != hipSuccess && rc == HWPA_OK) rc = Fail(HWPA_ERR_HIP, "EM kernel failed");
+        DeviceFree(tmp);
+    }
+    return rc;
+}
+
+}  // namespace hwpa

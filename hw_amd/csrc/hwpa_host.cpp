@@ -1,0 +1,698 @@
+// hwpa_host.cpp -- host side of libhwpa: error reporting, configuration
+// (defaults, JSON overrides, validation), derived tables, and the C-ABI
+// wrappers of the stage entry points.
+
+#include "internal.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+
+namespace hwpa {
+
+static thread_local std::string g_err;
+
+void SetError(const std::string& msg) { g_err = msg; }
+void ClearError() { g_err.clear(); }
+
+int32_t Fail(int32_t code, const std::string& msg)
+{
+    SetError(msg);
+    return code;
+}
+
+int32_t CheckLaunch(const char* what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return Fail(HWPA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return HWPA_OK;
+}
+
+int32_t DeviceAlloc(void** p, size_t bytes, const char* what)
+{
+    *p = nullptr;
+    if (bytes == 0) return HWPA_OK;
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return Fail(HWPA_ERR_OOM, std::string("hipMalloc(") + what + ", " + std::to_string(bytes) +
+                                      " bytes): " + hipGetErrorString(e));
+    }
+    return HWPA_OK;
+}
+
+void DeviceFree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+__global__ void Fill32Kernel(uint32_t* d, uint32_t v, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = v;
+}
+
+int32_t Fill32(void* d, uint32_t value, size_t words, hipStream_t s)
+{
+    if (words == 0) return HWPA_OK;
+    const size_t blocks = std::min<size_t>((words + 255) / 256, 8192);
+    hipLaunchKernelGGL(Fill32Kernel, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t*)d, value, words);
+    return CheckLaunch("Fill32Kernel");
+}
+
+// traceAnalysis/BaselinerParams.cpp:10-72, BaselinerParams.h:45-70
+hwpa_filter_params FilterParams(int32_t type)
+{
+    hwpa_filter_params p{};
+    auto set = [&](int n, const int* s, const int* w, float sb, float mb) {
+        p.num_stages = n;
+        for (int i = 0; i < n; ++i) {
+            p.strides[i] = s[i];
+            p.widths[i] = w[i];
+        }
+        p.sigma_bias = sb;
+        p.mean_bias = mb;
+    };
+    static const int s3[3] = {1, 2, 8}, s2[2] = {2, 8};
+    static const int wl3[3] = {11, 17, 61}, wm3[3] = {9, 17, 31}, ws3[3] = {7, 17, 17};
+    static const int wl2[2] = {11, 61}, wm2[2] = {9, 31}, ws2[2] = {7, 17};
+    switch (type) {
+    case HWPA_FILTER_MULTISCALE_LARGE: set(3, s3, wl3, 2.64f, 0.78f); break;
+    case HWPA_FILTER_MULTISCALE_MEDIUM: set(3, s3, wm3, 2.41f, 0.69f); break;
+    case HWPA_FILTER_MULTISCALE_SMALL: set(3, s3, ws3, 2.15f, 0.63f); break;
+    case HWPA_FILTER_TWOSCALE_LARGE: set(2, s2, wl2, 2.71f, 0.60f); break;
+    case HWPA_FILTER_TWOSCALE_MEDIUM: set(2, s2, wm2, 2.44f, 0.50f); break;
+    default: set(2, s2, ws2, 2.07f, 0.41f); break;
+    }
+    // LatentSize: next power of two >= sum(w_i * prod_{j<=i} s_j); AggregateStride = prod(s)
+    int cum = 1, minSize = 0;
+    for (int i = 0; i < p.num_stages; ++i) {
+        cum *= p.strides[i];
+        minSize += p.widths[i] * cum;
+    }
+    int ret = 1;
+    while (ret < minSize) ret *= 2;
+    p.latent_size = ret;
+    p.aggregate_stride = cum;
+    return p;
+}
+
+// CoreDMEstimator::Configuration (traceAnalysis/CoreDMEstimator.h:67-109)
+DmeConsts MakeDmeConsts(const hwpa_config& c)
+{
+    DmeConsts d{};
+    d.signalScaler = c.movie.photoelectron_sensitivity;
+    d.shotVarCoeff = c.movie.shot_var_coeff;
+    const float ss = std::max(d.signalScaler, 1.0f);
+    d.blVarMin = ss * ss * (1.0f / 12.0f);
+    const float t = 2.0f * d.signalScaler;
+    d.blVarNominal = t * t;
+    d.blVarMax = 36.0f * d.blVarNominal;
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// Minimal JSON reader: flattens objects to dotted keys and arrays to key[i].
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct JsonFlat {
+    const char* p;
+    std::map<std::string, std::string> out;
+    std::string err;
+
+    void Ws()
+    {
+        while (*p && std::isspace((unsigned char)*p)) ++p;
+    }
+    bool Str(std::string& s)
+    {
+        if (*p != '"') return false;
+        ++p;
+        while (*p && *p != '"') {
+            if (*p == '\\' && p[1]) ++p;
+            s.push_back(*p++);
+        }
+        if (*p != '"') return false;
+        ++p;
+        return true;
+    }
+    bool Value(const std::string& key)
+    {
+        Ws();
+        if (*p == '{') {
+            ++p;
+            Ws();
+            if (*p == '}') {
+                ++p;
+                return true;
+            }
+            while (true) {
+                Ws();
+                std::string k;
+                if (!Str(k)) return false;
+                Ws();
+                if (*p != ':') return false;
+                ++p;
+                if (!Value(key.empty() ? k : key + "." + k)) return false;
+                Ws();
+                if (*p == ',') {
+                    ++p;
+                    continue;
+                }
+                if (*p == '}') {
+                    ++p;
+                    return true;
+                }
+                return false;
+            }
+        }
+        if (*p == '[') {
+            ++p;
+            int i = 0;
+            Ws();
+            if (*p == ']') {
+                ++p;
+                return true;
+            }
+            while (true) {
+                if (!Value(key + "[" + std::to_string(i++) + "]")) return false;
+                Ws();
+                if (*p == ',') {
+                    ++p;
+                    continue;
+                }
+                if (*p == ']') {
+                    ++p;
+                    return true;
+                }
+                return false;
+            }
+        }
+        if (*p == '"') {
+            std::string s;
+            if (!Str(s)) return false;
+            out[key] = s;
+            return true;
+        }
+        const char* b = p;
+        while (*p && *p != ',' && *p != '}' && *p != ']' && !std::isspace((unsigned char)*p)) ++p;
+        if (p == b) return false;
+        out[key] = std::string(b, p);
+        return true;
+    }
+};
+
+bool ToF(const std::string& v, double* d)
+{
+    if (v == "true") {
+        *d = 1;
+        return true;
+    }
+    if (v == "false") {
+        *d = 0;
+        return true;
+    }
+    char* e = nullptr;
+    *d = std::strtod(v.c_str(), &e);
+    return e && *e == 0;
+}
+
+int FilterByName(const std::string& v)
+{
+    static const char* names[] = {"MultiScaleLarge", "MultiScaleMedium", "MultiScaleSmall",
+                                  "TwoScaleLarge",   "TwoScaleMedium",   "TwoScaleSmall"};
+    for (int i = 0; i < 6; ++i)
+        if (v == names[i]) return i;
+    return -1;
+}
+
+}  // namespace
+
+}  // namespace hwpa
+
+using namespace hwpa;
+
+extern "C" {
+
+const char* hwpa_last_error(void) { return g_err.c_str(); }
+
+int32_t hwpa_config_default(hwpa_config* c)
+{
+    if (!c) return Fail(HWPA_ERR_INVALID_ARG, "null config");
+    std::memset(c, 0, sizeof(*c));
+    // BasecallerBaselinerConfig (dataTypes/configs/BasecallerBaselinerConfig.h:48-89)
+    c->filter_type = HWPA_FILTER_MULTISCALE_LARGE;
+    c->mean_bias_adjust = 0.0f;
+    c->sigma_bias_adjust = 0.0f;
+    c->mean_ema_scale_strides = 0.0f;
+    c->sigma_ema_scale_strides = 512.0f;
+    c->jump_suppression = 0.0f;
+    // BasecallerTraceHistogramConfig (BasecallerTraceHistogramConfig.h:57-72)
+    c->bin_size_coeff = 0.2f;
+    c->bin_size_low_bound_coeff = 1.0f;
+    c->baseline_stat_min_frame_count = 50;
+    c->fallback_baseline_sigma = 10.0f;
+    // BasecallerDmeConfig (BasecallerDmeConfig.h:82-223)
+    c->analog_mix_fraction_threshold[0] = 0.039f / 3;
+    c->analog_mix_fraction_threshold[1] = 0.039f;
+    c->scale_snr_conf_tol = 1.0f;
+    c->em_iteration_limit = 20;
+    c->gtest_stat_factor = -1.0f;
+    c->iterate_to_limit = 0;
+    c->min_analog_snr_thresh0 = 2.0f;
+    c->min_analog_snr_thresh1 = 4.0f;
+    c->pulse_amp_regularization = 0.0f;
+    c->model_update_method = 1;
+    c->snr_drop_thresh = 1.0f;
+    c->success_confidence_thresh = 0.10f;
+    c->min_frames_for_estimate = 4000;
+    c->num_frames_pre_accum_stats = 1000;
+    c->confidence_half_life[0] = 4096.0f;
+    c->confidence_half_life[1] = 12000.0f;
+    // BasecallerFrameLabelerConfig (BasecallerFrameLabelerConfig.h:39-84)
+    c->viterbi_alpha = 1.0f;
+    c->viterbi_beta = 1.0f;
+    c->viterbi_gamma = 1.0f;
+    c->roi_upper_threshold = 7.0f;
+    c->roi_lower_threshold = 2.0f;
+    // BasecallerPulseAccumConfig (BasecallerPulseAccumConfig.h:89-98)
+    c->max_calls_per_zmw = 48;
+    c->xsp_amp_thresh = 0.70;
+    c->xsp_width_thresh = 3.5f;
+    c->learned_init = 0;
+    c->learned_conf_threshold = 0.5f;
+    c->learned_em_iteration_limit = 3;
+    // Movie description of the synthetic workload: 100 fps, 1 e-/DN, four analogs
+    // with amplitudes 180/120/80/50 over a 12-count baseline sigma.
+    c->movie.frame_rate = 100.0f;
+    c->movie.photoelectron_sensitivity = 1.0f;
+    c->movie.ref_snr = 50.0f / 12.0f;
+    c->movie.shot_var_coeff = 1.0f;
+    c->movie.pedestal = 0;
+    const float rel[4] = {3.6f, 2.4f, 1.6f, 1.0f};
+    const char base[4] = {'A', 'C', 'G', 'T'};
+    for (int a = 0; a < 4; ++a) {
+        hwpa_analog& an = c->movie.analogs[a];
+        an.rel_amplitude = rel[a];
+        an.excess_noise_cv = 0.1f;
+        an.inter_pulse_distance = 0.2f;
+        an.pulse_width = 0.08f;
+        an.pw2_slow_step_ratio = 3.2f;
+        an.ipd2_slow_step_ratio = 3.2f;
+        an.base_label = (uint8_t)base[a];
+    }
+    return HWPA_OK;
+}
+
+int32_t hwpa_config_load_json(hwpa_config* c, const char* json)
+{
+    if (!c || !json) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    JsonFlat jf{json, {}, {}};
+    if (!jf.Value("")) return Fail(HWPA_ERR_BAD_CONFIG, "malformed JSON near offset " + std::to_string(jf.p - json));
+    struct FieldF {
+        const char* key;
+        float* f;
+    };
+    struct FieldU {
+        const char* key;
+        uint32_t* u;
+    };
+    const FieldF ff[] = {
+        {"baselinerConfig.MeanBiasAdjust", &c->mean_bias_adjust},
+        {"baselinerConfig.SigmaBiasAdjust", &c->sigma_bias_adjust},
+        {"baselinerConfig.MeanEmaScaleStrides", &c->mean_ema_scale_strides},
+        {"baselinerConfig.SigmaEmaScaleStrides", &c->sigma_ema_scale_strides},
+        {"baselinerConfig.JumpSuppression", &c->jump_suppression},
+        {"traceHistogramConfig.BinSizeCoeff", &c->bin_size_coeff},
+        {"traceHistogramConfig.BinSizeLowBoundCoeff", &c->bin_size_low_bound_coeff},
+        {"traceHistogramConfig.FallBackBaselineSigma", &c->fallback_baseline_sigma},
+        {"dmeConfig.AnalogMixFractionThreshold[0]", &c->analog_mix_fraction_threshold[0]},
+        {"dmeConfig.AnalogMixFractionThreshold[1]", &c->analog_mix_fraction_threshold[1]},
+        {"dmeConfig.ScaleSnrConfTol", &c->scale_snr_conf_tol},
+        {"dmeConfig.GTestStatFactor", &c->gtest_stat_factor},
+        {"dmeConfig.MinAnalogSnrThresh0", &c->min_analog_snr_thresh0},
+        {"dmeConfig.MinAnalogSnrThresh1", &c->min_analog_snr_thresh1},
+        {"dmeConfig.PulseAmpRegularization", &c->pulse_amp_regularization},
+        {"dmeConfig.SnrDropThresh", &c->snr_drop_thresh},
+        {"dmeConfig.SuccessConfidenceThresh", &c->success_confidence_thresh},
+        {"dmeConfig.ConfidenceHalfLife[0]", &c->confidence_half_life[0]},
+        {"dmeConfig.ConfidenceHalfLife[1]", &c->confidence_half_life[1]},
+        {"frameLabelerConfig.viterbi.alpha", &c->viterbi_alpha},
+        {"frameLabelerConfig.viterbi.beta", &c->viterbi_beta},
+        {"frameLabelerConfig.viterbi.gamma", &c->viterbi_gamma},
+        {"frameLabelerConfig.roi.upperThreshold", &c->roi_upper_threshold},
+        {"frameLabelerConfig.roi.lowerThreshold", &c->roi_lower_threshold},
+        {"pulseAccumConfig.XspWidthThresh", &c->xsp_width_thresh},
+        {"learnedInit.ConfidenceThreshold", &c->learned_conf_threshold},
+        {"movie.frameRate", &c->movie.frame_rate},
+        {"movie.photoelectronSensitivity", &c->movie.photoelectron_sensitivity},
+        {"movie.refSnr", &c->movie.ref_snr},
+        {"movie.shotVarCoeff", &c->movie.shot_var_coeff},
+    };
+    const FieldU fu[] = {
+        {"traceHistogramConfig.BaselineStatMinFrameCount", &c->baseline_stat_min_frame_count},
+        {"dmeConfig.EmIterationLimit", &c->em_iteration_limit},
+        {"dmeConfig.ModelUpdateMethod", &c->model_update_method},
+        {"dmeConfig.MinFramesForEstimate", &c->min_frames_for_estimate},
+        {"dmeConfig.NumFramesPreAccumStats", &c->num_frames_pre_accum_stats},
+        {"pulseAccumConfig.maxCallsPerZmw", &c->max_calls_per_zmw},
+        {"learnedInit.EmIterationLimit", &c->learned_em_iteration_limit},
+    };
+    for (const auto& kv : jf.out) {
+        const std::string& k = kv.first;
+        const std::string& v = kv.second;
+        double d = 0;
+        bool done = false;
+        for (const auto& f : ff)
+            if (k == f.key) {
+                if (!ToF(v, &d)) return Fail(HWPA_ERR_BAD_CONFIG, "not a number: " + k);
+                *f.f = (float)d;
+                done = true;
+            }
+        for (const auto& f : fu)
+            if (k == f.key) {
+                if (!ToF(v, &d) || d < 0) return Fail(HWPA_ERR_BAD_CONFIG, "not a count: " + k);
+                *f.u = (uint32_t)d;
+                done = true;
+            }
+        if (done) continue;
+        if (k == "baselinerConfig.Filter") {
+            const int t = FilterByName(v);
+            if (t < 0) return Fail(HWPA_ERR_BAD_CONFIG, "unknown filter type: " + v);
+            c->filter_type = t;
+        } else if (k == "dmeConfig.IterateToLimit" || k == "learnedInit.Enable") {
+            if (!ToF(v, &d)) return Fail(HWPA_ERR_BAD_CONFIG, "not a bool: " + k);
+            if (k == "dmeConfig.IterateToLimit") c->iterate_to_limit = d != 0;
+            else c->learned_init = d != 0;
+        } else if (k == "pulseAccumConfig.XspAmpThresh") {
+            if (!ToF(v, &d)) return Fail(HWPA_ERR_BAD_CONFIG, "not a number: " + k);
+            c->xsp_amp_thresh = d;
+        } else if (k == "movie.pedestal") {
+            if (!ToF(v, &d)) return Fail(HWPA_ERR_BAD_CONFIG, "not a number: " + k);
+            c->movie.pedestal = (int16_t)d;
+        } else if (k.rfind("movie.analogs[", 0) == 0) {
+            const size_t rb = k.find(']');
+            const int a = std::atoi(k.c_str() + 14);
+            if (a < 0 || a >= 4 || rb == std::string::npos) return Fail(HWPA_ERR_BAD_CONFIG, "bad analog index: " + k);
+            const std::string f = k.substr(rb + 2);
+            hwpa_analog& an = c->movie.analogs[a];
+            if (f == "baseLabel") {
+                an.base_label = v.empty() ? 'N' : (uint8_t)v[0];
+                continue;
+            }
+            if (!ToF(v, &d)) return Fail(HWPA_ERR_BAD_CONFIG, "not a number: " + k);
+            if (f == "relAmplitude") an.rel_amplitude = (float)d;
+            else if (f == "excessNoiseCV") an.excess_noise_cv = (float)d;
+            else if (f == "interPulseDistance") an.inter_pulse_distance = (float)d;
+            else if (f == "pulseWidth") an.pulse_width = (float)d;
+            else if (f == "pw2SlowStepRatio") an.pw2_slow_step_ratio = (float)d;
+            else if (f == "ipd2SlowStepRatio") an.ipd2_slow_step_ratio = (float)d;
+            else return Fail(HWPA_ERR_BAD_CONFIG, "unknown analog field: " + k);
+        } else {
+            return Fail(HWPA_ERR_BAD_CONFIG, "unknown config key: " + k);
+        }
+    }
+    return hwpa_config_validate(c);
+}
+
+int32_t hwpa_config_validate(const hwpa_config* c)
+{
+    if (!c) return Fail(HWPA_ERR_INVALID_ARG, "null config");
+    // ValidateConfig<BasecallerBaselinerConfig> (BasecallerBaselinerConfig.h:101-150)
+    if (!std::isfinite(c->mean_bias_adjust) || !std::isfinite(c->sigma_bias_adjust))
+        return Fail(HWPA_ERR_BAD_CONFIG, "MeanBiasAdjust/SigmaBiasAdjust must be finite");
+    if (std::isnan(c->sigma_ema_scale_strides) || std::signbit(c->sigma_ema_scale_strides))
+        return Fail(HWPA_ERR_BAD_CONFIG, "SigmaEmaScaleStrides may not be negative, -0, or NaN");
+    if (c->mean_ema_scale_strides < 0) return Fail(HWPA_ERR_BAD_CONFIG, "MeanEmaScaleStrides must be >= 0");
+    if (c->jump_suppression < 0) return Fail(HWPA_ERR_BAD_CONFIG, "JumpSuppression must be >= 0");
+    if (c->filter_type < 0 || c->filter_type > 5) return Fail(HWPA_ERR_BAD_CONFIG, "unknown filter type");
+    // ValidateConfig<BasecallerDmeConfig>
+    if (!(std::isfinite(c->analog_mix_fraction_threshold[1]) && c->analog_mix_fraction_threshold[1] <= 0.25f))
+        return Fail(HWPA_ERR_BAD_CONFIG, "AnalogMixFractionThreshold[1] must be finite and <= 0.25");
+    if (c->model_update_method > 2) return Fail(HWPA_ERR_BAD_CONFIG, "ModelUpdateMethod must be 0, 1 or 2");
+    if (!(c->scale_snr_conf_tol > 0)) return Fail(HWPA_ERR_BAD_CONFIG, "ScaleSnrConfTol must be positive");
+    // PulseAccumulator::InitFactory (PulseAccumulator.cpp:48-76)
+    if (c->xsp_width_thresh < 0) return Fail(HWPA_ERR_BAD_CONFIG, "Bad XspWidthThresh: must be non-negative");
+    if (c->xsp_amp_thresh < 0.0 || c->xsp_amp_thresh > 1.0) return Fail(HWPA_ERR_BAD_CONFIG, "Bad XspAmpThresh: valid range is [0, 1]");
+    if (c->max_calls_per_zmw == 0) return Fail(HWPA_ERR_BAD_CONFIG, "maxCallsPerZmw must be positive");
+    if (!(c->movie.photoelectron_sensitivity > 0)) return Fail(HWPA_ERR_BAD_CONFIG, "photoelectronSensitivity must be positive");
+    // DmeEmHost::EstimateLaneDetModel relative amplitude check (DmeEmHost.cpp:301-305)
+    for (int a = 0; a < 4; ++a)
+        if (!(c->movie.analogs[a].rel_amplitude > 0.0f))
+            return Fail(HWPA_ERR_BAD_CONFIG, "Bad relative amplitude in analog properties. Relative amplitudes must be greater than zero.");
+    // Constants must fit half precision (CoreDMEstimator.h:99-106)
+    if (!(MakeDmeConsts(*c).blVarMax < 6.5e4f)) return Fail(HWPA_ERR_BAD_CONFIG, "BaselineVarianceMax exceeds half range");
+    if (c->learned_em_iteration_limit == 0 && c->learned_init) return Fail(HWPA_ERR_BAD_CONFIG, "learned EM iteration limit must be positive");
+    return HWPA_OK;
+}
+
+int32_t hwpa_layout_sizes(int64_t* out, int32_t n)
+{
+    const int64_t sz[] = {(int64_t)sizeof(hwpa_config),          (int64_t)sizeof(hwpa_movie_info),
+                          (int64_t)sizeof(hwpa_stat_acc),        (int64_t)sizeof(hwpa_autocorr_acc),
+                          (int64_t)sizeof(hwpa_baseliner_stats), (int64_t)sizeof(hwpa_lane_hist),
+                          (int64_t)sizeof(hwpa_lane_model),      (int64_t)sizeof(hwpa_pulse),
+                          (int64_t)sizeof(hwpa_tracegen_params), (int64_t)sizeof(hwpa_tracegen_state),
+                          (int64_t)sizeof(hwpa_transition)};
+    const int32_t m = (int32_t)(sizeof(sz) / sizeof(sz[0]));
+    if (!out) return m;
+    for (int32_t i = 0; i < n && i < m; ++i) out[i] = sz[i];
+    return std::min(n, m);
+}
+
+int32_t hwpa_filter_params_get(int32_t type, hwpa_filter_params* out)
+{
+    if (!out || type < 0 || type > 5) return Fail(HWPA_ERR_INVALID_ARG, "bad filter type");
+    *out = FilterParams(type);
+    return HWPA_OK;
+}
+
+// ---- stage C-ABI wrappers ------------------------------------------------
+
+struct hwpa_baseliner {
+    BaselinerDev* dev;
+};
+
+int32_t hwpa_baseliner_create(const hwpa_config* cfg, uint32_t lanes, hwpa_baseliner** out)
+{
+    if (!cfg || !out || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "baseliner_create: bad argument");
+    const int32_t v = hwpa_config_validate(cfg);
+    if (v != HWPA_OK) return v;
+    BaselinerDev* d = nullptr;
+    const int32_t rc = BaselinerCreate(*cfg, lanes, &d);
+    if (rc != HWPA_OK) return rc;
+    *out = new hwpa_baseliner{d};
+    return HWPA_OK;
+}
+
+int32_t hwpa_baseliner_destroy(hwpa_baseliner* b)
+{
+    if (!b) return HWPA_OK;
+    BaselinerDestroy(b->dev);
+    delete b;
+    return HWPA_OK;
+}
+
+int32_t hwpa_baseliner_run(hwpa_baseliner* b, const int16_t* d_raw, uint32_t frames, int16_t* d_out,
+                           hwpa_baseliner_stats* d_stats, void* stream)
+{
+    if (!b) return Fail(HWPA_ERR_INVALID_ARG, "null baseliner");
+    return BaselinerRun(b->dev, d_raw, frames, d_out, d_stats, AsStream(stream));
+}
+
+int32_t hwpa_stats_reset(hwpa_baseliner_stats* d_agg, uint32_t lanes, void* stream)
+{
+    return StatsReset(d_agg, lanes, AsStream(stream));
+}
+
+int32_t hwpa_stats_merge(hwpa_baseliner_stats* d_agg, const hwpa_baseliner_stats* d_chunk, uint32_t lanes,
+                         void* stream)
+{
+    return StatsMerge(d_agg, d_chunk, lanes, AsStream(stream));
+}
+
+size_t hwpa_edge_state_bytes(void) { return EdgeStateBytes(); }
+
+int32_t hwpa_edge_state_init(void* d_edge, uint32_t lanes, void* stream)
+{
+    return EdgeStateInit(d_edge, lanes, AsStream(stream));
+}
+
+int32_t hwpa_hist_reset(const hwpa_config* cfg, hwpa_lane_hist* d_hist, const hwpa_baseliner_stats* d_stats,
+                        uint32_t lanes, void* stream)
+{
+    if (!cfg) return Fail(HWPA_ERR_INVALID_ARG, "null config");
+    return HistReset(*cfg, d_hist, d_stats, lanes, AsStream(stream));
+}
+
+int32_t hwpa_hist_add(hwpa_lane_hist* d_hist, void* d_edge, const int16_t* d_traces, uint32_t frames,
+                      const hwpa_lane_model* d_models, uint32_t lanes, void* stream)
+{
+    return HistAdd(d_hist, d_edge, d_traces, frames, d_models, lanes, AsStream(stream));
+}
+
+int32_t hwpa_dme_init_models(const hwpa_config* cfg, const hwpa_baseliner_stats* d_stats,
+                             hwpa_lane_model* d_models, uint32_t lanes, void* stream)
+{
+    if (!cfg) return Fail(HWPA_ERR_INVALID_ARG, "null config");
+    return DmeInitModels(*cfg, d_stats, d_models, lanes, AsStream(stream));
+}
+
+int32_t hwpa_dme_estimate(const hwpa_config* cfg, const hwpa_lane_hist* d_hist, const hwpa_baseliner_stats* d_stats,
+                          hwpa_lane_model* d_models, const int32_t model_fi[2], const int32_t est_fi[2],
+                          uint32_t lanes, int32_t* d_iters, void* stream)
+{
+    if (!cfg || !model_fi || !est_fi) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    return DmeEstimate(*cfg, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, nullptr, AsStream(stream));
+}
+
+struct hwpa_labeler {
+    LabelerDev* dev;
+};
+
+int32_t hwpa_labeler_create(const hwpa_config* cfg, uint32_t lanes, uint32_t max_frames, hwpa_labeler** out)
+{
+    if (!cfg || !out || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "labeler_create: bad argument");
+    LabelerDev* d = nullptr;
+    const int32_t rc = LabelerCreate(*cfg, lanes, max_frames, &d);
+    if (rc != HWPA_OK) return rc;
+    *out = new hwpa_labeler{d};
+    return HWPA_OK;
+}
+
+int32_t hwpa_labeler_destroy(hwpa_labeler* l)
+{
+    if (!l) return HWPA_OK;
+    LabelerDestroy(l->dev);
+    delete l;
+    return HWPA_OK;
+}
+
+int32_t hwpa_labeler_run(hwpa_labeler* l, const int16_t* d_traces, uint32_t frames, const hwpa_lane_model* d_models,
+                         int16_t* d_labels, int16_t* d_latent_out, float* d_score, void* stream)
+{
+    if (!l) return Fail(HWPA_ERR_INVALID_ARG, "null labeler");
+    return LabelerRun(l->dev, d_traces, frames, d_models, d_labels, d_latent_out, d_score, AsStream(stream));
+}
+
+struct hwpa_pulser {
+    PulserDev* dev;
+};
+
+int32_t hwpa_pulser_create(const hwpa_config* cfg, uint32_t lanes, hwpa_pulser** out)
+{
+    if (!cfg || !out || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "pulser_create: bad argument");
+    const int32_t v = hwpa_config_validate(cfg);
+    if (v != HWPA_OK) return v;
+    PulserDev* d = nullptr;
+    const int32_t rc = PulserCreate(*cfg, lanes, &d);
+    if (rc != HWPA_OK) return rc;
+    *out = new hwpa_pulser{d};
+    return HWPA_OK;
+}
+
+int32_t hwpa_pulser_destroy(hwpa_pulser* p)
+{
+    if (!p) return HWPA_OK;
+    PulserDestroy(p->dev);
+    delete p;
+    return HWPA_OK;
+}
+
+int32_t hwpa_pulser_run(hwpa_pulser* p, const int16_t* d_labels, const int16_t* d_latent, const int16_t* d_traces,
+                        uint32_t frames, int32_t first_frame, const hwpa_lane_model* d_models, hwpa_pulse* d_pulses,
+                        uint32_t* d_counts, uint32_t* d_overflows, hwpa_stat_acc* d_bl_stats, void* stream)
+{
+    if (!p) return Fail(HWPA_ERR_INVALID_ARG, "null pulser");
+    return PulserRun(p->dev, d_labels, d_latent, d_traces, frames, first_frame, d_models, d_pulses, d_counts,
+                     d_overflows, d_bl_stats, AsStream(stream));
+}
+
+int32_t hwpa_device_count(int32_t* count)
+{
+    if (!count) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return HWPA_OK;
+}
+
+int32_t hwpa_device_sync(void)
+{
+    HWPA_HIP(hipDeviceSynchronize());
+    return HWPA_OK;
+}
+
+int32_t hwpa_dev_alloc(void** d_ptr, size_t bytes)
+{
+    if (!d_ptr) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    return DeviceAlloc(d_ptr, bytes, "caller buffer");
+}
+
+int32_t hwpa_dev_free(void* d_ptr)
+{
+    if (d_ptr) HWPA_HIP(hipFree(d_ptr));
+    return HWPA_OK;
+}
+
+int32_t hwpa_memcpy_htod(void* d_dst, const void* host_src, size_t bytes)
+{
+    if (!d_dst || !host_src) return Fail(HWPA_ERR_INVALID_ARG, "memcpy: null pointer");
+    HWPA_HIP(hipMemcpy(d_dst, host_src, bytes, hipMemcpyHostToDevice));
+    return HWPA_OK;
+}
+
+int32_t hwpa_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes)
+{
+    if (!d_dst || !d_src) return Fail(HWPA_ERR_INVALID_ARG, "memcpy: null pointer");
+    HWPA_HIP(hipMemcpy(d_dst, d_src, bytes, hipMemcpyDeviceToDevice));
+    return HWPA_OK;
+}
+
+int32_t hwpa_memcpy_dtoh(void* host_dst, const void* d_src, size_t bytes)
+{
+    if (!host_dst || !d_src) return Fail(HWPA_ERR_INVALID_ARG, "memcpy: null pointer");
+    HWPA_HIP(hipMemcpy(host_dst, d_src, bytes, hipMemcpyDeviceToHost));
+    return HWPA_OK;
+}
+
+int32_t hwpa_tracegen_default(hwpa_tracegen_params* p)
+{
+    if (!p) return Fail(HWPA_ERR_INVALID_ARG, "null params");
+    std::memset(p, 0, sizeof(*p));
+    // Plain workload: baseline N(0, 12^2), drift +-40 counts per 512 frames,
+    // amplitudes 180/120/80/50 with 10% per-frame multiplicative noise,
+    // geometric widths (mean 8) and gaps (mean 20), uniform analogs.
+    p->seed = 20240601ull;
+    p->baseline_mean = 0.0f;
+    p->baseline_sigma = 12.0f;
+    p->drift_per_chunk = 40.0f;
+    p->drift_chunk = 512.0f;
+    p->step_drift = 0.0f;
+    p->step_span = 1.0f;
+    p->empty_fraction = 0.0f;
+    p->amplitude[0] = 180.0f;
+    p->amplitude[1] = 120.0f;
+    p->amplitude[2] = 80.0f;
+    p->amplitude[3] = 50.0f;
+    p->amplitude_cv = 0.1f;
+    p->mean_pulse_width = 8.0f;
+    p->mean_ipd = 20.0f;
+    return HWPA_OK;
+}
+
+int32_t hwpa_tracegen_run(const hwpa_tracegen_params* p, uint32_t lanes, uint32_t lane_offset, uint32_t frames,
+                          uint32_t first_frame, hwpa_tracegen_state* d_state, int32_t init, int16_t* d_out,
+                          void* stream)
+{
+    if (!p || !d_state || !d_out) return Fail(HWPA_ERR_INVALID_ARG, "tracegen: null argument");
+    return TracegenRun(*p, lanes, lane_offset, frames, first_frame, d_state, init, d_out, AsStream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// internal.h -- shared declarations of libhwpa (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/hwpa.h"
+#include "../../include/hwpa_math.h"
+
+namespace hwpa {
+
+constexpr int LS = HWPA_LANE_SIZE;
+constexpr int NA = HWPA_NUM_ANALOGS;
+constexpr int NB = HWPA_NUM_BINS;
+constexpr int NS = HWPA_NUM_STATES;
+constexpr int NL = HWPA_LATENT_FRAMES;
+
+// Thread-local error text behind hwpa_last_error().
+void SetError(const std::string& msg);
+void ClearError();
+
+int32_t Fail(int32_t code, const std::string& msg);
+
+#define HWPA_HIP(expr)                                                                  \
+    do {                                                                                \
+        hipError_t hwpa_e_ = (expr);                                                    \
+        if (hwpa_e_ != hipSuccess)                                                      \
+            return ::hwpa::Fail(HWPA_ERR_HIP, std::string(#expr) + ": " +              \
+                                                  hipGetErrorString(hwpa_e_));          \
+    } while (0)
+
+inline hipStream_t AsStream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Reports and clears any error left by the last kernel launch.
+int32_t CheckLaunch(const char* what);
+
+// Device allocation helpers (hipMalloc / hipFree with error text).
+int32_t DeviceAlloc(void** p, size_t bytes, const char* what);
+void DeviceFree(void* p);
+
+// Fills `words` 32-bit words at d with `value` using a kernel (no memset
+// nodes, so the call is safe inside captured graphs).
+int32_t Fill32(void* d, uint32_t value, size_t words, hipStream_t s);
+
+// Host-side constant derivations shared by several stages.
+hwpa_filter_params FilterParams(int32_t type);
+
+struct DmeConsts {
+    float blVarMin, blVarMax, blVarNominal, signalScaler, shotVarCoeff;
+};
+DmeConsts MakeDmeConsts(const hwpa_config& c);
+
+// ---- stage launchers implemented in the .hip files ----------------------
+
+struct BaselinerDev;
+int32_t BaselinerCreate(const hwpa_config& cfg, uint32_t lanes, BaselinerDev** out);
+void BaselinerDestroy(BaselinerDev* b);
+int32_t BaselinerRun(BaselinerDev* b, const int16_t* raw, uint32_t frames, int16_t* out,
+                     hwpa_baseliner_stats* stats, hipStream_t s);
+uint32_t BaselinerStartupLatency(const hwpa_config& cfg);
+
+int32_t StatsReset(hwpa_baseliner_stats* agg, uint32_t lanes, hipStream_t s);
+int32_t StatsMerge(hwpa_baseliner_stats* agg, const hwpa_baseliner_stats* chunk, uint32_t lanes,
+                   hipStream_t s);
+
+size_t EdgeStateBytes();
+int32_t EdgeStateInit(void* edge, uint32_t lanes, hipStream_t s);
+int32_t HistReset(const hwpa_config& cfg, hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
+                  uint32_t lanes, hipStream_t s);
+int32_t HistAdd(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
+                const hwpa_lane_model* models, uint32_t lanes, hipStream_t s);
+
+int32_t DmeInitModels(const hwpa_config& cfg, const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
+                      uint32_t lanes, hipStream_t s);
+int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
+                    hwpa_lane_model* models, const int32_t model_fi[2], const int32_t est_fi[2],
+                    uint32_t lanes, int32_t* d_iters, float* scratch, hipStream_t s);
+size_t DmeScratchBytes(uint32_t lanes);
+
+struct LabelerDev;
+int32_t LabelerCreate(const hwpa_config& cfg, uint32_t lanes, uint32_t max_frames, LabelerDev** out);
+void LabelerDestroy(LabelerDev* l);
+int32_t LabelerRun(LabelerDev* l, const int16_t* traces, uint32_t frames, const hwpa_lane_model* models,
+                   int16_t* labels, int16_t* latent_out, float* score, hipStream_t s);
+
+struct PulserDev;
+int32_t PulserCreate(const hwpa_config& cfg, uint32_t lanes, PulserDev** out);
+void PulserDestroy(PulserDev* p);
+int32_t PulserRun(PulserDev* p, const int16_t* labels, const int16_t* latent, const int16_t* traces,
+                  uint32_t frames, int32_t first_frame, const hwpa_lane_model* models, hwpa_pulse* pulses,
+                  uint32_t* counts, uint32_t* overflows, hwpa_stat_acc* bl_stats, hipStream_t s);
+uint32_t PulserMaxCalls(const PulserDev* p);
+
+int32_t TracegenRun(const hwpa_tracegen_params& p, uint32_t lanes, uint32_t lane_offset, uint32_t frames,
+                    uint32_t first_frame, hwpa_tracegen_state* state, int32_t init, int16_t* out,
+                    hipStream_t s);
+
+}  // namespace hwpa

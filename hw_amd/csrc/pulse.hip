@@ -1,0 +1,202 @@
+// pulse.hip -- pulse accumulation: segments the frame labels into pulses
+// with fixed-point signal statistics and accumulates baseline statistics
+// (HostPulseAccumulator, traceAnalysis/HostPulseAccumulator.{h,cpp};
+// PulseAccumulator::InitFactory, traceAnalysis/PulseAccumulator.cpp:48-76;
+// Pulse fixed point, dataTypes/Pulse.h:212-222).
+//
+// Mapping: one wave64 per lane, one thread per ZMW; the open segment lives in
+// registers for the chunk and in HBM between chunks.  Labels and signal are
+// read once (coalesced 128 B rows); pulses go to the ZMW's slot list.
+
+#include "internal.h"
+
+namespace hwpa {
+
+struct SegLane {
+    uint32_t start[LS];
+    int16_t first[LS];
+    int16_t last[LS];
+    int16_t smax[LS];
+    int16_t label[LS];
+    float total[LS];
+    float m2[LS];
+};
+
+struct PulseConsts {
+    float widthThresh, ampThresh;
+    uint32_t maxCalls;
+    uint8_t analogMap[NA];
+};
+
+struct PulserDev {
+    PulseConsts k;
+    uint32_t lanes;
+    SegLane* seg = nullptr;
+};
+
+// std::min / std::max on floats (NaN handling of ToPulse's clamps).
+__device__ __forceinline__ float StdMin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float StdMax(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ uint16_t ToFixed(float v) { return (uint16_t)(v / 0.1f + 0.5f); }
+
+template <int UNROLL>
+__global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ labels, const int16_t* __restrict__ latent,
+                                                  const int16_t* __restrict__ traces, int frames, int32_t firstFrame,
+                                                  const hwpa_lane_model* __restrict__ models, SegLane* seg,
+                                                  hwpa_pulse* __restrict__ pulses, uint32_t* __restrict__ counts,
+                                                  uint32_t* __restrict__ overflows, hwpa_stat_acc* __restrict__ blStats,
+                                                  PulseConsts k)
+{
+    const int lane = blockIdx.x;
+    const int z = threadIdx.x;
+    SegLane& S = seg[lane];
+    uint32_t start = S.start[z];
+    int first = S.first[z], last = S.last[z], smax = S.smax[z], label = S.label[z];
+    float total = S.total[z], m2 = S.m2[z];
+    const float minMean = hw_h2f(models[lane].analogs[NA - 1].means[z]);
+    const float maxSignal = 65535 * 0.1f;
+    hwpa_pulse* out = pulses + ((size_t)lane * LS + z) * k.maxCalls;
+    uint32_t n = 0, ovf = 0;
+    float bm0 = 0.f, bm1 = 0.f, bm2 = 0.f;
+    const int16_t* lb = labels + (size_t)lane * frames * LS + z;
+    const int16_t* lt = latent + (size_t)lane * NL * LS + z;
+    const int16_t* tr = traces + (size_t)lane * frames * LS + z;
+
+    for (int f0 = 0; f0 < frames; f0 += UNROLL) {
+        int lv[UNROLL], sv[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int rf = f0 + u;
+            lv[u] = (rf < frames) ? (int)lb[(size_t)rf * LS] : 0;
+            sv[u] = (rf < NL) ? (int)lt[(size_t)rf * LS] : ((rf < frames) ? (int)tr[(size_t)(rf - NL) * LS] : 0);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int rf = f0 + u;
+            if (rf >= frames) break;
+            const uint32_t absf = (uint32_t)(rf + firstFrame);
+            const int lab = lv[u];
+            const int sig = sv[u];
+            const bool isUp = (lab > NA) && (lab <= 2 * NA);
+            const bool boundary = isUp || (lab == 0 && label != 0);
+            const bool isPulse = label != 0;
+            if (boundary && isPulse) {
+                const int width = (int)(absf - start);
+                const float rawMean = (float)(total + last + first) / width;
+                const float rawMid = (float)total / (width - 2);
+                const bool keep = (width >= k.widthThresh) || (rawMean * width >= k.ampThresh * minMean);
+                int l = label;
+                if (l > 2 * NA) l -= 2 * NA;
+                if (l > NA) l -= NA;
+                hwpa_pulse p;
+                p.start = start;
+                p.width = (uint16_t)width;
+                p.mean_signal = ToFixed(StdMin(maxSignal, StdMax(0.0f, rawMean)));
+                p.mid_signal = ToFixed(width < 3 ? 0.0f : StdMin(maxSignal, StdMax(0.0f, rawMid)));
+                p.max_signal = ToFixed(StdMin(maxSignal, StdMax(0.0f, (float)smax)));
+                p.signal_m2 = m2;
+                p.label = (l >= 1 && l <= NA) ? k.analogMap[l - 1] : (uint8_t)4;
+                p.is_reject = keep ? 0 : 1;
+                p.pad[0] = p.pad[1] = 0;
+                if (n < k.maxCalls) out[n++] = p;
+                else ++ovf;
+            }
+            if (boundary) {
+                start = absf;
+                first = sig;
+                last = 0;
+                smax = sig;
+                total = 0.0f;
+                m2 = 0.0f;
+                label = lab;
+            } else {
+                total = total + last;
+                m2 = m2 + (float)last * last;
+                last = sig;
+                smax = sig > smax ? sig : smax;
+            }
+            if (!isPulse && !boundary) {
+                const float x = (float)sig;
+                bm0 += 1.0f;
+                bm1 += x;
+                bm2 += x * x;
+            }
+        }
+    }
+    S.start[z] = start;
+    S.first[z] = (int16_t)first;
+    S.last[z] = (int16_t)last;
+    S.smax[z] = (int16_t)smax;
+    S.label[z] = (int16_t)label;
+    S.total[z] = total;
+    S.m2[z] = m2;
+    counts[(size_t)lane * LS + z] = n;
+    overflows[(size_t)lane * LS + z] = ovf;
+    hwpa_stat_acc& bs = blStats[lane];
+    bs.offset[z] = 0.f;
+    bs.m0[z] = bm0;
+    bs.m1[z] = bm1;
+    bs.m2[z] = bm2;
+}
+
+__global__ __launch_bounds__(64) void SegInitKernel(SegLane* seg)
+{
+    SegLane& S = seg[blockIdx.x];
+    const int z = threadIdx.x;
+    S.start[z] = 0;
+    S.first[z] = 0;
+    S.last[z] = 0;
+    S.smax[z] = 0;
+    S.label[z] = 0;
+    S.total[z] = 0.f;
+    S.m2[z] = 0.f;
+}
+
+int32_t PulserCreate(const hwpa_config& cfg, uint32_t lanes, PulserDev** out)
+{
+    auto* p = new PulserDev;
+    p->lanes = lanes;
+    p->k.widthThresh = cfg.xsp_width_thresh;
+    const double x = cfg.xsp_amp_thresh;
+    p->k.ampThresh = (float)(x / (1.0 - x));
+    p->k.maxCalls = cfg.max_calls_per_zmw;
+    for (int a = 0; a < NA; ++a) {
+        const uint8_t b = cfg.movie.analogs[a].base_label;
+        p->k.analogMap[a] = (b == 'A') ? 0 : (b == 'C') ? 1 : (b == 'G') ? 2 : (b == 'T') ? 3 : 4;
+    }
+    int32_t rc = DeviceAlloc((void**)&p->seg, sizeof(SegLane) * lanes, "pulse segments");
+    if (rc == HWPA_OK) {
+        hipLaunchKernelGGL(SegInitKernel, dim3(lanes), dim3(LS), 0, nullptr, p->seg);
+        rc = CheckLaunch("SegInitKernel");
+    }
+    if (rc == HWPA_OK && hipDeviceSynchronize() != hipSuccess) rc = Fail(HWPA_ERR_HIP, "pulser init failed");
+    if (rc != HWPA_OK) {
+        PulserDestroy(p);
+        return rc;
+    }
+    *out = p;
+    return HWPA_OK;
+}
+
+void PulserDestroy(PulserDev* p)
+{
+    if (!p) return;
+    DeviceFree(p->seg);
+    delete p;
+}
+
+uint32_t PulserMaxCalls(const PulserDev* p) { return p->k.maxCalls; }
+
+int32_t PulserRun(PulserDev* p, const int16_t* labels, const int16_t* latent, const int16_t* traces, uint32_t frames,
+                  int32_t first_frame, const hwpa_lane_model* models, hwpa_pulse* pulses, uint32_t* counts,
+                  uint32_t* overflows, hwpa_stat_acc* bl_stats, hipStream_t s)
+{
+    if (!p || !labels || !latent || !traces || !models || !pulses || !counts || !overflows || !bl_stats)
+        return Fail(HWPA_ERR_INVALID_ARG, "pulser_run: null argument");
+    if (frames <= (uint32_t)NL) return Fail(HWPA_ERR_INVALID_ARG, "pulser_run: frames must be > 24");
+    hipLaunchKernelGGL(PulseKernel<8>, dim3(p->lanes), dim3(LS), 0, s, labels, latent, traces, (int)frames,
+                       first_frame, models, p->seg, pulses, counts, overflows, bl_stats, p->k);
+    return CheckLaunch("PulseKernel");
+}
+
+}  // namespace hwpa

@@ -1,0 +1,175 @@
+/*
+ * hwpa.h -- C-ABI of the MI355X trace-analysis library (libhwpa.so).
+ *
+ * The analyzer entry points replace the reference's per-pool analyzer
+ * boundary; the per-stage entry points replace the Device* strategy
+ * classes behind it and exist so each stage can be checked on its own.
+ * All functions return an hwpa_status code and never throw; the message of
+ * the last failure on the calling thread is available from hwpa_last_error().
+ *
+ * Pointers named d_* are device pointers (HBM) in the [lane][frame][zmw]
+ * layout of include/hwpa_types.h; `stream` is a hipStream_t (NULL = the
+ * default stream).  Stage functions enqueue work and return without
+ * synchronising.
+ */
+#ifndef HWPA_H
+#define HWPA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "hwpa_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- errors / configuration ------------------------------------------- */
+
+/* Thread-local message of the last failed call ("" if none). */
+const char* hwpa_last_error(void);
+
+/* Fills *cfg with the reference defaults (dataTypes/configs/Basecaller*Config.h,
+ * CoreDMEstimator.cpp:36-87) and a default movie description. */
+int32_t hwpa_config_default(hwpa_config* cfg);
+
+/* Applies a JSON object of overrides, keyed like the reference's
+ * BasecallerAlgorithmConfig (e.g. {"baselinerConfig":{"SigmaEmaScaleStrides":0}}).
+ * Replaces PBConfig JSON loading (applications/SmrtBasecaller.cpp:985-1029). */
+int32_t hwpa_config_load_json(hwpa_config* cfg, const char* json);
+
+/* Checks knob ranges (ValidateConfig<...> specialisations in dataTypes/configs). */
+int32_t hwpa_config_validate(const hwpa_config* cfg);
+
+/* sizeof() of the public structures, in this order: hwpa_config,
+ * hwpa_movie_info, hwpa_stat_acc, hwpa_autocorr_acc, hwpa_baseliner_stats,
+ * hwpa_lane_hist, hwpa_lane_model, hwpa_pulse, hwpa_tracegen_params,
+ * hwpa_tracegen_state, hwpa_transition.  Returns how many were written. */
+int32_t hwpa_layout_sizes(int64_t* out, int32_t n);
+
+/* Filter table lookup (traceAnalysis/BaselinerParams.cpp:10-72). */
+int32_t hwpa_filter_params_get(int32_t filter_type, hwpa_filter_params* out);
+
+/* 13-state log transition matrix (traceAnalysis/SubframeScorer.cpp:35-223). */
+int32_t hwpa_transition_matrix(const hwpa_config* cfg, hwpa_transition* out);
+
+/* ---- per-pool analyzer (analyzer/BatchAnalyzer.h:51-106) -------------- */
+
+typedef struct hwpa_analyzer hwpa_analyzer;
+
+/* BatchAnalyzer ctor + DynamicEstimateBatchAnalyzer (analyzer/BatchAnalyzer.cpp:71-118). */
+int32_t hwpa_analyzer_create(uint32_t pool_id, uint32_t lanes, uint32_t frames_per_chunk,
+                             const hwpa_config* cfg, hwpa_analyzer** out);
+int32_t hwpa_analyzer_destroy(hwpa_analyzer* a);
+
+/* BatchAnalyzer::operator()(const TraceBatchVariant&) (analyzer/BatchAnalyzer.cpp:149-170).
+ * traces: int16 [lanes][frames][64]; traces_on_device selects d_ vs host pointer.
+ * Errors: HWPA_ERR_BAD_POOL (wrong pool), HWPA_ERR_BAD_FRAME (out of order). */
+int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t traces_on_device,
+                              uint32_t pool_id, int32_t first_frame);
+
+/* Blocks until the analyzer's stream is idle. */
+int32_t hwpa_analyzer_sync(hwpa_analyzer* a);
+
+/* Result accessors for the last process call (device pointers owned by the
+ * analyzer, valid until the next process call).  flags: [0] pulses emitted,
+ * [1] full estimation ran, [2] frame labels ran. */
+int32_t hwpa_result_flags(const hwpa_analyzer* a, int32_t flags[4]);
+int32_t hwpa_result_pulses(const hwpa_analyzer* a, const hwpa_pulse** d_pulses,
+                           const uint32_t** d_counts, const uint32_t** d_overflows,
+                           uint32_t* max_calls_per_zmw);
+int32_t hwpa_result_labels(const hwpa_analyzer* a, const int16_t** d_labels);
+int32_t hwpa_result_baselined(const hwpa_analyzer* a, const int16_t** d_traces);
+int32_t hwpa_result_models(const hwpa_analyzer* a, const hwpa_lane_model** d_models);
+int32_t hwpa_result_baseliner_stats(const hwpa_analyzer* a, const hwpa_baseliner_stats** d_stats);
+int32_t hwpa_result_pulse_baseline_stats(const hwpa_analyzer* a, const hwpa_stat_acc** d_stats);
+int32_t hwpa_result_viterbi_score(const hwpa_analyzer* a, const float** d_score);
+
+/* Per-stage device time of the last process call in milliseconds, measured
+ * with HIP events on the analyzer's stream when profiling is enabled:
+ * [0] baseline [1] aggregate [2] histogram [3] dme [4] labels [5] pulses. */
+int32_t hwpa_analyzer_set_profiling(hwpa_analyzer* a, int32_t enable);
+int32_t hwpa_analyzer_stage_ms(const hwpa_analyzer* a, float ms[8]);
+/* Number of EM iterations the last estimation executed (max over lanes). */
+int32_t hwpa_analyzer_em_iterations(const hwpa_analyzer* a, int32_t* iters);
+/* Stream used by the analyzer (hipStream_t). */
+int32_t hwpa_analyzer_stream(const hwpa_analyzer* a, void** stream);
+
+/* ---- stage entry points (Device* strategies) ----------------------------- */
+
+/* Baseliner: DeviceMultiScaleBaseliner / HostMultiScaleBaseliner::FilterBaseline
+ * (traceAnalysis/HostMultiScaleBaseliner.cpp:57-176).  State is per lane. */
+typedef struct hwpa_baseliner hwpa_baseliner;
+int32_t hwpa_baseliner_create(const hwpa_config* cfg, uint32_t lanes, hwpa_baseliner** out);
+int32_t hwpa_baseliner_destroy(hwpa_baseliner* b);
+int32_t hwpa_baseliner_run(hwpa_baseliner* b, const int16_t* d_raw, uint32_t frames,
+                           int16_t* d_out, hwpa_baseliner_stats* d_stats, void* stream);
+
+/* Stats aggregator: BaselineStatsAggregatorDevice::AddMetricsImpl / ResetImpl
+ * (traceAnalysis/BaselineStatsAggregatorHost.cpp:40-71). */
+int32_t hwpa_stats_reset(hwpa_baseliner_stats* d_agg, uint32_t lanes, void* stream);
+int32_t hwpa_stats_merge(hwpa_baseliner_stats* d_agg, const hwpa_baseliner_stats* d_chunk,
+                         uint32_t lanes, void* stream);
+
+/* Histogram: TraceHistogramAccumHost::{ResetImpl,AddBlock}
+ * (traceAnalysis/TraceHistogramAccumHost.cpp:92-169).  The edge-frame
+ * classifier state (d_edge, hwpa_edge_state_bytes() per lane) persists
+ * across resets. */
+size_t hwpa_edge_state_bytes(void);
+int32_t hwpa_edge_state_init(void* d_edge, uint32_t lanes, void* stream);
+int32_t hwpa_hist_reset(const hwpa_config* cfg, hwpa_lane_hist* d_hist,
+                        const hwpa_baseliner_stats* d_stats, uint32_t lanes, void* stream);
+int32_t hwpa_hist_add(hwpa_lane_hist* d_hist, void* d_edge, const int16_t* d_traces,
+                      uint32_t frames, const hwpa_lane_model* d_models, uint32_t lanes, void* stream);
+
+/* DME: DmeEmHost::InitDetectionModels / EstimateImpl
+ * (traceAnalysis/DmeEmHost.cpp:131-159,855-925).  Intervals are [lo, hi). */
+int32_t hwpa_dme_init_models(const hwpa_config* cfg, const hwpa_baseliner_stats* d_stats,
+                             hwpa_lane_model* d_models, uint32_t lanes, void* stream);
+int32_t hwpa_dme_estimate(const hwpa_config* cfg, const hwpa_lane_hist* d_hist,
+                          const hwpa_baseliner_stats* d_stats, hwpa_lane_model* d_models,
+                          const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
+                          int32_t* d_iters, void* stream);
+
+/* Frame labeler: FrameLabelerHost::Process (traceAnalysis/FrameLabelerHost.cpp:312-612).
+ * d_latent_out receives the 24 trace frames preceding this chunk (the
+ * LabelsBatch latent trace). */
+typedef struct hwpa_labeler hwpa_labeler;
+int32_t hwpa_labeler_create(const hwpa_config* cfg, uint32_t lanes, uint32_t max_frames,
+                            hwpa_labeler** out);
+int32_t hwpa_labeler_destroy(hwpa_labeler* l);
+int32_t hwpa_labeler_run(hwpa_labeler* l, const int16_t* d_traces, uint32_t frames,
+                         const hwpa_lane_model* d_models, int16_t* d_labels,
+                         int16_t* d_latent_out, float* d_score, void* stream);
+
+/* Pulse accumulator: HostPulseAccumulator::Process
+ * (traceAnalysis/HostPulseAccumulator.cpp:48-127). */
+typedef struct hwpa_pulser hwpa_pulser;
+int32_t hwpa_pulser_create(const hwpa_config* cfg, uint32_t lanes, hwpa_pulser** out);
+int32_t hwpa_pulser_destroy(hwpa_pulser* p);
+int32_t hwpa_pulser_run(hwpa_pulser* p, const int16_t* d_labels, const int16_t* d_latent,
+                        const int16_t* d_traces, uint32_t frames, int32_t first_frame,
+                        const hwpa_lane_model* d_models, hwpa_pulse* d_pulses, uint32_t* d_counts,
+                        uint32_t* d_overflows, hwpa_stat_acc* d_bl_stats, void* stream);
+
+/* Device memory helpers for callers without their own allocator (tests,
+ * benchmarks, language bindings).  Copies are synchronous. */
+int32_t hwpa_device_count(int32_t* count);
+int32_t hwpa_device_sync(void);
+int32_t hwpa_dev_alloc(void** d_ptr, size_t bytes);
+int32_t hwpa_dev_free(void* d_ptr);
+int32_t hwpa_memcpy_htod(void* d_dst, const void* host_src, size_t bytes);
+int32_t hwpa_memcpy_dtoh(void* host_dst, const void* d_src, size_t bytes);
+int32_t hwpa_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+
+/* ---- synthetic input (no reference counterpart: bench/test data) ------- */
+int32_t hwpa_tracegen_default(hwpa_tracegen_params* p);
+int32_t hwpa_tracegen_run(const hwpa_tracegen_params* p, uint32_t lanes, uint32_t lane_offset,
+                          uint32_t frames, uint32_t first_frame, hwpa_tracegen_state* d_state,
+                          int32_t init, int16_t* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HWPA_H */

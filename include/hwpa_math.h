@@ -1,0 +1,229 @@
+/*
+ * hwpa_math.h -- deterministic single-precision math shared by the gfx950
+ * kernels and their host-side checkers.
+ *
+ * Every routine here is written with IEEE-754 basic operations only
+ * (add, sub, mul, correctly rounded div and sqrt, floor, rint, bit casts),
+ * so a translation unit built with -ffp-contract=off produces the same bits
+ * on the host and on the GPU.  The transcendental functions are derived
+ * from their Taylor / atanh series, not from any vendor library:
+ *   exp(x)  = 2^k * exp(r),  r = x - k ln2 (two-part ln2),  |r| <= ln2/2,
+ *             exp(r) by its Taylor series to r^8 (truncation < 6e-9 rel).
+ *   log(x)  = e ln2 + 2 atanh(s),  s = (m-1)/(m+1),  m in [sqrt(1/2), sqrt(2)),
+ *             atanh series to s^13 (truncation < 2e-10).
+ * Accuracy is a few ulp; tests/test_math.py bounds the error against libm.
+ *
+ * Conversions follow the reference's SSE LaneArray semantics:
+ *   float -> int32 : round-to-nearest-even, 0x80000000 when out of range or NaN
+ *                    (_mm_cvtps_epi32 after _mm_round_ps, common/simd/m512f_SSE.h:126-132)
+ *   int32 -> int16 : wrap (common/simd/m512s_SSE.h:101-113)
+ *   float -> half  : IEEE round-to-nearest-even (PBHalf = cuda half)
+ */
+#ifndef HWPA_MATH_H
+#define HWPA_MATH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define HWPA_HD __host__ __device__ inline
+#else
+#define HWPA_HD inline
+#endif
+
+HWPA_HD uint32_t hw_f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+HWPA_HD float hw_u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+HWPA_HD float hw_floorf(float x) { return __builtin_floorf(x); }
+HWPA_HD float hw_rintf(float x) { return __builtin_rintf(x); }
+HWPA_HD float hw_sqrtf(float x) { return __builtin_sqrtf(x); }
+HWPA_HD float hw_fabsf(float x) { return hw_u2f(hw_f2u(x) & 0x7fffffffu); }
+HWPA_HD bool hw_isnan(float x) { return (hw_f2u(x) & 0x7fffffffu) > 0x7f800000u; }
+HWPA_HD bool hw_isinf(float x) { return (hw_f2u(x) & 0x7fffffffu) == 0x7f800000u; }
+HWPA_HD bool hw_isfinite(float x) { return (hw_f2u(x) & 0x7f800000u) != 0x7f800000u; }
+
+/* SSE min/max: the second operand is returned when either is NaN
+ * (_mm_min_ps / _mm_max_ps). */
+HWPA_HD float hw_minf(float a, float b) { return (a < b) ? a : b; }
+HWPA_HD float hw_maxf(float a, float b) { return (a > b) ? a : b; }
+HWPA_HD float hw_clampf(float v, float lo, float hi) { return hw_minf(hw_maxf(v, lo), hi); }
+/* satlin (common/NumericUtil.h:70-76). */
+HWPA_HD float hw_satlin(float a, float b, float x)
+{
+    const float r = (x - a) / (b - a);
+    return hw_minf(hw_maxf(r, 0.0f), 1.0f);
+}
+
+/* float -> int32 with the x86 cvtps semantics for out-of-range values. */
+HWPA_HD int32_t hw_cvt_i32(float x)
+{
+    if (!(x >= -2147483648.0f && x < 2147483648.0f)) return (int32_t)0x80000000u;
+    return (int32_t)x;
+}
+HWPA_HD int32_t hw_round_i32(float x) { return hw_cvt_i32(hw_rintf(x)); }
+HWPA_HD int32_t hw_floor_i32(float x) { return hw_cvt_i32(hw_floorf(x)); }
+HWPA_HD int16_t hw_wrap16(int32_t v) { return (int16_t)(uint16_t)(uint32_t)v; }
+HWPA_HD int16_t hw_round_i16(float x) { return hw_wrap16(hw_round_i32(x)); }
+
+/* 2^k for integer k in [-149, 127]. */
+HWPA_HD float hw_pow2i(int k)
+{
+    if (k >= -126) return hw_u2f((uint32_t)(k + 127) << 23);
+    return hw_u2f(1u << (k + 149));
+}
+
+HWPA_HD float hw_expf(float x)
+{
+    if (hw_isnan(x)) return x;
+    if (x > 88.72283935546875f) return hw_u2f(0x7f800000u);
+    if (x < -103.97208404541015625f) return 0.0f;
+    const float kf = hw_floorf(x * 1.44269502162933349609375f + 0.5f);
+    const int k = (int)kf;
+    /* ln2 split: hi has 16 significant bits so kf*hi is exact for |k| <= 150. */
+    const float ln2hi = 0.693145751953125f;
+    const float ln2lo = 1.428606765330187045e-06f;
+    const float r = (x - kf * ln2hi) - kf * ln2lo;
+    float p = 1.0f / 5040.0f;
+    p = p * r + 1.0f / 720.0f;
+    p = p * r + 1.0f / 120.0f;
+    p = p * r + 1.0f / 24.0f;
+    p = p * r + 1.0f / 6.0f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    if (k > 127) return (p * hw_pow2i(127)) * hw_pow2i(k - 127);
+    if (k < -126) return (p * hw_pow2i(-126)) * hw_pow2i(k + 126);
+    return p * hw_pow2i(k);
+}
+
+/* Returns log(m) for m in [sqrt(1/2), sqrt(2)) and the binary exponent e. */
+HWPA_HD float hw_log_reduce(float x, int* e_out)
+{
+    uint32_t u = hw_f2u(x);
+    int e = 0;
+    if (u < 0x00800000u) {           /* subnormal: scale by 2^24 */
+        x = x * 16777216.0f;
+        u = hw_f2u(x);
+        e = -24;
+    }
+    e += (int)((u >> 23) & 0xffu) - 127;
+    float m = hw_u2f((u & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421353816986083984375f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float q = 1.0f / 13.0f;
+    q = q * s2 + 1.0f / 11.0f;
+    q = q * s2 + 1.0f / 9.0f;
+    q = q * s2 + 1.0f / 7.0f;
+    q = q * s2 + 1.0f / 5.0f;
+    q = q * s2 + 1.0f / 3.0f;
+    q = q * s2;
+    *e_out = e;
+    return (s + s) + (s + s) * q;
+}
+
+HWPA_HD float hw_logf(float x)
+{
+    if (hw_isnan(x)) return x;
+    if (x < 0.0f) return hw_u2f(0x7fc00000u);
+    if (x == 0.0f) return hw_u2f(0xff800000u);
+    if (hw_isinf(x)) return x;
+    int e;
+    const float lm = hw_log_reduce(x, &e);
+    const float ef = (float)e;
+    const float ln2hi = 0.693145751953125f;
+    const float ln2lo = 1.428606765330187045e-06f;
+    return ef * ln2hi + (lm + ef * ln2lo);
+}
+
+HWPA_HD float hw_log2f(float x)
+{
+    if (hw_isnan(x)) return x;
+    if (x < 0.0f) return hw_u2f(0x7fc00000u);
+    if (x == 0.0f) return hw_u2f(0xff800000u);
+    if (hw_isinf(x)) return x;
+    int e;
+    const float lm = hw_log_reduce(x, &e);
+    return (float)e + lm * 1.44269502162933349609375f;
+}
+
+HWPA_HD float hw_exp2f(float x)
+{
+    if (hw_isnan(x)) return x;
+    if (x >= 128.0f) return hw_u2f(0x7f800000u);
+    if (x < -150.0f) return 0.0f;
+    const float kf = hw_rintf(x);
+    const int k = (int)kf;
+    const float r = (x - kf) * 0.693147182464599609375f;   /* x - kf is exact */
+    float p = 1.0f / 5040.0f;
+    p = p * r + 1.0f / 720.0f;
+    p = p * r + 1.0f / 120.0f;
+    p = p * r + 1.0f / 24.0f;
+    p = p * r + 1.0f / 6.0f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    if (k > 127) return (p * hw_pow2i(127)) * hw_pow2i(k - 127);
+    if (k < -126) return (p * hw_pow2i(-126)) * hw_pow2i(k + 126);
+    return p * hw_pow2i(k);
+}
+
+/* x^y for x > 0; exact for y == 0 and y == 1. */
+HWPA_HD float hw_powf(float x, float y)
+{
+    if (y == 0.0f) return 1.0f;
+    if (y == 1.0f) return x;
+    if (hw_isnan(x) || hw_isnan(y)) return hw_u2f(0x7fc00000u);
+    if (x == 1.0f) return 1.0f;
+    if (x < 0.0f) return hw_u2f(0x7fc00000u);
+    if (x == 0.0f) return (y > 0.0f) ? 0.0f : hw_u2f(0x7f800000u);
+    return hw_exp2f(y * hw_log2f(x));
+}
+
+/* IEEE binary16 <-> binary32, round to nearest even. */
+HWPA_HD uint16_t hw_f2h(float f)
+{
+    const uint32_t u = hw_f2u(f);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const uint32_t a = u & 0x7fffffffu;
+    if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u | ((a >> 13) & 0x3ffu));
+    if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);   /* rounds to >= 65520 */
+    if (a >= 0x38800000u) {                                     /* normal half */
+        const uint32_t mant = a & 0x7fffffu;
+        const uint32_t exp = (a >> 23) - 112u;
+        uint32_t h = (exp << 10) | (mant >> 13);
+        const uint32_t rem = mant & 0x1fffu;
+        if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h += 1u;
+        return (uint16_t)(sign | h);
+    }
+    if (a < 0x33000000u) return (uint16_t)sign;                 /* < 2^-25 rounds to 0 */
+    /* subnormal half: value = mant * 2^-24 */
+    const uint32_t e = a >> 23;                                  /* 102..112 */
+    const uint32_t mant = (a & 0x7fffffu) | 0x800000u;
+    const uint32_t shift = 126u - e;                             /* 14..24 */
+    uint32_t h = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u);
+    const uint32_t half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (h & 1u))) h += 1u;
+    return (uint16_t)(sign | h);
+}
+
+HWPA_HD float hw_h2f(uint16_t h)
+{
+    const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+    const uint32_t exp = ((uint32_t)h >> 10) & 0x1fu;
+    const uint32_t mant = (uint32_t)h & 0x3ffu;
+    if (exp == 0x1fu) return hw_u2f(sign | 0x7f800000u | (mant << 13));
+    if (exp != 0u) return hw_u2f(sign | ((exp + 112u) << 23) | (mant << 13));
+    if (mant == 0u) return hw_u2f(sign);
+    /* subnormal: mant * 2^-24 is exact in float */
+    const float v = (float)mant * 5.9604644775390625e-08f;
+    return sign ? -v : v;
+}
+
+/* Round a float through binary16 and back. */
+HWPA_HD float hw_half_round(float f) { return hw_h2f(hw_f2h(f)); }
+
+#endif /* HWPA_MATH_H */

@@ -1,0 +1,255 @@
+"""CPU oracle for the trace-analysis path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes view of liboracle.so, the scalar C++ restatement of the reference's
+Host* strategies (oracle/oracle.cpp).  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg may use it, as the checker and the CPU
+baseline.  All arrays are numpy, [lane][frame][zmw] like the GPU path.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("liboracle.so is not built; run `make -C oracle`")
+    return ctypes.CDLL(LIB_PATH)
+
+
+lib = _load()
+_vp = ctypes.c_void_p
+LANE = 64
+LATENT = 24
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp)
+
+
+class DmeDiag(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int32 * LANE), ("iter_count", ctypes.c_int32 * LANE),
+                ("status", ctypes.c_int32 * LANE), ("iterations_run", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("log_like", ctypes.c_float * LANE), ("conf_factors", (ctypes.c_float * LANE) * 8),
+                ("scale_factor", ctypes.c_float * LANE), ("confidence", ctypes.c_float * LANE),
+                ("est_fp32", ((ctypes.c_float * LANE) * 3) * 5), ("out_fp32", ((ctypes.c_float * LANE) * 3) * 5)]
+
+
+def threads(n):
+    return lib.orc_threads(int(n))
+
+
+def tracegen(params, lanes, frames, first_frame, state, init, lane_offset=0, truth=False):
+    """Seeded synthetic traces [lanes, frames, 64]; with truth=True also the
+    generating state per frame (0 = baseline, a+1 = analog a)."""
+    out = np.empty((lanes, frames, LANE), dtype=np.int16)
+    tr = np.empty((lanes, frames, LANE), dtype=np.int16) if truth else None
+    lib.orc_tracegen(ctypes.byref(params), ctypes.c_int32(lanes), ctypes.c_int32(lane_offset),
+                     ctypes.c_int32(frames), ctypes.c_uint32(first_frame), _p(state), ctypes.c_int32(1 if init else 0),
+                     _p(out), _p(tr) if truth else None)
+    return (out, tr) if truth else out
+
+
+def transition_matrix(cfg, tr):
+    lib.orc_transition_matrix(ctypes.byref(cfg), ctypes.byref(tr))
+    return tr
+
+
+class Baseliner:
+    def __init__(self, cfg, lanes):
+        lib.orc_baseliner_create.restype = _vp
+        self.h = _vp(lib.orc_baseliner_create(ctypes.byref(cfg), ctypes.c_int32(lanes)))
+        self.lanes = lanes
+
+    def run(self, raw, stats_dtype):
+        raw = np.ascontiguousarray(raw, dtype=np.int16)
+        frames = raw.shape[1]
+        out = np.empty_like(raw)
+        stats = np.zeros(self.lanes, dtype=stats_dtype)
+        rc = lib.orc_baseliner_process(self.h, _p(raw), ctypes.c_int32(frames), _p(out), _p(stats))
+        if rc != 0:
+            raise RuntimeError("oracle baseliner failed: %d" % rc)
+        return out, stats
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_baseliner_destroy(self.h)
+            self.h = None
+
+
+class Aggregator:
+    def __init__(self, lanes):
+        lib.orc_agg_create.restype = _vp
+        self.h = _vp(lib.orc_agg_create(ctypes.c_int32(lanes)))
+        self.lanes = lanes
+
+    def add(self, stats):
+        lib.orc_agg_add(self.h, _p(np.ascontiguousarray(stats)))
+
+    def get(self, stats_dtype):
+        out = np.zeros(self.lanes, dtype=stats_dtype)
+        lib.orc_agg_get(self.h, _p(out))
+        return out
+
+    def reset(self):
+        lib.orc_agg_reset(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_agg_destroy(self.h)
+            self.h = None
+
+
+class Histogram:
+    def __init__(self, cfg, lanes):
+        lib.orc_hist_create.restype = _vp
+        self.h = _vp(lib.orc_hist_create(ctypes.byref(cfg), ctypes.c_int32(lanes)))
+        self.lanes = lanes
+
+    def reset(self, stats):
+        return lib.orc_hist_reset(self.h, _p(np.ascontiguousarray(stats)))
+
+    def add(self, traces, models):
+        traces = np.ascontiguousarray(traces, dtype=np.int16)
+        lib.orc_hist_add(self.h, _p(traces), ctypes.c_int32(traces.shape[1]), _p(np.ascontiguousarray(models)))
+
+    def get(self, hist_dtype):
+        out = np.zeros(self.lanes, dtype=hist_dtype)
+        lib.orc_hist_get(self.h, _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_hist_destroy(self.h)
+            self.h = None
+
+
+def dme_init_models(cfg, stats, model_dtype):
+    out = np.zeros(len(stats), dtype=model_dtype)
+    lib.orc_dme_init_models(ctypes.byref(cfg), ctypes.c_int32(len(stats)), _p(np.ascontiguousarray(stats)), _p(out))
+    return out
+
+
+def dme_estimate(cfg, hist, stats, models, model_fi, est_fi, want_diag=False):
+    models = np.ascontiguousarray(models).copy()
+    lanes = len(models)
+    mfi = (ctypes.c_int32 * 2)(*model_fi)
+    efi = (ctypes.c_int32 * 2)(*est_fi)
+    diag = (DmeDiag * lanes)() if want_diag else None
+    lib.orc_dme_estimate(ctypes.byref(cfg), ctypes.c_int32(lanes), _p(np.ascontiguousarray(hist)),
+                         _p(np.ascontiguousarray(stats)), _p(models), mfi, efi, diag)
+    return (models, diag) if want_diag else models
+
+
+class Labeler:
+    def __init__(self, cfg, lanes):
+        lib.orc_labeler_create.restype = _vp
+        self.h = _vp(lib.orc_labeler_create(ctypes.byref(cfg), ctypes.c_int32(lanes)))
+        self.lanes = lanes
+
+    def run(self, traces, models):
+        traces = np.ascontiguousarray(traces, dtype=np.int16)
+        frames = traces.shape[1]
+        labels = np.empty_like(traces)
+        latent = np.empty((self.lanes, LATENT, LANE), dtype=np.int16)
+        score = np.empty((self.lanes, LANE), dtype=np.float32)
+        lib.orc_labeler_process(self.h, _p(traces), ctypes.c_int32(frames), _p(np.ascontiguousarray(models)),
+                                _p(labels), _p(latent), _p(score))
+        return labels, latent, score
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_labeler_destroy(self.h)
+            self.h = None
+
+
+class Pulser:
+    def __init__(self, cfg, lanes):
+        lib.orc_pulse_create.restype = _vp
+        self.h = _vp(lib.orc_pulse_create(ctypes.byref(cfg), ctypes.c_int32(lanes)))
+        self.lanes = lanes
+        self.max_calls = cfg.max_calls_per_zmw
+
+    def run(self, labels, latent, traces, first_frame, models, pulse_dtype, stat_dtype):
+        labels = np.ascontiguousarray(labels, dtype=np.int16)
+        frames = labels.shape[1]
+        n = self.lanes * LANE
+        pulses = np.zeros((n, self.max_calls), dtype=pulse_dtype)
+        counts = np.zeros(n, dtype=np.uint32)
+        ovf = np.zeros(n, dtype=np.uint32)
+        bl = np.zeros(self.lanes, dtype=stat_dtype)
+        lib.orc_pulse_process(self.h, _p(labels), _p(np.ascontiguousarray(latent, dtype=np.int16)),
+                              _p(np.ascontiguousarray(traces, dtype=np.int16)), ctypes.c_int32(frames),
+                              ctypes.c_int32(first_frame), _p(np.ascontiguousarray(models)), _p(pulses), _p(counts),
+                              _p(ovf), _p(bl))
+        return pulses, counts, ovf, bl
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_pulse_destroy(self.h)
+            self.h = None
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("pulses", _vp), ("pulse_counts", _vp), ("pulse_overflows", _vp), ("pulse_bl_stats", _vp),
+                ("baselined", _vp), ("bl_stats", _vp), ("labels", _vp), ("viterbi_score", _vp), ("models", _vp),
+                ("flags", ctypes.c_int32 * 4)]
+
+
+class Analyzer:
+    """DynamicEstimateBatchAnalyzer restated on the CPU."""
+
+    def __init__(self, cfg, pool_id, lanes, frames):
+        lib.orc_analyzer_create.restype = _vp
+        self.h = _vp(lib.orc_analyzer_create(ctypes.byref(cfg), ctypes.c_uint32(pool_id), ctypes.c_int32(lanes),
+                                             ctypes.c_int32(frames)))
+        self.lanes, self.frames, self.pool_id = lanes, frames, pool_id
+        self.max_calls = cfg.max_calls_per_zmw
+
+    def __call__(self, traces, first_frame, dtypes, pool_id=None, want=("pulses", "labels", "models")):
+        pulse_dt, stat_dt, bls_dt, model_dt = dtypes
+        L, F, n = self.lanes, self.frames, self.lanes * LANE
+        out = {}
+        r = Result()
+        if "pulses" in want:
+            out["pulses"] = np.zeros((n, self.max_calls), dtype=pulse_dt)
+            out["counts"] = np.zeros(n, dtype=np.uint32)
+            out["overflows"] = np.zeros(n, dtype=np.uint32)
+            out["pulse_bl_stats"] = np.zeros(L, dtype=stat_dt)
+            r.pulses, r.pulse_counts = _p(out["pulses"]), _p(out["counts"])
+            r.pulse_overflows, r.pulse_bl_stats = _p(out["overflows"]), _p(out["pulse_bl_stats"])
+        if "baselined" in want:
+            out["baselined"] = np.zeros((L, F, LANE), dtype=np.int16)
+            r.baselined = _p(out["baselined"])
+        if "bl_stats" in want:
+            out["bl_stats"] = np.zeros(L, dtype=bls_dt)
+            r.bl_stats = _p(out["bl_stats"])
+        if "labels" in want:
+            out["labels"] = np.zeros((L, F, LANE), dtype=np.int16)
+            r.labels = _p(out["labels"])
+        if "score" in want:
+            out["score"] = np.zeros((L, LANE), dtype=np.float32)
+            r.viterbi_score = _p(out["score"])
+        if "models" in want:
+            out["models"] = np.zeros(L, dtype=model_dt)
+            r.models = _p(out["models"])
+        traces = np.ascontiguousarray(traces, dtype=np.int16)
+        rc = lib.orc_analyzer_process(self.h, _p(traces), ctypes.c_uint32(self.pool_id if pool_id is None else pool_id),
+                                      ctypes.c_int32(first_frame), ctypes.byref(r))
+        out["rc"] = rc
+        out["flags"] = list(r.flags)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_analyzer_destroy(self.h)
+            self.h = None

@@ -1,0 +1,2007 @@
+// oracle.cpp -- CPU oracle for the trace-analysis path (TEST INFRASTRUCTURE).
+//
+// A scalar restatement of the reference's Host* strategies.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it (as the
+// checker and the CPU baseline); the product never does.
+//
+// Lane semantics: the reference runs every stage on 64-ZMW LaneArrays, so a
+// few decisions are lane-wide (all()/any()).  Those are kept lane-wide here;
+// everything else is a per-ZMW scalar loop with the reference's operation
+// order.  Arithmetic follows the SSE LaneArray rules (int16 wrap, RNE float
+// to int, SSE min/max NaN rule) via include/hwpa_math.h.
+//
+// Parity pins for each part are listed in the test that covers it.
+
+#include "oracle.h"
+#include "../include/hwpa_math.h"
+#include "../include/hwpa_tracegen.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr int LS = HWPA_LANE_SIZE;
+constexpr int NA = HWPA_NUM_ANALOGS;
+constexpr int NB = HWPA_NUM_BINS;
+constexpr int NS = HWPA_NUM_STATES;
+
+int g_threads = 1;
+
+template <typename F>
+void ParLanes(int lanes, F&& fn)
+{
+    const int nt = std::max(1, std::min(g_threads, lanes));
+    if (nt == 1) {
+        for (int l = 0; l < lanes; ++l) fn(l);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t) {
+        pool.emplace_back([&]() {
+            for (int l = next.fetch_add(1); l < lanes; l = next.fetch_add(1)) fn(l);
+        });
+    }
+    for (auto& th : pool) th.join();
+}
+
+// ---------------------------------------------------------------------------
+// Configuration-derived constants
+// ---------------------------------------------------------------------------
+
+// traceAnalysis/BaselinerParams.cpp:10-72 and BaselinerParams.h:45-70
+hwpa_filter_params FilterParams(int type)
+{
+    hwpa_filter_params p{};
+    auto set = [&](std::initializer_list<int> s, std::initializer_list<int> w, float sb, float mb) {
+        p.num_stages = (int)s.size();
+        int i = 0;
+        for (int v : s) p.strides[i++] = v;
+        i = 0;
+        for (int v : w) p.widths[i++] = v;
+        p.sigma_bias = sb;
+        p.mean_bias = mb;
+    };
+    switch (type) {
+    case HWPA_FILTER_MULTISCALE_LARGE: set({1, 2, 8}, {11, 17, 61}, 2.64f, 0.78f); break;
+    case HWPA_FILTER_MULTISCALE_MEDIUM: set({1, 2, 8}, {9, 17, 31}, 2.41f, 0.69f); break;
+    case HWPA_FILTER_MULTISCALE_SMALL: set({1, 2, 8}, {7, 17, 17}, 2.15f, 0.63f); break;
+    case HWPA_FILTER_TWOSCALE_LARGE: set({2, 8}, {11, 61}, 2.71f, 0.60f); break;
+    case HWPA_FILTER_TWOSCALE_MEDIUM: set({2, 8}, {9, 31}, 2.44f, 0.50f); break;
+    default: set({2, 8}, {7, 17}, 2.07f, 0.41f); break;
+    }
+    int cum = 1, minSize = 0;
+    for (int i = 0; i < p.num_stages; ++i) {
+        cum *= p.strides[i];
+        minSize += p.widths[i] * cum;
+    }
+    int ret = 1;
+    while (ret < minSize) ret *= 2;
+    p.latent_size = ret;
+    p.aggregate_stride = cum;
+    return p;
+}
+
+struct DmeConsts {
+    float blVarMin, blVarMax, blVarNominal, signalScaler, shotVarCoeff;
+};
+
+// traceAnalysis/CoreDMEstimator.h:67-109, CoreDMEstimator.cpp:36-87
+DmeConsts MakeDmeConsts(const hwpa_config& c)
+{
+    DmeConsts d{};
+    d.signalScaler = c.movie.photoelectron_sensitivity;
+    d.shotVarCoeff = c.movie.shot_var_coeff;
+    const float ss = std::max(d.signalScaler, 1.0f);
+    d.blVarMin = ss * ss * (1.0f / 12.0f);
+    const float t = 2.0f * d.signalScaler;
+    d.blVarNominal = t * t;
+    d.blVarMax = 36.0f * d.blVarNominal;
+    return d;
+}
+
+// DetectionModelHost::ModelSignalCovar (traceAnalysis/DetectionModelHost.cpp:247-256)
+inline float ModelSignalCovar(const DmeConsts& d, float cv2, float mean, float blVar)
+{
+    float r = blVar;
+    r += d.shotVarCoeff * mean;
+    r += cv2 * (mean * mean);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Stat accumulators (common/StatAccumulator.h, common/AutocorrAccumulator.cpp,
+// dataTypes/BaselinerStatAccumulator.cpp)
+// ---------------------------------------------------------------------------
+
+struct StatZ {
+    float off = 0, m0 = 0, m1 = 0, m2 = 0;
+    void Add(float v)
+    {
+        const float x = v - off;
+        m0 += 1.0f;
+        m1 += x;
+        m2 += x * x;
+    }
+    void AddMasked(float v, bool mask)
+    {
+        const float x = v - off;
+        if (mask) {
+            m0 = m0 + 1.0f;
+            m1 = m1 + x;
+            m2 = m2 + x * x;
+        }
+    }
+    float Mean() const { return m1 / m0 + off; }
+    float Variance() const
+    {
+        float var = m1 * m1 / m0;
+        var = (m2 - var) / (m0 - 1.0f);
+        var = hw_maxf(var, 0.0f);
+        return (m0 > 1.0f) ? var : NAN;
+    }
+};
+
+struct LaneStats {
+    // AutocorrAccumulator (lag 4) over all frames
+    StatZ corr[LS];
+    float lagM2[LS];
+    float fbuf[4][LS];
+    float bbuf[4][LS];
+    int fbi = 0, bbi = 0;
+    int16_t tmin[LS], tmax[LS];
+    StatZ bl[LS];
+    float rawBlSum[LS];
+    StatZ bg[LS];
+
+    LaneStats() { Reset(); }
+    void Reset()
+    {
+        for (int z = 0; z < LS; ++z) {
+            corr[z] = StatZ{};
+            lagM2[z] = 0;
+            for (int k = 0; k < 4; ++k) fbuf[k][z] = bbuf[k][z] = 0;
+            tmin[z] = 32767;
+            tmax[z] = -32768;
+            bl[z] = StatZ{};
+            rawBlSum[z] = 0;
+            bg[z] = StatZ{};
+        }
+        fbi = bbi = 0;
+    }
+
+    // BaselinerStatAccumulator::AddSample (dataTypes/BaselinerStatAccumulator.cpp:38-53)
+    void AddSample(const int16_t* raw, const int16_t* bls, const bool* mask)
+    {
+        const bool fillFront = fbi < 4;
+        const int bslot = bbi % 4;
+        for (int z = 0; z < LS; ++z) {
+            const float bs = (float)bls[z];
+            // AutocorrAccumulator::AddSample (common/AutocorrAccumulator.cpp:29-38)
+            const float v = bs - corr[z].off;
+            if (fillFront) fbuf[fbi][z] = v;
+            lagM2[z] += bbuf[bslot][z] * v;
+            bbuf[bslot][z] = v;
+            corr[z].Add(bs);
+            tmin[z] = std::min(bls[z], tmin[z]);
+            tmax[z] = std::max(bls[z], tmax[z]);
+            bl[z].AddMasked(bs, mask[z]);
+            rawBlSum[z] += (float)(mask[z] ? raw[z] : (int16_t)0);
+        }
+        if (fillFront) ++fbi;
+        bbi = (bbi + 1) % 4;
+    }
+
+    void AddBackground(const int16_t* v)
+    {
+        for (int z = 0; z < LS; ++z) bg[z].Add((float)v[z]);
+    }
+
+    void ToState(hwpa_baseliner_stats* s) const
+    {
+        for (int z = 0; z < LS; ++z) {
+            s->full.basic.offset[z] = corr[z].off;
+            s->full.basic.m0[z] = corr[z].m0;
+            s->full.basic.m1[z] = corr[z].m1;
+            s->full.basic.m2[z] = corr[z].m2;
+            s->full.lag_m2[z] = lagM2[z];
+            for (int k = 0; k < 4; ++k) {
+                s->full.fbuf[k][z] = fbuf[k][z];
+                s->full.bbuf[k][z] = bbuf[k][z];
+            }
+            s->full.bidx[0][z] = (uint8_t)fbi;
+            s->full.bidx[1][z] = (uint8_t)bbi;
+            s->trace_min[z] = tmin[z];
+            s->trace_max[z] = tmax[z];
+            s->baseline.offset[z] = bl[z].off;
+            s->baseline.m0[z] = bl[z].m0;
+            s->baseline.m1[z] = bl[z].m1;
+            s->baseline.m2[z] = bl[z].m2;
+            s->raw_baseline_sum[z] = rawBlSum[z];
+            s->background.offset[z] = bg[z].off;
+            s->background.m0[z] = bg[z].m0;
+            s->background.m1[z] = bg[z].m1;
+            s->background.m2[z] = bg[z].m2;
+        }
+    }
+
+    void FromState(const hwpa_baseliner_stats* s)
+    {
+        for (int z = 0; z < LS; ++z) {
+            corr[z] = StatZ{s->full.basic.offset[z], s->full.basic.m0[z], s->full.basic.m1[z], s->full.basic.m2[z]};
+            lagM2[z] = s->full.lag_m2[z];
+            for (int k = 0; k < 4; ++k) {
+                fbuf[k][z] = s->full.fbuf[k][z];
+                bbuf[k][z] = s->full.bbuf[k][z];
+            }
+            tmin[z] = s->trace_min[z];
+            tmax[z] = s->trace_max[z];
+            bl[z] = StatZ{s->baseline.offset[z], s->baseline.m0[z], s->baseline.m1[z], s->baseline.m2[z]};
+            rawBlSum[z] = s->raw_baseline_sum[z];
+            bg[z] = StatZ{s->background.offset[z], s->background.m0[z], s->background.m1[z], s->background.m2[z]};
+        }
+        fbi = s->full.bidx[0][0];
+        bbi = s->full.bidx[1][0];
+    }
+
+    // BaselinerStatAccumulator::Merge (dataTypes/BaselinerStatAccumulator.cpp:64-75) with
+    // AutocorrAccumulator::Merge (common/AutocorrAccumulator.cpp:72-104) and
+    // StatAccumulator::Merge (common/StatAccumulator.h:183-191).
+    void Merge(const LaneStats& o)
+    {
+        const int n1 = 4 - o.fbi;
+        const int n2 = 4 - fbi;
+        for (int z = 0; z < LS; ++z) {
+            corr[z].m0 += o.corr[z].m0;
+            corr[z].m1 += o.corr[z].m1;
+            corr[z].m2 += o.corr[z].m2;
+            lagM2[z] += o.lagM2[z];
+            for (int k = 0; k < 4 - n1; ++k) {
+                lagM2[z] += bbuf[(bbi + k) % 4][z] * o.fbuf[k][z];
+                bbuf[(bbi + k) % 4][z] = o.bbuf[(o.bbi + n1 + k) % 4][z];
+            }
+            for (int k = 0; k < n2; ++k) fbuf[fbi + k][z] = o.fbuf[k][z];
+            tmin[z] = std::min(tmin[z], o.tmin[z]);
+            tmax[z] = std::max(tmax[z], o.tmax[z]);
+            bl[z].m0 += o.bl[z].m0;
+            bl[z].m1 += o.bl[z].m1;
+            bl[z].m2 += o.bl[z].m2;
+            rawBlSum[z] += o.rawBlSum[z];
+            bg[z].m0 += o.bg[z].m0;
+            bg[z].m1 += o.bg[z].m1;
+            bg[z].m2 += o.bg[z].m2;
+        }
+        fbi += n2;
+        bbi = (bbi + (4 - n1)) % 4;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Baseliner: van Herk / Gil-Werman extremum filters
+// (traceAnalysis/TraceFilters.cpp:10-57, WindowBuffer.h:12-41, BlockFilterStage.*)
+// ---------------------------------------------------------------------------
+
+struct HgwFilter {
+    int W = 0, stride = 1, nf = 1;
+    bool isMax = false;
+    std::vector<int16_t> ring;  // [W][LS], ring[(head + j) % W] is element j
+    int head = 0;
+    int counter = 0;
+    int16_t hold[LS];
+
+    void Init(int w, int s, int n, bool mx)
+    {
+        W = w;
+        stride = s;
+        nf = n;
+        isMax = mx;
+        ring.assign((size_t)W * LS, 0);  // BlockFilterStage ctor pushes W zeros
+        head = 0;
+        counter = 0;  // W pushes leave the counter at W % W
+        for (int z = 0; z < LS; ++z) hold[z] = 0;
+    }
+    int16_t Op(int16_t a, int16_t b) const { return isMax ? std::max(a, b) : std::min(a, b); }
+    int16_t* R(int j) { return &ring[(size_t)((head + j) % W) * LS]; }
+
+    // BlockFilterStage::operator() + extremumFilterHgw, in place on buf[frames][LS].
+    void Run(int16_t* buf, int frames)
+    {
+        const int dist = frames / nf;
+        int16_t s[LS];
+        for (int z = 0; z < LS; ++z) s[z] = hold[z];
+        int k = 0;
+        for (int i = 0; i < dist; i += stride) {
+            if (counter == 0) {
+                for (int j = W - 1; j > 0; --j) {
+                    int16_t* rj = R(j);
+                    int16_t* rj1 = R(j - 1);
+                    for (int z = 0; z < LS; ++z) rj1[z] = Op(rj[z], rj1[z]);
+                }
+                const int16_t* back = R(W - 1);
+                for (int z = 0; z < LS; ++z) s[z] = back[z];
+            } else {
+                const int16_t* back = R(W - 1);
+                for (int z = 0; z < LS; ++z) s[z] = Op(s[z], back[z]);
+            }
+            int16_t front[LS];
+            std::memcpy(front, R(0), sizeof(front));
+            int16_t* slot = &ring[(size_t)head * LS];  // push_back overwrites the front slot
+            const int16_t* x = &buf[(size_t)i * LS];
+            for (int z = 0; z < LS; ++z) slot[z] = x[z];
+            head = (head + 1) % W;
+            counter = (counter + 1) % W;
+            int16_t* o = &buf[(size_t)k * LS];
+            for (int z = 0; z < LS; ++z) o[z] = Op(s[z], front[z]);
+            ++k;
+        }
+        for (int z = 0; z < LS; ++z) hold[z] = s[z];
+    }
+};
+
+struct LaneBaseliner {
+    std::vector<HgwFilter> lower, upper;
+    float emaSum[LS], emaWeight[LS], sigmaEma[LS];
+    int16_t latData[LS], latRaw[LS];
+    bool latL[LS], latH1[LS], latH2[LS];
+};
+
+struct BaselinerCtx {
+    hwpa_filter_params fp;
+    float cMeanBias, cSigmaBias, meanEmaAlpha, sigmaEmaAlpha, jumpTol, scaler;
+    int16_t pedestal;
+    int lanes;
+    std::vector<LaneBaseliner> lb;
+};
+
+// HostMultiScaleBaseliner::LaneBaseliner::GetSmoothedBlEstimate
+// (traceAnalysis/HostMultiScaleBaseliner.cpp:178-212)
+float SmoothedBlEstimate(const BaselinerCtx& c, LaneBaseliner& L, int z, int16_t lower, int16_t upper)
+{
+    const float minSigma = .288675135f;
+    const float sigma = hw_maxf((float)hw_wrap16((int32_t)upper - (int32_t)lower) / c.cSigmaBias, minSigma);
+    const float newSigmaEma = c.sigmaEmaAlpha * L.sigmaEma[z] + (1.0f - c.sigmaEmaAlpha) * sigma;
+    const float blEst = 0.5f * (float)hw_wrap16((int32_t)upper + (int32_t)lower) + c.cMeanBias * newSigmaEma;
+    const bool mask = (L.emaWeight[z] == 0.0f) ||
+                      (blEst - L.emaSum[z] / L.emaWeight[z] < c.jumpTol * L.sigmaEma[z]);
+    const float newWeight = c.meanEmaAlpha * L.emaWeight[z] + (1.0f - c.meanEmaAlpha);
+    const float newSum = c.meanEmaAlpha * L.emaSum[z] + (1.0f - c.meanEmaAlpha) * blEst;
+    if (mask) {
+        L.emaWeight[z] = newWeight;
+        L.emaSum[z] = newSum;
+        L.sigmaEma[z] = newSigmaEma;
+    }
+    return L.emaSum[z] / L.emaWeight[z];
+}
+
+// LaneBaseliner::EstimateBaseline + AddToBaselineStats
+// (traceAnalysis/HostMultiScaleBaseliner.cpp:93-176)
+void EstimateBaselineLane(BaselinerCtx& c, int l, const int16_t* raw, int frames, int16_t* out,
+                          hwpa_baseliner_stats* statsOut)
+{
+    LaneBaseliner& L = c.lb[l];
+    std::vector<int16_t> lo(raw, raw + (size_t)frames * LS), up(raw, raw + (size_t)frames * LS);
+    for (auto& f : L.lower) f.Run(lo.data(), frames);
+    for (auto& f : L.upper) f.Run(up.data(), frames);
+
+    LaneStats st;
+    const int stride = c.fp.aggregate_stride;
+    const int fd = frames / stride;
+    const float thrScale = 4.5f;
+    for (int i = 0; i < fd; ++i) {
+        float subBl[LS];
+        for (int z = 0; z < LS; ++z) {
+            const int16_t lv = hw_wrap16((int32_t)lo[(size_t)i * LS + z] - c.pedestal);
+            const int16_t uv = hw_wrap16((int32_t)up[(size_t)i * LS + z] - c.pedestal);
+            const float blEst = SmoothedBlEstimate(c, L, z, lv, uv);
+            subBl[z] = blEst * c.scaler;
+        }
+        for (int j = 0; j < stride; ++j) {
+            const size_t t = (size_t)(i * stride + j);
+            int16_t rawRounded[LS], bls[LS];
+            bool mask[LS];
+            for (int z = 0; z < LS; ++z) {
+                const float rawScaled = (float)hw_wrap16((int32_t)raw[t * LS + z] - c.pedestal) * c.scaler;
+                bls[z] = hw_round_i16(rawScaled - subBl[z]);
+                out[t * LS + z] = bls[z];
+                rawRounded[z] = hw_round_i16(rawScaled);
+                const float thrLow = thrScale * L.sigmaEma[z];
+                const float thrHigh = thrScale * L.sigmaEma[z];
+                const bool maskHp1 = (float)bls[z] < thrHigh * c.scaler;
+                mask[z] = L.latH1[z] && L.latL[z] && maskHp1;
+                L.latL[z] = (float)bls[z] < thrLow * c.scaler;
+                L.latH2[z] = L.latH1[z];
+                L.latH1[z] = maskHp1;
+            }
+            st.AddSample(L.latRaw, L.latData, mask);
+            for (int z = 0; z < LS; ++z) {
+                L.latRaw[z] = rawRounded[z];
+                L.latData[z] = bls[z];
+            }
+        }
+        int16_t bgv[LS];
+        for (int z = 0; z < LS; ++z) bgv[z] = hw_round_i16(subBl[z]);
+        st.AddBackground(bgv);
+    }
+    st.ToState(statsOut);
+}
+
+// ---------------------------------------------------------------------------
+// Trace histogram (traceAnalysis/TraceHistogramAccumHost.cpp:92-169,
+// dataTypes/UHistogramSimd.*, traceAnalysis/EdgeFrameClassifier.h)
+// ---------------------------------------------------------------------------
+
+struct LaneHistState {
+    float binSize[LS];
+    float binStart[NB + 1][LS];
+    uint16_t lowOut[LS], highOut[LS];
+    uint16_t count[NB + 1][LS];
+    // EdgeFrameClassifier: never reset by histogram resets
+    int16_t prevFrame[LS];
+    bool hist[2][LS];
+    int hi0 = 0, hi1 = -1;
+};
+
+struct HistCtx {
+    float binSizeCoeff, binSizeMin, fallbackSigma;
+    float minFrameCount;
+    int lanes;
+    std::vector<LaneHistState> h;
+};
+
+// TraceHistogramAccumHost::ResetImpl(BaselinerMetrics) + UHistogramSimd ctor
+int HistResetLane(HistCtx& c, int l, const hwpa_baseliner_stats& s)
+{
+    LaneHistState& H = c.h[l];
+    int err = 0;
+    for (int z = 0; z < LS; ++z) {
+        StatZ st{s.baseline.offset[z], s.baseline.m0[z], s.baseline.m1[z], s.baseline.m2[z]};
+        const bool suff = st.m0 >= c.minFrameCount;
+        const float mean = suff ? st.Mean() : 0.0f;
+        const float sigma = suff ? hw_sqrtf(st.Variance()) : c.fallbackSigma;
+        const float binSize = hw_maxf(c.binSizeCoeff * sigma, c.binSizeMin);
+        const float lo = mean - 4.0f * sigma;
+        const float up = lo + (float)NB * binSize;
+        if (!(lo < up)) err = 1;
+        const float bs = (up - lo) / (float)NB;
+        if (!(bs > 0.0f)) err = 1;
+        H.binSize[z] = bs;
+        H.binStart[0][z] = lo;
+        for (int i = 1; i <= NB; ++i) H.binStart[i][z] = bs * (float)i + lo;
+        H.binStart[NB][z] = up;
+        H.lowOut[z] = H.highOut[z] = 0;
+        for (int i = 0; i <= NB; ++i) H.count[i][z] = 0;
+    }
+    return err;
+}
+
+void HistAddLane(HistCtx& c, int l, const int16_t* tr, int frames, const hwpa_lane_model& m)
+{
+    LaneHistState& H = c.h[l];
+    int16_t thr[LS];
+    for (int z = 0; z < LS; ++z) {
+        const float var = hw_h2f(m.baseline.vars[z]);
+        const float mean = hw_h2f(m.baseline.means[z]);
+        thr[z] = hw_wrap16(hw_round_i32(2.0f * hw_sqrtf(var) + mean));
+    }
+    for (int t = 0; t < frames; ++t) {
+        const int16_t* f = tr + (size_t)t * LS;
+        const bool full = (H.hi1 >= 0) && (1 + (2 + H.hi1 - H.hi0) % 2) == 2;
+        bool isBl[LS], edge[LS];
+        for (int z = 0; z < LS; ++z) {
+            isBl[z] = f[z] < thr[z];
+            edge[z] = full ? (isBl[z] ^ H.hist[H.hi0][z]) : true;
+        }
+        // CircularArray<LaneMask,2>::push_back (common/CircularArray.h)
+        if (full) H.hi0 = (H.hi0 + 1) % 2;
+        H.hi1 = (H.hi1 + 1) % 2;
+        for (int z = 0; z < LS; ++z) H.hist[H.hi1][z] = isBl[z];
+        for (int z = 0; z < LS; ++z) {
+            const int16_t cand = H.prevFrame[z];
+            H.prevFrame[z] = f[z];
+            if (edge[z]) continue;
+            const float y = ((float)cand - H.binStart[0][z]) / H.binSize[z];
+            const int bin = hw_floor_i32(y);
+            if (bin < 0) H.lowOut[z] = (uint16_t)(H.lowOut[z] + 1);
+            else if (bin >= NB) H.highOut[z] = (uint16_t)(H.highOut[z] + 1);
+            else H.count[bin][z] = (uint16_t)(H.count[bin][z] + 1);
+        }
+    }
+}
+
+void HistGetLane(const HistCtx& c, int l, hwpa_lane_hist& o)
+{
+    const LaneHistState& H = c.h[l];
+    for (int z = 0; z < LS; ++z) {
+        o.low_bound[z] = H.binStart[0][z];
+        o.bin_size[z] = H.binSize[z];
+        o.outlier_low[z] = H.lowOut[z];
+        o.outlier_high[z] = H.highOut[z];
+        for (int b = 0; b < NB; ++b) o.bin_count[b][z] = H.count[b][z];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Detection model (traceAnalysis/DetectionModelHost.*, DmeEmHost.cpp)
+// ---------------------------------------------------------------------------
+
+struct ModeZ {
+    float w, mean, var;
+};
+struct ModelZ {
+    ModeZ bl;
+    ModeZ an[NA];
+    float conf;
+};
+
+ModelZ ModelFromHalf(const hwpa_lane_model& m, int z)
+{
+    ModelZ r{};
+    r.bl = {hw_h2f(m.baseline.weights[z]), hw_h2f(m.baseline.means[z]), hw_h2f(m.baseline.vars[z])};
+    for (int a = 0; a < NA; ++a)
+        r.an[a] = {hw_h2f(m.analogs[a].weights[z]), hw_h2f(m.analogs[a].means[z]), hw_h2f(m.analogs[a].vars[z])};
+    r.conf = hw_h2f(m.confidence[z]);
+    return r;
+}
+
+// SignalModeHost::ExportTo / DetectionModelHost::ExportTo (DetectionModelHost.cpp:225-245,270-278)
+void ModelToHalf(const ModelZ& r, hwpa_lane_model& m, int z)
+{
+    m.baseline.weights[z] = hw_f2h(r.bl.w);
+    m.baseline.means[z] = hw_f2h(r.bl.mean);
+    m.baseline.vars[z] = hw_f2h(hw_minf(r.bl.var, 65000.0f));
+    for (int a = 0; a < NA; ++a) {
+        m.analogs[a].weights[z] = hw_f2h(r.an[a].w);
+        m.analogs[a].means[z] = hw_f2h(r.an[a].mean);
+        m.analogs[a].vars[z] = hw_f2h(hw_minf(r.an[a].var, 65000.0f));
+    }
+    m.confidence[z] = hw_f2h(r.conf);
+}
+
+struct DmeCfg {
+    DmeConsts k;
+    float refSnr;
+    float relAmp[NA], cv[NA];
+    float mixThresh0, mixThresh1, scaleSnrConfTol, pulseAmpReg, snrDrop, snrThresh0, snrThresh1, successThresh;
+    float hl0, hl1;
+    unsigned iterLimit;
+    bool iterToLimit;
+    unsigned updateMethod;
+    float log2pi;
+};
+
+DmeCfg MakeDmeCfg(const hwpa_config& c)
+{
+    DmeCfg d{};
+    d.k = MakeDmeConsts(c);
+    d.refSnr = c.movie.ref_snr;
+    for (int a = 0; a < NA; ++a) {
+        d.relAmp[a] = c.movie.analogs[a].rel_amplitude;
+        d.cv[a] = c.movie.analogs[a].excess_noise_cv;
+    }
+    d.mixThresh0 = c.analog_mix_fraction_threshold[0];
+    d.mixThresh1 = c.analog_mix_fraction_threshold[1];
+    d.scaleSnrConfTol = c.scale_snr_conf_tol;
+    d.pulseAmpReg = c.pulse_amp_regularization;
+    d.snrDrop = c.snr_drop_thresh;
+    d.snrThresh0 = c.min_analog_snr_thresh0;
+    d.snrThresh1 = c.min_analog_snr_thresh1;
+    d.successThresh = c.success_confidence_thresh;
+    d.hl0 = c.confidence_half_life[0];
+    d.hl1 = c.confidence_half_life[1];
+    d.iterLimit = c.em_iteration_limit;
+    d.iterToLimit = c.iterate_to_limit != 0;
+    d.updateMethod = c.model_update_method;
+    d.log2pi = hw_logf(2.0f * 3.1415926536f);
+    return d;
+}
+
+// DetectionModelHost::Update0/1/2 (DetectionModelHost.cpp:69-165)
+void UpdateFraction(const DmeCfg& d, ModelZ& t, const ModelZ& o, float f)
+{
+    const float a = 1.0f - f;
+    const float b = f;
+    if (d.updateMethod == 0) {
+        t.bl.mean = a * t.bl.mean + b * o.bl.mean;
+        t.bl.var = a * t.bl.var + b * o.bl.var;
+        for (int i = 0; i < NA; ++i) {
+            t.an[i].mean = a * t.an[i].mean + b * o.an[i].mean;
+            t.an[i].var = a * t.an[i].var + b * o.an[i].var;
+        }
+        return;
+    }
+    const float prevBlVar = t.bl.var;
+    const float bw = a * t.bl.w + b * o.bl.w;
+    t.bl.w = bw;
+    t.bl.mean = a * t.bl.mean + b * o.bl.mean;
+    const float bv = hw_powf(t.bl.var, a) * hw_powf(o.bl.var, b);
+    t.bl.var = bv;
+    const float aw = 0.25f * (1.0f - bw);
+    for (int i = 0; i < NA; ++i) {
+        t.an[i].w = aw;
+        float cv2;
+        if (d.updateMethod == 2) {
+            auto xsn = [&](float m, float v, float blv) {
+                float r = v - blv;
+                r -= m * d.k.shotVarCoeff;
+                r /= (m * m);
+                return r;
+            };
+            const float tx = xsn(t.an[i].mean, t.an[i].var, prevBlVar);
+            const float ox = xsn(o.an[i].mean, o.an[i].var, o.bl.var);
+            cv2 = a * tx + b * ox;
+        } else {
+            cv2 = d.cv[i] * d.cv[i];
+        }
+        const float am = hw_powf(t.an[i].mean, a) * hw_powf(o.an[i].mean, b);
+        t.an[i].mean = am;
+        t.an[i].var = ModelSignalCovar(d.k, cv2, am, bv);
+    }
+}
+
+// DetectionModelHost::Update(const DetectionModelHost&) (DetectionModelHost.cpp:183-205)
+void UpdateConf(const DmeCfg& d, ModelZ& t, const ModelZ& o)
+{
+    const float confSum = t.conf + o.conf;
+    const float fraction = (confSum > 0.0f) ? o.conf / confSum : 0.0f;
+    UpdateFraction(d, t, o, fraction);
+    t.conf = confSum;
+}
+
+// DmeEmHost::InitLaneDetModel (DmeEmHost.cpp:871-925)
+void InitLaneModel(const DmeCfg& d, const hwpa_baseliner_stats& s, hwpa_lane_model& m)
+{
+    for (int z = 0; z < LS; ++z) {
+        StatZ bst{s.baseline.offset[z], s.baseline.m0[z], s.baseline.m1[z], s.baseline.m2[z]};
+        const float blWeight = s.baseline.m0[z] / s.full.basic.m0[z];
+        float blMean = bst.Mean();
+        float blVar = bst.Variance();
+        if (hw_isnan(blMean)) blMean = 0.0f;
+        if (hw_isnan(blVar)) blVar = 100.0f;
+        blVar = hw_clampf(blVar, d.k.blVarMin, d.k.blVarMax);
+        m.confidence[z] = hw_f2h(0.1f);
+        m.baseline.weights[z] = hw_f2h(blWeight);
+        m.baseline.means[z] = hw_f2h(blMean);
+        m.baseline.vars[z] = hw_f2h(blVar);
+        const float amw = (1.0f - blWeight) / (float)NA;
+        const float refSignal = d.refSnr * hw_sqrtf(blVar);
+        for (int a = 0; a < NA; ++a) {
+            m.analogs[a].weights[z] = hw_f2h(amw);
+            const float aMean = hw_maxf(0.0f, blMean + d.relAmp[a] * refSignal);
+            m.analogs[a].means[z] = hw_f2h(aMean);
+            m.analogs[a].vars[z] = hw_f2h(ModelSignalCovar(d.k, d.cv[a] * d.cv[a], aMean, blVar));
+        }
+    }
+}
+
+// Histogram view for the EM (UHistogramSimd(const LaneHistogram&), UHistogramSimd.cpp:100-118)
+struct HistZ {
+    float binSize;
+    float start[NB + 1];
+    uint16_t cnt[NB + 1];  // cnt[NB] is UHistogramSimd's always-empty overflow bin
+    uint16_t lowOut, highOut;
+    uint16_t InRange() const
+    {
+        uint16_t s = 0;
+        for (int i = 0; i < NB; ++i) s = (uint16_t)(s + cnt[i]);
+        return s;
+    }
+    uint16_t Total() const { return (uint16_t)(InRange() + (uint16_t)(lowOut + highOut)); }
+
+    // UHistogramSimd::CumulativeCount (UHistogramSimd.cpp:199-238)
+    float CumulativeCount(float x) const
+    {
+        const bool mNan = hw_isnan(x);
+        const bool mLow = x < start[0];
+        const bool mHigh = x >= start[NB];
+        const bool mIn = !(mNan || mLow || mHigh);
+        int xbin = hw_floor_i32((x - start[0]) / binSize);
+        if (!mIn) xbin = 0;
+        const float xrem = x - start[xbin];
+        const float xbinCount = (float)cnt[xbin];
+        float cc = (float)lowOut;
+        uint16_t below = 0;
+        for (int i = 0; i < xbin; ++i) below = (uint16_t)(below + cnt[i]);
+        cc += (float)below;
+        cc += xbinCount * xrem / binSize;
+        if (mLow) cc = (float)lowOut;
+        if (mHigh) cc = (float)(uint16_t)(lowOut + InRange());
+        if (mNan) cc = NAN;
+        return cc;
+    }
+
+    // UHistogramSimd::Fractile (UHistogramSimd.cpp:140-197)
+    float Fractile(float frac) const
+    {
+        const float nfz = frac * (float)Total();
+        uint16_t n = lowOut;
+        if (n > 0 && (float)n >= nfz) return -INFINITY;
+        int i = 0;
+        while ((n == 0 || (float)n < nfz) && i < NB) n = (uint16_t)(n + cnt[i++]);
+        if ((float)n < nfz) return INFINITY;
+        i -= 1;
+        const float x0 = start[i];
+        const uint16_t niz = cnt[i];
+        const int m = (int)n - (int)niz;
+        return x0 + binSize * (nfz - (float)m) / (float)(niz + 1);
+    }
+};
+
+// Eigen 3.3 reduction orders used by DmeEmHost (fixed-size 5: tree; dynamic: linear).
+inline float Sum5Tree(const float* v) { return (v[0] + v[1]) + (v[2] + (v[3] + v[4])); }
+
+struct EmZ {
+    ModelZ det;      // detModel (running)
+    ModelZ work;     // workModel
+    float scaleFactor;
+    float rho[5], mu[5], var[5];
+    float rhoEst[5], muEst[5], varEst[5];
+    float s, sExpect, sExpectWeight;
+    float logLike, logLikePrev;
+    float binProbMin;
+    int status;
+    bool converged;
+    int iterCount;
+    float mlLogLike, mlDelta;
+    uint16_t numFrames;
+    float njSum;
+    float varQuant;
+};
+
+// DmeEmHost::PrelimScaleFactor (DmeEmHost.cpp:618-654)
+float PrelimScaleFactor(const DmeCfg& d, const ModelZ& m, const HistZ& h)
+{
+    const float bgSigma = hw_sqrtf(m.bl.var);
+    const float thresh = 2.5f * bgSigma + m.bl.mean;
+    const float bgCount = h.CumulativeCount(thresh);
+    const float totalCount = (float)(uint16_t)(h.lowOut + h.InRange());
+    const float fractile = 0.5f * (bgCount + totalCount) / totalCount;
+    float avg = 0.0f;
+    for (int a = 0; a < NA; ++a) avg += m.an[a].mean;
+    avg /= (float)NA;
+    const float w = hw_satlin(0.0f, d.scaleSnrConfTol, m.conf);
+    float sf = h.Fractile(fractile) / avg;
+    sf = (1.0f - w) * sf + w;
+    return hw_minf(hw_maxf(sf, 0.1f), 10.0f);
+}
+
+// DmeEmHost::ComputeConfidence (DmeEmHost.cpp:717-803)
+void ComputeConfidence(const DmeCfg& d, bool converged, const ModelZ& ref, const ModelZ& est, float* cf)
+{
+    cf[0] = converged ? 1.0f : 0.0f;
+    cf[1] = hw_satlin(0.05f, 0.15f, est.bl.w);
+    float x = (est.bl.mean * est.bl.mean) / est.bl.var;
+    x = hw_expf(-x / (2.0f * (1.0f * 1.0f)));
+    cf[2] = x;
+    x = hw_log2f(est.bl.var / ref.bl.var);
+    const float tol = 1.5f / (0.5f + ref.conf);
+    x = hw_expf(-x * x / (2.0f * (tol * tol)));
+    cf[3] = x;
+    x = 1.0f;
+    if (d.mixThresh1 > 0.0f) {
+        x *= hw_satlin(d.mixThresh0, d.mixThresh1, est.an[0].w);
+        x *= hw_satlin(d.mixThresh0, d.mixThresh1, est.an[NA - 1].w);
+    }
+    cf[4] = x;
+    const float bgSigma = hw_sqrtf(est.bl.var);
+    x = est.an[NA - 1].mean;
+    x /= bgSigma;
+    cf[5] = hw_satlin(d.snrThresh0, d.snrThresh1, x);
+    if (d.snrDrop < 0.0f) {
+        cf[6] = 1.0f;
+    } else {
+        const float snrEst = est.an[0].mean / bgSigma;
+        const float refBgSigma = hw_sqrtf(ref.bl.var);
+        const float refSnr0 = ref.an[0].mean / refBgSigma;
+        float refSnr1 = ref.an[1].mean / refBgSigma;
+        refSnr1 *= d.snrDrop;
+        refSnr1 *= hw_minf(ref.conf, 1.0f);
+        cf[6] = hw_satlin(refSnr1, hw_sqrtf(refSnr0 * refSnr1), snrEst);
+    }
+    cf[7] = 1.0f;  // G-test disabled (GTestStatFactor < 0): pValue stays 1
+}
+
+// DmeEmHost::EstimateLaneDetModel (DmeEmHost.cpp:216-615) for one lane.
+void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseliner_stats& bs,
+                  hwpa_lane_model& lm, const int32_t* modelFi, const int32_t* estFi, orc_dme_diag* dx)
+{
+    static thread_local std::vector<EmZ> ez;
+    ez.resize(LS);
+    std::vector<HistZ> hz(LS);
+    // tau[z][mode][b], boundaryProb[z][b], binProb[z][b]
+    std::vector<float> tau((size_t)LS * 5 * (NB + 1)), bp((size_t)LS * (NB + 1)), binProb((size_t)LS * NB);
+    auto TAU = [&](int z, int m, int b) -> float& { return tau[((size_t)z * 5 + m) * (NB + 1) + b]; };
+
+    const float rpa[NA] = {d.relAmp[0], d.relAmp[1], d.relAmp[2], d.relAmp[3]};
+    const float rpaSqSum = (rpa[0] * rpa[0] + rpa[2] * rpa[2]) + (rpa[1] * rpa[1] + rpa[3] * rpa[3]);
+    const float rpaMin = std::min(std::min(rpa[0], rpa[1]), std::min(rpa[2], rpa[3]));
+
+    // EvolveModel (DmeEmHost.cpp:806-834) + EvolveConfidence (DetectionModelHost.h:153-163)
+    const int32_t tModel = modelFi[0] + (int32_t)((uint32_t)(modelFi[1] - modelFi[0]) / 2u);
+    const int32_t tEst = estFi[0] + (int32_t)((uint32_t)(estFi[1] - estFi[0]) / 2u);
+    const float thl = 0.5f * (float)(tModel + tEst);
+    const float hl = d.hl0 + (d.hl1 - d.hl0) * hw_satlin(56160.0f, 168480.0f, thl);
+    const float tDiff = (float)(tEst - tModel);
+    const float evolve = hw_exp2f(-hw_fabsf(tDiff) / hl);
+
+    for (int z = 0; z < LS; ++z) {
+        EmZ& e = ez[z];
+        HistZ& h = hz[z];
+        h.binSize = lh.bin_size[z];
+        for (int b = 0; b <= NB; ++b) h.start[b] = lh.low_bound[z] + (float)b * h.binSize;
+        for (int b = 0; b < NB; ++b) h.cnt[b] = lh.bin_count[b][z];
+        h.cnt[NB] = 0;
+        h.lowOut = lh.outlier_low[z];
+        h.highOut = lh.outlier_high[z];
+
+        e.det = ModelFromHalf(lm, z);
+        e.det.conf *= evolve;
+
+        // PrelimEstimate (DmeEmHost.cpp:161-214)
+        ModelZ w = e.det;
+        {
+            const float nBl = bs.baseline.m0[z];
+            const float total = bs.full.basic.m0[z];
+            const float blWeight = hw_maxf(nBl / total, 0.01f);
+            const bool mask = nBl >= 3.0f;
+            StatZ blsa{bs.baseline.offset[z], bs.baseline.m0[z], bs.baseline.m1[z], bs.baseline.m2[z]};
+            float blVar = mask ? blsa.Variance() : w.bl.var;
+            blVar = hw_clampf(blVar, d.k.blVarMin, d.k.blVarMax);
+            const float blMean = mask ? blsa.Mean() : w.bl.mean;
+            const float scale = hw_sqrtf(blVar / w.bl.var);
+            for (int a = 0; a < NA; ++a) {
+                const float mean = w.an[a].mean * scale;
+                const float var = ModelSignalCovar(d.k, d.cv[a] * d.cv[a], mean, blVar);
+                w.an[a].mean = mean;
+                w.an[a].var = var;
+                w.an[a].w = 0.25f * (1.0f - blWeight);
+            }
+            w.bl.w = blWeight;
+            w.bl.mean = blMean;
+            w.bl.var = blVar;
+            float conf = 0.1f * hw_satlin(0.0f, 500.0f, nBl - 3.0f);
+            conf *= hw_satlin(d.k.blVarMax, 0.5f * (d.k.blVarNominal + d.k.blVarMax), blVar);
+            w.conf = conf;
+        }
+        UpdateConf(d, e.det, w);
+        e.work = e.det;
+
+        e.numFrames = h.Total();
+        e.scaleFactor = PrelimScaleFactor(d, e.work, h);
+        // ScaleModelSnr (DmeEmHost.cpp:837-852)
+        for (int a = 0; a < NA; ++a) {
+            e.work.an[a].mean = e.scaleFactor * e.work.an[a].mean;
+            e.work.an[a].var = ModelSignalCovar(d.k, d.cv[a] * d.cv[a], e.work.an[a].mean, e.work.bl.var);
+        }
+        const float rhoMin = 0.001f;
+        e.rho[0] = hw_maxf(e.work.bl.w, rhoMin);
+        e.mu[0] = e.work.bl.mean;
+        e.var[0] = e.work.bl.var;
+        for (int a = 0; a < NA; ++a) {
+            e.rho[a + 1] = hw_maxf(e.work.an[a].w, rhoMin);
+            e.mu[a + 1] = e.work.an[a].mean;
+            e.var[a + 1] = e.work.an[a].var;
+        }
+        e.varQuant = (h.binSize * h.binSize) / 12.0f;
+        for (int k = 0; k < 5; ++k) e.var[k] += e.varQuant;
+        e.var[0] = hw_minf(e.var[0], d.k.blVarMax);
+        const float rs = Sum5Tree(e.rho);
+        for (int k = 0; k < 5; ++k) e.rho[k] = e.rho[k] / rs;
+        for (int k = 0; k < 5; ++k) {
+            e.rhoEst[k] = e.rho[k];
+            e.muEst[k] = e.mu[k];
+            e.varEst[k] = e.var[k];
+        }
+        e.s = ((e.mu[1] * rpa[0] + e.mu[2] * rpa[1]) + (e.mu[3] * rpa[2] + e.mu[4] * rpa[3])) / rpaSqSum;
+        e.sExpect = e.s / e.scaleFactor;
+        e.sExpectWeight = e.det.conf * d.pulseAmpReg;
+        e.logLike = -FLT_MAX;
+        e.logLikePrev = e.logLike;
+        e.binProbMin = hw_maxf((float)e.numFrames * h.binSize, 1.0f) * FLT_MIN;
+        e.status = (e.numFrames < 160) ? HWPA_ZS_INSUF_DATA : HWPA_ZS_OK;
+        e.converged = false;
+        e.iterCount = 0;
+        e.mlLogLike = -INFINITY;
+        e.mlDelta = -INFINITY;
+        float njs = 0.0f;
+        for (int b = 0; b < NB; ++b) njs += (float)h.cnt[b];
+        e.njSum = njs;
+    }
+
+    unsigned it = 0;
+    std::vector<float> sci((size_t)LS * 5), smom1((size_t)LS * 5), svarinv((size_t)LS * 5);
+    for (; it < d.iterLimit; ++it) {
+        // E-step for every ZMW of the lane (DmeEmHost.cpp:374-483)
+        bool allDone = true;
+        for (int z = 0; z < LS; ++z) {
+            EmZ& e = ez[z];
+            const HistZ& h = hz[z];
+            float varinv[5], pref[5];
+            for (int k = 0; k < 5; ++k) {
+                varinv[k] = 1.0f / e.var[k];
+                pref[k] = (hw_logf(e.rho[k]) - 0.5f * hw_logf(e.var[k])) - 0.5f * d.log2pi;
+            }
+            float* bpz = &bp[(size_t)z * (NB + 1)];
+            float* binp = &binProb[(size_t)z * NB];
+            auto corner = [&](int b) {
+                const float x = h.start[b];
+                float t[5];
+                for (int i = 0; i < 5; ++i) {
+                    const float y = x - e.mu[i];
+                    t[i] = pref[i] - 0.5f * y * y * varinv[i];
+                }
+                float tmax = t[0];
+                for (int i = 1; i < 5; ++i) tmax = hw_maxf(tmax, t[i]);
+                float es = hw_expf(t[0] - tmax);
+                for (int i = 1; i < 5; ++i) es = es + hw_expf(t[i] - tmax);
+                const float llb = tmax + hw_logf(es);
+                for (int i = 0; i < 5; ++i) TAU(z, i, b) = hw_expf(t[i] - llb);
+                bpz[b] = hw_expf(llb);
+            };
+            float probSum = 0.0f, wps = 0.0f;
+            float ci[5] = {0, 0, 0, 0, 0}, mom1[5] = {0, 0, 0, 0, 0};
+            const float halfBin = 0.5f * h.binSize;
+            corner(0);
+            for (int b = 0; b < NB; ++b) {
+                corner(b + 1);
+                binp[b] = hw_maxf(halfBin * (bpz[b] + bpz[b + 1]), e.binProbMin);
+                probSum += binp[b];
+                const float nj = (float)h.cnt[b];
+                wps += nj * hw_logf(binp[b]);
+                const float factor = nj / binp[b];
+                const float x0 = h.start[b], x1 = h.start[b + 1];
+                for (int m = 0; m < 5; ++m) {
+                    const float tmp1 = bpz[b] * TAU(z, m, b);
+                    const float tmp2 = bpz[b + 1] * TAU(z, m, b + 1);
+                    ci[m] += (tmp1 + tmp2) * factor;
+                    mom1[m] += (tmp1 * x0 + tmp2 * x1) * factor;
+                }
+            }
+            for (int m = 0; m < 5; ++m) {
+                ci[m] *= halfBin;
+                mom1[m] *= halfBin;
+            }
+            e.logLikePrev = e.logLike;
+            e.logLike = wps - e.njSum * hw_logf(probSum);
+            const float ds = e.s - e.sExpect;
+            e.logLike -= 0.5f * (ds * ds) * e.sExpectWeight;
+            const float delta = e.logLike - e.logLikePrev;
+            const float convTol = 1e-4f * hw_fabsf(e.logLike);
+            bool conv = (delta >= 0.0f) && (delta < convTol);
+            conv = conv && (e.logLike >= e.mlLogLike);
+            // MaxLikelihoodDiagnostics::Converged (MaxLikelihoodDiagnostics.h:90-101)
+            if (conv && !e.converged) e.iterCount = (int)it;
+            if (conv) {
+                e.mlLogLike = e.logLike;
+                e.mlDelta = delta;
+                for (int k = 0; k < 5; ++k) {
+                    e.rhoEst[k] = e.rho[k];
+                    e.muEst[k] = e.mu[k];
+                    e.varEst[k] = e.var[k];
+                }
+            }
+            e.converged = e.converged || conv;
+            if (!(e.converged || e.status != HWPA_ZS_OK)) allDone = false;
+            for (int m = 0; m < 5; ++m) {
+                sci[(size_t)z * 5 + m] = ci[m];
+                smom1[(size_t)z * 5 + m] = mom1[m];
+                svarinv[(size_t)z * 5 + m] = varinv[m];
+            }
+        }
+        // Lane-wide early exit (DmeEmHost.cpp:485)
+        if (!d.iterToLimit && allDone) break;
+
+        // M-step (DmeEmHost.cpp:494-564)
+        for (int z = 0; z < LS; ++z) {
+            EmZ& E = ez[z];
+            const HistZ& H = hz[z];
+            float c[5], mm[5], vi[5];
+            for (int m = 0; m < 5; ++m) {
+                c[m] = hw_maxf(sci[(size_t)z * 5 + m], 0.1f);
+                mm[m] = smom1[(size_t)z * 5 + m];
+                vi[m] = svarinv[(size_t)z * 5 + m];
+            }
+            for (int m = 0; m < 5; ++m) E.rho[m] = c[m] / E.njSum;
+            const float rs = Sum5Tree(E.rho);
+            for (int m = 0; m < 5; ++m) E.rho[m] = E.rho[m] / rs;
+            E.mu[0] = mm[0] / c[0];
+            float numer = 0.0f, denom = 0.0f;
+            for (int a = 0; a < NA; ++a) {
+                const int i = a + 1;
+                numer += rpa[a] * vi[i] * mm[i];
+                denom += (rpa[a] * rpa[a]) * vi[i] * c[i];
+            }
+            numer += E.sExpect * E.sExpectWeight;
+            denom += E.sExpectWeight;
+            E.s = numer / denom;
+            const float minPulseMean = hw_maxf(1.0f, d.k.signalScaler);
+            const float sMin = hw_maxf(0.5f * hw_sqrtf(E.var[0]), minPulseMean) / rpaMin;
+            if (E.s < sMin) {
+                E.s = sMin;
+                E.status |= HWPA_ZS_VLOW_SIGNAL;
+            }
+            for (int a = 0; a < NA; ++a) E.mu[a + 1] = E.s * rpa[a];
+            const float* bpz = &bp[(size_t)z * (NB + 1)];
+            const float* binpz = &binProb[(size_t)z * NB];
+            float mom2 = 0.0f;
+            for (int j = 0; j < NB; ++j) {
+                const float x0 = H.start[j] - E.mu[0];
+                const float x1 = H.start[j + 1] - E.mu[0];
+                float tmp = bpz[j] * x0 * x0 * TAU(z, 0, j);
+                tmp += bpz[j + 1] * x1 * x1 * TAU(z, 0, j + 1);
+                tmp *= (float)H.cnt[j] / binpz[j];
+                mom2 += tmp;
+            }
+            mom2 *= 0.5f * H.binSize;
+            E.var[0] = mom2 / c[0] + E.varQuant;
+            E.var[0] = hw_clampf(E.var[0], d.k.blVarMin, d.k.blVarMax);
+            for (int a = 0; a < NA; ++a)
+                E.var[a + 1] = ModelSignalCovar(d.k, d.cv[a] * d.cv[a], E.mu[a + 1], E.var[0]);
+        }
+    }
+    for (int z = 0; z < LS; ++z) {
+        EmZ& e = ez[z];
+        if (!e.converged) e.status |= HWPA_ZS_NO_CONVERGE;
+        e.work.bl.w = e.rhoEst[0];
+        e.work.bl.mean = e.muEst[0];
+        e.work.bl.var = e.varEst[0];
+        for (int a = 0; a < NA; ++a) {
+            e.work.an[a].w = e.rhoEst[a + 1];
+            e.work.an[a].mean = e.muEst[a + 1];
+            e.work.an[a].var = e.varEst[a + 1];
+        }
+        float cf[8];
+        ComputeConfidence(d, e.converged, e.det, e.work, cf);
+        float conf = 1.0f;
+        for (int k = 0; k < 8; ++k) conf *= cf[k];
+        conf = hw_minf(hw_maxf(0.0f, conf), 1.0f);
+        conf = (conf >= d.successThresh) ? conf : 0.0f;
+        e.work.conf = conf;
+        if (dx) {
+            dx->converged[z] = e.converged;
+            dx->iter_count[z] = e.iterCount;
+            dx->status[z] = e.status;
+            dx->log_like[z] = e.mlLogLike;
+            for (int k = 0; k < 8; ++k) dx->conf_factors[k][z] = cf[k];
+            dx->scale_factor[z] = e.scaleFactor;
+            dx->confidence[z] = conf;
+            const ModeZ* modes[5] = {&e.work.bl, &e.work.an[0], &e.work.an[1], &e.work.an[2], &e.work.an[3]};
+            for (int m = 0; m < 5; ++m) {
+                dx->est_fp32[m][0][z] = modes[m]->w;
+                dx->est_fp32[m][1][z] = modes[m]->mean;
+                dx->est_fp32[m][2][z] = modes[m]->var;
+            }
+        }
+        UpdateConf(d, e.det, e.work);
+        if (dx) {
+            const ModeZ* modes[5] = {&e.det.bl, &e.det.an[0], &e.det.an[1], &e.det.an[2], &e.det.an[3]};
+            for (int m = 0; m < 5; ++m) {
+                dx->out_fp32[m][0][z] = modes[m]->w;
+                dx->out_fp32[m][1][z] = modes[m]->mean;
+                dx->out_fp32[m][2][z] = modes[m]->var;
+            }
+        }
+        ModelToHalf(e.det, lm, z);
+    }
+    if (dx) dx->iterations_run = (int)it;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C API
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int orc_threads(int n)
+{
+    const int prev = g_threads;
+    if (n > 0) g_threads = n;
+    return prev;
+}
+
+void orc_filter_params(int32_t type, hwpa_filter_params* out) { *out = FilterParams(type); }
+
+void* orc_baseliner_create(const hwpa_config* cfg, int32_t lanes)
+{
+    auto* c = new BaselinerCtx;
+    c->fp = FilterParams(cfg->filter_type);
+    // LaneBaseliner ctor (HostMultiScaleBaseliner.h:58-66) and MultiScaleBaseliner::Configure
+    // (MultiScaleBaseliner.cpp:53-75)
+    c->cMeanBias = c->fp.mean_bias * hw_exp2f(cfg->mean_bias_adjust);
+    c->cSigmaBias = c->fp.sigma_bias * hw_exp2f(cfg->sigma_bias_adjust);
+    c->meanEmaAlpha = hw_powf(0.5f, 1.0f / cfg->mean_ema_scale_strides);
+    c->sigmaEmaAlpha = hw_exp2f(-1.0f / cfg->sigma_ema_scale_strides);
+    c->jumpTol = (cfg->jump_suppression > 0.0f) ? 1.0f / cfg->jump_suppression : INFINITY;
+    c->scaler = cfg->movie.photoelectron_sensitivity;
+    c->pedestal = cfg->movie.pedestal;
+    c->lanes = lanes;
+    c->lb.resize(lanes);
+    const auto& fp = c->fp;
+    for (auto& L : c->lb) {
+        // MultiStageFilter ctor (HostMultiScaleBaseliner.h:115-125)
+        L.lower.resize(2 * fp.num_stages);
+        L.upper.resize(2 * fp.num_stages);
+        L.lower[0].Init(fp.widths[0], fp.strides[0], 1, false);
+        L.lower[1].Init(fp.widths[0], 1, fp.strides[0], true);
+        L.upper[0].Init(fp.widths[0], fp.strides[0], 1, true);
+        L.upper[1].Init(fp.widths[0], 1, fp.strides[0], false);
+        for (int i = 1; i < fp.num_stages; ++i) {
+            for (auto* v : {&L.lower, &L.upper}) {
+                (*v)[2 * i].Init(fp.widths[i], fp.strides[i], fp.strides[i - 1], false);
+                (*v)[2 * i + 1].Init(fp.widths[i], 1, fp.strides[i - 1] * fp.strides[i], true);
+            }
+        }
+        for (int z = 0; z < LS; ++z) {
+            L.emaSum[z] = 0.0f;
+            L.emaWeight[z] = 0.0f;
+            L.sigmaEma[z] = 2.0f;
+            L.latData[z] = L.latRaw[z] = 0;
+            L.latL[z] = L.latH1[z] = L.latH2[z] = false;
+        }
+    }
+    return c;
+}
+
+void orc_baseliner_destroy(void* h) { delete static_cast<BaselinerCtx*>(h); }
+
+int32_t orc_baseliner_process(void* h, const int16_t* raw, int32_t frames, int16_t* out,
+                              hwpa_baseliner_stats* stats)
+{
+    auto* c = static_cast<BaselinerCtx*>(h);
+    if (frames <= 0 || frames % c->fp.aggregate_stride != 0) return HWPA_ERR_INVALID_ARG;
+    ParLanes(c->lanes, [&](int l) {
+        const size_t off = (size_t)l * frames * LS;
+        EstimateBaselineLane(*c, l, raw + off, frames, out + off, stats + l);
+    });
+    return HWPA_OK;
+}
+
+struct AggCtx {
+    int lanes;
+    std::vector<LaneStats> st;
+};
+
+void* orc_agg_create(int32_t lanes)
+{
+    auto* a = new AggCtx;
+    a->lanes = lanes;
+    a->st.resize(lanes);
+    return a;
+}
+void orc_agg_destroy(void* h) { delete static_cast<AggCtx*>(h); }
+
+// BaselineStatsAggregatorHost::AddMetricsImpl (BaselineStatsAggregatorHost.cpp:40-51)
+void orc_agg_add(void* h, const hwpa_baseliner_stats* stats)
+{
+    auto* a = static_cast<AggCtx*>(h);
+    ParLanes(a->lanes, [&](int l) {
+        LaneStats o;
+        o.FromState(stats + l);
+        a->st[l].Merge(o);
+    });
+}
+void orc_agg_get(void* h, hwpa_baseliner_stats* out)
+{
+    auto* a = static_cast<AggCtx*>(h);
+    for (int l = 0; l < a->lanes; ++l) a->st[l].ToState(out + l);
+}
+void orc_agg_reset(void* h)
+{
+    auto* a = static_cast<AggCtx*>(h);
+    for (auto& s : a->st) s.Reset();
+}
+
+void* orc_hist_create(const hwpa_config* cfg, int32_t lanes)
+{
+    auto* c = new HistCtx;
+    c->binSizeCoeff = cfg->bin_size_coeff;
+    c->binSizeMin = cfg->bin_size_low_bound_coeff * std::max(1.0f, cfg->movie.photoelectron_sensitivity);
+    c->fallbackSigma = cfg->fallback_baseline_sigma;
+    c->minFrameCount = (float)cfg->baseline_stat_min_frame_count;
+    c->lanes = lanes;
+    c->h.resize(lanes);
+    for (auto& H : c->h) {
+        std::memset(H.prevFrame, 0, sizeof(H.prevFrame));
+        std::memset(H.hist, 0, sizeof(H.hist));
+        H.hi0 = 0;
+        H.hi1 = -1;
+        std::memset(H.count, 0, sizeof(H.count));
+        for (int z = 0; z < LS; ++z) H.binSize[z] = 1.0f;
+        std::memset(H.binStart, 0, sizeof(H.binStart));
+        std::memset(H.lowOut, 0, sizeof(H.lowOut));
+        std::memset(H.highOut, 0, sizeof(H.highOut));
+    }
+    return c;
+}
+void orc_hist_destroy(void* h) { delete static_cast<HistCtx*>(h); }
+
+int32_t orc_hist_reset(void* h, const hwpa_baseliner_stats* stats)
+{
+    auto* c = static_cast<HistCtx*>(h);
+    int err = 0;
+    for (int l = 0; l < c->lanes; ++l) err |= HistResetLane(*c, l, stats[l]);
+    return err ? HWPA_ERR_INVALID_ARG : HWPA_OK;
+}
+
+void orc_hist_add(void* h, const int16_t* traces, int32_t frames, const hwpa_lane_model* models)
+{
+    auto* c = static_cast<HistCtx*>(h);
+    ParLanes(c->lanes, [&](int l) {
+        HistAddLane(*c, l, traces + (size_t)l * frames * LS, frames, models[l]);
+    });
+}
+
+void orc_hist_get(void* h, hwpa_lane_hist* out)
+{
+    auto* c = static_cast<HistCtx*>(h);
+    for (int l = 0; l < c->lanes; ++l) HistGetLane(*c, l, out[l]);
+}
+
+void orc_dme_init_models(const hwpa_config* cfg, int32_t lanes, const hwpa_baseliner_stats* stats,
+                         hwpa_lane_model* out)
+{
+    const DmeCfg d = MakeDmeCfg(*cfg);
+    for (int l = 0; l < lanes; ++l) InitLaneModel(d, stats[l], out[l]);
+}
+
+void orc_dme_estimate(const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,
+                      const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
+                      const int32_t model_fi[2], const int32_t est_fi[2], orc_dme_diag* diag)
+{
+    const DmeCfg d = MakeDmeCfg(*cfg);
+    ParLanes(lanes, [&](int l) {
+        EstimateLane(d, hist[l], stats[l], models[l], model_fi, est_fi, diag ? diag + l : nullptr);
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Frame labeler (traceAnalysis/FrameLabelerHost.cpp, SubframeScorer.*, RoiHost.h)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// TransitionMatrix<float> ctor (traceAnalysis/SubframeScorer.cpp:35-223)
+void BuildTransition(const hwpa_config& cfg, hwpa_transition& T)
+{
+    hwpa_analog analogs[NA];
+    for (int i = 0; i < NA; ++i) analogs[i] = cfg.movie.analogs[i];
+    float data[NS][NS];
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) data[i][j] = 0.0f;
+    for (int i = 0; i < NA; ++i) {
+        const float delta = 0.01f;
+        if (std::abs(analogs[i].ipd2_slow_step_ratio - 1.0f) < delta)
+            analogs[i].pw2_slow_step_ratio = (analogs[i].pw2_slow_step_ratio < 1.0f) ? 1.0f - delta : 1.0f + delta;
+        if (std::abs(analogs[i].ipd2_slow_step_ratio - 1.0f) < delta)
+            analogs[i].ipd2_slow_step_ratio = (analogs[i].ipd2_slow_step_ratio < 1.0f) ? 1.0f - delta : 1.0f + delta;
+    }
+    float meanIpd = 0.0f;
+    for (int i = 0; i < NA; ++i) meanIpd += analogs[i].inter_pulse_distance;
+    meanIpd = meanIpd / NA;
+    const double frameRate = cfg.movie.frame_rate;
+    auto lenProbs = [&](float widthSeconds, float ratio, float* ret) {
+        const double widthFrames = frameRate * widthSeconds;
+        ratio = std::max(ratio, 1e-6f);
+        const double k1 = (1.0f + ratio) / widthFrames;
+        const double k2 = (1.0f + 1.0f / ratio) / widthFrames;
+        const double dk = k1 - k2;
+        const double c1 = 1.0f - std::exp(-k1);
+        const double c2 = 1.0f - std::exp(-k2);
+        ret[0] = (float)(1 - k1 * c2 / k2 / dk + k2 * c1 / k1 / dk);
+        ret[1] = (float)(k1 * (c2 * c2) / k2 / dk - k2 * (c1 * c1) / k1 / dk);
+        ret[2] = (float)(k1 * (c2 * c2) * (1 - c2) / k2 / dk - k2 * (c1 * c1) * (1 - c1) / k1 / dk);
+    };
+    float ipdLen[NA][3];
+    for (int i = 0; i < NA; ++i) lenProbs(analogs[i].inter_pulse_distance, analogs[i].ipd2_slow_step_ratio, ipdLen[i]);
+    float alpha[NA];
+    for (int i = 0; i < NA; ++i) alpha[i] = cfg.viterbi_alpha * 0.25f * (ipdLen[i][0] + ipdLen[i][1]);
+    float alphaSum = 0.0f;
+    for (int i = 0; i < NA; ++i) alphaSum += alpha[i];
+    float meanLen[3];
+    lenProbs(meanIpd, 0.0f, meanLen);
+    const float meanIpd3m = 1 - meanLen[0] - meanLen[1];
+    const float pulseStartProb = meanLen[2] / meanIpd3m;
+    data[0][0] = 1.0f - pulseStartProb;
+    for (int i = 0; i < NA; ++i) {
+        const int full = i + 1, up = i + 1 + NA, down = i + 1 + 2 * NA;
+        float al[3];
+        lenProbs(analogs[i].pulse_width, analogs[i].pw2_slow_step_ratio, al);
+        const float gamma = cfg.viterbi_gamma * al[0] / (1 + alphaSum);
+        const float beta = cfg.viterbi_beta * al[1];
+        data[up][0] = 0.25f * pulseStartProb;
+        data[0][up] = gamma;
+        data[down][up] = beta;
+        data[full][up] = 1 - gamma - gamma * alphaSum - beta;
+        const float prob3ms = 1.0f - al[0] - al[1];
+        data[down][full] = al[2] / prob3ms;
+        data[full][full] = 1.0f - data[down][full];
+        data[0][down] = 1 - alphaSum;
+        for (int j = 0; j < NA; ++j) {
+            const int upPrime = j + 1 + NA;
+            data[upPrime][down] = alpha[j];
+            data[upPrime][up] = gamma * alpha[j];
+        }
+    }
+    // Sparsity pattern (SubframeScorer.h:54-68): rows B and U: {B} + [TU..AD];
+    // rows F and D: {F_i} + {U_i}.
+    for (int r = 0; r < NS; ++r) {
+        if (r == 0 || (r >= 5 && r <= 8)) {
+            T.first_col[r] = 0;
+            T.seg_first[r] = 5;
+            T.seg_last[r] = 13;
+        } else {
+            const int a = (r <= 4) ? r - 1 : r - 9;
+            T.first_col[r] = a + 1;
+            T.seg_first[r] = a + 5;
+            T.seg_last[r] = a + 6;
+        }
+    }
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) T.logp[i][j] = (data[i][j] > 0.0f) ? std::log(data[i][j]) : 0.0f;
+}
+
+struct Scorer {
+    // BlockStateSubframeScorer<VF> per-ZMW terms (SubframeScorer.h:143-318)
+    float bgMean, bInvVar, bgFixed, x0;
+    float ffMean[NA], pInvVar[NA], ffFixed[NA], logPMean[NA], x1[NA];
+    float dimMean, dimVar, dimNorm, dimScale;
+
+    void Setup(const hwpa_lane_model& m, int z)
+    {
+        const float log2pi = 1.8378770664f;
+        const float pi = 3.1415926536f;
+        const float sqrtHalfPi = hw_sqrtf(0.5f * pi);
+        const float bMean = hw_h2f(m.baseline.means[z]);
+        const float bVar = hw_h2f(m.baseline.vars[z]);
+        const float bSigma = hw_sqrtf(bVar);
+        x0 = bMean + bSigma * sqrtHalfPi;
+        bInvVar = 1.0f / bVar;
+        for (int a = 0; a < NA; ++a) {
+            const float aMean = hw_h2f(m.analogs[a].means[z]);
+            logPMean[a] = hw_logf(aMean - bMean);
+            const float aVar = hw_h2f(m.analogs[a].vars[z]);
+            const float aSigma = hw_sqrtf(aVar);
+            x1[a] = aMean - aSigma * sqrtHalfPi;
+            pInvVar[a] = 1.0f / aVar;
+        }
+        {
+            const float aMean = hw_h2f(m.analogs[3].means[z]);
+            const float aVar = hw_h2f(m.analogs[3].vars[z]);
+            float eMean = 0.5f * (bMean + aMean);
+            float eVar = (1.0f / 6.0f) * (aMean - bMean);
+            eVar += bVar + aVar;
+            eVar *= 0.5f;
+            dimMean = eMean;
+            dimVar = eVar;
+            dimNorm = -0.5f * (log2pi + hw_logf(eVar));
+            dimScale = 0.5f / eVar;
+        }
+        const float normConst = log2pi / 2.0f;
+        bgFixed = -0.5f * hw_logf(bVar) - normConst;
+        bgMean = bMean;
+        for (int a = 0; a < NA; ++a) {
+            ffFixed[a] = -0.5f * hw_logf(hw_h2f(m.analogs[a].vars[z])) - normConst;
+            ffMean[a] = hw_h2f(m.analogs[a].means[z]);
+        }
+    }
+
+    float Score(float d, int state) const
+    {
+        if (state == 0) {
+            const float y = d - bgMean;
+            return -0.5f * y * bInvVar * y + bgFixed;
+        }
+        if (state <= 4) {
+            const int i = state - 1;
+            const float y = d - ffMean[i];
+            return -0.5f * y * pInvVar[i] * y + ffFixed[i];
+        }
+        const int i = (state - 1) % 4;
+        // SubframeScore
+        const bool lowExt = d < x0;
+        const float cap = lowExt ? x0 : x1[i];
+        float fixedTerm = lowExt ? bInvVar : pInvVar[i];
+        fixedTerm = -0.5f * fixedTerm;
+        const bool ext = lowExt || (d > x1[i]);
+        const float dc = d - cap;
+        float s = (ext ? fixedTerm * (dc * dc) : 0.0f) - logPMean[i];
+        if (i == NA - 1) {
+            const float dm = d - dimMean;
+            const float fb = dimNorm - (dm * dm) * dimScale;
+            s = (x0 <= x1[i]) ? s : fb;
+        }
+        const float y = d - ffMean[i];
+        const float fScore = -0.5f * y * pInvVar[i] * y + ffFixed[i];
+        if (d > ffMean[i]) s = hw_minf(hw_minf(s, fScore - 1.0f), 0.0f);
+        return s;
+    }
+};
+
+struct LaneLabeler {
+    hwpa_lane_model oldModel;
+    int16_t labelBoundary[LS];
+    uint16_t roiBoundary[LS];
+    int16_t latentTrc[LS];
+    int16_t prevLat[HWPA_LATENT_FRAMES][LS];
+};
+
+struct LabelerCtx {
+    hwpa_transition T;
+    float roiUpper, roiLower;
+    int lanes;
+    std::vector<LaneLabeler> L;
+};
+
+// LabelBlock (FrameLabelerHost.cpp:312-448) for one lane.
+void LabelLane(const LabelerCtx& c, LaneLabeler& L, const int16_t* tr, int frames,
+               const hwpa_lane_model& model, int16_t* labelsOut, int16_t* latOut, float* score)
+{
+    const int NL = HWPA_LATENT_FRAMES;
+    const int nroi = frames + HWPA_VITERBI_LOOKBACK + HWPA_ROI_STITCH;
+    const int nlab = frames + HWPA_VITERBI_LOOKBACK;
+    std::vector<uint16_t> roi((size_t)nroi * LS);
+    std::vector<uint8_t> tb((size_t)nlab * NS * LS);
+    for (int i = 0; i < NL; ++i)
+        for (int z = 0; z < LS; ++z) latOut[i * LS + z] = L.prevLat[i][z];
+
+    for (int z = 0; z < LS; ++z) {
+        // ComputeRoi (FrameLabelerHost.cpp:283-309, RoiHost.h:137-248)
+        const float inv1 = 1.0f / hw_sqrtf(hw_h2f(L.oldModel.baseline.vars[z]));
+        const float inv2 = 1.0f / hw_sqrtf(hw_h2f(model.baseline.vars[z]));
+        auto seq = [&](int k) -> float {  // concatenated prevLat ++ traces, sigma-normalised
+            return (k < NL) ? (float)L.prevLat[k][z] * inv1 : (float)tr[(size_t)(k - NL) * LS + z] * inv2;
+        };
+        float prev[3] = {(float)L.latentTrc[z] * inv1, seq(0), seq(1)};
+        uint16_t prevRoi = L.roiBoundary[z];
+        const float s2 = hw_sqrtf(2.0f), s3 = hw_sqrtf(3.0f);
+        for (int k = 0; k < nroi; ++k) {
+            const float nv = seq(k + 2);
+            float mval = prev[1];
+            mval = hw_maxf(mval, (prev[0] + prev[1]) / s2);
+            mval = hw_maxf(mval, (prev[0] + prev[1] + prev[2]) / s3);
+            mval = hw_maxf(mval, (prev[0] + prev[1] + prev[2] + nv) / 2.0f);
+            const float thresh = (prevRoi & 1u) ? c.roiLower : c.roiUpper;
+            const uint16_t v = (uint16_t)((mval >= thresh ? 1u : 0u) | (mval >= c.roiLower ? 2u : 0u));
+            roi[(size_t)k * LS + z] = v;
+            prevRoi = v;
+            prev[0] = prev[1];
+            prev[1] = prev[2];
+            prev[2] = nv;
+        }
+        uint16_t pr = 0;
+        for (int k = nroi - 1; k >= 0; --k) {
+            const uint16_t r = roi[(size_t)k * LS + z];
+            const bool b = (r & 1u) || ((r & 2u) && pr != 0);
+            pr = b ? 1 : 0;
+            roi[(size_t)k * LS + z] = b ? 1 : 0;
+        }
+
+        // Viterbi recursion (FrameLabelerHost.cpp:213-264)
+        float ll[NS];
+        for (int s = 0; s < NS; ++s) ll[s] = (L.labelBoundary[z] == s) ? 0.0f : -INFINITY;
+        Scorer sc;
+        sc.Setup(L.oldModel, z);
+        float anchorLL[NS];
+        const int anchor = frames - NL;
+        for (int f = 0; f < nlab; ++f) {
+            if (f == NL) sc.Setup(model, z);
+            const float d = (f < NL) ? (float)L.prevLat[f][z] : (float)tr[(size_t)(f - NL) * LS + z];
+            const bool isRoi = roi[(size_t)f * LS + z] != 0;
+            float acc[NS];
+            for (int r = 0; r < NS; ++r) {
+                const float sco = sc.Score(d, r);
+                const int fc = c.T.first_col[r];
+                float mx = sco + c.T.logp[r][fc] + ll[fc];
+                int mi = fc;
+                for (int p = c.T.seg_first[r]; p < c.T.seg_last[r]; ++p) {
+                    const float v = sco + c.T.logp[r][p] + ll[p];
+                    if (v >= mx) {
+                        mx = v;
+                        mi = p;
+                    }
+                }
+                const float pen = (r == 0 || isRoi) ? 0.0f : INFINITY;
+                acc[r] = mx - pen;
+                tb[((size_t)f * NS + r) * LS + z] = (uint8_t)mi;
+            }
+            for (int r = 0; r < NS; ++r) ll[r] = acc[r];
+            if (f == NL + anchor - 1) {
+                for (int r = 0; r < NS; ++r) anchorLL[r] = ll[r];
+            }
+        }
+        // Normalize (FrameLabelerHost.cpp:185-205)
+        float prob[NS];
+        float mx = ll[0];
+        for (int i = 1; i < NS; ++i) mx = hw_maxf(ll[i], mx);
+        float sum = 0.0f;
+        for (int i = 0; i < NS; ++i) {
+            prob[i] = hw_expf(ll[i] - mx);
+            sum += prob[i];
+        }
+        const float inv = 1 / sum;
+        for (int i = 0; i < NS; ++i) prob[i] *= inv;
+        for (int i = nlab - 1; i > frames - 1; --i) {
+            float np[NS];
+            for (int s = 0; s < NS; ++s) np[s] = 0.0f;
+            for (int from = 0; from < NS; ++from) {
+                const int prevState = tb[((size_t)i * NS + from) * LS + z];
+                for (int to = 0; to < NS; ++to) np[to] += (to == prevState) ? prob[from] : 0.0f;
+            }
+            for (int s = 0; s < NS; ++s) prob[s] = np[s];
+        }
+        float maxProb = prob[0];
+        int anchorState = 0;
+        for (int i = 1; i < NS; ++i) {
+            const bool cond = maxProb > prob[i];
+            maxProb = cond ? maxProb : prob[i];
+            anchorState = cond ? anchorState : i;
+        }
+        int st = anchorState;
+        for (int f = frames - 1; f >= 0; --f) {
+            labelsOut[(size_t)f * LS + z] = (int16_t)st;
+            st = tb[((size_t)f * NS + st) * LS + z];
+        }
+        score[z] = (frames >= NL) ? anchorLL[anchorState] : NAN;
+        L.roiBoundary[z] = roi[(size_t)(frames - 1) * LS + z];
+        L.labelBoundary[z] = (int16_t)anchorState;
+        L.latentTrc[z] = tr[(size_t)(anchor - 1) * LS + z];
+    }
+    L.oldModel = model;
+    for (int i = 0; i < NL; ++i)
+        for (int z = 0; z < LS; ++z) L.prevLat[i][z] = tr[(size_t)(frames - NL + i) * LS + z];
+}
+
+// ---------------------------------------------------------------------------
+// Pulse accumulator (traceAnalysis/HostPulseAccumulator.{h,cpp}, PulseAccumulator.cpp,
+// dataTypes/Pulse.h)
+// ---------------------------------------------------------------------------
+
+struct Segment {
+    uint32_t start[LS], end[LS];
+    int16_t first[LS], last[LS], smax[LS];
+    float total[LS], m2[LS];
+    int16_t label[LS];
+};
+
+struct PulseCtx {
+    float widthThresh, ampThresh;
+    uint32_t maxCalls;
+    uint8_t analogMap[NA];
+    int lanes;
+    std::vector<Segment> seg;
+};
+
+inline uint16_t ToFixed(float v) { return (uint16_t)(v / 0.1f + 0.5f); }
+
+inline uint8_t NucleotideOf(const PulseCtx& c, int label)
+{
+    if (label > 2 * NA) label -= 2 * NA;
+    if (label > NA) label -= NA;
+    return (label >= 1 && label <= NA) ? c.analogMap[label - 1] : (uint8_t)4;
+}
+
+void PulseLane(PulseCtx& c, int l, const int16_t* labels, const int16_t* lat, const int16_t* cur, int frames,
+               int32_t firstFrame, const hwpa_lane_model& model, hwpa_pulse* pulses, uint32_t* counts,
+               uint32_t* ovf, hwpa_stat_acc& blst)
+{
+    Segment& S = c.seg[l];
+    const int NL = HWPA_LATENT_FRAMES;
+    const float maxSignal = 65535 * 0.1f;
+    StatZ bls[LS];
+    for (int z = 0; z < LS; ++z) {
+        counts[z] = 0;
+        ovf[z] = 0;
+    }
+    for (int z = 0; z < LS; ++z) {
+        const float minMean = hw_h2f(model.analogs[NA - 1].means[z]);
+        for (int rf = 0; rf < frames; ++rf) {
+            const uint32_t absf = (uint32_t)(rf + firstFrame);
+            const int16_t lab = labels[(size_t)rf * LS + z];
+            const int16_t sig = (rf < NL) ? lat[(size_t)rf * LS + z] : cur[(size_t)(rf - NL) * LS + z];
+            const bool isUp = (lab > NA) && (lab <= 2 * NA);
+            const bool boundary = isUp || (lab == 0 && S.label[z] != 0);
+            const bool isPulse = S.label[z] != 0;
+            if (boundary && isPulse) {
+                // LabelsSegment::ToPulse (HostPulseAccumulator.h:95-126)
+                S.end[z] = absf;
+                const int width = (int)(absf - S.start[z]);
+                const float rawMean = (float)(S.total[z] + S.last[z] + S.first[z]) / width;
+                const float rawMid = (float)S.total[z] / (width - 2);
+                const bool keep = (width >= c.widthThresh) || (rawMean * width >= c.ampThresh * minMean);
+                hwpa_pulse p{};
+                p.start = S.start[z];
+                p.width = (uint16_t)width;
+                // std::min / std::max on floats (HostPulseAccumulator.h:95-126)
+                auto smin = [](float a, float b) { return (b < a) ? b : a; };
+                auto smax = [](float a, float b) { return (a < b) ? b : a; };
+                p.mean_signal = ToFixed(smin(maxSignal, smax(0.0f, rawMean)));
+                p.mid_signal = ToFixed(width < 3 ? 0.0f : smin(maxSignal, smax(0.0f, rawMid)));
+                p.max_signal = ToFixed(smin(maxSignal, smax(0.0f, (float)S.smax[z])));
+                p.signal_m2 = S.m2[z];
+                p.label = NucleotideOf(c, S.label[z]);
+                p.is_reject = !keep;
+                // LaneVectorView::push_back (dataTypes/BatchVectors.h:105-118)
+                if (counts[z] < c.maxCalls) pulses[(size_t)z * c.maxCalls + counts[z]++] = p;
+                else ovf[z]++;
+            }
+            if (boundary) {
+                // ResetSegment (HostPulseAccumulator.h:128-139)
+                S.start[z] = absf;
+                S.end[z] = 0;
+                S.first[z] = sig;
+                S.last[z] = 0;
+                S.smax[z] = sig;
+                S.total[z] = 0.0f;
+                S.m2[z] = 0.0f;
+                S.label[z] = lab;
+            } else {
+                // AddSignal (HostPulseAccumulator.h:141-148)
+                S.total[z] = S.total[z] + S.last[z];
+                S.m2[z] = S.m2[z] + (float)S.last[z] * S.last[z];
+                S.last[z] = sig;
+                S.smax[z] = std::max(S.smax[z], sig);
+            }
+            if (!isPulse && !boundary) bls[z].Add((float)sig);
+        }
+        blst.offset[z] = bls[z].off;
+        blst.m0[z] = bls[z].m0;
+        blst.m1[z] = bls[z].m1;
+        blst.m2[z] = bls[z].m2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Analyzer (analyzer/BatchAnalyzer.cpp:288-373, traceAnalysis/DetectionModelEstimator.cpp:93-160)
+// ---------------------------------------------------------------------------
+
+struct Interval {
+    int32_t lo = 0, hi = 0;
+    bool Empty() const { return lo == hi; }
+    void Hull(int32_t a, int32_t b)
+    {
+        if (Empty()) {
+            lo = a;
+            hi = b;
+        } else {
+            lo = std::min(lo, a);
+            hi = std::max(hi, b);
+        }
+    }
+};
+
+enum { ST_HIST_INIT = 0, ST_DME_INIT = 1, ST_SEQUENCING = 2 };
+
+}  // namespace
+
+struct OrcAnalyzer {
+    hwpa_config cfg;
+    uint32_t poolId;
+    int lanes, frames;
+    int32_t nextFrame = 0;
+    uint32_t baselinerLatency, dmeDelay;
+    bool fullEstimation = false;
+    int status = ST_HIST_INIT;
+    int32_t framesRemaining;
+    void* bl;
+    void* agg;
+    void* hist;
+    void* lab;
+    void* pulse;
+    Interval aggFi, histFi, modelFi;
+    std::vector<hwpa_lane_model> models;
+    std::vector<int16_t> baselined, labels, latent;
+    std::vector<hwpa_baseliner_stats> stats, aggStats;
+    std::vector<hwpa_lane_hist> hists;
+    std::vector<float> score;
+};
+
+extern "C" {
+
+void orc_transition_matrix(const hwpa_config* cfg, hwpa_transition* out) { BuildTransition(*cfg, *out); }
+
+void* orc_labeler_create(const hwpa_config* cfg, int32_t lanes)
+{
+    auto* c = new LabelerCtx;
+    BuildTransition(*cfg, c->T);
+    c->roiUpper = cfg->roi_upper_threshold;
+    c->roiLower = cfg->roi_lower_threshold;
+    c->lanes = lanes;
+    c->L.resize(lanes);
+    for (auto& L : c->L) {
+        // LatentViterbi ctor (FrameLabelerHost.cpp:58-81); weights/confidence unused
+        std::memset(&L.oldModel, 0, sizeof(L.oldModel));
+        for (int z = 0; z < LS; ++z) {
+            L.oldModel.baseline.means[z] = hw_f2h(0.0f);
+            L.oldModel.baseline.vars[z] = hw_f2h(50.0f);
+            for (int a = 0; a < NA; ++a) {
+                L.oldModel.analogs[a].means[z] = hw_f2h(1000.0f);
+                L.oldModel.analogs[a].vars[z] = hw_f2h(10.0f);
+            }
+            L.labelBoundary[z] = 0;
+            L.roiBoundary[z] = 0;
+            L.latentTrc[z] = 0;
+        }
+        std::memset(L.prevLat, 0, sizeof(L.prevLat));
+    }
+    return c;
+}
+void orc_labeler_destroy(void* h) { delete static_cast<LabelerCtx*>(h); }
+
+void orc_labeler_process(void* h, const int16_t* traces, int32_t frames, const hwpa_lane_model* models,
+                         int16_t* labels, int16_t* latent_trace, float* scores)
+{
+    auto* c = static_cast<LabelerCtx*>(h);
+    ParLanes(c->lanes, [&](int l) {
+        const size_t off = (size_t)l * frames * LS;
+        LabelLane(*c, c->L[l], traces + off, frames, models[l], labels + off,
+                  latent_trace + (size_t)l * HWPA_LATENT_FRAMES * LS, scores + (size_t)l * LS);
+    });
+}
+
+void* orc_pulse_create(const hwpa_config* cfg, int32_t lanes)
+{
+    auto* c = new PulseCtx;
+    c->widthThresh = cfg->xsp_width_thresh;
+    const double x = cfg->xsp_amp_thresh;
+    c->ampThresh = (float)(x / (1.0 - x));
+    c->maxCalls = cfg->max_calls_per_zmw;
+    for (int a = 0; a < NA; ++a) {
+        const uint8_t b = cfg->movie.analogs[a].base_label;
+        c->analogMap[a] = (b == 'A') ? 0 : (b == 'C') ? 1 : (b == 'G') ? 2 : (b == 'T') ? 3 : 4;
+    }
+    c->lanes = lanes;
+    c->seg.resize(lanes);
+    for (auto& S : c->seg) {
+        std::memset(&S, 0, sizeof(S));  // default LabelsSegment: baseline label 0
+    }
+    return c;
+}
+void orc_pulse_destroy(void* h) { delete static_cast<PulseCtx*>(h); }
+
+void orc_pulse_process(void* h, const int16_t* labels, const int16_t* latent_trace, const int16_t* cur_trace,
+                       int32_t frames, int32_t first_frame, const hwpa_lane_model* models, hwpa_pulse* pulses,
+                       uint32_t* counts, uint32_t* overflows, hwpa_stat_acc* bl_stats)
+{
+    auto* c = static_cast<PulseCtx*>(h);
+    ParLanes(c->lanes, [&](int l) {
+        const size_t off = (size_t)l * frames * LS;
+        PulseLane(*c, l, labels + off, latent_trace + (size_t)l * HWPA_LATENT_FRAMES * LS, cur_trace + off,
+                  frames, first_frame, models[l], pulses + (size_t)l * LS * c->maxCalls, counts + (size_t)l * LS,
+                  overflows + (size_t)l * LS, bl_stats[l]);
+    });
+}
+
+void orc_tracegen(const hwpa_tracegen_params* p, int32_t lanes, int32_t lane_offset, int32_t frames,
+                  uint32_t first_frame, hwpa_tracegen_state* state, int32_t init, int16_t* out, int16_t* truth)
+{
+    ParLanes(lanes, [&](int l) {
+        hwpa_tracegen_state& st = state[l];
+        for (int i = 0; i < LS; ++i) {
+            const uint64_t z = (uint64_t)(l + lane_offset) * LS + (uint64_t)i;
+            if (init) hw_tracegen_init_zmw(p, z, &st, i);
+            for (int t = 0; t < frames; ++t) {
+                const size_t o = ((size_t)l * frames + t) * LS + i;
+                if (truth) truth[o] = st.in_pulse[i] ? (int16_t)(st.analog[i] + 1) : (int16_t)0;
+                out[o] = hw_tracegen_frame(p, z, first_frame + (uint32_t)t, &st, i);
+            }
+        }
+    });
+}
+
+void* orc_analyzer_create(const hwpa_config* cfg, uint32_t pool_id, int32_t lanes, int32_t frames)
+{
+    auto* a = new OrcAnalyzer;
+    a->cfg = *cfg;
+    a->poolId = pool_id;
+    a->lanes = lanes;
+    a->frames = frames;
+    const hwpa_filter_params fp = FilterParams(cfg->filter_type);
+    // MultiScaleBaseliner::StartupLatency (MultiScaleBaseliner.h:56-99)
+    const float meanAlpha = hw_powf(0.5f, 1.0f / cfg->mean_ema_scale_strides);
+    const float sigmaAlpha = hw_exp2f(-1.0f / cfg->sigma_ema_scale_strides);
+    const float mess = -1.0f / std::log2(meanAlpha);
+    const float sess = -1.0f / std::log2(sigmaAlpha);
+    a->baselinerLatency = (uint32_t)((float)fp.latent_size + std::max(mess, sess) * (float)fp.aggregate_stride);
+    // DynamicEstimateBatchAnalyzer ctor stagger (BatchAnalyzer.cpp:100-118)
+    const uint32_t cpe = (std::max(cfg->min_frames_for_estimate, 1u) + frames - 1) / frames;
+    a->dmeDelay = (pool_id % cpe) * (uint32_t)frames;
+    a->framesRemaining = (int32_t)cfg->num_frames_pre_accum_stats;
+    a->bl = orc_baseliner_create(cfg, lanes);
+    a->agg = orc_agg_create(lanes);
+    a->hist = orc_hist_create(cfg, lanes);
+    a->lab = orc_labeler_create(cfg, lanes);
+    a->pulse = orc_pulse_create(cfg, lanes);
+    a->models.resize(lanes);
+    std::memset(a->models.data(), 0, sizeof(hwpa_lane_model) * lanes);
+    a->baselined.resize((size_t)lanes * frames * LS);
+    a->labels.resize((size_t)lanes * frames * LS);
+    a->latent.resize((size_t)lanes * HWPA_LATENT_FRAMES * LS);
+    a->stats.resize(lanes);
+    a->aggStats.resize(lanes);
+    a->hists.resize(lanes);
+    a->score.resize((size_t)lanes * LS);
+    return a;
+}
+
+void orc_analyzer_destroy(void* h)
+{
+    auto* a = static_cast<OrcAnalyzer*>(h);
+    orc_baseliner_destroy(a->bl);
+    orc_agg_destroy(a->agg);
+    orc_hist_destroy(a->hist);
+    orc_labeler_destroy(a->lab);
+    orc_pulse_destroy(a->pulse);
+    delete a;
+}
+
+int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id, int32_t first_frame,
+                             orc_result* res)
+{
+    auto* a = static_cast<OrcAnalyzer*>(h);
+    if (pool_id != a->poolId) return HWPA_ERR_BAD_POOL;
+    if (first_frame != a->nextFrame) return HWPA_ERR_BAD_FRAME;
+    const int L = a->lanes, F = a->frames;
+    res->flags[0] = res->flags[1] = res->flags[2] = res->flags[3] = 0;
+    int32_t rc = orc_baseliner_process(a->bl, traces, F, a->baselined.data(), a->stats.data());
+    if (rc != HWPA_OK) return rc;
+    a->nextFrame = first_frame + F;
+    if (res->baselined) std::memcpy(res->baselined, a->baselined.data(), a->baselined.size() * 2);
+    if (res->bl_stats) std::memcpy(res->bl_stats, a->stats.data(), sizeof(hwpa_baseliner_stats) * L);
+    if (res->pulse_counts) std::memset(res->pulse_counts, 0, sizeof(uint32_t) * L * LS);
+    if (res->pulse_overflows) std::memset(res->pulse_overflows, 0, sizeof(uint32_t) * L * LS);
+    if (first_frame < (int32_t)(a->baselinerLatency + a->dmeDelay)) return HWPA_OK;
+
+    // DetectionModelEstimator::AddBatch
+    orc_agg_add(a->agg, a->stats.data());
+    a->aggFi.Hull(first_frame, first_frame + F);
+    if (a->status != ST_HIST_INIT) {
+        orc_hist_add(a->hist, a->baselined.data(), F, a->models.data());
+        a->histFi.Hull(first_frame, first_frame + F);
+    }
+    if (a->status != ST_SEQUENCING) {
+        orc_agg_get(a->agg, a->aggStats.data());
+        orc_dme_init_models(&a->cfg, L, a->aggStats.data(), a->models.data());
+        a->modelFi = a->aggFi;
+    }
+    a->framesRemaining -= F;
+    if (a->framesRemaining <= 0) {
+        if (a->status == ST_DME_INIT) a->status = ST_SEQUENCING;
+        if (a->status == ST_SEQUENCING) {
+            res->flags[1] = 1;
+            orc_hist_get(a->hist, a->hists.data());
+            orc_agg_get(a->agg, a->aggStats.data());
+            const int32_t mfi[2] = {a->modelFi.lo, a->modelFi.hi};
+            const int32_t efi[2] = {a->histFi.lo, a->histFi.hi};
+            orc_dme_estimate(&a->cfg, L, a->hists.data(), a->aggStats.data(), a->models.data(), mfi, efi, nullptr);
+            a->modelFi = a->histFi;
+        }
+        orc_agg_get(a->agg, a->aggStats.data());
+        rc = orc_hist_reset(a->hist, a->aggStats.data());
+        if (rc != HWPA_OK) return rc;
+        a->histFi = Interval{};
+        orc_agg_reset(a->agg);
+        a->aggFi = Interval{};
+        if (a->status == ST_HIST_INIT) a->status = ST_DME_INIT;
+        a->framesRemaining = (int32_t)a->cfg.min_frames_for_estimate;
+        if (res->flags[1]) a->fullEstimation = true;
+    }
+    if (res->models) std::memcpy(res->models, a->models.data(), sizeof(hwpa_lane_model) * L);
+
+    orc_labeler_process(a->lab, a->baselined.data(), F, a->models.data(), a->labels.data(), a->latent.data(),
+                        a->score.data());
+    res->flags[2] = 1;
+    if (res->labels) std::memcpy(res->labels, a->labels.data(), a->labels.size() * 2);
+    if (res->viterbi_score) std::memcpy(res->viterbi_score, a->score.data(), a->score.size() * 4);
+
+    const uint32_t mc = a->cfg.max_calls_per_zmw;
+    std::vector<hwpa_pulse> pulses((size_t)L * LS * mc);
+    std::vector<uint32_t> counts((size_t)L * LS), ovf((size_t)L * LS);
+    std::vector<hwpa_stat_acc> pbl(L);
+    orc_pulse_process(a->pulse, a->labels.data(), a->latent.data(), a->baselined.data(), F,
+                      first_frame - HWPA_LATENT_FRAMES, a->models.data(), pulses.data(), counts.data(),
+                      ovf.data(), pbl.data());
+    if (a->fullEstimation) {
+        res->flags[0] = 1;
+        if (res->pulses) std::memcpy(res->pulses, pulses.data(), pulses.size() * sizeof(hwpa_pulse));
+        if (res->pulse_counts) std::memcpy(res->pulse_counts, counts.data(), counts.size() * 4);
+        if (res->pulse_overflows) std::memcpy(res->pulse_overflows, ovf.data(), ovf.size() * 4);
+        if (res->pulse_bl_stats) std::memcpy(res->pulse_bl_stats, pbl.data(), sizeof(hwpa_stat_acc) * L);
+    }
+    return HWPA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Hooks used by the known-answer tests
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+// Element-wise evaluation of the shared math routines of include/hwpa_math.h.
+// kind: 0 exp, 1 log, 2 log2, 3 exp2, 4 pow(a,b), 5 f->half->f, 6 round_i16, 7 floor_i32
+void orc_math(int32_t kind, const float* a, const float* b, float* out, int32_t n)
+{
+    for (int32_t i = 0; i < n; ++i) {
+        switch (kind) {
+        case 0: out[i] = hw_expf(a[i]); break;
+        case 1: out[i] = hw_logf(a[i]); break;
+        case 2: out[i] = hw_log2f(a[i]); break;
+        case 3: out[i] = hw_exp2f(a[i]); break;
+        case 4: out[i] = hw_powf(a[i], b[i]); break;
+        case 5: out[i] = hw_h2f(hw_f2h(a[i])); break;
+        case 6: out[i] = (float)hw_round_i16(a[i]); break;
+        default: out[i] = (float)hw_floor_i32(a[i]); break;
+        }
+    }
+}
+
+// Runs one fresh extremum filter (ExtremumFilterHgw + BlockFilterStage) over
+// `chunks` consecutive blocks of `len` lane-frames; outputs are compacted per
+// block into out[chunk][k][64] with k < ceil(len / stride).
+void orc_hgw(int32_t width, int32_t stride, int32_t is_max, int32_t chunks, int32_t len, const int16_t* x,
+             int16_t* out)
+{
+    HgwFilter f;
+    f.Init(width, stride, 1, is_max != 0);
+    std::vector<int16_t> buf((size_t)len * LS);
+    const int nout = (len + stride - 1) / stride;
+    for (int c = 0; c < chunks; ++c) {
+        std::memcpy(buf.data(), x + (size_t)c * len * LS, buf.size() * 2);
+        f.Run(buf.data(), len);
+        std::memcpy(out + (size_t)c * nout * LS, buf.data(), (size_t)nout * LS * 2);
+    }
+}
+
+// The Viterbi recursion of FrameLabelerHost.cpp:213-264 over given per-frame
+// state scores: scores[f][13], roi[f], initial log-likelihoods ll0[13].
+// Writes the traceback-decoded best path ending in argmax of the final
+// log-likelihood (later state wins ties) and the final log-likelihoods.
+void orc_viterbi_core(const hwpa_config* cfg, int32_t frames, const float* scores, const uint8_t* roi,
+                      const float* ll0, int16_t* path, float* llOut)
+{
+    hwpa_transition T;
+    BuildTransition(*cfg, T);
+    float ll[NS];
+    for (int s = 0; s < NS; ++s) ll[s] = ll0[s];
+    std::vector<uint8_t> tb((size_t)frames * NS);
+    for (int f = 0; f < frames; ++f) {
+        float acc[NS];
+        for (int r = 0; r < NS; ++r) {
+            const float sco = scores[f * NS + r];
+            const int fc = T.first_col[r];
+            float mx = sco + T.logp[r][fc] + ll[fc];
+            int mi = fc;
+            for (int p = T.seg_first[r]; p < T.seg_last[r]; ++p) {
+                const float v = sco + T.logp[r][p] + ll[p];
+                if (v >= mx) {
+                    mx = v;
+                    mi = p;
+                }
+            }
+            acc[r] = mx - ((r == 0 || roi[f]) ? 0.0f : INFINITY);
+            tb[(size_t)f * NS + r] = (uint8_t)mi;
+        }
+        for (int r = 0; r < NS; ++r) ll[r] = acc[r];
+    }
+    int st = 0;
+    for (int s = 1; s < NS; ++s)
+        if (!(ll[st] > ll[s])) st = s;
+    for (int f = frames - 1; f >= 0; --f) {
+        path[f] = (int16_t)st;
+        st = tb[(size_t)f * NS + st];
+    }
+    for (int s = 0; s < NS; ++s) llOut[s] = ll[s];
+}
+
+}  // extern "C"

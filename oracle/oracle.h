@@ -1,0 +1,101 @@
+/*
+ * oracle.h -- CPU oracle for the trace-analysis path (TEST INFRASTRUCTURE).
+ *
+ * A scalar C++ restatement of the reference's Host* strategies, used only
+ * by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+ * checker and CPU baseline.  Nothing in the product links or calls it.
+ * Each function cites the reference file:line it follows.
+ */
+#ifndef HWPA_ORACLE_H
+#define HWPA_ORACLE_H
+
+#include <stdint.h>
+#include "../include/hwpa_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-lane DME diagnostics (traceAnalysis/DmeDiagnostics.h, MaxLikelihoodDiagnostics.h). */
+typedef struct orc_dme_diag {
+    int32_t converged[HWPA_LANE_SIZE];
+    int32_t iter_count[HWPA_LANE_SIZE];
+    int32_t status[HWPA_LANE_SIZE];
+    int32_t iterations_run;                 /* lane-wide EM iterations executed */
+    int32_t pad;
+    float log_like[HWPA_LANE_SIZE];
+    float conf_factors[8][HWPA_LANE_SIZE];
+    float scale_factor[HWPA_LANE_SIZE];
+    float confidence[HWPA_LANE_SIZE];       /* estimate confidence before blending */
+    float est_fp32[5][3][HWPA_LANE_SIZE];   /* [mode][weight,mean,var] of the estimate */
+    float out_fp32[5][3][HWPA_LANE_SIZE];   /* blended model before half export */
+} orc_dme_diag;
+
+typedef struct orc_result {
+    hwpa_pulse* pulses;              /* [L*64*max_calls] */
+    uint32_t* pulse_counts;          /* [L*64] */
+    uint32_t* pulse_overflows;       /* [L*64] */
+    hwpa_stat_acc* pulse_bl_stats;   /* [L] */
+    int16_t* baselined;              /* [L][F][64] */
+    hwpa_baseliner_stats* bl_stats;  /* [L] */
+    int16_t* labels;                 /* [L][F][64] */
+    float* viterbi_score;            /* [L*64] */
+    hwpa_lane_model* models;         /* [L] */
+    int32_t flags[4];                /* [0] pulses emitted, [1] estimation ran, [2] labels ran */
+} orc_result;
+
+int orc_threads(int n);              /* set worker thread count, returns previous */
+
+void orc_filter_params(int32_t type, hwpa_filter_params* out);
+void orc_transition_matrix(const hwpa_config* cfg, hwpa_transition* out);
+
+void* orc_baseliner_create(const hwpa_config* cfg, int32_t lanes);
+void orc_baseliner_destroy(void* h);
+int32_t orc_baseliner_process(void* h, const int16_t* raw, int32_t frames,
+                              int16_t* out, hwpa_baseliner_stats* stats);
+
+void* orc_agg_create(int32_t lanes);
+void orc_agg_destroy(void* h);
+void orc_agg_add(void* h, const hwpa_baseliner_stats* stats);
+void orc_agg_get(void* h, hwpa_baseliner_stats* out);
+void orc_agg_reset(void* h);
+
+void* orc_hist_create(const hwpa_config* cfg, int32_t lanes);
+void orc_hist_destroy(void* h);
+int32_t orc_hist_reset(void* h, const hwpa_baseliner_stats* stats);
+void orc_hist_add(void* h, const int16_t* traces, int32_t frames, const hwpa_lane_model* models);
+void orc_hist_get(void* h, hwpa_lane_hist* out);
+
+void orc_dme_init_models(const hwpa_config* cfg, int32_t lanes,
+                         const hwpa_baseliner_stats* stats, hwpa_lane_model* out);
+void orc_dme_estimate(const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,
+                      const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
+                      const int32_t model_fi[2], const int32_t est_fi[2], orc_dme_diag* diag);
+
+void* orc_labeler_create(const hwpa_config* cfg, int32_t lanes);
+void orc_labeler_destroy(void* h);
+void orc_labeler_process(void* h, const int16_t* traces, int32_t frames,
+                         const hwpa_lane_model* models, int16_t* labels,
+                         int16_t* latent_trace, float* scores);
+
+void* orc_pulse_create(const hwpa_config* cfg, int32_t lanes);
+void orc_pulse_destroy(void* h);
+void orc_pulse_process(void* h, const int16_t* labels, const int16_t* latent_trace,
+                       const int16_t* cur_trace, int32_t frames, int32_t first_frame,
+                       const hwpa_lane_model* models, hwpa_pulse* pulses, uint32_t* counts,
+                       uint32_t* overflows, hwpa_stat_acc* bl_stats);
+
+void* orc_analyzer_create(const hwpa_config* cfg, uint32_t pool_id, int32_t lanes, int32_t frames);
+void orc_analyzer_destroy(void* h);
+int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id,
+                             int32_t first_frame, orc_result* res);
+
+void orc_tracegen(const hwpa_tracegen_params* p, int32_t lanes, int32_t lane_offset,
+                  int32_t frames, uint32_t first_frame, hwpa_tracegen_state* state,
+                  int32_t init, int16_t* out, int16_t* truth /* nullable: 0 = baseline, a+1 = analog a */);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HWPA_ORACLE_H */

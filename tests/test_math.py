@@ -1,0 +1,92 @@
+"""The deterministic math of include/hwpa_math.h (shared by the kernels and
+the oracle) against libm/numpy: ulp-level accuracy, exact special cases, and
+IEEE half conversion matching numpy's float16."""
+import ctypes
+
+import numpy as np
+import pytest
+
+
+def _eval(O, kind, a, b=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b if b is not None else np.zeros_like(a), dtype=np.float32)
+    out = np.empty_like(a)
+    O.lib.orc_math(ctypes.c_int32(kind), a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p),
+                   out.ctypes.data_as(ctypes.c_void_p), ctypes.c_int32(a.size))
+    return out
+
+
+def _ulps(x, ref):
+    x = x.astype(np.float64)
+    ref = ref.astype(np.float64)
+    spacing = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    return np.abs(x - ref) / spacing
+
+
+def test_exp_accuracy(O):
+    x = np.concatenate([np.linspace(-87, 88, 200001, dtype=np.float32),
+                        np.float32([0.0, -0.0, 1e-30, -1e-30, 0.5, -0.5])])
+    got = _eval(O, 0, x)
+    ref = np.exp(x.astype(np.float64))
+    assert _ulps(got, ref).max() <= 3.0
+    assert _eval(O, 0, np.float32([0.0]))[0] == 1.0
+
+
+def test_exp_limits(O):
+    got = _eval(O, 0, np.float32([100.0, -110.0, np.inf, -np.inf, np.nan, -100.0]))
+    assert got[0] == np.inf and got[1] == 0.0 and got[2] == np.inf and got[3] == 0.0 and np.isnan(got[4])
+    # subnormal results keep their value to within a few subnormal steps
+    assert abs(got[5] - np.float32(np.exp(-100.0))) <= 4 * np.float32(1.4e-45)
+
+
+def test_log_accuracy(O):
+    x = np.concatenate([np.exp(np.linspace(-80, 80, 200001)).astype(np.float32),
+                        np.float32([1.0, 2.0, 0.5, 1.5e-42, 3.4e38, 1.0000001, 0.99999994])])
+    got = _eval(O, 1, x)
+    ref = np.log(x.astype(np.float64))
+    err = np.abs(got.astype(np.float64) - ref)
+    tol = np.maximum(3 * np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64), 2e-7 * np.abs(ref) + 1e-12)
+    assert (err <= np.maximum(tol, 3e-8)).all()
+    assert _eval(O, 1, np.float32([1.0]))[0] == 0.0
+    sp = _eval(O, 1, np.float32([0.0, -1.0, np.inf, np.nan]))
+    assert sp[0] == -np.inf and np.isnan(sp[1]) and sp[2] == np.inf and np.isnan(sp[3])
+
+
+def test_log2_exp2_pow(O):
+    x = np.exp(np.linspace(-20, 20, 50001)).astype(np.float32)
+    assert _ulps(_eval(O, 2, x), np.log2(x.astype(np.float64))).max() <= 4 or \
+        np.abs(_eval(O, 2, x) - np.log2(x.astype(np.float64))).max() < 1e-6
+    y = np.linspace(-120, 120, 50001).astype(np.float32)
+    assert _ulps(_eval(O, 3, y), np.exp2(y.astype(np.float64))).max() <= 3.0
+    # exact powers of two
+    k = np.arange(-126, 128, dtype=np.float32)
+    assert np.array_equal(_eval(O, 3, k), np.ldexp(np.float64(1.0), k.astype(np.int64)).astype(np.float32))
+    a = np.exp(np.linspace(-5, 9, 2001)).astype(np.float32)
+    b = np.linspace(0, 1, 2001).astype(np.float32)
+    got = _eval(O, 4, a, b)
+    ref = np.power(a.astype(np.float64), b.astype(np.float64))
+    assert (np.abs(got - ref) / ref).max() < 2e-6
+    # pow(x, 0) == 1 and pow(x, 1) == x exactly (confidence-weighted update with fraction 0/1)
+    assert np.array_equal(_eval(O, 4, a, np.zeros_like(a)), np.ones_like(a))
+    assert np.array_equal(_eval(O, 4, a, np.ones_like(a)), a)
+
+
+def test_half_round_trip_matches_numpy(O):
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.normal(0, 1, 100000) * 10.0 ** rng.integers(-9, 6, 100000),
+                        [65504.0, 65519.99, 65520.0, 1e9, 6.1e-5, 5.96e-8, 2.98e-8, 2.99e-8, 0.0, -0.0, 0.1]]
+                       ).astype(np.float32)
+    got = _eval(O, 5, x)
+    with np.errstate(over="ignore"):
+        ref = x.astype(np.float16).astype(np.float32)
+    same = (got == ref) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), x[~same][:5]
+
+
+def test_round_and_floor_conversions(O):
+    x = np.float32([0.5, 1.5, 2.5, -0.5, -1.5, 32767.4, 32768.0, -32769.0, 70000.0, 3e9, np.nan])
+    r = _eval(O, 6, x)
+    # RNE, then int32 -> int16 wrap; out of int32 range or NaN -> 0x80000000 -> 0
+    assert list(r) == [0, 2, 2, 0, -2, 32767, -32768, 32767, 4464, 0, 0]
+    f = _eval(O, 7, np.float32([-0.5, 0.99999994, 299.99997, -1e10]))
+    assert list(f) == [-1, 0, 299, -2147483648.0]
