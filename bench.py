@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Throughput of the trace-analysis path on one MI355X.
+
+A step is one 512-frame chunk of every ZMW of the pool through the whole
+path (baseline filter, statistics aggregation, histogram, EM detection-model
+estimation, Viterbi labeling, pulse accumulation) in the measurement
+configuration: sigma EMA off and one model estimate per chunk, so every
+stage runs on every chunk.  Workload: 2,097,152 ZMWs (32,768 lanes x 64) of
+int16 traces, 16 chunks streamed (the first `warmup` chunks cover the
+baseliner/DME start-up and are not timed).  All chunks are generated on the
+GPU before timing starts, so inputs are resident in HBM.
+
+Prints one JSON line.  The CPU baseline is the scalar C++ oracle timed on a
+bounded sample on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per ZMW-frame (MultiScaleLarge filter, F = 512):
+#  baseline  = 2 in + 2 out + 2.78 filter windows in/out + 0.09 EMA/latent + 0.18 stats
+#  histogram = 2 in + 2.39 (300 x u16 counters + bounds, in and out) + 0.02 edge state
+ALG_BYTES = {"baseline": 7.05, "histogram": 4.41}
+STAGES = ["baseline", "aggregate", "histogram", "dme", "labels", "pulses"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(lanes, frames, chunks, timed_from, threads):
+    import oracle as O
+    import hw_amd as H
+    from hw_amd import types as T
+    O.threads(threads)
+    cfg = H.bench_config(frames)
+    p = H.tracegen_default()
+    st = np.zeros(lanes, dtype=T.TRACEGEN_STATE)
+    data = [O.tracegen(p, lanes, frames, c * frames, st, c == 0) for c in range(chunks)]
+    an = O.Analyzer(cfg, 0, lanes, frames)
+    dts = (T.PULSE, T.STAT_ACC, T.BASELINER_STATS, T.LANE_MODEL)
+    t0 = None
+    for c in range(chunks):
+        if c == timed_from:
+            t0 = time.perf_counter()
+        r = an(data[c], c * frames, dts, want=("pulses",))
+        if r["rc"] != 0:
+            raise RuntimeError("oracle analyzer failed")
+    dt = time.perf_counter() - t0
+    zf = lanes * 64 * frames * (chunks - timed_from)
+    return zf / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--lanes", type=int, default=32768)
+    ap.add_argument("--frames", type=int, default=512)
+    ap.add_argument("--cpu-lanes", type=int, default=64)
+    ap.add_argument("--cpu-chunks", type=int, default=12)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the path shards by pool with no exchange, "
+              "so --gpus %d is not run (use --gpus 1)" % args.gpus, file=sys.stderr)
+        return 2
+    if args.warmup < 4:
+        log("warmup raised to 4 chunks (baseliner start-up 1024 frames + first estimate)")
+        args.warmup = 4
+
+    import hw_amd as H
+    from hw_amd import types as T
+
+    L, F = args.lanes, args.frames
+    chunks = args.warmup + args.steps
+    cfg = H.bench_config(F)
+    gen = H.tracegen_default()
+    log("generating %d chunks of %d lanes x %d frames on the GPU" % (chunks, L, F))
+    d_state = H.dev_bytes(L * T.TRACEGEN_STATE.itemsize)
+    bufs = []
+    for c in range(chunks):
+        b = H.dev_bytes(L * F * 64 * 2)
+        H.tracegen(gen, L, F, c * F, d_state, c == 0, b)
+        bufs.append(b)
+    H.sync()
+    an = H.BatchAnalyzer(0, L, F, cfg)
+    an.set_profiling(True)
+    stage_ms = {s: [] for s in STAGES}
+    em_iters = []
+    for c in range(args.warmup):
+        an(bufs[c], c * F)
+        an.sync()
+    log("warmup done (%d chunks); timing %d chunks" % (args.warmup, args.steps))
+    H.sync()
+    t0 = time.perf_counter()
+    for c in range(args.warmup, chunks):
+        an(bufs[c], c * F)
+        ms = an.stage_ms()   # waits for this chunk's last event
+        for s, v in zip(STAGES, ms):
+            stage_ms[s].append(v)
+    H.sync()
+    dt = time.perf_counter() - t0
+    for c in range(1):
+        em_iters.append(an.em_iterations())
+    flags = an.flags()
+    zf_step = L * 64 * F
+    value = zf_step * args.steps / dt
+    mean_ms = {s: float(np.mean(v)) for s, v in stage_ms.items()}
+    dominant = max(mean_ms, key=mean_ms.get)
+    bl_gbs = ALG_BYTES["baseline"] * zf_step / (mean_ms["baseline"] * 1e-3) / 1e9
+    hist_gbs = ALG_BYTES["histogram"] * zf_step / (mean_ms["histogram"] * 1e-3) / 1e9
+    log("stage ms:", {k: round(v, 3) for k, v in mean_ms.items()}, "EM iterations", em_iters, "flags", flags)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log("timing the CPU oracle on %d lanes x %d chunks with %d threads" % (args.cpu_lanes, args.cpu_chunks,
+                                                                             threads))
+        cpu_value, cpu_dt = cpu_baseline(args.cpu_lanes, F, args.cpu_chunks, 4, threads)
+        cpu = {"value": cpu_value, "unit": "ZMW-frames/s", "cores": threads, "kind": "port",
+               "sample": "%d ZMWs (%d lanes) x %d-frame chunks, chunks 4..%d timed (%.1f s), same pipeline and "
+                         "configuration" % (args.cpu_lanes * 64, args.cpu_lanes, F, args.cpu_chunks - 1, cpu_dt)}
+
+    out = {
+        "metric": "ZMW-frames/s on 1 GPU, plain vs learned-DME init, at matched label agreement",
+        "value": value,
+        "unit": "ZMW-frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16+f32",
+        "data": "synthetic (seeded GPU generator: N(0,12^2) baseline, +-40 counts/chunk drift, "
+                "amplitudes 180/120/80/50, geometric widths 8 / gaps 20)",
+        "config": {"workload": "plain pipeline, %d ZMWs (%d lanes x 64) x %d-frame int16 chunks, %d chunks "
+                               "streamed, one estimate per chunk" % (L * 64, L, F, chunks),
+                   "lanes": L, "frames_per_chunk": F, "chunks": chunks, "filter": "MultiScaleLarge",
+                   "init": "plain EM"},
+        "roofline": {"bound": "hbm", "kernel": "BaselineKernel", "achieved": bl_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": bl_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_zmw_frame": ALG_BYTES["baseline"]},
+        "histogram_roofline": {"kernel": "HistAddKernel", "achieved": hist_gbs, "unit": "GB/s",
+                               "frac": hist_gbs / HBM_PEAK_GBS, "alg_bytes_per_zmw_frame": ALG_BYTES["histogram"]},
+        "stage_ms": mean_ms,
+        "dominant_stage": dominant,
+        "em_iterations_last_chunk": em_iters[-1] if em_iters else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
