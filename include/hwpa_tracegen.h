@@ -33,7 +33,8 @@ HWPA_HD uint64_t hw_mix64(uint64_t x)
 HWPA_HD uint64_t hw_hash(uint64_t seed, uint64_t zmw, uint64_t ctr, uint32_t stream)
 {
     uint64_t h = hw_mix64(seed ^ 0x9e3779b97f4a7c15ull);
-    h = hw_mix64(h ^ (zmw * 0xd1b54a32d192ed03ull) ^ ((uint64_t)stream << 56));
+    h = hw_mix64(h ^ (zmw * 0xd1b54a32d192ed03ull));
+    h = hw_mix64(h ^ ((uint64_t)stream * 0xa0761d6478bd642full));
     h = hw_mix64(h ^ (ctr * 0x8cb92ba72f3d8dd7ull));
     return h;
 }
