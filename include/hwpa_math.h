@@ -3,12 +3,14 @@
  * kernels and their host-side checkers.
  *
  * Every routine here is written with IEEE-754 basic operations only
- * (add, sub, mul, correctly rounded div and sqrt, floor, rint, bit casts),
- * so a translation unit built with -ffp-contract=off produces the same bits
- * on the host and on the GPU.  The transcendental functions are derived
+ * (add, sub, mul, explicit fused multiply-add, correctly rounded div and
+ * sqrt, floor, rint, ldexp, bit casts), so a translation unit built with
+ * -ffp-contract=off produces the same bits on the host and on the GPU
+ * (an explicit fmaf is a single correctly rounded operation on both).  The transcendental functions are derived
  * from their Taylor / atanh series, not from any vendor library:
  *   exp(x)  = 2^k * exp(r),  r = x - k ln2 (two-part ln2),  |r| <= ln2/2,
- *             exp(r) by its Taylor series to r^8 (truncation < 6e-9 rel).
+ *             exp(r) by its Taylor series to r^7 (truncation < 6e-9 rel),
+ *             2^k applied with an exact ldexp.
  *   log(x)  = e ln2 + 2 atanh(s),  s = (m-1)/(m+1),  m in [sqrt(1/2), sqrt(2)),
  *             atanh series to s^13 (truncation < 2e-10).
  * Accuracy is a few ulp; tests/test_math.py bounds the error against libm.
@@ -36,6 +38,7 @@ HWPA_HD float hw_u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 HWPA_HD float hw_floorf(float x) { return __builtin_floorf(x); }
 HWPA_HD float hw_rintf(float x) { return __builtin_rintf(x); }
 HWPA_HD float hw_sqrtf(float x) { return __builtin_sqrtf(x); }
+HWPA_HD float hw_fmaf(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 HWPA_HD float hw_fabsf(float x) { return hw_u2f(hw_f2u(x) & 0x7fffffffu); }
 HWPA_HD bool hw_isnan(float x) { return (hw_f2u(x) & 0x7fffffffu) > 0x7f800000u; }
 HWPA_HD bool hw_isinf(float x) { return (hw_f2u(x) & 0x7fffffffu) == 0x7f800000u; }
@@ -71,28 +74,58 @@ HWPA_HD float hw_pow2i(int k)
     return hw_u2f(1u << (k + 149));
 }
 
+/* x * 2^k, exact (one rounding for subnormal results).  The GPU has it as one
+ * instruction; the host builds the bits when the result stays normal. */
+HWPA_HD float hw_ldexpf(float x, int k)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_ldexpf(x, k);
+#else
+    const uint32_t u = hw_f2u(x);
+    const int e = (int)((u >> 23) & 0xffu);
+    if (e != 0 && e != 255 && k > -e && k < 255 - e) return hw_u2f(u + ((uint32_t)k << 23));
+    return __builtin_ldexpf(x, k);
+#endif
+}
+
+/* Clamp for reduction arguments; a NaN argument gives an unspecified finite
+ * value, so callers restore NaN themselves. */
+HWPA_HD float hw_clamp_arg(float x, float lo, float hi)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fminf(__builtin_fmaxf(x, lo), hi);
+#else
+    return hw_minf(hw_maxf(x, lo), hi);
+#endif
+}
+
+/* exp(r) for |r| <= ln2/2: Taylor polynomial to r^7 in Horner form. */
+HWPA_HD float hw_exp_poly(float r)
+{
+    float p = 1.0f / 5040.0f;
+    p = hw_fmaf(p, r, 1.0f / 720.0f);
+    p = hw_fmaf(p, r, 1.0f / 120.0f);
+    p = hw_fmaf(p, r, 1.0f / 24.0f);
+    p = hw_fmaf(p, r, 1.0f / 6.0f);
+    p = hw_fmaf(p, r, 0.5f);
+    p = hw_fmaf(p, r, 1.0f);
+    return hw_fmaf(p, r, 1.0f);
+}
+
+/* exp(x): k = rint(x / ln2), r = x - k ln2 (Cody-Waite, |r| <= ln2/2),
+ * exp(r) by its Taylor polynomial, scaled by 2^k with an exact ldexp (which
+ * also produces the correctly rounded subnormal / overflow results).  The
+ * argument is clamped to +-200 so k stays in int range; NaN propagates. */
 HWPA_HD float hw_expf(float x)
 {
-    if (hw_isnan(x)) return x;
-    if (x > 88.72283935546875f) return hw_u2f(0x7f800000u);
-    if (x < -103.97208404541015625f) return 0.0f;
-    const float kf = hw_floorf(x * 1.44269502162933349609375f + 0.5f);
-    const int k = (int)kf;
-    /* ln2 split: hi has 16 significant bits so kf*hi is exact for |k| <= 150. */
+    const float xc = hw_clamp_arg(x, -200.0f, 200.0f);
+    const float kf = hw_rintf(xc * 1.44269502162933349609375f);
+    /* ln2 = ln2hi + ln2lo; with fused steps the reduction keeps ~48 bits. */
     const float ln2hi = 0.693145751953125f;
     const float ln2lo = 1.428606765330187045e-06f;
-    const float r = (x - kf * ln2hi) - kf * ln2lo;
-    float p = 1.0f / 5040.0f;
-    p = p * r + 1.0f / 720.0f;
-    p = p * r + 1.0f / 120.0f;
-    p = p * r + 1.0f / 24.0f;
-    p = p * r + 1.0f / 6.0f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
-    if (k > 127) return (p * hw_pow2i(127)) * hw_pow2i(k - 127);
-    if (k < -126) return (p * hw_pow2i(-126)) * hw_pow2i(k + 126);
-    return p * hw_pow2i(k);
+    const float r = hw_fmaf(-kf, ln2lo, hw_fmaf(-kf, ln2hi, xc));
+    const float v = hw_ldexpf(hw_exp_poly(r), (int)kf);
+    return hw_isnan(x) ? x : v;
 }
 
 /* Returns log(m) for m in [sqrt(1/2), sqrt(2)) and the binary exponent e. */
@@ -114,14 +147,14 @@ HWPA_HD float hw_log_reduce(float x, int* e_out)
     const float s = (m - 1.0f) / (m + 1.0f);
     const float s2 = s * s;
     float q = 1.0f / 13.0f;
-    q = q * s2 + 1.0f / 11.0f;
-    q = q * s2 + 1.0f / 9.0f;
-    q = q * s2 + 1.0f / 7.0f;
-    q = q * s2 + 1.0f / 5.0f;
-    q = q * s2 + 1.0f / 3.0f;
+    q = hw_fmaf(q, s2, 1.0f / 11.0f);
+    q = hw_fmaf(q, s2, 1.0f / 9.0f);
+    q = hw_fmaf(q, s2, 1.0f / 7.0f);
+    q = hw_fmaf(q, s2, 1.0f / 5.0f);
+    q = hw_fmaf(q, s2, 1.0f / 3.0f);
     q = q * s2;
     *e_out = e;
-    return (s + s) + (s + s) * q;
+    return hw_fmaf(s + s, q, s + s);
 }
 
 HWPA_HD float hw_logf(float x)
@@ -135,7 +168,7 @@ HWPA_HD float hw_logf(float x)
     const float ef = (float)e;
     const float ln2hi = 0.693145751953125f;
     const float ln2lo = 1.428606765330187045e-06f;
-    return ef * ln2hi + (lm + ef * ln2lo);
+    return hw_fmaf(ef, ln2hi, hw_fmaf(ef, ln2lo, lm));
 }
 
 HWPA_HD float hw_log2f(float x)
@@ -146,28 +179,16 @@ HWPA_HD float hw_log2f(float x)
     if (hw_isinf(x)) return x;
     int e;
     const float lm = hw_log_reduce(x, &e);
-    return (float)e + lm * 1.44269502162933349609375f;
+    return hw_fmaf(lm, 1.44269502162933349609375f, (float)e);
 }
 
 HWPA_HD float hw_exp2f(float x)
 {
-    if (hw_isnan(x)) return x;
-    if (x >= 128.0f) return hw_u2f(0x7f800000u);
-    if (x < -150.0f) return 0.0f;
-    const float kf = hw_rintf(x);
-    const int k = (int)kf;
-    const float r = (x - kf) * 0.693147182464599609375f;   /* x - kf is exact */
-    float p = 1.0f / 5040.0f;
-    p = p * r + 1.0f / 720.0f;
-    p = p * r + 1.0f / 120.0f;
-    p = p * r + 1.0f / 24.0f;
-    p = p * r + 1.0f / 6.0f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
-    if (k > 127) return (p * hw_pow2i(127)) * hw_pow2i(k - 127);
-    if (k < -126) return (p * hw_pow2i(-126)) * hw_pow2i(k + 126);
-    return p * hw_pow2i(k);
+    const float xc = hw_clamp_arg(x, -300.0f, 300.0f);
+    const float kf = hw_rintf(xc);
+    const float r = (xc - kf) * 0.693147182464599609375f;   /* xc - kf is exact */
+    const float v = hw_ldexpf(hw_exp_poly(r), (int)kf);
+    return hw_isnan(x) ? x : v;
 }
 
 /* x^y for x > 0; exact for y == 0 and y == 1. */
