@@ -93,6 +93,8 @@ def main():
         H.tracegen(gen, L, F, c * F, d_state, c == 0, b)
         bufs.append(b)
     H.sync()
+    copy_ms, copy_gbs = H.copy_bandwidth(1 << 30, 10)
+    log("copy kernel: 1 GiB in %.3f ms = %.0f GB/s read+write" % (copy_ms, copy_gbs))
     an = H.BatchAnalyzer(0, L, F, cfg)
     an.set_profiling(True)
     stage_ms = {s: [] for s in STAGES}
@@ -151,7 +153,7 @@ def main():
                    "init": "plain EM"},
         "roofline": {"bound": "hbm", "kernel": "BaselineKernel", "achieved": bl_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": bl_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_zmw_frame": ALG_BYTES["baseline"]},
+                     "alg_bytes_per_zmw_frame": ALG_BYTES["baseline"], "copy_kernel_gbs": copy_gbs},
         "histogram_roofline": {"kernel": "HistAddKernel", "achieved": hist_gbs, "unit": "GB/s",
                                "frac": hist_gbs / HBM_PEAK_GBS, "alg_bytes_per_zmw_frame": ALG_BYTES["histogram"]},
         "stage_ms": mean_ms,

@@ -138,10 +138,28 @@ class DevBuf:
         out = _dtoh(self.p.value, n).view(dtype)
         return out.reshape(shape) if shape is not None else out
 
+    def view(self, offset, nbytes):
+        """A non-owning window [offset, offset + nbytes) of this buffer."""
+        if offset < 0 or nbytes < 0 or offset + nbytes > self.nbytes:
+            raise HwpaError(1, "view outside the buffer")
+        return DevView(self, offset, nbytes)
+
     def __del__(self):
         if getattr(self, "p", None) is not None and self.p.value:
             lib.hwpa_dev_free(self.p)
             self.p = None
+
+
+class DevView(DevBuf):
+    """Window into a DevBuf; keeps the parent alive, frees nothing."""
+
+    def __init__(self, parent, offset, nbytes):
+        self.parent = parent
+        self.nbytes = nbytes
+        self.p = _vp(parent.data_ptr() + offset)
+
+    def __del__(self):
+        pass
 
 
 def dev_bytes(nbytes):
@@ -164,6 +182,15 @@ def ptr(buf):
 
 def sync():
     check(lib.hwpa_device_sync())
+
+
+def copy_bandwidth(nbytes=1 << 30, launches=10):
+    """Mean ms per launch of the library's plain copy kernel over nbytes, and
+    the resulting read+write GB/s."""
+    src, dst = dev_bytes(nbytes), dev_bytes(nbytes)
+    ms = ctypes.c_float(0.0)
+    check(lib.hwpa_copy_bandwidth(ptr(dst), ptr(src), ctypes.c_size_t(nbytes), launches, ctypes.byref(ms)))
+    return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 # ---- stage wrappers ------------------------------------------------------------
@@ -222,6 +249,78 @@ def dme_estimate(cfg, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iter
     efi = (ctypes.c_int32 * 2)(*est_fi)
     check(lib.hwpa_dme_estimate(ctypes.byref(cfg), ptr(d_hist), ptr(d_stats), ptr(d_models), mfi, efi,
                                 ctypes.c_uint32(lanes), ptr(d_iters) if d_iters is not None else None, _vp(stream)))
+
+
+def dme_estimate_diag(cfg, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_diag, d_iters=None, stream=None):
+    """As dme_estimate, also writing the raw per-ZMW EM result to d_diag
+    (float32 [lanes][EM_DIAG_ROWS][64])."""
+    mfi = (ctypes.c_int32 * 2)(*model_fi)
+    efi = (ctypes.c_int32 * 2)(*est_fi)
+    check(lib.hwpa_dme_estimate_diag(ctypes.byref(cfg), ptr(d_hist), ptr(d_stats), ptr(d_models), mfi, efi,
+                                     ctypes.c_uint32(lanes), ptr(d_iters) if d_iters is not None else None,
+                                     ptr(d_diag), _vp(stream)))
+
+
+def dme_estimate_learned(cfg, init, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters=None, stream=None):
+    """Learned-start estimation; returns the number of ZMWs re-estimated by the plain EM."""
+    mfi = (ctypes.c_int32 * 2)(*model_fi)
+    efi = (ctypes.c_int32 * 2)(*est_fi)
+    n = ctypes.c_uint32(0)
+    check(lib.hwpa_dme_estimate_learned(ctypes.byref(cfg), init.h, ptr(d_hist), ptr(d_stats), ptr(d_models), mfi,
+                                        efi, ctypes.c_uint32(lanes), ptr(d_iters) if d_iters is not None else None,
+                                        ctypes.byref(n), _vp(stream)))
+    return n.value
+
+
+MLP_IN, MLP_HIDDEN, MLP_OUT = 128, 256, 11
+MLP_PARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_OUT * MLP_HIDDEN + MLP_OUT
+EM_DIAG_ROWS = 18
+
+
+class Initializer:
+    """The learned EM initialiser (128 -> 256 -> 256 -> 11 perceptron)."""
+
+    def __init__(self, seed=1):
+        self.h = _vp()
+        check(lib.hwpa_initializer_create(ctypes.c_uint64(seed), ctypes.byref(self.h)))
+
+    def params(self):
+        p = np.empty(MLP_PARAMS, dtype=np.float32)
+        check(lib.hwpa_initializer_get_params(self.h, p.ctypes.data_as(_vp), ctypes.c_size_t(MLP_PARAMS)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib.hwpa_initializer_set_params(self.h, p.ctypes.data_as(_vp), ctypes.c_size_t(p.size)))
+
+    def forward(self, d_x, rows, d_y, stream=None):
+        check(lib.hwpa_mlp_forward(self.h, ptr(d_x), ctypes.c_uint32(rows), ptr(d_y), _vp(stream)))
+
+    def gradients(self, d_x, d_t, batch, d_grads, d_loss, d_valid=None, stream=None):
+        check(lib.hwpa_mlp_gradients(self.h, ptr(d_x), ptr(d_t), _opt(d_valid), ctypes.c_uint32(batch), ptr(d_grads),
+                                     ptr(d_loss), _vp(stream)))
+
+    def train_step(self, d_x, d_t, batch, lr, d_loss=None, d_valid=None, stream=None):
+        check(lib.hwpa_mlp_train_step(self.h, ptr(d_x), ptr(d_t), _opt(d_valid), ctypes.c_uint32(batch),
+                                      ctypes.c_float(lr), _opt(d_loss), _vp(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.hwpa_initializer_destroy(self.h)
+            self.h = _vp()
+
+
+def _opt(buf):
+    return ptr(buf) if buf is not None else None
+
+
+def mlp_features(d_hist, lanes, d_x, stream=None):
+    check(lib.hwpa_mlp_features(ptr(d_hist), ctypes.c_uint32(lanes), ptr(d_x), _vp(stream)))
+
+
+def mlp_targets(d_hist, d_diag, lanes, min_conf, d_t, d_valid, stream=None):
+    check(lib.hwpa_mlp_targets(ptr(d_hist), ptr(d_diag), ctypes.c_uint32(lanes), ctypes.c_float(min_conf), ptr(d_t),
+                               ptr(d_valid), _vp(stream)))
 
 
 class Labeler:
@@ -369,6 +468,21 @@ class BatchAnalyzer:
         it = ctypes.c_int32()
         check(lib.hwpa_analyzer_em_iterations(self.h, ctypes.byref(it)))
         return it.value
+
+    def set_initializer(self, init):
+        self._init = init
+        check(lib.hwpa_analyzer_set_initializer(self.h, init.h if init is not None else None))
+
+    def capture_training(self, d_x, d_t, d_valid, min_conf=0.1):
+        """Arm (or with d_x None, disarm) capture of (input, target, valid) at each plain estimate."""
+        self._cap = (d_x, d_t, d_valid)
+        check(lib.hwpa_analyzer_capture_training(self.h, _opt(d_x), _opt(d_t), _opt(d_valid),
+                                                 ctypes.c_float(min_conf)))
+
+    def fallback_count(self):
+        n = ctypes.c_uint32()
+        check(lib.hwpa_analyzer_fallback_count(self.h, ctypes.byref(n)))
+        return n.value
 
     def __del__(self):
         if getattr(self, "h", None):

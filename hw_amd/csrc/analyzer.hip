@@ -81,6 +81,15 @@ struct hwpa_analyzer {
     float* dEmScratch = nullptr;
     int32_t* dIters = nullptr;
     bool ranEstimate = false;
+    hwpa_initializer* mlp = nullptr;
+    LearnedWs lws;
+    bool ranLearned = false;
+    // training-pair capture
+    uint16_t* capX = nullptr;
+    float* capT = nullptr;
+    uint8_t* capValid = nullptr;
+    float capMinConf = 0.0f;
+    float* dDiag = nullptr;
 };
 
 static void FreeAnalyzer(hwpa_analyzer* a)
@@ -94,6 +103,8 @@ static void FreeAnalyzer(hwpa_analyzer* a)
                     a->dModels, a->dLabels,    a->dLatent, a->dScore,  a->dPulses,    a->dCounts,
                     a->dOvf,    a->dPulseBl,   a->dEmScratch, a->dIters};
     for (void* p : bufs) DeviceFree(p);
+    LearnedWsFree(a->lws);
+    DeviceFree(a->dDiag);
     for (auto& e : a->ev)
         if (e) (void)hipEventDestroy(e);
     if (a->stream) (void)hipStreamDestroy(a->stream);
@@ -259,8 +270,20 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
         if (a->status == kSequencing) {
             const int32_t mfi[2] = {a->modelFi.lo, a->modelFi.hi};
             const int32_t efi[2] = {a->histFi.lo, a->histFi.hi};
-            if ((rc = DmeEstimate(a->cfg, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, a->dEmScratch, s)) != HWPA_OK)
-                return rc;
+            if (a->mlp && a->cfg.learned_init) {
+                rc = DmeEstimateLearned(a->cfg, a->mlp, a->lws, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters,
+                                        a->dEmScratch, s);
+                a->ranLearned = true;
+            } else {
+                const bool cap = a->capX != nullptr;
+                rc = DmeEstimate(a->cfg, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, a->dEmScratch, s,
+                                 cap ? a->dDiag : nullptr);
+                if (rc == HWPA_OK && cap) rc = MlpFeatures(a->dHist, L, a->capX, s);
+                if (rc == HWPA_OK && cap)
+                    rc = MlpTargets(a->dHist, a->dDiag, L, a->capMinConf, a->capT, a->capValid, s);
+                a->ranLearned = false;
+            }
+            if (rc != HWPA_OK) return rc;
             a->modelFi = a->histFi;
             a->flags[1] = 1;
             a->ranEstimate = true;
@@ -395,6 +418,47 @@ int32_t hwpa_analyzer_em_iterations(const hwpa_analyzer* a, int32_t* iters)
     HWPA_HIP(hipStreamSynchronize(a->stream));
     HWPA_HIP(hipMemcpy(h.data(), a->dIters, sizeof(int32_t) * a->lanes, hipMemcpyDeviceToHost));
     for (int32_t v : h) *iters = std::max(*iters, v);
+    return HWPA_OK;
+}
+
+int32_t hwpa_analyzer_set_initializer(hwpa_analyzer* a, hwpa_initializer* m)
+{
+    if (!a) return Fail(HWPA_ERR_INVALID_ARG, "null analyzer");
+    HWPA_HIP(hipStreamSynchronize(a->stream));
+    if (m && a->lws.lanes < a->lanes) {
+        const int32_t rc = LearnedWsAlloc(a->lws, a->lanes);
+        if (rc != HWPA_OK) return rc;
+    }
+    a->mlp = m;
+    return HWPA_OK;
+}
+
+int32_t hwpa_analyzer_capture_training(hwpa_analyzer* a, uint16_t* d_x, float* d_t, uint8_t* d_valid,
+                                       float min_confidence)
+{
+    if (!a) return Fail(HWPA_ERR_INVALID_ARG, "null analyzer");
+    if (d_x && (!d_t || !d_valid)) return Fail(HWPA_ERR_INVALID_ARG, "capture: targets and valid flags required");
+    if (d_x && ((uintptr_t)d_x & 15) != 0) return Fail(HWPA_ERR_INVALID_ARG, "capture: input must be 16-byte aligned");
+    HWPA_HIP(hipStreamSynchronize(a->stream));
+    if (d_x && !a->dDiag) {
+        const int32_t rc = DeviceAlloc((void**)&a->dDiag, (size_t)a->lanes * HWPA_EM_DIAG_ROWS * LS * sizeof(float),
+                                       "EM diagnostics");
+        if (rc != HWPA_OK) return rc;
+    }
+    a->capX = d_x;
+    a->capT = d_x ? d_t : nullptr;
+    a->capValid = d_x ? d_valid : nullptr;
+    a->capMinConf = min_confidence;
+    return HWPA_OK;
+}
+
+int32_t hwpa_analyzer_fallback_count(const hwpa_analyzer* a, uint32_t* count)
+{
+    if (!a || !count) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *count = 0;
+    if (!a->ranEstimate || !a->ranLearned) return HWPA_OK;
+    HWPA_HIP(hipStreamSynchronize(a->stream));
+    HWPA_HIP(hipMemcpy(count, a->lws.total, sizeof(uint32_t), hipMemcpyDeviceToHost));
     return HWPA_OK;
 }
 

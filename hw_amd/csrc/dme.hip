@@ -248,14 +248,33 @@ __device__ float Fractile(const HistView& h, uint32_t lowOut, uint32_t total, fl
 
 __device__ __forceinline__ float Sum5Tree(const float* v) { return (v[0] + v[1]) + (v[2] + (v[3] + v[4])); }
 
-// DmeEmHost::EstimateLaneDetModel (DmeEmHost.cpp:216-615)
+enum { EM_PLAIN = 0, EM_LEARNED = 1 };
+
+// Extra inputs/outputs of the EM kernel beyond the reference's.
+struct EmExtra {
+    const float* init;        // EM_LEARNED: network outputs [lane*64 + z][HWPA_MLP_OUT]
+    uint8_t* fallback;        // EM_LEARNED: 1 = start rejected, model left for the plain EM
+    float* diag;              // optional raw estimate [lane][HWPA_EM_DIAG_ROWS][64]
+    const uint32_t* active;   // optional: number of valid ZMWs; lanes at or beyond it exit
+    float acceptConf;         // EM_LEARNED: minimum confidence to accept the capped EM
+    float convTol;            // EM_LEARNED: relative log-likelihood convergence tolerance
+    int32_t iterLimit;        // EM_LEARNED: iteration cap
+};
+
+// DmeEmHost::EstimateLaneDetModel (DmeEmHost.cpp:216-615).  EM_LEARNED replaces
+// the start model of the EM loop (after PrelimEstimate and the scale factor,
+// which still set the amplitude prior) by the network's prediction, caps the
+// iterations, uses its own log-likelihood convergence tolerance and accepts
+// the result only for converged ZMWs with status OK and enough confidence.
+template <int MODE>
 __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __restrict__ hists,
                                                         const hwpa_baseliner_stats* __restrict__ stats,
                                                         hwpa_lane_model* models, float* __restrict__ scratch,
-                                                        int32_t* iters, DmeDevCfg d, float evolve)
+                                                        int32_t* iters, DmeDevCfg d, float evolve, EmExtra x)
 {
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
+    if (x.active && (uint32_t)lane * LS >= *x.active) return;
     const hwpa_lane_hist& lh = hists[lane];
     const hwpa_baseliner_stats& bs = stats[lane];
     hwpa_lane_model& lm = models[lane];
@@ -363,9 +382,46 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
     float njSum = 0.0f;
     for (int b = 0; b < NB; ++b) njSum += (float)h.Cnt(b);
     const float halfBin = 0.5f * h.bs;
+    int iterLimit = d.iterLimit;
+    if (MODE == EM_LEARNED) {
+        // Network start: outputs in units of u = 2 binSize from the lower bound,
+        // log variance / 4 and softmax mixture logits.
+        const float* y = x.init + ((size_t)lane * LS + z) * HWPA_MLP_OUT;
+        const float u = 2.0f * h.bs;
+        const float su = 32.0f * u;
+        float mx = y[6];
+#pragma unroll
+        for (int k = 1; k < 5; ++k) mx = hw_maxf(mx, y[6 + k]);
+        float e[5], se = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            e[k] = hw_expf(y[6 + k] - mx);
+            se += e[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rho[k] = hw_maxf(e[k] / se, rhoMin);
+        const float rs = Sum5Tree(rho);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rho[k] = rho[k] / rs;
+        mu[0] = h.lb + su * y[0];
+        var[0] = hw_clampf((u * u) * hw_expf(4.0f * y[1]), d.blVarMin, d.blVarMax);
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            mu[a + 1] = h.lb + su * y[2 + a];
+            var[a + 1] = Covar(d, d.cv[a] * d.cv[a], mu[a + 1], var[0]);
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            rhoEst[k] = rho[k];
+            muEst[k] = mu[k];
+            varEst[k] = var[k];
+        }
+        s = ((mu[1] * rpa[0] + mu[2] * rpa[1]) + (mu[3] * rpa[2] + mu[4] * rpa[3])) / rpaSqSum;
+        iterLimit = x.iterLimit;
+    }
 
     int it = 0;
-    for (; it < d.iterLimit; ++it) {
+    for (; it < iterLimit; ++it) {
         // ---- E-step
         float varinv[5], pref[5];
 #pragma unroll
@@ -431,7 +487,7 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
         const float ds = s - sExpect;
         logLike -= 0.5f * (ds * ds) * sExpectWeight;
         const float delta = logLike - logLikePrev;
-        const float convTol = 1e-4f * hw_fabsf(logLike);
+        const float convTol = (MODE == EM_LEARNED ? x.convTol : 1e-4f) * hw_fabsf(logLike);
         bool conv = (delta >= 0.0f) && (delta < convTol);
         conv = conv && (logLike >= mlLogLike);
         if (conv) {
@@ -550,9 +606,160 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
     }
     conf = hw_minf(hw_maxf(0.0f, conf), 1.0f);
     conf = (conf >= d.successThresh) ? conf : 0.0f;
+    if (x.diag) {
+        float* dg = x.diag + (size_t)lane * HWPA_EM_DIAG_ROWS * LS + z;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            dg[k * LS] = rhoEst[k];
+            dg[(5 + k) * LS] = muEst[k];
+            dg[(10 + k) * LS] = varEst[k];
+        }
+        dg[15 * LS] = conf;
+        dg[16 * LS] = converged ? 1.0f : 0.0f;
+        dg[17 * LS] = (float)status;
+    }
+    if (MODE == EM_LEARNED) {
+        const bool accept = converged && status == HWPA_ZS_OK && conf >= x.acceptConf;
+        x.fallback[(size_t)lane * LS + z] = accept ? 0 : 1;
+        if (!accept) return;
+    }
     work.conf = conf;
     UpdateConf(d, det, work);
     StoreModel(det, lm, z);
+}
+
+// ---- fallback compaction for the learned start ---------------------------------
+// ZMWs whose capped EM was rejected are packed densely (in lane order, so the
+// grouping is deterministic) into compact lanes, run through the plain EM and
+// scattered back.
+
+__device__ __forceinline__ void ColStatAcc(const hwpa_stat_acc& s, hwpa_stat_acc& d, int sz, int dz)
+{
+    d.offset[dz] = s.offset[sz];
+    d.m0[dz] = s.m0[sz];
+    d.m1[dz] = s.m1[sz];
+    d.m2[dz] = s.m2[sz];
+}
+
+__device__ void ColStats(const hwpa_baseliner_stats& s, hwpa_baseliner_stats& d, int sz, int dz)
+{
+    ColStatAcc(s.full.basic, d.full.basic, sz, dz);
+    d.full.lag_m2[dz] = s.full.lag_m2[sz];
+    for (int k = 0; k < HWPA_AUTOCORR_LAG; ++k) {
+        d.full.fbuf[k][dz] = s.full.fbuf[k][sz];
+        d.full.bbuf[k][dz] = s.full.bbuf[k][sz];
+    }
+    d.full.bidx[0][dz] = s.full.bidx[0][sz];
+    d.full.bidx[1][dz] = s.full.bidx[1][sz];
+    d.trace_min[dz] = s.trace_min[sz];
+    d.trace_max[dz] = s.trace_max[sz];
+    ColStatAcc(s.baseline, d.baseline, sz, dz);
+    d.raw_baseline_sum[dz] = s.raw_baseline_sum[sz];
+    ColStatAcc(s.background, d.background, sz, dz);
+}
+
+__device__ void ColHist(const hwpa_lane_hist& s, hwpa_lane_hist& d, int sz, int dz)
+{
+    d.low_bound[dz] = s.low_bound[sz];
+    d.bin_size[dz] = s.bin_size[sz];
+    d.outlier_low[dz] = s.outlier_low[sz];
+    d.outlier_high[dz] = s.outlier_high[sz];
+    for (int b = 0; b < NB; ++b) d.bin_count[b][dz] = s.bin_count[b][sz];
+}
+
+__device__ void ColModel(const hwpa_lane_model& s, hwpa_lane_model& d, int sz, int dz)
+{
+    for (int a = 0; a < NA; ++a) {
+        d.analogs[a].means[dz] = s.analogs[a].means[sz];
+        d.analogs[a].vars[dz] = s.analogs[a].vars[sz];
+        d.analogs[a].weights[dz] = s.analogs[a].weights[sz];
+    }
+    d.baseline.means[dz] = s.baseline.means[sz];
+    d.baseline.vars[dz] = s.baseline.vars[sz];
+    d.baseline.weights[dz] = s.baseline.weights[sz];
+    d.confidence[dz] = s.confidence[sz];
+}
+
+__global__ __launch_bounds__(64) void FallbackCountKernel(const uint8_t* __restrict__ fb, uint32_t* __restrict__ cnt)
+{
+    const uint64_t m = __ballot(fb[(size_t)blockIdx.x * LS + threadIdx.x] != 0);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = (uint32_t)__popcll(m);
+}
+
+// Exclusive scan of per-lane counts in one workgroup (fixed order).
+__global__ __launch_bounds__(1024) void FallbackScanKernel(const uint32_t* __restrict__ cnt, uint32_t lanes,
+                                                           uint32_t* __restrict__ off, uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (lanes + 1023) / 1024;
+    const uint32_t b0 = threadIdx.x * per, b1 = min(lanes, b0 + per);
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < b1; ++i) s += cnt[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < 1024; ++i) {
+            const uint32_t v = part[i];
+            part[i] = run;
+            run += v;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x];
+    for (uint32_t i = b0; i < b1; ++i) {
+        off[i] = run;
+        run += cnt[i];
+    }
+}
+
+__global__ __launch_bounds__(64) void FallbackGatherKernel(const uint8_t* __restrict__ fb, const uint32_t* __restrict__ off,
+                                                           const hwpa_lane_hist* __restrict__ hist,
+                                                           const hwpa_baseliner_stats* __restrict__ stats,
+                                                           const hwpa_lane_model* __restrict__ models,
+                                                           hwpa_lane_hist* cHist, hwpa_baseliner_stats* cStats,
+                                                           hwpa_lane_model* cModels, uint32_t* __restrict__ map)
+{
+    const int lane = blockIdx.x, z = threadIdx.x;
+    const bool f = fb[(size_t)lane * LS + z] != 0;
+    const uint64_t m = __ballot(f);
+    if (!f) return;
+    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << z) - 1ull));
+    const uint32_t pos = off[lane] + rank;
+    const uint32_t cl = pos / LS;
+    const int dz = (int)(pos % LS);
+    ColHist(hist[lane], cHist[cl], z, dz);
+    ColStats(stats[lane], cStats[cl], z, dz);
+    ColModel(models[lane], cModels[cl], z, dz);
+    map[pos] = (uint32_t)lane * LS + (uint32_t)z;
+}
+
+// Fills the unused slots of the last, partly filled compact lane with copies of
+// its first ZMW so that its EM behaves like a full lane.
+__global__ __launch_bounds__(64) void FallbackPadKernel(const uint32_t* __restrict__ total, hwpa_lane_hist* cHist,
+                                                        hwpa_baseliner_stats* cStats, hwpa_lane_model* cModels)
+{
+    const uint32_t t = *total;
+    const uint32_t used = t % LS;
+    if (used == 0) return;
+    const uint32_t cl = t / LS;
+    const int z = threadIdx.x;
+    if ((uint32_t)z < used) return;
+    ColHist(cHist[cl], cHist[cl], 0, z);
+    ColStats(cStats[cl], cStats[cl], 0, z);
+    ColModel(cModels[cl], cModels[cl], 0, z);
+}
+
+__global__ __launch_bounds__(64) void FallbackScatterKernel(const uint32_t* __restrict__ total,
+                                                            const uint32_t* __restrict__ map,
+                                                            const hwpa_lane_model* __restrict__ cModels,
+                                                            hwpa_lane_model* models)
+{
+    const uint32_t pos = blockIdx.x * LS + threadIdx.x;
+    if (pos >= *total) return;
+    const uint32_t dst = map[pos];
+    ColModel(cModels[blockIdx.x], models[dst / LS], (int)threadIdx.x, (int)(dst % LS));
 }
 
 size_t DmeScratchBytes(uint32_t lanes) { return (size_t)lanes * (NB + 1) * 2 * LS * sizeof(float); }
@@ -565,19 +772,24 @@ int32_t DmeInitModels(const hwpa_config& cfg, const hwpa_baseliner_stats* stats,
     return CheckLaunch("DmeInitKernel");
 }
 
-int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
-                    hwpa_lane_model* models, const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
-                    int32_t* d_iters, float* scratch, hipStream_t s)
+// DmeEmHost::EvolveModel (DmeEmHost.cpp:806-834), DetectionModelHost::EvolveConfidence
+static float EvolveFactor(const DmeDevCfg& d, const int32_t model_fi[2], const int32_t est_fi[2])
 {
-    if (!hist || !stats || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "dme_estimate: bad argument");
-    const DmeDevCfg d = MakeDevCfg(cfg);
-    // DmeEmHost::EvolveModel (DmeEmHost.cpp:806-834), DetectionModelHost::EvolveConfidence
     const int32_t tModel = model_fi[0] + (int32_t)((uint32_t)(model_fi[1] - model_fi[0]) / 2u);
     const int32_t tEst = est_fi[0] + (int32_t)((uint32_t)(est_fi[1] - est_fi[0]) / 2u);
     const float thl = 0.5f * (float)(tModel + tEst);
     const float hl = d.hl0 + (d.hl1 - d.hl0) * hw_satlin(56160.0f, 168480.0f, thl);
     const float tDiff = (float)(tEst - tModel);
-    const float evolve = hw_exp2f(-hw_fabsf(tDiff) / hl);
+    return hw_exp2f(-hw_fabsf(tDiff) / hl);
+}
+
+int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
+                    hwpa_lane_model* models, const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
+                    int32_t* d_iters, float* scratch, hipStream_t s, float* diag)
+{
+    if (!hist || !stats || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "dme_estimate: bad argument");
+    const DmeDevCfg d = MakeDevCfg(cfg);
+    const float evolve = EvolveFactor(d, model_fi, est_fi);
     float* tmp = nullptr;
     if (!scratch) {
         // Stand-alone call: private scratch for this call only.
@@ -585,14 +797,97 @@ int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hw
         if (rc != HWPA_OK) return rc;
         scratch = tmp;
     }
-    hipLaunchKernelGGL(DmeEstimateKernel, dim3(lanes), dim3(LS), 0, s, hist, stats, models, scratch, d_iters, d,
-                       evolve);
+    EmExtra x{};
+    x.diag = diag;
+    hipLaunchKernelGGL(DmeEstimateKernel<EM_PLAIN>, dim3(lanes), dim3(LS), 0, s, hist, stats, models, scratch, d_iters,
+                       d, evolve, x);
     int32_t rc = CheckLaunch("DmeEstimateKernel");
     if (tmp) {
         if (hipStreamSynchronize(s) != hipSuccess && rc == HWPA_OK) rc = Fail(HWPA_ERR_HIP, "EM kernel failed");
         DeviceFree(tmp);
     }
     return rc;
+}
+
+void LearnedWsFree(LearnedWs& w)
+{
+    DeviceFree(w.x);
+    DeviceFree(w.y);
+    DeviceFree(w.fallback);
+    DeviceFree(w.cnt);
+    DeviceFree(w.off);
+    DeviceFree(w.total);
+    DeviceFree(w.map);
+    DeviceFree(w.cHist);
+    DeviceFree(w.cStats);
+    DeviceFree(w.cModels);
+    DeviceFree(w.cIters);
+    w = LearnedWs{};
+}
+
+int32_t LearnedWsAlloc(LearnedWs& w, uint32_t lanes)
+{
+    LearnedWsFree(w);
+    const size_t Z = (size_t)lanes * LS;
+    struct {
+        void** p;
+        size_t n;
+    } req[] = {
+        {(void**)&w.x, Z * HWPA_MLP_IN * 2},      {(void**)&w.y, Z * HWPA_MLP_OUT * 4},
+        {(void**)&w.fallback, Z},                 {(void**)&w.cnt, (size_t)lanes * 4},
+        {(void**)&w.off, (size_t)lanes * 4},      {(void**)&w.total, 16},
+        {(void**)&w.map, Z * 4},                  {(void**)&w.cHist, (size_t)lanes * sizeof(hwpa_lane_hist)},
+        {(void**)&w.cStats, (size_t)lanes * sizeof(hwpa_baseliner_stats)},
+        {(void**)&w.cModels, (size_t)lanes * sizeof(hwpa_lane_model)}, {(void**)&w.cIters, (size_t)lanes * 4},
+    };
+    for (auto& r : req) {
+        const int32_t rc = DeviceAlloc(r.p, r.n, "learned-start workspace");
+        if (rc != HWPA_OK) {
+            LearnedWsFree(w);
+            return rc;
+        }
+    }
+    w.lanes = lanes;
+    return HWPA_OK;
+}
+
+int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, LearnedWs& w, const hwpa_lane_hist* hist,
+                           const hwpa_baseliner_stats* stats, hwpa_lane_model* models, const int32_t model_fi[2],
+                           const int32_t est_fi[2], uint32_t lanes, int32_t* d_iters, float* scratch, hipStream_t s)
+{
+    if (!mlp || !hist || !stats || !models || !scratch || lanes == 0 || lanes > w.lanes)
+        return Fail(HWPA_ERR_INVALID_ARG, "dme_estimate_learned: bad argument");
+    if (cfg.learned_em_iteration_limit == 0) return Fail(HWPA_ERR_BAD_CONFIG, "learned EM iteration limit must be positive");
+    const DmeDevCfg d = MakeDevCfg(cfg);
+    const float evolve = EvolveFactor(d, model_fi, est_fi);
+    int32_t rc = MlpFeatures(hist, lanes, w.x, s);
+    if (rc == HWPA_OK) rc = MlpForward(mlp, w.x, lanes * LS, w.y, s);
+    if (rc != HWPA_OK) return rc;
+    EmExtra x{};
+    x.init = w.y;
+    x.fallback = w.fallback;
+    x.acceptConf = cfg.learned_conf_threshold;
+    x.convTol = cfg.learned_conv_tol;
+    x.iterLimit = (int32_t)cfg.learned_em_iteration_limit;
+    hipLaunchKernelGGL(DmeEstimateKernel<EM_LEARNED>, dim3(lanes), dim3(LS), 0, s, hist, stats, models, scratch,
+                       d_iters, d, evolve, x);
+    if ((rc = CheckLaunch("DmeEstimateKernel")) != HWPA_OK) return rc;
+    hipLaunchKernelGGL(FallbackCountKernel, dim3(lanes), dim3(LS), 0, s, w.fallback, w.cnt);
+    if ((rc = CheckLaunch("FallbackCountKernel")) != HWPA_OK) return rc;
+    hipLaunchKernelGGL(FallbackScanKernel, dim3(1), dim3(1024), 0, s, w.cnt, lanes, w.off, w.total);
+    if ((rc = CheckLaunch("FallbackScanKernel")) != HWPA_OK) return rc;
+    hipLaunchKernelGGL(FallbackGatherKernel, dim3(lanes), dim3(LS), 0, s, w.fallback, w.off, hist, stats, models,
+                       w.cHist, w.cStats, w.cModels, w.map);
+    if ((rc = CheckLaunch("FallbackGatherKernel")) != HWPA_OK) return rc;
+    hipLaunchKernelGGL(FallbackPadKernel, dim3(1), dim3(LS), 0, s, w.total, w.cHist, w.cStats, w.cModels);
+    if ((rc = CheckLaunch("FallbackPadKernel")) != HWPA_OK) return rc;
+    EmExtra p{};
+    p.active = w.total;
+    hipLaunchKernelGGL(DmeEstimateKernel<EM_PLAIN>, dim3(lanes), dim3(LS), 0, s, w.cHist, w.cStats, w.cModels, scratch,
+                       w.cIters, d, evolve, p);
+    if ((rc = CheckLaunch("DmeEstimateKernel")) != HWPA_OK) return rc;
+    hipLaunchKernelGGL(FallbackScatterKernel, dim3(lanes), dim3(LS), 0, s, w.total, w.map, w.cModels, models);
+    return CheckLaunch("FallbackScatterKernel");
 }
 
 }  // namespace hwpa

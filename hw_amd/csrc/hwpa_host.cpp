@@ -286,8 +286,9 @@ int32_t hwpa_config_default(hwpa_config* c)
     c->xsp_amp_thresh = 0.70;
     c->xsp_width_thresh = 3.5f;
     c->learned_init = 0;
-    c->learned_conf_threshold = 0.5f;
+    c->learned_conf_threshold = 0.0f;
     c->learned_em_iteration_limit = 3;
+    c->learned_conv_tol = 1e-3f;
     // Movie description of the synthetic workload: 100 fps, 1 e-/DN, four analogs
     // with amplitudes 180/120/80/50 over a 12-count baseline sigma.
     c->movie.frame_rate = 100.0f;
@@ -350,6 +351,7 @@ int32_t hwpa_config_load_json(hwpa_config* c, const char* json)
         {"frameLabelerConfig.roi.lowerThreshold", &c->roi_lower_threshold},
         {"pulseAccumConfig.XspWidthThresh", &c->xsp_width_thresh},
         {"learnedInit.ConfidenceThreshold", &c->learned_conf_threshold},
+        {"learnedInit.ConvergenceTolerance", &c->learned_conv_tol},
         {"movie.frameRate", &c->movie.frame_rate},
         {"movie.photoelectronSensitivity", &c->movie.photoelectron_sensitivity},
         {"movie.refSnr", &c->movie.ref_snr},
@@ -449,6 +451,7 @@ int32_t hwpa_config_validate(const hwpa_config* c)
     // Constants must fit half precision (CoreDMEstimator.h:99-106)
     if (!(MakeDmeConsts(*c).blVarMax < 6.5e4f)) return Fail(HWPA_ERR_BAD_CONFIG, "BaselineVarianceMax exceeds half range");
     if (c->learned_em_iteration_limit == 0 && c->learned_init) return Fail(HWPA_ERR_BAD_CONFIG, "learned EM iteration limit must be positive");
+    if (!(c->learned_conv_tol > 0.0f)) return Fail(HWPA_ERR_BAD_CONFIG, "learned EM convergence tolerance must be positive");
     return HWPA_OK;
 }
 
@@ -552,6 +555,39 @@ int32_t hwpa_dme_estimate(const hwpa_config* cfg, const hwpa_lane_hist* d_hist, 
     return DmeEstimate(*cfg, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, nullptr, AsStream(stream));
 }
 
+int32_t hwpa_dme_estimate_diag(const hwpa_config* cfg, const hwpa_lane_hist* d_hist,
+                               const hwpa_baseliner_stats* d_stats, hwpa_lane_model* d_models,
+                               const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
+                               int32_t* d_iters, float* d_diag, void* stream)
+{
+    if (!cfg || !model_fi || !est_fi || !d_diag) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    return DmeEstimate(*cfg, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, nullptr, AsStream(stream),
+                       d_diag);
+}
+
+int32_t hwpa_dme_estimate_learned(const hwpa_config* cfg, hwpa_initializer* m, const hwpa_lane_hist* d_hist,
+                                  const hwpa_baseliner_stats* d_stats, hwpa_lane_model* d_models,
+                                  const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
+                                  int32_t* d_iters, uint32_t* fallback_count, void* stream)
+{
+    if (!cfg || !m || !model_fi || !est_fi || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "null argument or no lanes");
+    hipStream_t s = AsStream(stream);
+    LearnedWs w;
+    float* scratch = nullptr;
+    int32_t rc = LearnedWsAlloc(w, lanes);
+    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&scratch, DmeScratchBytes(lanes), "EM scratch");
+    if (rc == HWPA_OK)
+        rc = DmeEstimateLearned(*cfg, m, w, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, scratch, s);
+    if (hipStreamSynchronize(s) != hipSuccess && rc == HWPA_OK) rc = Fail(HWPA_ERR_HIP, "learned EM failed");
+    if (rc == HWPA_OK && fallback_count) {
+        if (hipMemcpy(fallback_count, w.total, 4, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = Fail(HWPA_ERR_HIP, "fallback count copy failed");
+    }
+    DeviceFree(scratch);
+    LearnedWsFree(w);
+    return rc;
+}
+
 struct hwpa_labeler {
     LabelerDev* dev;
 };
@@ -653,6 +689,46 @@ int32_t hwpa_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes)
     if (!d_dst || !d_src) return Fail(HWPA_ERR_INVALID_ARG, "memcpy: null pointer");
     HWPA_HIP(hipMemcpy(d_dst, d_src, bytes, hipMemcpyDeviceToDevice));
     return HWPA_OK;
+}
+
+// Plain streaming copy, 16 bytes per thread per step: the reference point for
+// achievable HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void CopyKernel(uint4* __restrict__ d, const uint4* __restrict__ s, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) d[i] = s[i];
+}
+
+int32_t hwpa_copy_bandwidth(void* d_dst, const void* d_src, size_t bytes, int32_t launches, float* ms_per_launch)
+{
+    if (!d_dst || !d_src || !ms_per_launch || launches <= 0)
+        return Fail(HWPA_ERR_INVALID_ARG, "copy: null pointer or no launches");
+    if ((bytes % 16) != 0 || ((uintptr_t)d_dst % 16) != 0 || ((uintptr_t)d_src % 16) != 0)
+        return Fail(HWPA_ERR_INVALID_ARG, "copy: pointers and size must be 16-byte multiples");
+    const size_t n = bytes / 16;
+    if (n == 0) return Fail(HWPA_ERR_INVALID_ARG, "copy: empty");
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    hipEvent_t e0, e1;
+    HWPA_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return Fail(HWPA_ERR_HIP, "hipEventCreate failed");
+    }
+    int32_t rc = HWPA_OK;
+    (void)hipEventRecord(e0, nullptr);
+    for (int32_t i = 0; i < launches && rc == HWPA_OK; ++i) {
+        hipLaunchKernelGGL(CopyKernel, dim3(blocks), dim3(256), 0, nullptr, (uint4*)d_dst, (const uint4*)d_src, n);
+        rc = CheckLaunch("CopyKernel");
+    }
+    (void)hipEventRecord(e1, nullptr);
+    float ms = 0.0f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        if (rc == HWPA_OK) rc = Fail(HWPA_ERR_HIP, "copy kernel failed");
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_launch = ms / (float)launches;
+    return rc;
 }
 
 int32_t hwpa_memcpy_dtoh(void* host_dst, const void* d_src, size_t bytes)

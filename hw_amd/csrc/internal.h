@@ -75,9 +75,35 @@ int32_t HistAdd(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_
 int32_t DmeInitModels(const hwpa_config& cfg, const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
                       uint32_t lanes, hipStream_t s);
 int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
-                    hwpa_lane_model* models, const int32_t model_fi[2], const int32_t est_fi[2],
-                    uint32_t lanes, int32_t* d_iters, float* scratch, hipStream_t s);
+                    hwpa_lane_model* models, const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
+                    int32_t* d_iters, float* scratch, hipStream_t s, float* diag = nullptr);
 size_t DmeScratchBytes(uint32_t lanes);
+
+// Learned start of the EM: network input/output and fallback compaction buffers
+// for up to `lanes` lanes.
+struct LearnedWs {
+    uint32_t lanes = 0;
+    uint16_t* x = nullptr;            // [Z][128] bf16 network input
+    float* y = nullptr;               // [Z][11] network output
+    uint8_t* fallback = nullptr;      // [Z]
+    uint32_t* cnt = nullptr;          // [lanes] fallback ZMWs per lane
+    uint32_t* off = nullptr;          // [lanes] exclusive prefix
+    uint32_t* total = nullptr;        // number of fallback ZMWs
+    uint32_t* map = nullptr;          // compact slot -> lane * 64 + zmw
+    hwpa_lane_hist* cHist = nullptr;  // compacted fallback inputs
+    hwpa_baseliner_stats* cStats = nullptr;
+    hwpa_lane_model* cModels = nullptr;
+    int32_t* cIters = nullptr;
+};
+int32_t LearnedWsAlloc(LearnedWs& w, uint32_t lanes);
+void LearnedWsFree(LearnedWs& w);
+int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, LearnedWs& w, const hwpa_lane_hist* hist,
+                           const hwpa_baseliner_stats* stats, hwpa_lane_model* models, const int32_t model_fi[2],
+                           const int32_t est_fi[2], uint32_t lanes, int32_t* d_iters, float* scratch, hipStream_t s);
+int32_t MlpFeatures(const hwpa_lane_hist* hist, uint32_t lanes, uint16_t* x, hipStream_t s);
+int32_t MlpForward(hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, hipStream_t s);
+int32_t MlpTargets(const hwpa_lane_hist* hist, const float* diag, uint32_t lanes, float minConf, float* t,
+                   uint8_t* valid, hipStream_t s);
 
 struct LabelerDev;
 int32_t LabelerCreate(const hwpa_config& cfg, uint32_t lanes, uint32_t max_frames, LabelerDev** out);

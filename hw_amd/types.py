@@ -58,7 +58,7 @@ class Config(ctypes.Structure):
         ("max_calls_per_zmw", ctypes.c_uint32), ("xsp_amp_thresh", ctypes.c_double),
         ("xsp_width_thresh", ctypes.c_float), ("learned_init", ctypes.c_int32),
         ("learned_conf_threshold", ctypes.c_float), ("learned_em_iteration_limit", ctypes.c_uint32),
-        ("pad", ctypes.c_int32), ("movie", MovieInfo)]
+        ("learned_conv_tol", ctypes.c_float), ("movie", MovieInfo)]
 
 
 class FilterParams(ctypes.Structure):
@@ -77,7 +77,8 @@ class TracegenParams(ctypes.Structure):
                 ("drift_per_chunk", ctypes.c_float), ("drift_chunk", ctypes.c_float), ("step_drift", ctypes.c_float),
                 ("step_span", ctypes.c_float), ("empty_fraction", ctypes.c_float),
                 ("amplitude", ctypes.c_float * NUM_ANALOGS), ("amplitude_cv", ctypes.c_float),
-                ("mean_pulse_width", ctypes.c_float), ("mean_ipd", ctypes.c_float)]
+                ("mean_pulse_width", ctypes.c_float), ("mean_ipd", ctypes.c_float), ("snr_lo", ctypes.c_float),
+                ("snr_hi", ctypes.c_float)]
 
 
 def _f(n):

@@ -131,6 +131,17 @@ int32_t hwpa_dme_estimate(const hwpa_config* cfg, const hwpa_lane_hist* d_hist,
                           const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
                           int32_t* d_iters, void* stream);
 
+/* As hwpa_dme_estimate, and also writes each ZMW's raw EM result (before the
+ * confidence-weighted update) to d_diag, fp32 [lanes][HWPA_EM_DIAG_ROWS][64]:
+ * rows 0-4 mixture weights, 5-9 means, 10-14 variances (baseline first),
+ * 15 confidence, 16 converged (0/1), 17 status bits.  No reference
+ * counterpart (the reference's DmeDiagnostics are not exported per ZMW);
+ * these are the training targets of the learned initialiser. */
+int32_t hwpa_dme_estimate_diag(const hwpa_config* cfg, const hwpa_lane_hist* d_hist,
+                               const hwpa_baseliner_stats* d_stats, hwpa_lane_model* d_models,
+                               const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
+                               int32_t* d_iters, float* d_diag, void* stream);
+
 /* Frame labeler: FrameLabelerHost::Process (traceAnalysis/FrameLabelerHost.cpp:312-612).
  * d_latent_out receives the 24 trace frames preceding this chunk (the
  * LabelsBatch latent trace). */
@@ -152,6 +163,74 @@ int32_t hwpa_pulser_run(hwpa_pulser* p, const int16_t* d_labels, const int16_t* 
                         const hwpa_lane_model* d_models, hwpa_pulse* d_pulses, uint32_t* d_counts,
                         uint32_t* d_overflows, hwpa_stat_acc* d_bl_stats, void* stream);
 
+/* ---- learned initialiser of the EM (no reference counterpart) -----------
+ * A 128 -> 256 -> 256 -> 11 perceptron with ReLU hidden layers maps each
+ * ZMW's histogram (re-binned to 128 bins: bin i = bins 2i and 2i+1 of the
+ * 300, scaled by 16 / their total) to a start model for the EM: baseline
+ * mean, log variance / 4 and four analog means in units of u = 2 binSize from
+ * the histogram's lower bound (means scaled by 1/32), and five mixture-weight
+ * logits.  Matrices are bfloat16 for the matrix cores with fp32 accumulation;
+ * an fp32 master copy and Adam moments live on the device.  The start replaces
+ * the EM's (DmeEmHost.cpp:236-249 onwards); the EM then runs at most
+ * cfg->learned_em_iteration_limit iterations with the convergence test
+ * delta LL < cfg->learned_conv_tol * |LL| (the plain EM uses 1e-4), and ZMWs
+ * that did not converge (or whose model confidence is below
+ * cfg->learned_conf_threshold, default 0) are re-estimated by the plain EM. */
+#define HWPA_MLP_IN 128
+#define HWPA_MLP_HIDDEN 256
+#define HWPA_MLP_OUT 11
+/* W1[256][128] b1[256] W2[256][256] b2[256] W3[11][256] b3[11] */
+#define HWPA_MLP_PARAMS (HWPA_MLP_HIDDEN * HWPA_MLP_IN + HWPA_MLP_HIDDEN + HWPA_MLP_HIDDEN * HWPA_MLP_HIDDEN + \
+                         HWPA_MLP_HIDDEN + HWPA_MLP_OUT * HWPA_MLP_HIDDEN + HWPA_MLP_OUT)
+#define HWPA_EM_DIAG_ROWS 18
+
+typedef struct hwpa_initializer hwpa_initializer;
+/* He-uniform weights from a 64-bit seed, zero biases. */
+int32_t hwpa_initializer_create(uint64_t seed, hwpa_initializer** out);
+int32_t hwpa_initializer_destroy(hwpa_initializer* m);
+/* fp32 parameters in the order above (n must equal HWPA_MLP_PARAMS);
+ * setting them also resets the optimiser state. */
+int32_t hwpa_initializer_get_params(const hwpa_initializer* m, float* host_params, size_t n);
+int32_t hwpa_initializer_set_params(hwpa_initializer* m, const float* host_params, size_t n);
+/* Network input from lane histograms: bf16 [lanes*64][128] (16-byte aligned). */
+int32_t hwpa_mlp_features(const hwpa_lane_hist* d_hist, uint32_t lanes, uint16_t* d_x, void* stream);
+/* Training targets from hwpa_dme_estimate_diag output: fp32 [lanes*64][11] in
+ * the output units above (weights as probabilities) and d_valid u8 [lanes*64]
+ * = converged, status OK, confidence >= min_confidence. */
+int32_t hwpa_mlp_targets(const hwpa_lane_hist* d_hist, const float* d_diag, uint32_t lanes,
+                         float min_confidence, float* d_t, uint8_t* d_valid, void* stream);
+/* Network outputs fp32 [rows][11]. */
+int32_t hwpa_mlp_forward(hwpa_initializer* m, const uint16_t* d_x, uint32_t rows, float* d_y, void* stream);
+/* Mean loss over a batch (squared error on outputs 0-5, softmax cross-entropy
+ * on 6-10) and its gradient, fp32 [HWPA_MLP_PARAMS] in parameter order.
+ * Samples with d_valid[i] == 0 (d_valid may be NULL) contribute nothing; the
+ * mean is still taken over the whole batch. */
+int32_t hwpa_mlp_gradients(hwpa_initializer* m, const uint16_t* d_x, const float* d_t, const uint8_t* d_valid,
+                           uint32_t batch, float* d_grads, float* d_loss, void* stream);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on a batch; d_loss (optional)
+ * receives the batch loss before the step. */
+int32_t hwpa_mlp_train_step(hwpa_initializer* m, const uint16_t* d_x, const float* d_t, const uint8_t* d_valid,
+                            uint32_t batch, float lr, float* d_loss, void* stream);
+/* Learned-start estimation of one pool: *fallback_count (host, optional)
+ * receives the number of ZMWs re-estimated by the plain EM.  Synchronous. */
+int32_t hwpa_dme_estimate_learned(const hwpa_config* cfg, hwpa_initializer* m, const hwpa_lane_hist* d_hist,
+                                  const hwpa_baseliner_stats* d_stats, hwpa_lane_model* d_models,
+                                  const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
+                                  int32_t* d_iters, uint32_t* fallback_count, void* stream);
+/* Attaches an initialiser to an analyzer (NULL detaches); with
+ * cfg->learned_init set, its estimations use the learned start.  The
+ * initialiser must outlive the analyzer or be detached first. */
+int32_t hwpa_analyzer_set_initializer(hwpa_analyzer* a, hwpa_initializer* m);
+/* Fallback count of the last learned estimation (synchronises the stream). */
+int32_t hwpa_analyzer_fallback_count(const hwpa_analyzer* a, uint32_t* count);
+/* Arms training-pair capture: every later plain estimation also writes the
+ * network input of each ZMW (d_x, bf16 [lanes*64][128], 16-byte aligned),
+ * its target from the EM result (d_t, fp32 [lanes*64][11]) and d_valid
+ * (u8 [lanes*64], see hwpa_mlp_targets with min_confidence), overwriting the
+ * previous capture.  NULL d_x disarms. */
+int32_t hwpa_analyzer_capture_training(hwpa_analyzer* a, uint16_t* d_x, float* d_t, uint8_t* d_valid,
+                                       float min_confidence);
+
 /* Device memory helpers for callers without their own allocator (tests,
  * benchmarks, language bindings).  Copies are synchronous. */
 int32_t hwpa_device_count(int32_t* count);
@@ -161,6 +240,12 @@ int32_t hwpa_dev_free(void* d_ptr);
 int32_t hwpa_memcpy_htod(void* d_dst, const void* host_src, size_t bytes);
 int32_t hwpa_memcpy_dtoh(void* host_dst, const void* d_src, size_t bytes);
 int32_t hwpa_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+/* Runs a plain 16-byte-per-thread copy kernel `launches` times on the null
+ * stream and returns the mean duration per launch (HIP events).  No reference
+ * counterpart: the achievable-bandwidth yardstick of the roofline figures.
+ * bytes and both pointers must be multiples of 16. */
+int32_t hwpa_copy_bandwidth(void* d_dst, const void* d_src, size_t bytes, int32_t launches,
+                            float* ms_per_launch);
 
 /* ---- synthetic input (no reference counterpart: bench/test data) ------- */
 int32_t hwpa_tracegen_default(hwpa_tracegen_params* p);

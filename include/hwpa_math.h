@@ -244,6 +244,16 @@ HWPA_HD float hw_h2f(uint16_t h)
     return sign ? -v : v;
 }
 
+/* bfloat16 <-> binary32: round to nearest even on the top 16 bits, NaN kept
+ * quiet.  Used for the initialiser network's weights and activations. */
+HWPA_HD uint16_t hw_f2bf(float f)
+{
+    const uint32_t u = hw_f2u(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+HWPA_HD float hw_bf2f(uint16_t b) { return hw_u2f((uint32_t)b << 16); }
+
 /* Round a float through binary16 and back. */
 HWPA_HD float hw_half_round(float f) { return hw_h2f(hw_f2h(f)); }
 

@@ -9,7 +9,8 @@
  * Model per frame t of ZMW z:
  *   value = mu + slope * t / drift_chunk + step(t) + sigma * N(0,1)
  *           + [in pulse] amp[a] * (1 + cv * N(0,1))
- * rounded to nearest-even and saturated to int16.  The pulse process is a
+ * rounded to nearest-even and saturated to int16.  Optionally all amplitudes
+ * of a ZMW are scaled by a per-ZMW factor drawn from a signal-to-noise range.  The pulse process is a
  * two-state renewal chain with per-frame exits, so gaps and widths are
  * geometric with means mean_ipd and mean_pulse_width; the analog a is drawn
  * uniformly at each pulse start.  Normals use Marsaglia's polar method.
@@ -60,7 +61,17 @@ HWPA_HD float hw_normal(uint64_t seed, uint64_t zmw, uint64_t ctr, uint32_t stre
 }
 
 enum { HW_STREAM_NOISE = 1, HW_STREAM_EXIT = 2, HW_STREAM_ANALOG = 3, HW_STREAM_AMP = 4,
-       HW_STREAM_SLOPE = 5, HW_STREAM_STEP = 6, HW_STREAM_EMPTY = 7 };
+       HW_STREAM_SLOPE = 5, HW_STREAM_STEP = 6, HW_STREAM_EMPTY = 7, HW_STREAM_SNR = 8 };
+
+/* Amplitude scale of ZMW z: 1, or snr * sigma / amplitude[0] with
+ * snr ~ U[snr_lo, snr_hi] when snr_hi > 0. */
+HWPA_HD float hw_tracegen_amp_scale(const hwpa_tracegen_params* p, uint64_t z)
+{
+    if (!(p->snr_hi > 0.0f)) return 1.0f;
+    const float u = hw_uniform(hw_hash(p->seed, z, 0, HW_STREAM_SNR));
+    const float snr = p->snr_lo + (p->snr_hi - p->snr_lo) * u;
+    return (snr * p->baseline_sigma) / p->amplitude[0];
+}
 
 /* Initialise the carried state of ZMW z (slot i of its lane). */
 HWPA_HD void hw_tracegen_init_zmw(const hwpa_tracegen_params* p, uint64_t z,
@@ -86,7 +97,8 @@ HWPA_HD int16_t hw_tracegen_frame(const hwpa_tracegen_params* p, uint64_t z, uin
     v = v + p->baseline_sigma * hw_normal(p->seed, z, t, HW_STREAM_NOISE);
     const float ux = hw_uniform(hw_hash(p->seed, z, t, HW_STREAM_EXIT));
     if (st->in_pulse[i]) {
-        const float amp = p->amplitude[st->analog[i]];
+        float amp = p->amplitude[st->analog[i]];
+        if (p->snr_hi > 0.0f) amp = amp * hw_tracegen_amp_scale(p, z);
         v = v + amp * (1.0f + p->amplitude_cv * hw_normal(p->seed, z, t, HW_STREAM_AMP));
         if (ux * p->mean_pulse_width < 1.0f) st->in_pulse[i] = 0;
     } else if (!st->empty[i]) {

@@ -189,7 +189,7 @@ typedef struct hwpa_config {
     int32_t learned_init;
     float learned_conf_threshold;
     uint32_t learned_em_iteration_limit;
-    int32_t pad;
+    float learned_conv_tol;      /* relative log-likelihood tolerance of the capped EM */
     hwpa_movie_info movie;
 } hwpa_config;
 
@@ -228,6 +228,8 @@ typedef struct hwpa_tracegen_params {
     float amplitude_cv;          /* multiplicative per-frame noise */
     float mean_pulse_width;      /* frames, geometric */
     float mean_ipd;              /* frames, geometric */
+    float snr_lo, snr_hi;        /* if snr_hi > 0: per-ZMW amplitude scale so that
+                                    amplitude[0] / baseline_sigma ~ U[snr_lo, snr_hi] */
 } hwpa_tracegen_params;
 
 /* Per-ZMW generator state carried between chunks. */

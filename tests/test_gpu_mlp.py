@@ -1,0 +1,161 @@
+"""GPU checks of the learned initialiser's kernels against the CPU oracle
+(oracle/mlp.py), through the C-ABI.
+
+* Forward pass with integer-valued weights and inputs: every product and
+  partial sum is an exactly representable integer, so the result does not
+  depend on the accumulation order and must equal the oracle bit for bit.
+  This pins the matrix-core operand and accumulator lane layouts and the
+  bias / ReLU / bf16 epilogues.
+* Features from lane histograms: bit-exact.
+* Gradients on realistic data: relative Frobenius error per tensor below
+  2e-2 (the two sides accumulate in different orders, so an element near a
+  bfloat16 rounding boundary can land one bf16 step apart), loss within 1e-4.
+* Training with the GPU's own forward/backward/Adam lowers the loss above
+  its floor (the entropy of the target weights) tenfold.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _upload_bf16(H, x):
+    return H.dev_from_numpy(M.bf16_bits(np.asarray(x, dtype=np.float32)))
+
+
+def _int_params(rng):
+    d = {}
+    for name, shape in M.SIZES:
+        if name.startswith("W"):
+            d[name] = rng.integers(-1, 2, size=shape).astype(np.float32)
+        else:
+            d[name] = rng.integers(-2, 3, size=shape).astype(np.float32)
+    return M.pack(d).astype(np.float32)
+
+
+@pytest.mark.parametrize("rows", [64, 200, 1024])
+def test_forward_exact_integer_network(H, gpu, rows):
+    rng = np.random.default_rng(rows)
+    init = H.Initializer(seed=5)
+    p = _int_params(rng)
+    init.set_params(p)
+    assert np.array_equal(init.params(), p)
+    x = rng.integers(0, 2, size=(rows, M.IN)).astype(np.float32)
+    d_x = _upload_bf16(H, x)
+    d_y = H.dev_bytes(rows * M.OUT * 4)
+    init.forward(d_x, rows, d_y)
+    H.sync()
+    y = H.to_numpy(d_y, np.float32, (rows, M.OUT))
+    ref = M.forward(p, x)["y"]
+    assert np.abs(ref).max() > 100          # the products are not trivially small
+    assert np.array_equal(y, ref)
+
+
+def test_features_match_oracle(H, T, gpu):
+    rng = np.random.default_rng(2)
+    lanes = 3
+    hist = np.zeros(lanes, dtype=T.LANE_HIST)
+    hist["bin_count"] = rng.integers(0, 60, size=(lanes, 300, 64)).astype(np.uint16)
+    hist["bin_count"][1, :, 5] = 0            # an empty ZMW
+    hist["bin_count"][2, 200:, :] = 0
+    d_h = H.dev_from_numpy(hist)
+    d_x = H.dev_bytes(lanes * 64 * M.IN * 2)
+    H.mlp_features(d_h, lanes, d_x)
+    H.sync()
+    got = H.to_numpy(d_x, np.uint16, (lanes * 64, M.IN))
+    counts = hist["bin_count"].transpose(0, 2, 1).reshape(lanes * 64, 300)
+    want = M.bf16_bits(M.features(counts))
+    assert np.array_equal(got, want)
+
+
+def _realistic_batch(rng, B):
+    # histogram-like non-negative inputs and plausible targets
+    x = np.zeros((B, M.IN), dtype=np.float32)
+    centers = rng.uniform(5, 15, B)
+    for i in range(B):
+        k = np.arange(M.IN)
+        x[i] = 12 * np.exp(-0.5 * ((k - centers[i]) / 2.5) ** 2) / (2.5 * np.sqrt(2 * np.pi))
+        x[i] += rng.uniform(0, 0.3, M.IN) * (k < 60)
+    x = M.bf16(x)
+    t = np.concatenate([rng.normal(0.5, 0.3, size=(B, 6)), rng.dirichlet(np.ones(5) * 2, size=B)],
+                       axis=1).astype(np.float32)
+    return x, t
+
+
+@pytest.mark.parametrize("batch", [256, 200])
+def test_gradients_match_oracle(H, gpu, batch):
+    rng = np.random.default_rng(batch)
+    init = H.Initializer(seed=9)
+    p = init.params()
+    x, t = _realistic_batch(rng, batch)
+    valid = (rng.uniform(size=batch) > 0.1).astype(np.uint8)
+    d_x, d_t, d_v = _upload_bf16(H, x), H.dev_from_numpy(t), H.dev_from_numpy(valid)
+    d_g = H.dev_bytes(M.NPARAM * 4)
+    d_l = H.dev_bytes(4)
+    init.gradients(d_x, d_t, batch, d_g, d_l, d_valid=d_v)
+    H.sync()
+    g = H.to_numpy(d_g, np.float32, (M.NPARAM,))
+    loss = H.to_numpy(d_l, np.float32, (1,))[0]
+    # oracle: invalid samples contribute nothing, the mean stays over the batch
+    tm = t.copy()
+    L_ref, g_ref = M.gradients(p, x, tm)
+    f = M.forward(p, x)
+    per = M.loss_terms(f["y"], t) * valid
+    assert abs(loss - per.mean()) <= 1e-4 * abs(per.mean())
+    _, g_ref = _masked_gradients(p, x, t, valid)
+    o = 0
+    for name, shape in M.SIZES:
+        n = int(np.prod(shape))
+        a, b = g[o:o + n], g_ref[o:o + n]
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        assert err < 2e-2, (name, err)
+        o += n
+
+
+def _masked_gradients(p, x, t, valid):
+    """Oracle gradients with per-sample validity (dY = 0 for invalid samples)."""
+    f = M.forward(p, x)
+    y = f["y"]
+    B = y.shape[0]
+    dy = np.empty_like(y)
+    dy[:, :6] = 2 * (y[:, :6] - t[:, :6]) / B
+    e = np.exp(y[:, 6:] - y[:, 6:].max(axis=1, keepdims=True))
+    dy[:, 6:] = (e / e.sum(axis=1, keepdims=True) - t[:, 6:]) / B
+    dy *= valid[:, None]
+    dyb = M.bf16(dy.astype(np.float32))
+    g = {"W3": dyb.T @ f["h2"], "b3": dy.sum(axis=0)}
+    dh2 = M.bf16((dyb @ f["W3"]) * (f["h2"] > 0))
+    g["W2"] = dh2.T @ f["h1"]
+    g["b2"] = dh2.sum(axis=0)
+    dh1 = M.bf16((dh2 @ f["W2"]) * (f["h1"] > 0))
+    g["W1"] = dh1.T @ f["x"]
+    g["b1"] = dh1.sum(axis=0)
+    return None, M.pack(g)
+
+
+def test_training_lowers_loss(H, gpu):
+    """Adam on the GPU fits a teacher network's outputs."""
+    rng = np.random.default_rng(4)
+    B = 512
+    x, _ = _realistic_batch(rng, 4 * B)
+    teacher = M.forward(H.Initializer(seed=77).params(), x)["y"]
+    t = np.concatenate([teacher[:, :6] * 0.1, np.exp(teacher[:, 6:]) / np.exp(teacher[:, 6:]).sum(1, keepdims=True)],
+                       axis=1).astype(np.float32)
+    init = H.Initializer(seed=3)
+    d_x, d_t = _upload_bf16(H, x), H.dev_from_numpy(t)
+    d_l = H.dev_bytes(4)
+    losses = []
+    for step in range(300):
+        off = (step % 4) * B
+        init.train_step(d_x.view(off * M.IN * 2, B * M.IN * 2), d_t.view(off * M.OUT * 4, B * M.OUT * 4), B, 1e-3,
+                        d_loss=d_l)
+        if step % 50 == 0 or step == 299:
+            H.sync()
+            losses.append(float(H.to_numpy(d_l, np.float32, (1,))[0]))
+    assert np.all(np.isfinite(losses))
+    # the cross-entropy term cannot go below the entropy of the target weights
+    floor = float(np.mean(-(t[:, 6:] * np.log(t[:, 6:])).sum(axis=1)))
+    excess = [v - floor for v in losses]
+    assert excess[-1] < 0.1 * excess[0], (floor, losses)
