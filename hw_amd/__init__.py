@@ -150,6 +150,16 @@ class DevBuf:
             self.p = None
 
 
+class _Ptr:
+    """A borrowed device pointer."""
+
+    def __init__(self, p):
+        self.p = p
+
+    def data_ptr(self):
+        return self.p
+
+
 class DevView(DevBuf):
     """Window into a DevBuf; keeps the parent alive, frees nothing."""
 
@@ -182,6 +192,13 @@ def ptr(buf):
 
 def sync():
     check(lib.hwpa_device_sync())
+
+
+def count_equal_i16(a, b, n):
+    """Equal elements among the first n int16 values of two device buffers."""
+    out = ctypes.c_uint64(0)
+    check(lib.hwpa_count_equal_i16(ptr(a), ptr(b), ctypes.c_size_t(n), ctypes.byref(out)))
+    return out.value
 
 
 def copy_bandwidth(nbytes=1 << 30, launches=10):
@@ -417,6 +434,12 @@ class BatchAnalyzer:
         check(getter(self.h, ctypes.byref(p)))
         self.sync()
         return _dtoh(p.value, nbytes).view(dtype).reshape(shape)
+
+    def labels_device(self):
+        """Device pointer of the last labels (valid until the next call)."""
+        p = ctypes.c_void_p()
+        check(lib.hwpa_result_labels(self.h, ctypes.byref(p)))
+        return _Ptr(p.value)
 
     def labels(self):
         return self._fetch(lib.hwpa_result_labels, self.lanes * self.frames * T.LANE * 2, np.int16,

@@ -731,6 +731,42 @@ int32_t hwpa_copy_bandwidth(void* d_dst, const void* d_src, size_t bytes, int32_
     return rc;
 }
 
+__global__ __launch_bounds__(256) void CountEqualKernel(const int16_t* __restrict__ a, const int16_t* __restrict__ b,
+                                                        size_t n, unsigned long long* __restrict__ count)
+{
+    __shared__ unsigned long long part[256];
+    unsigned long long c = 0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) c += (a[i] == b[i]) ? 1ull : 0ull;
+    part[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(count, part[0]);
+}
+
+int32_t hwpa_count_equal_i16(const int16_t* d_a, const int16_t* d_b, size_t n, uint64_t* equal)
+{
+    if (!d_a || !d_b || !equal) return Fail(HWPA_ERR_INVALID_ARG, "count_equal: null argument");
+    *equal = 0;
+    if (n == 0) return HWPA_OK;
+    unsigned long long* d = nullptr;
+    int32_t rc = DeviceAlloc((void**)&d, sizeof(*d), "count");
+    if (rc != HWPA_OK) return rc;
+    rc = Fill32(d, 0, 2, nullptr);
+    if (rc == HWPA_OK) {
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+        hipLaunchKernelGGL(CountEqualKernel, dim3(blocks), dim3(256), 0, nullptr, d_a, d_b, n, d);
+        rc = CheckLaunch("CountEqualKernel");
+    }
+    if (rc == HWPA_OK && hipMemcpy(equal, d, sizeof(*d), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = Fail(HWPA_ERR_HIP, "count_equal copy failed");
+    DeviceFree(d);
+    return rc;
+}
+
 int32_t hwpa_memcpy_dtoh(void* host_dst, const void* d_src, size_t bytes)
 {
     if (!host_dst || !d_src) return Fail(HWPA_ERR_INVALID_ARG, "memcpy: null pointer");

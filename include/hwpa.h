@@ -247,6 +247,10 @@ int32_t hwpa_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
 int32_t hwpa_copy_bandwidth(void* d_dst, const void* d_src, size_t bytes, int32_t launches,
                             float* ms_per_launch);
 
+/* Number of equal elements of two int16 device arrays (synchronous); used to
+ * measure frame-label agreement between two pipelines without a host copy. */
+int32_t hwpa_count_equal_i16(const int16_t* d_a, const int16_t* d_b, size_t n, uint64_t* equal);
+
 /* ---- synthetic input (no reference counterpart: bench/test data) ------- */
 int32_t hwpa_tracegen_default(hwpa_tracegen_params* p);
 int32_t hwpa_tracegen_run(const hwpa_tracegen_params* p, uint32_t lanes, uint32_t lane_offset,
