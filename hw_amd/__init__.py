@@ -508,8 +508,67 @@ class BatchAnalyzer:
         return n.value
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owned", True):
             lib.hwpa_analyzer_destroy(self.h)
+        self.h = None
+
+
+class _PoolView(BatchAnalyzer):
+    """Result view of one pool of a Batch (the batch owns the analyzer)."""
+
+    def __init__(self, batch, index, pool_id, lanes, frames, cfg):
+        self.h = _vp()
+        check(lib.hwpa_batch_pool(batch.h, ctypes.c_uint32(index), ctypes.byref(self.h)))
+        self._owned = False
+        self._batch = batch
+        self.pool_id, self.lanes, self.frames, self.cfg = pool_id, lanes, frames, cfg
+
+
+class Batch:
+    """Several pools run one chunk step at a time, captured into one hipGraph
+    per step (pools concurrent)."""
+
+    def __init__(self, pools):
+        """pools: list of (pool_id, lanes, frames_per_chunk, cfg)."""
+        n = len(pools)
+        ids = (ctypes.c_uint32 * n)(*[p[0] for p in pools])
+        lanes = (ctypes.c_uint32 * n)(*[p[1] for p in pools])
+        frames = (ctypes.c_uint32 * n)(*[p[2] for p in pools])
+        cfgs = (Config * n)(*[p[3] for p in pools])
+        self.h = _vp()
+        check(lib.hwpa_batch_create(ctypes.c_uint32(n), ids, lanes, frames, cfgs, ctypes.byref(self.h)))
+        self.specs = list(pools)
+        self.views = [_PoolView(self, i, *p) for i, p in enumerate(pools)]
+
+    def set_graph(self, on=True):
+        check(lib.hwpa_batch_set_graph(self.h, ctypes.c_int32(1 if on else 0)))
+
+    def set_initializer(self, init):
+        self._init = init
+        check(lib.hwpa_batch_set_initializer(self.h, init.h if init is not None else None))
+
+    def process(self, chunks, first_frames):
+        n = len(self.specs)
+        ptrs = (ctypes.c_void_p * n)(*[c.data_ptr() for c in chunks])
+        ff = (ctypes.c_int32 * n)(*first_frames)
+        check(lib.hwpa_batch_process(self.h, ptrs, ff))
+
+    def sync(self):
+        check(lib.hwpa_batch_sync(self.h))
+
+    def graph_stats(self):
+        st = (ctypes.c_uint64 * 3)()
+        check(lib.hwpa_batch_graph_stats(self.h, st))
+        return {"steps": st[0], "instantiations": st[1], "updates": st[2]}
+
+    def pool(self, i):
+        return self.views[i]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            for v in getattr(self, "views", []):
+                v.h = None
+            lib.hwpa_batch_destroy(self.h)
             self.h = None
 
 

@@ -41,6 +41,13 @@ struct Interval {
 
 enum { kHistInit = 0, kDmeInit = 1, kSequencing = 2 };
 enum { kStBaseline = 0, kStAggregate, kStHistogram, kStDme, kStLabels, kStPulses, kNumStages };
+// Kernel groups one process call enqueued; equal plans enqueue the same launch sequence.
+enum : uint32_t {
+    kPlanCopyIn = 1u << 0, kPlanEarlyZero = 1u << 1, kPlanAggregate = 1u << 2, kPlanHistAdd = 1u << 3,
+    kPlanDmeInit = 1u << 4, kPlanEstimate = 1u << 5, kPlanLearned = 1u << 6, kPlanCapture = 1u << 7,
+    kPlanReset = 1u << 8, kPlanLabels = 1u << 9, kPlanPulses = 1u << 10, kPlanLateZero = 1u << 11,
+    kPlanEvents = 1u << 12
+};
 
 }  // namespace hwpa
 
@@ -84,6 +91,7 @@ struct hwpa_analyzer {
     hwpa_initializer* mlp = nullptr;
     LearnedWs lws;
     bool ranLearned = false;
+    uint32_t plan = 0;  // work enqueued by the last process call (kPlan* bits)
     // training-pair capture
     uint16_t* capX = nullptr;
     float* capT = nullptr;
@@ -222,11 +230,13 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
     std::memset(a->flags, 0, sizeof(a->flags));
     std::fill(std::begin(a->stageRan), std::end(a->stageRan), false);
     a->ranEstimate = false;
+    a->plan = a->profiling ? kPlanEvents : 0u;
 
     const int16_t* src = traces;
     if (!on_device) {
         HWPA_HIP(hipMemcpyAsync(a->dTraces, traces, traceBytes, hipMemcpyHostToDevice, s));
         src = a->dTraces;
+        a->plan |= kPlanCopyIn;
     }
     HWPA_STAGE_BEGIN(a, kStBaseline);
     if ((rc = BaselinerRun(a->bl, src, F, a->dBaselined, a->dStats, s)) != HWPA_OK) return rc;
@@ -246,21 +256,25 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
     };
     // Early empty result during the baseliner start-up and the pool stagger (BatchAnalyzer.cpp:331-337)
     if (first_frame < (int32_t)(a->baselinerLatency + a->dmeDelay)) {
+        a->plan |= kPlanEarlyZero;
         if ((rc = zeroCounts()) != HWPA_OK) return rc;
         return finish(HWPA_OK);
     }
 
     // DetectionModelEstimator::AddBatch
     HWPA_STAGE_BEGIN(a, kStAggregate);
+    a->plan |= kPlanAggregate;
     if ((rc = StatsMerge(a->dAgg, a->dStats, L, s)) != HWPA_OK) return rc;
     a->aggFi.Hull(first_frame, first_frame + (int32_t)F);
     if (a->status != kHistInit) {
         HWPA_STAGE_BEGIN(a, kStHistogram);
+        a->plan |= kPlanHistAdd;
         if ((rc = HistAdd(a->dHist, a->dEdge, a->dBaselined, F, a->dModels, L, s)) != HWPA_OK) return rc;
         a->histFi.Hull(first_frame, first_frame + (int32_t)F);
     }
     HWPA_STAGE_BEGIN(a, kStDme);
     if (a->status != kSequencing) {
+        a->plan |= kPlanDmeInit;
         if ((rc = DmeInitModels(a->cfg, a->dAgg, a->dModels, L, s)) != HWPA_OK) return rc;
         a->modelFi = a->aggFi;
     }
@@ -274,6 +288,7 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
                 rc = DmeEstimateLearned(a->cfg, a->mlp, a->lws, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters,
                                         a->dEmScratch, s);
                 a->ranLearned = true;
+                a->plan |= kPlanLearned;
             } else {
                 const bool cap = a->capX != nullptr;
                 rc = DmeEstimate(a->cfg, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, a->dEmScratch, s,
@@ -282,6 +297,7 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
                 if (rc == HWPA_OK && cap)
                     rc = MlpTargets(a->dHist, a->dDiag, L, a->capMinConf, a->capT, a->capValid, s);
                 a->ranLearned = false;
+                a->plan |= kPlanEstimate | (cap ? kPlanCapture : 0u);
             }
             if (rc != HWPA_OK) return rc;
             a->modelFi = a->histFi;
@@ -289,6 +305,7 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
             a->ranEstimate = true;
             a->fullEstimation = true;
         }
+        a->plan |= kPlanReset;
         if ((rc = HistReset(a->cfg, a->dHist, a->dAgg, L, s)) != HWPA_OK) return rc;
         a->histFi = Interval{};
         if ((rc = StatsReset(a->dAgg, L, s)) != HWPA_OK) return rc;
@@ -298,6 +315,7 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
     }
 
     HWPA_STAGE_BEGIN(a, kStLabels);
+    a->plan |= kPlanLabels | kPlanPulses;
     if ((rc = LabelerRun(a->lab, a->dBaselined, F, a->dModels, a->dLabels, a->dLatent, a->dScore, s)) != HWPA_OK)
         return rc;
     a->flags[2] = 1;
@@ -307,8 +325,9 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
         return rc;
     if (a->fullEstimation) {
         a->flags[0] = 1;
-    } else if ((rc = zeroCounts()) != HWPA_OK) {
-        return rc;
+    } else {
+        a->plan |= kPlanLateZero;
+        if ((rc = zeroCounts()) != HWPA_OK) return rc;
     }
     return finish(HWPA_OK);
 }
@@ -466,6 +485,207 @@ int32_t hwpa_analyzer_stream(const hwpa_analyzer* a, void** stream)
 {
     if (!a || !stream) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
     *stream = (void*)a->stream;
+    return HWPA_OK;
+}
+
+}  // extern "C"
+
+// ---- batched pools: one captured graph per chunk step ----------------------
+// Every pool keeps its own analyzer (state machine, buffers, stream).  A step
+// captures all pools' work from a master stream that forks to the pool
+// streams and joins them back, so the whole step is one graph launch with the
+// pools running concurrently.  The analyzers' host state machines run during
+// capture, exactly as for direct calls; the captured graph is launched at once.
+// Executable graphs are cached by the pools' plans (the kernel sequence each
+// pool enqueued) and refreshed with hipGraphExecUpdate, which only rewrites
+// kernel arguments (chunk pointers, frame intervals) when the topology matches.
+
+struct hwpa_batch {
+    std::vector<hwpa_analyzer*> pools;
+    hipStream_t master = nullptr;
+    hipEvent_t fork = nullptr, done = nullptr;
+    std::vector<hipEvent_t> joins;
+    bool useGraph = true;
+    struct Exec {
+        std::vector<uint32_t> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Exec> cache;
+    uint64_t steps = 0, updates = 0, instantiations = 0;
+};
+
+static void FreeBatch(hwpa_batch* b)
+{
+    if (!b) return;
+    if (b->master) (void)hipStreamSynchronize(b->master);
+    for (auto& e : b->cache)
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    for (auto* a : b->pools) FreeAnalyzer(a);
+    for (auto& e : b->joins)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->master) (void)hipStreamDestroy(b->master);
+    delete b;
+}
+
+extern "C" {
+
+int32_t hwpa_batch_create(uint32_t n_pools, const uint32_t* pool_ids, const uint32_t* lanes, const uint32_t* frames,
+                          const hwpa_config* cfgs, hwpa_batch** out)
+{
+    if (!out || !pool_ids || !lanes || !frames || !cfgs || n_pools == 0)
+        return Fail(HWPA_ERR_INVALID_ARG, "batch_create: bad argument");
+    *out = nullptr;
+    hwpa_batch* b = new (std::nothrow) hwpa_batch();
+    if (!b) return Fail(HWPA_ERR_OOM, "out of host memory");
+    for (uint32_t i = 0; i < n_pools; ++i) {
+        hwpa_analyzer* a = nullptr;
+        const int32_t rc = hwpa_analyzer_create(pool_ids[i], lanes[i], frames[i], &cfgs[i], &a);
+        if (rc != HWPA_OK) {
+            FreeBatch(b);
+            return rc;
+        }
+        b->pools.push_back(a);
+    }
+    b->joins.assign(n_pools, nullptr);
+    bool ok = hipStreamCreate(&b->master) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess;
+    for (auto& e : b->joins) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        FreeBatch(b);
+        return Fail(HWPA_ERR_HIP, "batch: stream/event creation failed");
+    }
+    *out = b;
+    return HWPA_OK;
+}
+
+int32_t hwpa_batch_destroy(hwpa_batch* b)
+{
+    FreeBatch(b);
+    return HWPA_OK;
+}
+
+int32_t hwpa_batch_pool(hwpa_batch* b, uint32_t index, hwpa_analyzer** a)
+{
+    if (!b || !a || index >= b->pools.size()) return Fail(HWPA_ERR_INVALID_ARG, "batch_pool: bad argument");
+    *a = b->pools[index];
+    return HWPA_OK;
+}
+
+int32_t hwpa_batch_set_graph(hwpa_batch* b, int32_t enable)
+{
+    if (!b) return Fail(HWPA_ERR_INVALID_ARG, "null batch");
+    b->useGraph = enable != 0;
+    return HWPA_OK;
+}
+
+int32_t hwpa_batch_set_initializer(hwpa_batch* b, hwpa_initializer* m)
+{
+    if (!b) return Fail(HWPA_ERR_INVALID_ARG, "null batch");
+    for (auto* a : b->pools) {
+        const int32_t rc = hwpa_analyzer_set_initializer(a, m);
+        if (rc != HWPA_OK) return rc;
+    }
+    return HWPA_OK;
+}
+
+int32_t hwpa_batch_process(hwpa_batch* b, const int16_t* const* d_traces, const int32_t* first_frames)
+{
+    if (!b || !d_traces || !first_frames) return Fail(HWPA_ERR_INVALID_ARG, "batch_process: null argument");
+    const size_t n = b->pools.size();
+    if (!b->useGraph) {
+        for (size_t i = 0; i < n; ++i) {
+            hwpa_analyzer* a = b->pools[i];
+            const int32_t rc = hwpa_analyzer_process(a, d_traces[i], 1, a->poolId, first_frames[i]);
+            if (rc != HWPA_OK) return rc;
+        }
+        b->steps += 1;
+        return HWPA_OK;
+    }
+    for (auto* a : b->pools)
+        if (a->profiling) return Fail(HWPA_ERR_UNSUPPORTED, "batch: stage profiling is not captured into graphs");
+    HWPA_HIP(hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal));
+    int32_t rc = HWPA_OK;
+    if (hipEventRecord(b->fork, b->master) != hipSuccess) rc = Fail(HWPA_ERR_HIP, "batch: fork record failed");
+    for (size_t i = 0; i < n && rc == HWPA_OK; ++i) {
+        hwpa_analyzer* a = b->pools[i];
+        if (hipStreamWaitEvent(a->stream, b->fork, 0) != hipSuccess) {
+            rc = Fail(HWPA_ERR_HIP, "batch: fork wait failed");
+            break;
+        }
+        rc = hwpa_analyzer_process(a, d_traces[i], 1, a->poolId, first_frames[i]);
+        if (rc != HWPA_OK) break;
+        if (hipEventRecord(b->joins[i], a->stream) != hipSuccess ||
+            hipStreamWaitEvent(b->master, b->joins[i], 0) != hipSuccess)
+            rc = Fail(HWPA_ERR_HIP, "batch: join failed");
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(b->master, &g);
+    if (rc != HWPA_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess || !g) return Fail(HWPA_ERR_HIP, "batch: stream capture failed");
+    std::vector<uint32_t> key(n);
+    for (size_t i = 0; i < n; ++i) key[i] = b->pools[i]->plan;
+    hipGraphExec_t exec = nullptr;
+    for (auto& e : b->cache) {
+        if (e.key != key) continue;
+        hipGraphNode_t errNode = nullptr;
+        hipGraphExecUpdateResult res;
+        if (hipGraphExecUpdate(e.exec, g, &errNode, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            exec = e.exec;
+            b->updates += 1;
+        } else {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(e.exec);
+            e.exec = nullptr;
+            if (hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0) != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                return Fail(HWPA_ERR_HIP, "batch: graph instantiation failed");
+            }
+            exec = e.exec;
+            b->instantiations += 1;
+        }
+        break;
+    }
+    if (!exec) {
+        hwpa_batch::Exec e;
+        e.key = key;
+        if (hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return Fail(HWPA_ERR_HIP, "batch: graph instantiation failed");
+        }
+        exec = e.exec;
+        b->cache.push_back(e);
+        b->instantiations += 1;
+    }
+    const hipError_t le = hipGraphLaunch(exec, b->master);
+    (void)hipGraphDestroy(g);
+    if (le != hipSuccess) return Fail(HWPA_ERR_HIP, "batch: graph launch failed");
+    // Let each pool's stream (and so hwpa_analyzer_sync / result accessors) see the step.
+    HWPA_HIP(hipEventRecord(b->done, b->master));
+    for (auto* a : b->pools) HWPA_HIP(hipStreamWaitEvent(a->stream, b->done, 0));
+    b->steps += 1;
+    return HWPA_OK;
+}
+
+int32_t hwpa_batch_sync(hwpa_batch* b)
+{
+    if (!b) return Fail(HWPA_ERR_INVALID_ARG, "null batch");
+    HWPA_HIP(hipStreamSynchronize(b->master));
+    for (auto* a : b->pools) HWPA_HIP(hipStreamSynchronize(a->stream));
+    return HWPA_OK;
+}
+
+int32_t hwpa_batch_graph_stats(const hwpa_batch* b, uint64_t stats[3])
+{
+    if (!b || !stats) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    stats[0] = b->steps;
+    stats[1] = b->instantiations;
+    stats[2] = b->updates;
     return HWPA_OK;
 }
 

@@ -4,16 +4,25 @@
 // Semantics follow HostMultiScaleBaseliner (traceAnalysis/HostMultiScaleBaseliner.cpp:57-212):
 // lower envelope = opening cascade, upper = closing then openings; every
 // extremum filter returns op(previous W inputs) of its (decimated) stream
-// (traceAnalysis/TraceFilters.cpp:10-57), state carried across chunks.
+// (traceAnalysis/TraceFilters.cpp:10-57), state carried across chunks.  Any
+// algorithm that returns that window extremum gives the reference's bits.
 //
-// Mapping: one wave64 per lane, one lane (thread) per ZMW, frames streamed
-// in groups of the aggregate stride (16): the group's raw frames are read
-// once into registers, pushed through both cascades, and the same registers
-// feed the subtraction and the statistics, so HBM sees the trace once in and
-// once out.  The van Herk/Gil-Werman ring of every non-final stage lives in
-// LDS for the duration of the chunk (one shared, lane-uniform counter per
-// stage, since all four filters of a stage advance together); the final
-// stage runs once per group and keeps its ring directly in HBM.
+// Mapping: one wave64 per lane, one thread per ZMW, frames streamed in
+// groups of the aggregate stride (16).  The group's raw frames are read once
+// into registers, pushed through both cascades, and the same registers feed
+// the subtraction and the statistics, so HBM sees the trace once in and once
+// out.
+//
+// Filters run on packed 16-bit pairs so one v_pk_min_i16 / v_pk_max_i16
+// serves two filters: stage 0 pairs (x, ~x) -- bitwise NOT reverses the
+// order, so the upper cascade's dilate-then-erode becomes erode-then-dilate
+// of ~x -- and later stages pair (lower, upper), which both run
+// erode-then-dilate.  The non-final stages keep the last W inputs of each
+// filter pair in registers and compute a group's window extrema with an
+// unrolled block prefix/suffix pass (van Herk / Gil-Werman with blocks of W
+// over history ++ group), so they need no ring and no LDS.  The final stage,
+// one sample per group, keeps a van Herk / Gil-Werman ring per filter pair in
+// HBM whose lane-uniform head is shared by both pairs.
 
 #include "internal.h"
 
@@ -21,6 +30,25 @@
 #include <cmath>
 
 namespace hwpa {
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t PMin(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_elementwise_min(__builtin_bit_cast(s16x2, a), __builtin_bit_cast(s16x2, b)));
+}
+__device__ __forceinline__ uint32_t PMax(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_elementwise_max(__builtin_bit_cast(s16x2, a), __builtin_bit_cast(s16x2, b)));
+}
+template <bool MX>
+__device__ __forceinline__ uint32_t POp(uint32_t a, uint32_t b) { return MX ? PMax(a, b) : PMin(a, b); }
+__device__ __forceinline__ uint32_t Pack(int lo, int hi) { return (uint32_t)(uint16_t)lo | ((uint32_t)(uint16_t)hi << 16); }
+__device__ __forceinline__ int LoHalf(uint32_t p) { return (int)(int16_t)(uint16_t)(p & 0xffffu); }
+__device__ __forceinline__ int HiHalf(uint32_t p) { return (int)(int16_t)(uint16_t)(p >> 16); }
+constexpr uint32_t HI_NOT = 0xffff0000u;
 
 struct BlLaneState {
     float ema_sum[LS];
@@ -30,9 +58,9 @@ struct BlLaneState {
     int16_t lat_raw[LS];
     uint8_t lat_l[LS];
     uint8_t lat_h1[LS];
-    int16_t hold[4 * HWPA_MAX_FILTER_STAGES][LS];
-    int16_t last[4 * HWPA_MAX_FILTER_STAGES][LS];
-    int32_t counter[LS];  // [0..stages) used; lane-uniform
+    uint32_t hold[2][LS];  // final-stage van Herk / Gil-Werman hold, packed (erode pair, dilate pair)
+    uint32_t last[2][LS];
+    int32_t counter[LS];   // [0]: final-stage ring head (lane-uniform)
 };
 
 struct BlConsts {
@@ -44,35 +72,43 @@ struct BaselinerDev {
     hwpa_filter_params fp;
     BlConsts k;
     uint32_t lanes;
-    size_t winPerLane;  // int16 ring entries per ZMW column
+    size_t winPerLane;  // packed u32 rows per lane
     BlLaneState* state = nullptr;
-    int16_t* win = nullptr;
+    uint32_t* win = nullptr;
 };
 
-template <bool MX>
-__device__ __forceinline__ int Ext(int a, int b) { return MX ? (a > b ? a : b) : (a < b ? a : b); }
-
-// One van Herk / Gil-Werman step on a ring of W int16 (ZMW-minor, stride LS).
-// ctr is the lane-uniform ring head; when it wraps to 0 the ring (now one
-// complete block) is replaced in place by its suffix extrema.
-template <int W, bool MX, typename P>
-__device__ __forceinline__ int HgwStep(P ring, int ctr, int z, int x, int& s, int& last)
+// out[k] = op(c[k .. k+W-1]) for c = hist ++ in, k < N; then hist := c[N .. N+W-1].
+// Blocks of W start at index 0 of c: a window either is one block or spans
+// the suffix of one block and the prefix of the next.  Prefixes are formed
+// forward for c[W ..], suffixes backward and combined as they are formed.
+template <int W, int N, bool MX>
+__device__ __forceinline__ void Slide(uint32_t (&hist)[W], const uint32_t (&in)[N], uint32_t (&out)[N])
 {
-    if (ctr == 0) {
-        int v = last;
+    constexpr int L = W + N;
+    constexpr int E = ((N - 1) / W) * W + W - 1;  // end of the block holding index N-1 (< L - 1)
+    uint32_t c[L], P[L];
 #pragma unroll
-        for (int j = W - 2; j >= 0; --j) {
-            v = Ext<MX>(v, (int)ring[j * LS + z]);
-            ring[j * LS + z] = (int16_t)v;
-        }
-        s = last;
-    } else {
-        s = Ext<MX>(s, last);
+    for (int i = 0; i < W; ++i) c[i] = hist[i];
+#pragma unroll
+    for (int i = 0; i < N; ++i) c[W + i] = in[i];
+#pragma unroll
+    for (int i = W; i < L; ++i) {
+        if (i % W == 0)
+            P[i] = c[i];
+        else
+            P[i] = POp<MX>(P[i - 1], c[i]);
     }
-    const int front = ring[ctr * LS + z];
-    ring[ctr * LS + z] = (int16_t)x;
-    last = x;
-    return Ext<MX>(s, front);
+    uint32_t run = 0;
+#pragma unroll
+    for (int i = E; i >= 0; --i) {
+        if (i % W == W - 1)
+            run = c[i];
+        else
+            run = POp<MX>(c[i], run);
+        if (i < N) out[i] = (i % W == 0) ? run : POp<MX>(run, P[i + W - 1]);
+    }
+#pragma unroll
+    for (int i = 0; i < W; ++i) hist[i] = c[N + i];
 }
 
 template <int NST, int S0, int S1, int S2, int W0, int W1, int W2>
@@ -81,32 +117,43 @@ struct Table {
     static constexpr int s[3] = {S0, S1, S2};
     static constexpr int w[3] = {W0, W1, W2};
     static constexpr int S = S0 * S1 * (NST == 3 ? S2 : 1);
-    static constexpr int ldsW = (NST == 3) ? (W0 + W1) : W0;  // widths of the LDS stages
-    static constexpr int lastW = (NST == 3) ? W2 : W1;
-    static constexpr int off(int i) { return i == 0 ? 0 : (i == 1 ? 4 * W0 : 4 * (W0 + W1)); }
-    static constexpr int winPerLane = 4 * (W0 + W1 + (NST == 3 ? W2 : 0));
+    static constexpr int N0 = S / S0;                       // stage-0 samples per group
+    static constexpr int N1 = (NST == 3) ? N0 / S1 : 1;     // stage-1 samples per group
+    static constexpr int WF = (NST == 3) ? W2 : W1;         // final-stage width
+    static constexpr int W1R = (NST == 3) ? W1 : 1;         // register stage-1 width (unused for 2 stages)
+    // packed rows per lane: stage-0 histories, stage-1 histories, final rings
+    static constexpr int rowsHist = 2 * W0 + (NST == 3 ? 2 * W1 : 0);
+    static constexpr int winPerLane = rowsHist + 2 * WF;
 };
 
-// Runs stage I of the cascade on n consumed samples in[] -> lo[], up[].
-template <class T, int I, int N, typename P>
-__device__ __forceinline__ void RunStage(P base, int z, int& ctr, const int* inLo, const int* inUp, int stepIn,
-                                         int* outLo, int* outUp, int (&s)[12], int (&last)[12])
+// One van Herk / Gil-Werman step of the final stage on a packed ring (HBM).
+// `front` is ring[ctr] read at the start of the group (after any suffix refill).
+template <bool MX>
+__device__ __forceinline__ uint32_t RingResult(int ctr, uint32_t front, uint32_t& s, uint32_t last)
 {
-    constexpr int W = T::w[I];
-    constexpr bool upperFirstMax = (I == 0);
-    P rLE = base + 0 * W * LS;
-    P rLD = base + 1 * W * LS;
-    P rU1 = base + 2 * W * LS;
-    P rU2 = base + 3 * W * LS;
+    s = (ctr == 0) ? last : POp<MX>(s, last);
+    return POp<MX>(s, front);
+}
+
+template <int W, bool MX>
+__device__ __forceinline__ void RingRefill(uint32_t* ring, uint32_t last)
+{
+    // The ring now holds one complete block: replace it by its suffix
+    // extrema, working down from the top in batches of 12 loads.
+    constexpr int B = 12;
+    uint32_t acc = last;
+#pragma unroll 1
+    for (int top = W - 2; top >= 0; top -= B) {
+        uint32_t v[B];
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const int xl = inLo[k * stepIn];
-        const int xu = inUp[k * stepIn];
-        const int a = HgwStep<W, false>(rLE, ctr, z, xl, s[4 * I + 0], last[4 * I + 0]);
-        outLo[k] = HgwStep<W, true>(rLD, ctr, z, a, s[4 * I + 1], last[4 * I + 1]);
-        const int b = HgwStep<W, upperFirstMax>(rU1, ctr, z, xu, s[4 * I + 2], last[4 * I + 2]);
-        outUp[k] = HgwStep<W, !upperFirstMax>(rU2, ctr, z, b, s[4 * I + 3], last[4 * I + 3]);
-        ctr = (ctr + 1 == W) ? 0 : ctr + 1;
+        for (int q = 0; q < B; ++q) v[q] = (top - q >= 0) ? ring[(top - q) * LS] : 0u;
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            if (top - q >= 0) {
+                acc = POp<MX>(acc, v[q]);
+                ring[(top - q) * LS] = acc;
+            }
+        }
     }
 }
 
@@ -115,36 +162,39 @@ __global__ __launch_bounds__(64) void BaselineKernel(const int16_t* __restrict__
                                                      int16_t* __restrict__ out,
                                                      hwpa_baseliner_stats* __restrict__ stats,
                                                      BlLaneState* __restrict__ state,
-                                                     int16_t* __restrict__ win, BlConsts k)
+                                                     uint32_t* __restrict__ win, BlConsts k)
 {
     constexpr int S = T::S;
     constexpr int NST = T::nst;
-    __shared__ int16_t lds[4 * T::ldsW * LS];
+    constexpr int W0 = T::w[0], N0 = T::N0, N1 = T::N1, W1 = T::W1R, WF = T::WF;
 
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
     BlLaneState& st = state[lane];
-    int16_t* wl = win + (size_t)lane * T::winPerLane * LS;
+    uint32_t* wl = win + (size_t)lane * T::winPerLane * LS + z;
 
-    // Load the LDS rings of the non-final stages.
-#pragma unroll 4
-    for (int i = 0; i < 4 * T::ldsW; ++i) lds[i * LS + z] = wl[i * LS + z];
-    int16_t* lastRing = wl + T::off(NST - 1) * LS;
-
-    int s[12], last[12];
+    uint32_t h0a[W0], h0b[W0], h1a[W1], h1b[W1];
 #pragma unroll
-    for (int f = 0; f < 12; ++f) {
-        s[f] = (f < 4 * NST) ? (int)st.hold[f][z] : 0;
-        last[f] = (f < 4 * NST) ? (int)st.last[f][z] : 0;
+    for (int i = 0; i < W0; ++i) {
+        h0a[i] = wl[i * LS];
+        h0b[i] = wl[(W0 + i) * LS];
     }
-    int ctr0 = st.counter[0];
-    int ctr1 = st.counter[1];
-    int ctr2 = (NST == 3) ? st.counter[2] : 0;
+    if constexpr (NST == 3) {
+#pragma unroll
+        for (int i = 0; i < W1; ++i) {
+            h1a[i] = wl[(2 * W0 + i) * LS];
+            h1b[i] = wl[(2 * W0 + W1 + i) * LS];
+        }
+    }
+    uint32_t* ringA = wl + T::rowsHist * LS;
+    uint32_t* ringB = ringA + WF * LS;
+    uint32_t sA = st.hold[0][z], sB = st.hold[1][z];
+    uint32_t lastA = st.last[0][z], lastB = st.last[1][z];
+    int ctr = st.counter[0];
 
     float emaSum = st.ema_sum[z], emaWeight = st.ema_weight[z], sigmaEma = st.sigma_ema[z];
     int latData = st.lat_data[z], latRaw = st.lat_raw[z];
     bool latL = st.lat_l[z] != 0, latH1 = st.lat_h1[z] != 0;
-    __syncthreads();
 
     // per-chunk statistics (offsets are 0)
     float cm0 = 0.f, cm1 = 0.f, cm2 = 0.f, lagM2 = 0.f;
@@ -158,31 +208,53 @@ __global__ __launch_bounds__(64) void BaselineKernel(const int16_t* __restrict__
     const int groups = frames / S;
     const float minSigma = .288675135f;
 
-    for (int g = 0; g < groups; ++g) {
-        int rv[S];
+    int rv[S];
 #pragma unroll
-        for (int j = 0; j < S; ++j) rv[j] = rl[(size_t)(g * S + j) * LS];
+    for (int j = 0; j < S; ++j) rv[j] = rl[(size_t)j * LS];
 
-        int lo, up;
-        if constexpr (NST == 3) {
-            constexpr int n0 = S / T::s[0];
-            constexpr int n1 = n0 / T::s[1];
-            int lo0[n0], up0[n0], lo1[n1], up1[n1];
-            RunStage<T, 0, n0>(lds, z, ctr0, rv, rv, T::s[0], lo0, up0, s, last);
-            RunStage<T, 1, n1>(lds + 4 * T::w[0] * LS, z, ctr1, lo0, up0, T::s[1], lo1, up1, s, last);
-            int lo2[1], up2[1];
-            RunStage<T, 2, 1>(lastRing, z, ctr2, lo1, up1, T::s[2], lo2, up2, s, last);
-            lo = lo2[0];
-            up = up2[0];
-        } else {
-            constexpr int n0 = S / T::s[0];
-            int lo0[n0], up0[n0];
-            RunStage<T, 0, n0>(lds, z, ctr0, rv, rv, T::s[0], lo0, up0, s, last);
-            int lo1[1], up1[1];
-            RunStage<T, 1, 1>(lastRing, z, ctr1, lo0, up0, T::s[1], lo1, up1, s, last);
-            lo = lo1[0];
-            up = up1[0];
+    for (int g = 0; g < groups; ++g) {
+        // Final-stage ring: refill on wrap, then read the fronts early.
+        if (ctr == 0) {
+            RingRefill<WF, false>(ringA, lastA);
+            RingRefill<WF, true>(ringB, lastB);
         }
+        const uint32_t frontA = ringA[ctr * LS];
+        const uint32_t frontB = ringB[ctr * LS];
+        // Prefetch the next group's raw frames.
+        int nv[S];
+        const int gn = (g + 1 < groups) ? g + 1 : g;
+#pragma unroll
+        for (int j = 0; j < S; ++j) nv[j] = rl[(size_t)(gn * S + j) * LS];
+
+        // Stage 0 on (x, ~x): erode pair, then dilate pair -> (lower, ~upper).
+        uint32_t a0[N0], e0[N0], d0[N0];
+#pragma unroll
+        for (int i = 0; i < N0; ++i) {
+            const int x = rv[i * T::s[0]];
+            a0[i] = Pack(x, ~x);
+        }
+        Slide<W0, N0, false>(h0a, a0, e0);
+        Slide<W0, N0, true>(h0b, e0, d0);
+        uint32_t fin;
+        if constexpr (NST == 3) {
+            uint32_t a1[N1], e1[N1], d1[N1];
+#pragma unroll
+            for (int i = 0; i < N1; ++i) a1[i] = d0[i * T::s[1]] ^ HI_NOT;
+            Slide<W1, N1, false>(h1a, a1, e1);
+            Slide<W1, N1, true>(h1b, e1, d1);
+            fin = d1[0];
+        } else {
+            fin = d0[0] ^ HI_NOT;
+        }
+        // Final stage: erode pair then dilate pair.
+        const uint32_t fe = RingResult<false>(ctr, frontA, sA, lastA);
+        ringA[ctr * LS] = fin;
+        lastA = fin;
+        const uint32_t fd = RingResult<true>(ctr, frontB, sB, lastB);
+        ringB[ctr * LS] = fe;
+        lastB = fe;
+        ctr = (ctr + 1 == WF) ? 0 : ctr + 1;
+        const int lo = LoHalf(fd), up = HiHalf(fd);
 
         // GetSmoothedBlEstimate (HostMultiScaleBaseliner.cpp:178-212)
         const int16_t lv = hw_wrap16(lo - k.pedestal);
@@ -222,11 +294,10 @@ __global__ __launch_bounds__(64) void BaselineKernel(const int16_t* __restrict__
             cm2 += bs * bs;
             tmin = latData < tmin ? latData : tmin;
             tmax = latData > tmax ? latData : tmax;
-            if (m) {
-                bm0 = bm0 + 1.0f;
-                bm1 = bm1 + bs;
-                bm2 = bm2 + bs * bs;
-            }
+            // Branch-free masked sums: adding +0 leaves a sum of integers unchanged.
+            bm0 = bm0 + (m ? 1.0f : 0.0f);
+            bm1 = bm1 + (m ? bs : 0.0f);
+            bm2 = bm2 + (m ? bs * bs : 0.0f);
             rawBl += (float)(m ? latRaw : 0);
             latRaw = rr;
             latData = bls;
@@ -235,22 +306,28 @@ __global__ __launch_bounds__(64) void BaselineKernel(const int16_t* __restrict__
         gm0 += 1.0f;
         gm1 += bgv;
         gm2 += bgv * bgv;
+#pragma unroll
+        for (int j = 0; j < S; ++j) rv[j] = nv[j];
     }
 
     // Persist carried state.
-    __syncthreads();
-#pragma unroll 4
-    for (int i = 0; i < 4 * T::ldsW; ++i) wl[i * LS + z] = lds[i * LS + z];
 #pragma unroll
-    for (int f = 0; f < 4 * NST; ++f) {
-        st.hold[f][z] = (int16_t)s[f];
-        st.last[f][z] = (int16_t)last[f];
+    for (int i = 0; i < W0; ++i) {
+        wl[i * LS] = h0a[i];
+        wl[(W0 + i) * LS] = h0b[i];
     }
-    if (z == 0) {
-        st.counter[0] = ctr0;
-        st.counter[1] = ctr1;
-        st.counter[2] = ctr2;
+    if constexpr (NST == 3) {
+#pragma unroll
+        for (int i = 0; i < W1; ++i) {
+            wl[(2 * W0 + i) * LS] = h1a[i];
+            wl[(2 * W0 + W1 + i) * LS] = h1b[i];
+        }
     }
+    st.hold[0][z] = sA;
+    st.hold[1][z] = sB;
+    st.last[0][z] = lastA;
+    st.last[1][z] = lastB;
+    if (z == 0) st.counter[0] = ctr;
     st.ema_sum[z] = emaSum;
     st.ema_weight[z] = emaWeight;
     st.sigma_ema[z] = sigmaEma;
@@ -285,7 +362,9 @@ __global__ __launch_bounds__(64) void BaselineKernel(const int16_t* __restrict__
     o.background.m2[z] = gm2;
 }
 
-__global__ void BaselineInitKernel(BlLaneState* state, int16_t* win, int winPerLane)
+// Initial state: every filter window holds W zeros (traceAnalysis/BlockFilterStage.h:28-31);
+// in the stage-0 (x, ~x) domain the upper half of a zero window is ~0.
+__global__ void BaselineInitKernel(BlLaneState* state, uint32_t* win, int winPerLane, int stage0Rows)
 {
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
@@ -297,13 +376,13 @@ __global__ void BaselineInitKernel(BlLaneState* state, int16_t* win, int winPerL
     st.lat_raw[z] = 0;
     st.lat_l[z] = 0;
     st.lat_h1[z] = 0;
-    for (int f = 0; f < 4 * HWPA_MAX_FILTER_STAGES; ++f) {
+    for (int f = 0; f < 2; ++f) {
         st.hold[f][z] = 0;
         st.last[f][z] = 0;
     }
     st.counter[z] = 0;
-    int16_t* wl = win + (size_t)lane * winPerLane * LS;
-    for (int i = 0; i < winPerLane; ++i) wl[i * LS + z] = 0;
+    uint32_t* wl = win + (size_t)lane * winPerLane * LS;
+    for (int i = 0; i < winPerLane; ++i) wl[i * LS + z] = (i < stage0Rows) ? HI_NOT : 0u;
 }
 
 using TMSL = Table<3, 1, 2, 8, 11, 17, 61>;
@@ -318,6 +397,7 @@ static int32_t Launch(BaselinerDev* b, const int16_t* raw, uint32_t frames, int1
                       hwpa_baseliner_stats* stats, hipStream_t s)
 {
     if (frames % T::S != 0) return Fail(HWPA_ERR_INVALID_ARG, "frames must be a multiple of the aggregate stride");
+    if ((size_t)T::winPerLane != b->winPerLane) return Fail(HWPA_ERR_UNSUPPORTED, "baseliner: window layout mismatch");
     hipLaunchKernelGGL(BaselineKernel<T>, dim3(b->lanes), dim3(LS), 0, s, raw, (int)frames, out, stats,
                        b->state, b->win, b->k);
     return CheckLaunch("BaselineKernel");
@@ -348,16 +428,18 @@ int32_t BaselinerCreate(const hwpa_config& cfg, uint32_t lanes, BaselinerDev** o
     b->k.jumpTol = (cfg.jump_suppression > 0.0f) ? 1.0f / cfg.jump_suppression : INFINITY;
     b->k.scaler = cfg.movie.photoelectron_sensitivity;
     b->k.pedestal = cfg.movie.pedestal;
+    // Packed (two filters per u32) windows: 2 rows per width of every stage.
     size_t w = 0;
-    for (int i = 0; i < b->fp.num_stages; ++i) w += 4 * (size_t)b->fp.widths[i];
+    for (int i = 0; i < b->fp.num_stages; ++i) w += 2 * (size_t)b->fp.widths[i];
     b->winPerLane = w;
     int32_t rc = DeviceAlloc((void**)&b->state, sizeof(BlLaneState) * lanes, "baseliner state");
-    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&b->win, w * LS * sizeof(int16_t) * lanes, "baseliner windows");
+    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&b->win, w * LS * sizeof(uint32_t) * lanes, "baseliner windows");
     if (rc != HWPA_OK) {
         BaselinerDestroy(b);
         return rc;
     }
-    hipLaunchKernelGGL(BaselineInitKernel, dim3(lanes), dim3(LS), 0, nullptr, b->state, b->win, (int)w);
+    hipLaunchKernelGGL(BaselineInitKernel, dim3(lanes), dim3(LS), 0, nullptr, b->state, b->win, (int)w,
+                       2 * b->fp.widths[0]);
     rc = CheckLaunch("BaselineInitKernel");
     if (rc == HWPA_OK && hipDeviceSynchronize() != hipSuccess) rc = Fail(HWPA_ERR_HIP, "baseliner init failed");
     if (rc != HWPA_OK) {

@@ -5,8 +5,9 @@
 //
 // Mapping: one wave64 per lane, one thread per ZMW.  The lane's 300x64 u16
 // counters (38.4 KB) are staged in LDS for the chunk; each thread owns one
-// ZMW column, so increments need no atomics, and the [bin][zmw] layout puts
-// the 64 lanes of every increment on distinct LDS banks whatever the bins.
+// ZMW column, and the [bin pair][zmw] layout puts the 64 lanes of every
+// increment on distinct LDS banks whatever the bins.  Frames are read in
+// batches of 16 with the next batch in flight while one is binned.
 
 #include "internal.h"
 
@@ -74,20 +75,56 @@ __global__ __launch_bounds__(64) void HistResetKernel(hwpa_lane_hist* hist, cons
     for (int b = 0; b < NB; ++b) h.bin_count[b][z] = 0;
 }
 
-template <int UNROLL>
+// Bin of a sample: floor((x - lo) / bs) with the reference's IEEE divide
+// (UHistogramSimd::BinIndex, dataTypes/UHistogramSimd.h:199-229).  The
+// quotient is first formed as (x - lo) * (1 / bs), which is within
+// 3 * 2^-24 |y| of the divided value; for |y| < 2048 that is below 3.7e-4, so
+// when the product is more than 1/1024 away from an integer both share the
+// same floor, and only near a bin edge is the divide itself evaluated.  For
+// 2048 <= |y| < 2^30 the sample is an outlier on the same side either way;
+// NaN, infinities and huge quotients take the divide (whose float -> int
+// conversion gives 0x80000000 out of range, a low outlier, as on the host).
+__device__ __forceinline__ int BinOf(float a, float bs, float inv)
+{
+    const float yp = a * inv;
+    const float ay = hw_fabsf(yp);
+    if (ay >= 2048.0f && ay < 1073741824.0f) return (yp > 0.0f) ? NB : -1;
+    if (ay < 2048.0f) {
+        const float fl = hw_floorf(yp);
+        const float d = yp - fl;
+        if (d >= 0.0009765625f && d <= 0.9990234375f) return (int)fl;
+    }
+    return hw_floor_i32(a / bs);
+}
+
+// TraceHistogramAccumHost::AddBlock (TraceHistogramAccumHost.cpp:128-169).
+// Counters live in LDS as words holding bins 2j (low half) and 2j+1 (high
+// half) of one ZMW, so an increment is a single ds_add_u32 that nothing waits
+// on.  That cannot carry into the neighbouring half while every count stays
+// below 65536 -- checked before the chunk; otherwise the wave increments with
+// a 16-bit wrapping read-modify-write, the reference's u16 counter semantics.
+template <int B>
 __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLane* edge,
                                                     const int16_t* __restrict__ traces, int frames,
                                                     const hwpa_lane_model* __restrict__ models)
 {
-    __shared__ uint16_t cnt[NB * LS];
+    __shared__ uint32_t cnt[(NB / 2) * LS];
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
     hwpa_lane_hist& h = hist[lane];
     EdgeLane& e = edge[lane];
-    for (int b = 0; b < NB; ++b) cnt[b * LS + z] = h.bin_count[b][z];
+    uint32_t mx = 0;
+    for (int j = 0; j < NB / 2; ++j) {
+        const uint32_t c0 = h.bin_count[2 * j][z], c1 = h.bin_count[2 * j + 1][z];
+        mx = c0 > mx ? c0 : mx;
+        mx = c1 > mx ? c1 : mx;
+        cnt[j * LS + z] = c0 | (c1 << 16);
+    }
+    const bool fast = __all(mx + (uint32_t)frames <= 65535u);
     uint32_t lowOut = h.outlier_low[z], highOut = h.outlier_high[z];
     const float lo = h.low_bound[z];
     const float bs = h.bin_size[z];
+    const float inv = 1.0f / bs;
 
     // Edge threshold from the current model (TraceHistogramAccumHost.cpp:156-157)
     const hwpa_lane_model& m = models[lane];
@@ -98,15 +135,18 @@ __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLa
     int prev = e.prev[z];
     bool h0 = e.hist[0][z] != 0, h1 = e.hist[1][z] != 0;
     int i0 = e.i0, i1 = e.i1;
-    __syncthreads();
 
     const int16_t* tr = traces + (size_t)lane * frames * LS + z;
-    for (int t0 = 0; t0 < frames; t0 += UNROLL) {
-        int fv[UNROLL];
+    int fv[B];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) fv[u] = (t0 + u < frames) ? (int)tr[(size_t)(t0 + u) * LS] : 0;
+    for (int u = 0; u < B; ++u) fv[u] = (u < frames) ? (int)tr[(size_t)u * LS] : 0;
+    for (int t0 = 0; t0 < frames; t0 += B) {
+        int nv[B];
+        const int tn = t0 + B;
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
+        for (int u = 0; u < B; ++u) nv[u] = (tn + u < frames) ? (int)tr[(size_t)(tn + u) * LS] : 0;
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
             if (t0 + u >= frames) break;
             const int f = fv[u];
             const bool isBl = f < thr;
@@ -121,16 +161,32 @@ __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLa
             const int cand = prev;
             prev = f;
             if (!isEdge) {
-                const float y = ((float)cand - lo) / bs;
-                const int bin = hw_floor_i32(y);
-                if (bin < 0) lowOut = (lowOut + 1) & 0xffffu;
-                else if (bin >= NB) highOut = (highOut + 1) & 0xffffu;
-                else cnt[bin * LS + z] = (uint16_t)(cnt[bin * LS + z] + 1);
+                const int bin = BinOf((float)cand - lo, bs, inv);
+                if (bin < 0) {
+                    lowOut = (lowOut + 1) & 0xffffu;
+                } else if (bin >= NB) {
+                    highOut = (highOut + 1) & 0xffffu;
+                } else {
+                    uint32_t* w = &cnt[(bin >> 1) * LS + z];
+                    const uint32_t sh = (uint32_t)(bin & 1) * 16u;
+                    if (fast) {
+                        atomicAdd(w, 1u << sh);
+                    } else {
+                        const uint32_t v = *w;
+                        const uint32_t c = ((v >> sh) + 1u) & 0xffffu;
+                        *w = (v & ~(0xffffu << sh)) | (c << sh);
+                    }
+                }
             }
         }
+#pragma unroll
+        for (int u = 0; u < B; ++u) fv[u] = nv[u];
     }
-    __syncthreads();
-    for (int b = 0; b < NB; ++b) h.bin_count[b][z] = cnt[b * LS + z];
+    for (int j = 0; j < NB / 2; ++j) {
+        const uint32_t v = cnt[j * LS + z];
+        h.bin_count[2 * j][z] = (uint16_t)(v & 0xffffu);
+        h.bin_count[2 * j + 1][z] = (uint16_t)(v >> 16);
+    }
     h.outlier_low[z] = (uint16_t)lowOut;
     h.outlier_high[z] = (uint16_t)highOut;
     e.prev[z] = (int16_t)prev;
@@ -159,7 +215,7 @@ int32_t HistAdd(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_
                 const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
 {
     if (!hist || !edge || !traces || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "hist_add: bad argument");
-    hipLaunchKernelGGL(HistAddKernel<8>, dim3(lanes), dim3(LS), 0, s, hist, (EdgeLane*)edge, traces, (int)frames,
+    hipLaunchKernelGGL(HistAddKernel<16>, dim3(lanes), dim3(LS), 0, s, hist, (EdgeLane*)edge, traces, (int)frames,
                        models);
     return CheckLaunch("HistAddKernel");
 }

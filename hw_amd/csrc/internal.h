@@ -85,6 +85,8 @@ struct LearnedWs {
     uint32_t lanes = 0;
     uint16_t* x = nullptr;            // [Z][128] bf16 network input
     float* y = nullptr;               // [Z][11] network output
+    uint16_t* h1 = nullptr;           // [Z][256] bf16 hidden activations
+    uint16_t* h2 = nullptr;
     uint8_t* fallback = nullptr;      // [Z]
     uint32_t* cnt = nullptr;          // [lanes] fallback ZMWs per lane
     uint32_t* off = nullptr;          // [lanes] exclusive prefix
@@ -101,7 +103,8 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
                            const hwpa_baseliner_stats* stats, hwpa_lane_model* models, const int32_t model_fi[2],
                            const int32_t est_fi[2], uint32_t lanes, int32_t* d_iters, float* scratch, hipStream_t s);
 int32_t MlpFeatures(const hwpa_lane_hist* hist, uint32_t lanes, uint16_t* x, hipStream_t s);
-int32_t MlpForward(hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, hipStream_t s);
+int32_t MlpForward(const hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, uint16_t* h1, uint16_t* h2,
+                   hipStream_t s);
 int32_t MlpTargets(const hwpa_lane_hist* hist, const float* diag, uint32_t lanes, float minConf, float* t,
                    uint8_t* valid, hipStream_t s);
 

@@ -410,25 +410,26 @@ static int32_t ColSum(const void* d, bool bf16, uint32_t rows, uint32_t cols, fl
     return CheckLaunch("ColSumFinalKernel");
 }
 
-// Forward pass over `rows` inputs into m->H1, m->H2 and y.
-static int32_t Forward(hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, hipStream_t s)
+// Forward pass over `rows` inputs through the activation buffers h1, h2 into y.
+static int32_t Forward(const hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, uint16_t* h1,
+                       uint16_t* h2, hipStream_t s)
 {
     const uint16_t* Wb = m->Pb;
     GemmArgs g = MakeGemm((int)rows, MH, MI, x, MI, 1, Wb + OFF_W1, 1, MI);
     g.bias = m->P + OFF_B1;
-    g.Cb = m->H1;
+    g.Cb = h1;
     g.c_m = MH;
     g.c_n = 1;
     int32_t rc = Gemm<true, true, EPI_BIAS_RELU_BF16>(g, s);
     if (rc != HWPA_OK) return rc;
-    g = MakeGemm((int)rows, MH, MH, m->H1, MH, 1, Wb + OFF_W2, 1, MH);
+    g = MakeGemm((int)rows, MH, MH, h1, MH, 1, Wb + OFF_W2, 1, MH);
     g.bias = m->P + OFF_B2;
-    g.Cb = m->H2;
+    g.Cb = h2;
     g.c_m = MH;
     g.c_n = 1;
     rc = Gemm<true, true, EPI_BIAS_RELU_BF16>(g, s);
     if (rc != HWPA_OK) return rc;
-    g = MakeGemm((int)rows, MO, MH, m->H2, MH, 1, Wb + OFF_W3, 1, MH);
+    g = MakeGemm((int)rows, MO, MH, h2, MH, 1, Wb + OFF_W3, 1, MH);
     g.bias = m->P + OFF_B3;
     g.Cf = y;
     g.c_m = MO;
@@ -440,7 +441,7 @@ static int32_t Forward(hwpa_initializer* m, const uint16_t* x, uint32_t rows, fl
 static int32_t Backward(hwpa_initializer* m, const uint16_t* x, const float* t, const uint8_t* valid, uint32_t batch,
                         hipStream_t s)
 {
-    int32_t rc = Forward(m, x, batch, m->Y, s);
+    int32_t rc = Forward(m, x, batch, m->Y, m->H1, m->H2, s);
     if (rc != HWPA_OK) return rc;
     hipLaunchKernelGGL(LossKernel, dim3((batch + 255) / 256), dim3(256), 0, s, m->Y, t, valid, batch, m->dY, m->dYb,
                        m->lossPer);
@@ -525,11 +526,10 @@ int32_t MlpTargets(const hwpa_lane_hist* hist, const float* diag, uint32_t lanes
     return CheckLaunch("TargetsKernel");
 }
 
-int32_t MlpForward(hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, hipStream_t s)
+int32_t MlpForward(const hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, uint16_t* h1, uint16_t* h2,
+                   hipStream_t s)
 {
-    int32_t rc = EnsureRows(m, rows);
-    if (rc != HWPA_OK) return rc;
-    return Forward(m, x, rows, y, s);
+    return Forward(m, x, rows, y, h1, h2, s);
 }
 
 }  // namespace hwpa
@@ -623,7 +623,9 @@ int32_t hwpa_mlp_forward(hwpa_initializer* m, const uint16_t* d_x, uint32_t rows
 {
     if (!m || !d_x || !d_y || rows == 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: null argument or no rows");
     if (((uintptr_t)d_x & 15) != 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: input must be 16-byte aligned");
-    return MlpForward(m, d_x, rows, d_y, (hipStream_t)stream);
+    int32_t rc = EnsureRows(m, rows);
+    if (rc != HWPA_OK) return rc;
+    return Forward(m, d_x, rows, d_y, m->H1, m->H2, (hipStream_t)stream);
 }
 
 int32_t hwpa_mlp_gradients(hwpa_initializer* m, const uint16_t* d_x, const float* d_t, const uint8_t* d_valid,

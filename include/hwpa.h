@@ -56,6 +56,7 @@ int32_t hwpa_transition_matrix(const hwpa_config* cfg, hwpa_transition* out);
 /* ---- per-pool analyzer (analyzer/BatchAnalyzer.h:51-106) -------------- */
 
 typedef struct hwpa_analyzer hwpa_analyzer;
+typedef struct hwpa_initializer hwpa_initializer;  /* learned EM initialiser, below */
 
 /* BatchAnalyzer ctor + DynamicEstimateBatchAnalyzer (analyzer/BatchAnalyzer.cpp:71-118). */
 int32_t hwpa_analyzer_create(uint32_t pool_id, uint32_t lanes, uint32_t frames_per_chunk,
@@ -94,6 +95,28 @@ int32_t hwpa_analyzer_stage_ms(const hwpa_analyzer* a, float ms[8]);
 int32_t hwpa_analyzer_em_iterations(const hwpa_analyzer* a, int32_t* iters);
 /* Stream used by the analyzer (hipStream_t). */
 int32_t hwpa_analyzer_stream(const hwpa_analyzer* a, void** stream);
+
+/* ---- batched pools (no reference counterpart: the reference runs one
+ * analyzer per pool on per-thread streams, Basecaller.h:160,246-250) -------
+ * n_pools analyzers (pool_ids[i], lanes[i], frames[i], cfgs[i]), each with its
+ * own state and stream.  hwpa_batch_process runs one chunk of every pool:
+ * d_traces[i] is pool i's device chunk, first_frames[i] its first frame.  By
+ * default the whole step is captured into one hipGraph (pools concurrent) and
+ * launched; executable graphs are cached per combination of the pools' stage
+ * sequences and updated in place.  Per-pool results come from the pool's
+ * analyzer (hwpa_batch_pool) after hwpa_batch_sync.  Stage profiling must be
+ * off while graphs are on. */
+typedef struct hwpa_batch hwpa_batch;
+int32_t hwpa_batch_create(uint32_t n_pools, const uint32_t* pool_ids, const uint32_t* lanes, const uint32_t* frames,
+                          const hwpa_config* cfgs, hwpa_batch** out);
+int32_t hwpa_batch_destroy(hwpa_batch* b);
+int32_t hwpa_batch_pool(hwpa_batch* b, uint32_t index, hwpa_analyzer** a);
+int32_t hwpa_batch_set_graph(hwpa_batch* b, int32_t enable);
+int32_t hwpa_batch_set_initializer(hwpa_batch* b, hwpa_initializer* m);
+int32_t hwpa_batch_process(hwpa_batch* b, const int16_t* const* d_traces, const int32_t* first_frames);
+int32_t hwpa_batch_sync(hwpa_batch* b);
+/* [0] steps run, [1] graph instantiations, [2] in-place graph updates. */
+int32_t hwpa_batch_graph_stats(const hwpa_batch* b, uint64_t stats[3]);
 
 /* ---- stage entry points (Device* strategies) ----------------------------- */
 
@@ -184,7 +207,6 @@ int32_t hwpa_pulser_run(hwpa_pulser* p, const int16_t* d_labels, const int16_t* 
                          HWPA_MLP_HIDDEN + HWPA_MLP_OUT * HWPA_MLP_HIDDEN + HWPA_MLP_OUT)
 #define HWPA_EM_DIAG_ROWS 18
 
-typedef struct hwpa_initializer hwpa_initializer;
 /* He-uniform weights from a 64-bit seed, zero biases. */
 int32_t hwpa_initializer_create(uint64_t seed, hwpa_initializer** out);
 int32_t hwpa_initializer_destroy(hwpa_initializer* m);

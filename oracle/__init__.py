@@ -127,6 +127,9 @@ class Histogram:
         lib.orc_hist_get(self.h, _p(out))
         return out
 
+    def set(self, hist):
+        lib.orc_hist_set(self.h, _p(np.ascontiguousarray(hist)))
+
     def __del__(self):
         if getattr(self, "h", None):
             lib.orc_hist_destroy(self.h)

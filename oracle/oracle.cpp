@@ -1239,6 +1239,26 @@ void orc_hist_add(void* h, const int16_t* traces, int32_t frames, const hwpa_lan
     });
 }
 
+// Loads bounds, counts and outliers of each lane (the LaneHistogram view of
+// UHistogramSimd, dataTypes/UHistogramSimd.cpp:100-118); edge state is kept.
+void orc_hist_set(void* h, const hwpa_lane_hist* in)
+{
+    auto* c = static_cast<HistCtx*>(h);
+    for (int l = 0; l < c->lanes; ++l) {
+        LaneHistState& H = c->h[l];
+        for (int z = 0; z < LS; ++z) {
+            const float lo = in[l].low_bound[z], bs = in[l].bin_size[z];
+            H.binSize[z] = bs;
+            H.binStart[0][z] = lo;
+            for (int i = 1; i <= NB; ++i) H.binStart[i][z] = bs * (float)i + lo;
+            H.lowOut[z] = in[l].outlier_low[z];
+            H.highOut[z] = in[l].outlier_high[z];
+            for (int b = 0; b < NB; ++b) H.count[b][z] = in[l].bin_count[b][z];
+            H.count[NB][z] = 0;
+        }
+    }
+}
+
 void orc_hist_get(void* h, hwpa_lane_hist* out)
 {
     auto* c = static_cast<HistCtx*>(h);

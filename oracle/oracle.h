@@ -65,6 +65,7 @@ void orc_hist_destroy(void* h);
 int32_t orc_hist_reset(void* h, const hwpa_baseliner_stats* stats);
 void orc_hist_add(void* h, const int16_t* traces, int32_t frames, const hwpa_lane_model* models);
 void orc_hist_get(void* h, hwpa_lane_hist* out);
+void orc_hist_set(void* h, const hwpa_lane_hist* in);
 
 void orc_dme_init_models(const hwpa_config* cfg, int32_t lanes,
                          const hwpa_baseliner_stats* stats, hwpa_lane_model* out);

@@ -127,6 +127,38 @@ def test_histogram_bit_exact(H, O, T, gpu):
         _eq_struct(H.to_numpy(d_hist, T.LANE_HIST, (L,)), hist.get(T.LANE_HIST), "hist[%d]" % c)
 
 
+def test_histogram_bin_edges_wrap_and_degenerate_bounds(H, O, T, gpu):
+    """Samples exactly on and next to bin edges (1/binSize inexact), counters at
+    the 16-bit wrap, and zero / NaN bin sizes, against the oracle."""
+    L, F = 3, 512
+    rng = np.random.default_rng(5)
+    cfg = H.bench_config(F)
+    h0 = np.zeros(L, dtype=T.LANE_HIST)
+    h0["low_bound"] = -40.0
+    h0["bin_size"] = rng.choice(np.array([2.4, 3.3, 1.7, 2.43, 0.3, 1.1, 7.9], dtype=np.float32), size=(L, 64))
+    h0["bin_size"][2, :4] = [0.0, np.nan, 1e-30, 3e30]
+    h0["low_bound"][2, 4:8] = [np.nan, np.inf, -1e9, 1e9]
+    h0["bin_count"][1] = rng.integers(65000, 65536, size=(300, 64)).astype(np.uint16)   # wrap path
+    h0["outlier_low"][1] = 65535
+    h0["outlier_high"][1] = 65534
+    # every integer in [-60, 450) plus the int16 extremes, shuffled per ZMW
+    vals = np.concatenate([np.arange(-60, 450), [-32768, 32767]]).astype(np.int16)
+    tr = np.stack([np.stack([rng.permutation(vals) for _ in range(64)], axis=1) for _ in range(L)])
+    models = np.zeros(L, dtype=T.LANE_MODEL)
+    models["baseline"]["means"] = np.float16(20000.0).view(np.uint16)   # every frame is "baseline": none are edges
+    models["baseline"]["vars"] = np.float16(1.0).view(np.uint16)
+    oh = O.Histogram(cfg, L)
+    oh.set(h0)
+    oh.add(tr, models)
+    d_h = _dev(H, h0)
+    d_edge = H.dev_bytes(L * H.edge_state_bytes())
+    H.edge_state_init(d_edge, L)
+    H.hist_add(d_h, d_edge, _dev(H, tr), F, _dev(H, models), L)
+    got, want = H.to_numpy(d_h, T.LANE_HIST, (L,)), oh.get(T.LANE_HIST)
+    assert want["bin_count"][0].sum() > 1000
+    _eq_struct(got, want, "hist")
+
+
 def test_dme_init_bit_exact(H, O, T, gpu):
     L, F, C = 4, 512, 3
     cfg, chunks = _pipeline_inputs(H, O, T, L, F, C, seed=9)
