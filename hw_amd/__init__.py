@@ -201,6 +201,37 @@ def count_equal_i16(a, b, n):
     return out.value
 
 
+TUNE_HISTOGRAM, TUNE_BASELINE = 0, 1
+
+
+def tune_variants(kernel):
+    n = ctypes.c_int32(0)
+    check(lib.hwpa_tune_variants(ctypes.c_int32(kernel), ctypes.byref(n)))
+    return n.value
+
+
+def tune_set(kernel, lanes, frames, variant):
+    check(lib.hwpa_tune_set(ctypes.c_int32(kernel), ctypes.c_uint32(lanes), ctypes.c_uint32(frames),
+                            ctypes.c_int32(variant)))
+
+
+def tune_get(kernel, lanes, frames):
+    v = ctypes.c_int32(0)
+    check(lib.hwpa_tune_get(ctypes.c_int32(kernel), ctypes.c_uint32(lanes), ctypes.c_uint32(frames), ctypes.byref(v)))
+    return v.value
+
+
+def autotune(cfg, lanes, frames, repeats=3):
+    """Times every kernel variant on this shape and keeps the fastest; returns
+    {"histogram": [ms...], "baseline": [ms...], "chosen": (hist, baseline)}."""
+    nh, nb = tune_variants(TUNE_HISTOGRAM), tune_variants(TUNE_BASELINE)
+    ms = (ctypes.c_float * (nh + nb))()
+    check(lib.hwpa_autotune(ctypes.byref(cfg), ctypes.c_uint32(lanes), ctypes.c_uint32(frames),
+                            ctypes.c_int32(repeats), ms, ctypes.c_int32(nh + nb)))
+    return {"histogram": list(ms)[:nh], "baseline": list(ms)[nh:],
+            "chosen": (tune_get(TUNE_HISTOGRAM, lanes, frames), tune_get(TUNE_BASELINE, lanes, frames))}
+
+
 def copy_bandwidth(nbytes=1 << 30, launches=10):
     """Mean ms per launch of the library's plain copy kernel over nbytes, and
     the resulting read+write GB/s."""
@@ -309,6 +340,17 @@ class Initializer:
     def set_params(self, p):
         p = np.ascontiguousarray(p, dtype=np.float32)
         check(lib.hwpa_initializer_set_params(self.h, p.ctypes.data_as(_vp), ctypes.c_size_t(p.size)))
+
+    def set_output_map(self, offset, scale, weight):
+        off = (ctypes.c_float * 6)(*[float(v) for v in offset])
+        sc = (ctypes.c_float * 6)(*[float(v) for v in scale])
+        w = (ctypes.c_float * 7)(*[float(v) for v in weight])
+        check(lib.hwpa_initializer_set_output_map(self.h, off, sc, w))
+
+    def output_map(self):
+        out = (ctypes.c_float * 19)()
+        check(lib.hwpa_initializer_get_output_map(self.h, out))
+        return np.array(out, dtype=np.float32)
 
     def forward(self, d_x, rows, d_y, stream=None):
         check(lib.hwpa_mlp_forward(self.h, ptr(d_x), ctypes.c_uint32(rows), ptr(d_y), _vp(stream)))
@@ -555,6 +597,9 @@ class Batch:
 
     def sync(self):
         check(lib.hwpa_batch_sync(self.h))
+
+    def autotune(self, repeats=3):
+        check(lib.hwpa_batch_autotune(self.h, ctypes.c_int32(repeats)))
 
     def graph_stats(self):
         st = (ctypes.c_uint64 * 3)()

@@ -680,6 +680,22 @@ int32_t hwpa_batch_sync(hwpa_batch* b)
     return HWPA_OK;
 }
 
+int32_t hwpa_batch_autotune(hwpa_batch* b, int32_t repeats)
+{
+    if (!b) return Fail(HWPA_ERR_INVALID_ARG, "null batch");
+    HWPA_HIP(hipStreamSynchronize(b->master));
+    for (size_t i = 0; i < b->pools.size(); ++i) {
+        const hwpa_analyzer* a = b->pools[i];
+        bool seen = false;
+        for (size_t j = 0; j < i && !seen; ++j)
+            seen = b->pools[j]->lanes == a->lanes && b->pools[j]->frames == a->frames;
+        if (seen) continue;
+        const int32_t rc = hwpa_autotune(&a->cfg, a->lanes, a->frames, repeats, nullptr, 0);
+        if (rc != HWPA_OK) return rc;
+    }
+    return HWPA_OK;
+}
+
 int32_t hwpa_batch_graph_stats(const hwpa_batch* b, uint64_t stats[3])
 {
     if (!b || !stats) return Fail(HWPA_ERR_INVALID_ARG, "null argument");

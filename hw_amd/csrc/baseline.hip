@@ -157,19 +157,21 @@ __device__ __forceinline__ void RingRefill(uint32_t* ring, uint32_t last)
     }
 }
 
-template <class T>
-__global__ __launch_bounds__(64) void BaselineKernel(const int16_t* __restrict__ raw, int frames,
-                                                     int16_t* __restrict__ out,
-                                                     hwpa_baseliner_stats* __restrict__ stats,
-                                                     BlLaneState* __restrict__ state,
-                                                     uint32_t* __restrict__ win, BlConsts k)
+// LPB lanes (waves) per workgroup -- the autotuned lane tile.
+template <class T, int LPB>
+__global__ __launch_bounds__(64 * LPB) void BaselineKernel(const int16_t* __restrict__ raw, int frames, int lanes,
+                                                           int16_t* __restrict__ out,
+                                                           hwpa_baseliner_stats* __restrict__ stats,
+                                                           BlLaneState* __restrict__ state,
+                                                           uint32_t* __restrict__ win, BlConsts k)
 {
     constexpr int S = T::S;
     constexpr int NST = T::nst;
     constexpr int W0 = T::w[0], N0 = T::N0, N1 = T::N1, W1 = T::W1R, WF = T::WF;
 
-    const int lane = blockIdx.x;
-    const int z = threadIdx.x;
+    const int lane = blockIdx.x * LPB + (int)(threadIdx.x >> 6);
+    const int z = threadIdx.x & 63;
+    if (lane >= lanes) return;
     BlLaneState& st = state[lane];
     uint32_t* wl = win + (size_t)lane * T::winPerLane * LS + z;
 
@@ -398,8 +400,18 @@ static int32_t Launch(BaselinerDev* b, const int16_t* raw, uint32_t frames, int1
 {
     if (frames % T::S != 0) return Fail(HWPA_ERR_INVALID_ARG, "frames must be a multiple of the aggregate stride");
     if ((size_t)T::winPerLane != b->winPerLane) return Fail(HWPA_ERR_UNSUPPORTED, "baseliner: window layout mismatch");
-    hipLaunchKernelGGL(BaselineKernel<T>, dim3(b->lanes), dim3(LS), 0, s, raw, (int)frames, out, stats,
-                       b->state, b->win, b->k);
+    const int v = TuneGet(kTuneBaseline, b->lanes, frames);
+    const int lpb = (v == 1) ? 2 : (v == 2 ? 4 : 1);
+    const unsigned grid = (b->lanes + (unsigned)lpb - 1) / (unsigned)lpb;
+    if (lpb == 1)
+        hipLaunchKernelGGL((BaselineKernel<T, 1>), dim3(grid), dim3(64), 0, s, raw, (int)frames, (int)b->lanes, out,
+                           stats, b->state, b->win, b->k);
+    else if (lpb == 2)
+        hipLaunchKernelGGL((BaselineKernel<T, 2>), dim3(grid), dim3(128), 0, s, raw, (int)frames, (int)b->lanes, out,
+                           stats, b->state, b->win, b->k);
+    else
+        hipLaunchKernelGGL((BaselineKernel<T, 4>), dim3(grid), dim3(256), 0, s, raw, (int)frames, (int)b->lanes, out,
+                           stats, b->state, b->win, b->k);
     return CheckLaunch("BaselineKernel");
 }
 

@@ -253,6 +253,7 @@ enum { EM_PLAIN = 0, EM_LEARNED = 1 };
 // Extra inputs/outputs of the EM kernel beyond the reference's.
 struct EmExtra {
     const float* init;        // EM_LEARNED: network outputs [lane*64 + z][HWPA_MLP_OUT]
+    const float* omap;        // EM_LEARNED: output map {off[6], scale[6], ...}
     uint8_t* fallback;        // EM_LEARNED: 1 = start rejected, model left for the plain EM
     float* diag;              // optional raw estimate [lane][HWPA_EM_DIAG_ROWS][64]
     const uint32_t* active;   // optional: number of valid ZMWs; lanes at or beyond it exit
@@ -384,11 +385,14 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
     const float halfBin = 0.5f * h.bs;
     int iterLimit = d.iterLimit;
     if (MODE == EM_LEARNED) {
-        // Network start: outputs in units of u = 2 binSize from the lower bound,
-        // log variance / 4 and softmax mixture logits.
+        // Network start: output j < 6 stands for off[j] + scale[j] * y[j] in units
+        // of u = 2 binSize from the lower bound (means) and log(var / u^2);
+        // outputs 6..10 are softmax mixture logits.
         const float* y = x.init + ((size_t)lane * LS + z) * HWPA_MLP_OUT;
         const float u = 2.0f * h.bs;
-        const float su = 32.0f * u;
+        float r[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) r[j] = x.omap[j] + x.omap[6 + j] * y[j];
         float mx = y[6];
 #pragma unroll
         for (int k = 1; k < 5; ++k) mx = hw_maxf(mx, y[6 + k]);
@@ -403,11 +407,11 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
         const float rs = Sum5Tree(rho);
 #pragma unroll
         for (int k = 0; k < 5; ++k) rho[k] = rho[k] / rs;
-        mu[0] = h.lb + su * y[0];
-        var[0] = hw_clampf((u * u) * hw_expf(4.0f * y[1]), d.blVarMin, d.blVarMax);
+        mu[0] = h.lb + u * r[0];
+        var[0] = hw_clampf((u * u) * hw_expf(r[1]), d.blVarMin, d.blVarMax);
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
-            mu[a + 1] = h.lb + su * y[2 + a];
+            mu[a + 1] = h.lb + u * r[2 + a];
             var[a + 1] = Covar(d, d.cv[a] * d.cv[a], mu[a + 1], var[0]);
         }
 #pragma unroll
@@ -868,6 +872,7 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
     if (rc != HWPA_OK) return rc;
     EmExtra x{};
     x.init = w.y;
+    x.omap = MlpOutputMap(mlp);
     x.fallback = w.fallback;
     x.acceptConf = cfg.learned_conf_threshold;
     x.convTol = cfg.learned_conv_tol;

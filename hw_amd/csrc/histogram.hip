@@ -3,8 +3,10 @@
 // UHistogramSimd binning, dataTypes/UHistogramSimd.{h,cpp}; EdgeFrameClassifier,
 // traceAnalysis/EdgeFrameClassifier.h).
 //
-// Mapping: one wave64 per lane, one thread per ZMW.  The lane's 300x64 u16
-// counters (38.4 KB) are staged in LDS for the chunk; each thread owns one
+// Mapping: one wave64 per lane, one thread per ZMW.  The counters of bins
+// [0, 128) -- where baseline and pulses of ordinary signal levels land -- are
+// staged in LDS for the chunk (16 KB per lane, so about nine waves fit a CU);
+// higher bins are counted with global atomics in place.  Each thread owns one
 // ZMW column, and the [bin pair][zmw] layout puts the 64 lanes of every
 // increment on distinct LDS banks whatever the bins.  Frames are read in
 // batches of 16 with the next batch in flight while one is binned.
@@ -21,7 +23,8 @@ struct EdgeLane {
     uint8_t hist[2][LS];
     int32_t i0;
     int32_t i1;
-    int32_t pad[30];
+    int32_t needWrap;  // set by the fast pass for lanes left to the 16-bit wrapping pass
+    int32_t pad[29];
 };
 
 size_t EdgeStateBytes() { return sizeof(EdgeLane); }
@@ -36,6 +39,7 @@ __global__ __launch_bounds__(64) void EdgeInitKernel(EdgeLane* e)
     if (z == 0) {
         s.i0 = 0;
         s.i1 = -1;
+        s.needWrap = 0;
     }
 }
 
@@ -80,47 +84,134 @@ __global__ __launch_bounds__(64) void HistResetKernel(hwpa_lane_hist* hist, cons
 // quotient is first formed as (x - lo) * (1 / bs), which is within
 // 3 * 2^-24 |y| of the divided value; for |y| < 2048 that is below 3.7e-4, so
 // when the product is more than 1/1024 away from an integer both share the
-// same floor, and only near a bin edge is the divide itself evaluated.  For
-// 2048 <= |y| < 2^30 the sample is an outlier on the same side either way;
-// NaN, infinities and huge quotients take the divide (whose float -> int
-// conversion gives 0x80000000 out of range, a low outlier, as on the host).
+// same floor.  For 2048 <= |y| < 2^30 the sample is an outlier on the same side
+// either way.  Everything else -- near a bin edge, NaN, infinities, huge
+// quotients -- takes the divide (whose float -> int conversion gives
+// 0x80000000 out of range, a low outlier, as on the host); the divide runs
+// only when some lane of the wave needs it.
 __device__ __forceinline__ int BinOf(float a, float bs, float inv)
 {
     const float yp = a * inv;
     const float ay = hw_fabsf(yp);
-    if (ay >= 2048.0f && ay < 1073741824.0f) return (yp > 0.0f) ? NB : -1;
-    if (ay < 2048.0f) {
-        const float fl = hw_floorf(yp);
-        const float d = yp - fl;
-        if (d >= 0.0009765625f && d <= 0.9990234375f) return (int)fl;
+    const bool far = ay >= 2048.0f && ay < 1073741824.0f;
+    const float fl = hw_floorf(yp);
+    const float d = yp - fl;
+    const bool clear = ay < 2048.0f && d >= 0.0009765625f && d <= 0.9990234375f;
+    // Branch-free: clamp into [-1, NB] in float, pick the outlier side, convert once.
+    float bf = __builtin_fminf(__builtin_fmaxf(fl, -1.0f), (float)NB);
+    const float side = (yp > 0.0f) ? (float)NB : -1.0f;
+    bf = far ? side : bf;
+    int bin = (int)bf;
+    const bool exact = !far && !clear;
+    if (__any(exact)) {
+        const int e = hw_floor_i32(a / bs);
+        bin = exact ? e : bin;
     }
-    return hw_floor_i32(a / bs);
+    return bin;
 }
 
 // TraceHistogramAccumHost::AddBlock (TraceHistogramAccumHost.cpp:128-169).
 // Counters live in LDS as words holding bins 2j (low half) and 2j+1 (high
 // half) of one ZMW, so an increment is a single ds_add_u32 that nothing waits
-// on.  That cannot carry into the neighbouring half while every count stays
-// below 65536 -- checked before the chunk; otherwise the wave increments with
-// a 16-bit wrapping read-modify-write, the reference's u16 counter semantics.
-template <int B>
+// on (frames that are not counted add 0).  That cannot carry into the
+// neighbouring half while every count stays below 65536 -- checked before the
+// chunk; otherwise the wave takes the WRAP variant, a 16-bit wrapping
+// read-modify-write, the reference's u16 counter semantics.
+// Bins [0, NL) are counted in LDS; the fast pass (NL < NB) counts the rest with
+// 32-bit global atomics on the lane's histogram in HBM, each word holding the
+// 16-bit counters of ZMWs z and z^1 for one bin (no carry under the same
+// 65536 guard).
+template <int B, bool WRAP, int NL>
+__device__ __forceinline__ void HistFrames(uint32_t* cnt, hwpa_lane_hist& hg, const int16_t* tr, int frames, int z,
+                                           float lo, float bs, float inv, int thr, int& prev, bool& bl1, bool& bl2,
+                                           int& pushes, uint32_t& lowOut, uint32_t& highOut)
+{
+    // One frame: EdgeFrameClassifier::IsEdgeFrame then UHistogramSimd::AddDatum.
+    // The classifier's two-entry circular array is a shift register here:
+    // bl1/bl2 = baseline flags of the previous two frames, pushes saturates
+    // at 2 (lane-uniform); the previous frame counts when the array was full
+    // and its two neighbours agree.
+    auto frame = [&](int f) {
+        const bool isBl = f < thr;
+        const bool counted = (pushes >= 2) && (isBl == bl2);
+        bl2 = bl1;
+        bl1 = isBl;
+        pushes = (pushes < 2) ? pushes + 1 : 2;
+        const int cand = prev;
+        prev = f;
+        const int bin = BinOf((float)cand - lo, bs, inv);
+        const bool low = bin < 0, high = bin >= NB;
+        lowOut = (lowOut + ((counted && low) ? 1u : 0u)) & 0xffffu;
+        highOut = (highOut + ((counted && high) ? 1u : 0u)) & 0xffffu;
+        const bool in = counted && !low && !high;
+        const int bc = low ? 0 : (high ? NB - 1 : bin);
+        if (NL < NB) {
+            const bool glob = in && bc >= NL;
+            if (__any(glob)) {
+                if (glob) {
+                    uint32_t* gw = reinterpret_cast<uint32_t*>(&hg.bin_count[bc][z & ~1]);
+                    atomicAdd(gw, 1u << ((uint32_t)(z & 1) * 16u));
+                }
+            }
+        }
+        const int bl = (NL < NB && bc >= NL) ? NL - 1 : bc;
+        uint32_t* w = &cnt[(bl >> 1) * LS + z];
+        const uint32_t sh = (uint32_t)(bl & 1) * 16u;
+        const bool inl = in && (NL == NB || bc < NL);
+        if (!WRAP) {
+            atomicAdd(w, inl ? (1u << sh) : 0u);
+        } else if (inl) {
+            const uint32_t v = *w;
+            const uint32_t c = ((v >> sh) + 1u) & 0xffffu;
+            *w = (v & ~(0xffffu << sh)) | (c << sh);
+        }
+    };
+    const int full = frames / B;
+    int fv[B];
+    if (full > 0) {
+#pragma clang loop unroll(full)
+        for (int u = 0; u < B; ++u) fv[u] = tr[(size_t)u * LS];
+    }
+    for (int g = 0; g < full; ++g) {
+        int nv[B];
+        const int gn = (g + 1 < full) ? g + 1 : g;
+#pragma clang loop unroll(full)
+        for (int u = 0; u < B; ++u) nv[u] = tr[(size_t)(gn * B + u) * LS];
+#pragma clang loop unroll(full)
+        for (int u = 0; u < B; ++u) frame(fv[u]);
+#pragma clang loop unroll(full)
+        for (int u = 0; u < B; ++u) fv[u] = nv[u];
+    }
+    for (int t = full * B; t < frames; ++t) frame(tr[(size_t)t * LS]);
+}
+
+// Two passes so each is compiled on its own: the fast pass runs every lane
+// whose counts stay below 65536 and flags the others (leaving them untouched);
+// the wrap pass then runs only the flagged lanes.
+template <int B, bool WRAP, int NL>
 __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLane* edge,
                                                     const int16_t* __restrict__ traces, int frames,
                                                     const hwpa_lane_model* __restrict__ models)
 {
-    __shared__ uint32_t cnt[(NB / 2) * LS];
+    static_assert(NL % 2 == 0 && NL <= NB, "LDS bins");
+    __shared__ uint32_t cnt[(NL / 2) * LS];
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
     hwpa_lane_hist& h = hist[lane];
     EdgeLane& e = edge[lane];
+    if (WRAP && e.needWrap == 0) return;
     uint32_t mx = 0;
     for (int j = 0; j < NB / 2; ++j) {
         const uint32_t c0 = h.bin_count[2 * j][z], c1 = h.bin_count[2 * j + 1][z];
         mx = c0 > mx ? c0 : mx;
         mx = c1 > mx ? c1 : mx;
-        cnt[j * LS + z] = c0 | (c1 << 16);
+        if (2 * j < NL) cnt[j * LS + z] = c0 | (c1 << 16);
     }
     const bool fast = __all(mx + (uint32_t)frames <= 65535u);
+    if (!WRAP && !fast) {
+        if (z == 0) e.needWrap = 1;
+        return;
+    }
     uint32_t lowOut = h.outlier_low[z], highOut = h.outlier_high[z];
     const float lo = h.low_bound[z];
     const float bs = h.bin_size[z];
@@ -132,69 +223,31 @@ __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLa
     const float bmean = hw_h2f(m.baseline.means[z]);
     const int thr = hw_wrap16(hw_round_i32(2.0f * hw_sqrtf(bvar) + bmean));
 
+    // CircularArray state (h[2], head i0, tail i1; i1 < 0 when empty) -> shift register.
     int prev = e.prev[z];
-    bool h0 = e.hist[0][z] != 0, h1 = e.hist[1][z] != 0;
-    int i0 = e.i0, i1 = e.i1;
+    const int i0 = e.i0, i1 = e.i1;
+    int pushes = (i1 < 0) ? 0 : ((2 + i1 - i0) & 1) + 1;
+    bool bl1 = (i1 >= 0) && e.hist[i1 & 1][z] != 0;
+    bool bl2 = (pushes == 2) && e.hist[i0 & 1][z] != 0;
 
     const int16_t* tr = traces + (size_t)lane * frames * LS + z;
-    int fv[B];
-#pragma unroll
-    for (int u = 0; u < B; ++u) fv[u] = (u < frames) ? (int)tr[(size_t)u * LS] : 0;
-    for (int t0 = 0; t0 < frames; t0 += B) {
-        int nv[B];
-        const int tn = t0 + B;
-#pragma unroll
-        for (int u = 0; u < B; ++u) nv[u] = (tn + u < frames) ? (int)tr[(size_t)(tn + u) * LS] : 0;
-#pragma unroll
-        for (int u = 0; u < B; ++u) {
-            if (t0 + u >= frames) break;
-            const int f = fv[u];
-            const bool isBl = f < thr;
-            // CircularArray<LaneMask,2>: full when two entries have been pushed
-            const bool full = (i1 >= 0) && (((2 + i1 - i0) & 1) + 1) == 2;
-            const bool front = (i0 == 0) ? h0 : h1;
-            const bool isEdge = full ? (isBl != front) : true;
-            if (full) i0 = (i0 + 1) & 1;
-            i1 = (i1 + 1) & 1;
-            if (i1 == 0) h0 = isBl;
-            else h1 = isBl;
-            const int cand = prev;
-            prev = f;
-            if (!isEdge) {
-                const int bin = BinOf((float)cand - lo, bs, inv);
-                if (bin < 0) {
-                    lowOut = (lowOut + 1) & 0xffffu;
-                } else if (bin >= NB) {
-                    highOut = (highOut + 1) & 0xffffu;
-                } else {
-                    uint32_t* w = &cnt[(bin >> 1) * LS + z];
-                    const uint32_t sh = (uint32_t)(bin & 1) * 16u;
-                    if (fast) {
-                        atomicAdd(w, 1u << sh);
-                    } else {
-                        const uint32_t v = *w;
-                        const uint32_t c = ((v >> sh) + 1u) & 0xffffu;
-                        *w = (v & ~(0xffffu << sh)) | (c << sh);
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < B; ++u) fv[u] = nv[u];
-    }
-    for (int j = 0; j < NB / 2; ++j) {
+    HistFrames<B, WRAP, NL>(cnt, h, tr, frames, z, lo, bs, inv, thr, prev, bl1, bl2, pushes, lowOut, highOut);
+
+    for (int j = 0; j < NL / 2; ++j) {
         const uint32_t v = cnt[j * LS + z];
         h.bin_count[2 * j][z] = (uint16_t)(v & 0xffffu);
         h.bin_count[2 * j + 1][z] = (uint16_t)(v >> 16);
     }
     h.outlier_low[z] = (uint16_t)lowOut;
     h.outlier_high[z] = (uint16_t)highOut;
+    // Back to the circular-array form: oldest flag at the head.
     e.prev[z] = (int16_t)prev;
-    e.hist[0][z] = h0 ? 1 : 0;
-    e.hist[1][z] = h1 ? 1 : 0;
+    e.hist[0][z] = (pushes == 2 ? bl2 : bl1) ? 1 : 0;
+    e.hist[1][z] = bl1 ? 1 : 0;
     if (z == 0) {
-        e.i0 = i0;
-        e.i1 = i1;
+        e.i0 = 0;
+        e.i1 = (pushes == 0) ? -1 : (pushes == 1 ? 0 : 1);
+        e.needWrap = 0;
     }
 }
 
@@ -211,13 +264,43 @@ int32_t HistReset(const hwpa_config& cfg, hwpa_lane_hist* hist, const hwpa_basel
     return CheckLaunch("HistResetKernel");
 }
 
+template <int B, int NL>
+static int32_t HistAddLaunch(hwpa_lane_hist* hist, EdgeLane* edge, const int16_t* traces, uint32_t frames,
+                             const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
+{
+    hipLaunchKernelGGL((HistAddKernel<B, false, NL>), dim3(lanes), dim3(LS), 0, s, hist, edge, traces, (int)frames,
+                       models);
+    const int32_t rc = CheckLaunch("HistAddKernel");
+    if (rc != HWPA_OK) return rc;
+    hipLaunchKernelGGL((HistAddKernel<16, true, NB>), dim3(lanes), dim3(LS), 0, s, hist, edge, traces, (int)frames,
+                       models);
+    return CheckLaunch("HistAddKernel");
+}
+
+// Histogram variants: frames per prefetched batch x bins kept in LDS.
+int32_t HistAddVariant(int32_t variant, hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
+                       const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
+{
+    if (!hist || !edge || !traces || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "hist_add: bad argument");
+    EdgeLane* e = (EdgeLane*)edge;
+    switch (variant) {
+    case 0: return HistAddLaunch<16, 128>(hist, e, traces, frames, models, lanes, s);
+    case 1: return HistAddLaunch<8, 128>(hist, e, traces, frames, models, lanes, s);
+    case 2: return HistAddLaunch<32, 128>(hist, e, traces, frames, models, lanes, s);
+    case 3: return HistAddLaunch<16, 96>(hist, e, traces, frames, models, lanes, s);
+    case 4: return HistAddLaunch<16, 64>(hist, e, traces, frames, models, lanes, s);
+    case 5: return HistAddLaunch<32, 96>(hist, e, traces, frames, models, lanes, s);
+    case 6: return HistAddLaunch<32, 64>(hist, e, traces, frames, models, lanes, s);
+    case 7: return HistAddLaunch<8, 96>(hist, e, traces, frames, models, lanes, s);
+    case 8: return HistAddLaunch<8, 64>(hist, e, traces, frames, models, lanes, s);
+    default: return Fail(HWPA_ERR_INVALID_ARG, "hist_add: unknown variant");
+    }
+}
+
 int32_t HistAdd(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
                 const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
 {
-    if (!hist || !edge || !traces || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "hist_add: bad argument");
-    hipLaunchKernelGGL(HistAddKernel<16>, dim3(lanes), dim3(LS), 0, s, hist, (EdgeLane*)edge, traces, (int)frames,
-                       models);
-    return CheckLaunch("HistAddKernel");
+    return HistAddVariant(TuneGet(kTuneHistogram, lanes, frames), hist, edge, traces, frames, models, lanes, s);
 }
 
 }  // namespace hwpa

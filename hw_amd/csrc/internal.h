@@ -72,6 +72,15 @@ int32_t HistReset(const hwpa_config& cfg, hwpa_lane_hist* hist, const hwpa_basel
 int32_t HistAdd(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
                 const hwpa_lane_model* models, uint32_t lanes, hipStream_t s);
 
+// Kernel shape variants chosen per (lanes, frames) by the autotuner; every
+// variant computes the same bits.  Variant 0 is the default.
+enum TuneKernel { kTuneHistogram = 0, kTuneBaseline = 1, kTuneKernels = 2 };
+int32_t TuneVariants(int kernel);
+int32_t TuneGet(int kernel, uint32_t lanes, uint32_t frames);
+void TuneSet(int kernel, uint32_t lanes, uint32_t frames, int32_t variant);
+int32_t HistAddVariant(int32_t variant, hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
+                       const hwpa_lane_model* models, uint32_t lanes, hipStream_t s);
+
 int32_t DmeInitModels(const hwpa_config& cfg, const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
                       uint32_t lanes, hipStream_t s);
 int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
@@ -105,6 +114,7 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
 int32_t MlpFeatures(const hwpa_lane_hist* hist, uint32_t lanes, uint16_t* x, hipStream_t s);
 int32_t MlpForward(const hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, uint16_t* h1, uint16_t* h2,
                    hipStream_t s);
+const float* MlpOutputMap(const hwpa_initializer* m);  // device {off[6], scale[6], w[7]}
 int32_t MlpTargets(const hwpa_lane_hist* hist, const float* diag, uint32_t lanes, float minConf, float* t,
                    uint8_t* valid, hipStream_t s);
 

@@ -166,9 +166,9 @@ __global__ __launch_bounds__(64) void FeaturesKernel(const hwpa_lane_hist* __res
 }
 
 // ---- training targets from the EM diagnostics ---------------------------------
-// Units: u = 2 * binSize (one re-binned bin), origin at the histogram's lower bound.
-//   t0 = (mu_bl - lo) / (32 u)   t1 = log(var_bl / u^2) / 4   t2..5 = (mu_a - lo) / (32 u)
-//   t6..10 = mixture weights (softmax targets).
+// Raw units: u = 2 * binSize (one re-binned bin), origin at the histogram's
+// lower bound:  r0 = (mu_bl - lo) / u,  r1 = log(var_bl / u^2),
+// r2..5 = (mu_a - lo) / u,  r6..10 = mixture weights (softmax targets).
 __global__ __launch_bounds__(64) void TargetsKernel(const hwpa_lane_hist* __restrict__ hists,
                                                     const float* __restrict__ diag, float minConf,
                                                     float* __restrict__ t, uint8_t* __restrict__ valid)
@@ -176,23 +176,25 @@ __global__ __launch_bounds__(64) void TargetsKernel(const hwpa_lane_hist* __rest
     const int lane = blockIdx.x, z = threadIdx.x;
     const float* dg = diag + (size_t)lane * HWPA_EM_DIAG_ROWS * LS + z;
     const float lo = hists[lane].low_bound[z], u = 2.0f * hists[lane].bin_size[z];
-    const float inv = 1.0f / (32.0f * u);
     float* o = t + ((size_t)lane * LS + z) * MO;
     const float var0 = dg[(10 + 0) * LS];
-    o[0] = (dg[(5 + 0) * LS] - lo) * inv;
-    o[1] = hw_logf(var0 / (u * u)) * 0.25f;
-    for (int a = 0; a < NA; ++a) o[2 + a] = (dg[(5 + 1 + a) * LS] - lo) * inv;
+    o[0] = (dg[(5 + 0) * LS] - lo) / u;
+    o[1] = hw_logf(var0 / (u * u));
+    for (int a = 0; a < NA; ++a) o[2 + a] = (dg[(5 + 1 + a) * LS] - lo) / u;
     for (int k = 0; k < 5; ++k) o[6 + k] = dg[k * LS];
     const bool ok = dg[16 * LS] > 0.0f && dg[17 * LS] == 0.0f && dg[15 * LS] >= minConf &&
                     hw_isfinite(o[0]) && hw_isfinite(o[1]) && var0 > 0.0f;
     valid[(size_t)lane * LS + z] = ok ? 1 : 0;
 }
 
-// ---- loss: squared error on t0..t5, softmax cross-entropy on the weights ----
+// ---- loss ----------------------------------------------------------------------
+// Network output j < 6 stands for the raw target off[j] + scale[j] * y[j]; the
+// loss is sum_j w[j] (y[j] - (r[j] - off[j]) / scale[j])^2 + w[6] * CE(softmax(y[6..10]), r[6..10]).
+// omap = {off[6], scale[6], w[7]}.
 __global__ __launch_bounds__(256) void LossKernel(const float* __restrict__ y, const float* __restrict__ t,
                                                   const uint8_t* __restrict__ valid, uint32_t batch,
-                                                  float* __restrict__ dy, uint16_t* __restrict__ dyb,
-                                                  float* __restrict__ lossPer)
+                                                  const float* __restrict__ omap, float* __restrict__ dy,
+                                                  uint16_t* __restrict__ dyb, float* __restrict__ lossPer)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= batch) return;
@@ -210,10 +212,13 @@ __global__ __launch_bounds__(256) void LossKernel(const float* __restrict__ y, c
     float loss = 0.0f;
     float g[MO];
     for (int j = 0; j < 6; ++j) {
-        const float d = yb[j] - tb[j];
-        loss += d * d;
-        g[j] = (2.0f * d) * invB;
+        const float tn = (tb[j] - omap[j]) / omap[6 + j];
+        const float d = yb[j] - tn;
+        const float w = omap[12 + j];
+        loss += w * (d * d);
+        g[j] = ((2.0f * w) * d) * invB;
     }
+    const float wce = omap[18];
     float mx = yb[6];
     for (int k = 1; k < 5; ++k) mx = hw_maxf(mx, yb[6 + k]);
     float e[5], se = 0.0f;
@@ -224,8 +229,8 @@ __global__ __launch_bounds__(256) void LossKernel(const float* __restrict__ y, c
     const float lse = mx + hw_logf(se);
     for (int k = 0; k < 5; ++k) {
         const float p = e[k] / se;
-        loss -= tb[6 + k] * (yb[6 + k] - lse);
-        g[6 + k] = (p - tb[6 + k]) * invB;
+        loss -= wce * (tb[6 + k] * (yb[6 + k] - lse));
+        g[6 + k] = (wce * (p - tb[6 + k])) * invB;
     }
     for (int j = 0; j < MO; ++j) {
         dy[(size_t)b * MO + j] = g[j];
@@ -330,6 +335,8 @@ struct hwpa_initializer {
     float* lossPer = nullptr;
     float* colPart = nullptr;
     float* lossDev = nullptr;
+    float* omap = nullptr;      // device {off[6], scale[6], w[7]}
+    float omapHost[19] = {};
 };
 
 namespace hwpa {
@@ -443,8 +450,8 @@ static int32_t Backward(hwpa_initializer* m, const uint16_t* x, const float* t, 
 {
     int32_t rc = Forward(m, x, batch, m->Y, m->H1, m->H2, s);
     if (rc != HWPA_OK) return rc;
-    hipLaunchKernelGGL(LossKernel, dim3((batch + 255) / 256), dim3(256), 0, s, m->Y, t, valid, batch, m->dY, m->dYb,
-                       m->lossPer);
+    hipLaunchKernelGGL(LossKernel, dim3((batch + 255) / 256), dim3(256), 0, s, m->Y, t, valid, batch, m->omap, m->dY,
+                       m->dYb, m->lossPer);
     if ((rc = CheckLaunch("LossKernel")) != HWPA_OK) return rc;
     hipLaunchKernelGGL(MeanKernel, dim3(1), dim3(256), 0, s, m->lossPer, batch, m->lossDev);
     if ((rc = CheckLaunch("MeanKernel")) != HWPA_OK) return rc;
@@ -519,6 +526,8 @@ int32_t MlpFeatures(const hwpa_lane_hist* hist, uint32_t lanes, uint16_t* x, hip
     return CheckLaunch("FeaturesKernel");
 }
 
+const float* MlpOutputMap(const hwpa_initializer* m) { return m->omap; }
+
 int32_t MlpTargets(const hwpa_lane_hist* hist, const float* diag, uint32_t lanes, float minConf, float* t,
                    uint8_t* valid, hipStream_t s)
 {
@@ -548,9 +557,21 @@ int32_t hwpa_initializer_create(uint64_t seed, hwpa_initializer** out)
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->Mo, NPARAM * 4, "mlp moments");
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->Ve, NPARAM * 4, "mlp moments");
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->lossDev, 16, "mlp loss");
+    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->omap, 19 * sizeof(float), "mlp output map");
     if (rc != HWPA_OK) {
         hwpa_initializer_destroy(m);
         return rc;
+    }
+    // Identity output map, unit weights, until the trainer sets one.
+    for (int j = 0; j < 6; ++j) {
+        m->omapHost[j] = 0.0f;
+        m->omapHost[6 + j] = 1.0f;
+        m->omapHost[12 + j] = 1.0f;
+    }
+    m->omapHost[18] = 1.0f;
+    if (hipMemcpy(m->omap, m->omapHost, sizeof(m->omapHost), hipMemcpyHostToDevice) != hipSuccess) {
+        hwpa_initializer_destroy(m);
+        return Fail(HWPA_ERR_HIP, "output map upload failed");
     }
     std::vector<float> p;
     InitParams(p, seed);
@@ -574,6 +595,7 @@ int32_t hwpa_initializer_destroy(hwpa_initializer* m)
     DeviceFree(m->Mo);
     DeviceFree(m->Ve);
     DeviceFree(m->lossDev);
+    DeviceFree(m->omap);
     delete m;
     return HWPA_OK;
 }
@@ -601,6 +623,33 @@ int32_t hwpa_initializer_set_params(hwpa_initializer* m, const float* host_param
     int32_t rc = CheckLaunch("ToBf16Kernel");
     if (rc != HWPA_OK) return rc;
     HWPA_HIP(hipDeviceSynchronize());
+    return HWPA_OK;
+}
+
+int32_t hwpa_initializer_set_output_map(hwpa_initializer* m, const float offset[6], const float scale[6],
+                                        const float weight[7])
+{
+    if (!m || !offset || !scale || !weight) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    for (int j = 0; j < 6; ++j) {
+        if (!(scale[j] > 0.0f) || !hw_isfinite(offset[j]) || !(weight[j] >= 0.0f))
+            return Fail(HWPA_ERR_INVALID_ARG, "output map: scales must be positive, weights non-negative");
+    }
+    if (!(weight[6] >= 0.0f)) return Fail(HWPA_ERR_INVALID_ARG, "output map: weights must be non-negative");
+    for (int j = 0; j < 6; ++j) {
+        m->omapHost[j] = offset[j];
+        m->omapHost[6 + j] = scale[j];
+        m->omapHost[12 + j] = weight[j];
+    }
+    m->omapHost[18] = weight[6];
+    HWPA_HIP(hipDeviceSynchronize());
+    HWPA_HIP(hipMemcpy(m->omap, m->omapHost, sizeof(m->omapHost), hipMemcpyHostToDevice));
+    return HWPA_OK;
+}
+
+int32_t hwpa_initializer_get_output_map(const hwpa_initializer* m, float out[19])
+{
+    if (!m || !out) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    for (int j = 0; j < 19; ++j) out[j] = m->omapHost[j];
     return HWPA_OK;
 }
 

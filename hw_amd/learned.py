@@ -73,12 +73,37 @@ def generate_pairs(lanes, chunks, seed, snr=(4.0, 20.0), frames=512, cfg=None, p
     return pairs
 
 
-def train(init, pairs, epochs=4, batch=4096, lr=2e-3, lr_end=1e-4, log=None):
+# Loss weights per squared raw unit (u = 2 binSize = 0.4 sigma for sigma >= 5):
+# the approximate EM log-likelihood lost by a start that is off by one unit, for
+# a 512-frame histogram with ~360 baseline frames and analog modes ~2 sigma
+# wide -- baseline mean N_bl (0.4)^2 / 2, log variance N_bl / 4, analog mean
+# N_a (0.2)^2 / 2 -- and N x KL for the mixture weights, all divided by N.
+LL_WEIGHTS = (29.0, 90.0, 0.74, 0.74, 0.74, 0.74)
+LL_FRAMES = 512.0
+
+
+def fit_output_map(init, pairs):
+    """Sets the network's output map from the valid training targets: offset
+    and scale standardise each raw target, weights follow LL_WEIGHTS."""
+    n = pairs.n
+    t = pairs.t.download(np.float32, (pairs.capacity, MLP_OUT))[:n]
+    v = pairs.v.download(np.uint8, (pairs.capacity,))[:n].astype(bool)
+    tv = t[v][:, :6].astype(np.float64)
+    off = tv.mean(axis=0)
+    sc = np.maximum(tv.std(axis=0), 1e-3)
+    w = [LL_WEIGHTS[j] * sc[j] ** 2 / LL_FRAMES for j in range(6)] + [1.0]
+    init.set_output_map(off, sc, w)
+    return off, sc, w
+
+
+def train(init, pairs, epochs=4, batch=4096, lr=2e-3, lr_end=1e-4, log=None, fit_map=True):
     """Adam over the pairs in order, cosine-decayed learning rate; returns the
     loss of every 50th step (and the last)."""
     n_batches = pairs.n // batch
     if n_batches == 0:
         raise ValueError("fewer pairs than one batch")
+    if fit_map:
+        fit_output_map(init, pairs)
     steps = epochs * n_batches
     d_loss = dev_bytes(4)
     losses = []

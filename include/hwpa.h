@@ -118,6 +118,24 @@ int32_t hwpa_batch_sync(hwpa_batch* b);
 /* [0] steps run, [1] graph instantiations, [2] in-place graph updates. */
 int32_t hwpa_batch_graph_stats(const hwpa_batch* b, uint64_t stats[3]);
 
+/* Autotunes every pool's shape (hwpa_autotune on its lanes, frames, config). */
+int32_t hwpa_batch_autotune(hwpa_batch* b, int32_t repeats);
+
+/* ---- kernel-shape autotuner (no reference counterpart) -------------------
+ * Tunable kernels run in variants that give the same bits with different
+ * shapes: kernel 0 = histogram (frames per prefetched batch x bins kept in
+ * LDS, 9 variants), kernel 1 = baseline filter (1, 2 or 4 lanes per
+ * workgroup).  The launchers look up the variant for their (lanes, frames);
+ * unset shapes use variant 0.  hwpa_autotune times every variant on
+ * synthetic traces of the shape (`repeats` timed runs each, HIP events) and
+ * keeps the fastest; ms (optional, ms_len entries) receives the mean times,
+ * histogram variants first. */
+int32_t hwpa_tune_variants(int32_t kernel, int32_t* count);
+int32_t hwpa_tune_set(int32_t kernel, uint32_t lanes, uint32_t frames, int32_t variant);
+int32_t hwpa_tune_get(int32_t kernel, uint32_t lanes, uint32_t frames, int32_t* variant);
+int32_t hwpa_autotune(const hwpa_config* cfg, uint32_t lanes, uint32_t frames, int32_t repeats, float* ms,
+                      int32_t ms_len);
+
 /* ---- stage entry points (Device* strategies) ----------------------------- */
 
 /* Baseliner: DeviceMultiScaleBaseliner / HostMultiScaleBaseliner::FilterBaseline
@@ -190,9 +208,9 @@ int32_t hwpa_pulser_run(hwpa_pulser* p, const int16_t* d_labels, const int16_t* 
  * A 128 -> 256 -> 256 -> 11 perceptron with ReLU hidden layers maps each
  * ZMW's histogram (re-binned to 128 bins: bin i = bins 2i and 2i+1 of the
  * 300, scaled by 16 / their total) to a start model for the EM: baseline
- * mean, log variance / 4 and four analog means in units of u = 2 binSize from
- * the histogram's lower bound (means scaled by 1/32), and five mixture-weight
- * logits.  Matrices are bfloat16 for the matrix cores with fp32 accumulation;
+ * mean, log variance and four analog means in units of u = 2 binSize from the
+ * histogram's lower bound (through the output map below), and five
+ * mixture-weight logits.  Matrices are bfloat16 for the matrix cores with fp32 accumulation;
  * an fp32 master copy and Adam moments live on the device.  The start replaces
  * the EM's (DmeEmHost.cpp:236-249 onwards); the EM then runs at most
  * cfg->learned_em_iteration_limit iterations with the convergence test
@@ -214,11 +232,20 @@ int32_t hwpa_initializer_destroy(hwpa_initializer* m);
  * setting them also resets the optimiser state. */
 int32_t hwpa_initializer_get_params(const hwpa_initializer* m, float* host_params, size_t n);
 int32_t hwpa_initializer_set_params(hwpa_initializer* m, const float* host_params, size_t n);
+/* Output map: network output j < 6 stands for offset[j] + scale[j] * y[j] in
+ * the raw target units of hwpa_mlp_targets; weight[0..5] weigh their squared
+ * errors in the loss, weight[6] the cross-entropy of the mixture weights.
+ * Default: offset 0, scale 1, weights 1.  get returns {offset, scale, weight}. */
+int32_t hwpa_initializer_set_output_map(hwpa_initializer* m, const float offset[6], const float scale[6],
+                                        const float weight[7]);
+int32_t hwpa_initializer_get_output_map(const hwpa_initializer* m, float out[19]);
 /* Network input from lane histograms: bf16 [lanes*64][128] (16-byte aligned). */
 int32_t hwpa_mlp_features(const hwpa_lane_hist* d_hist, uint32_t lanes, uint16_t* d_x, void* stream);
 /* Training targets from hwpa_dme_estimate_diag output: fp32 [lanes*64][11] in
- * the output units above (weights as probabilities) and d_valid u8 [lanes*64]
- * = converged, status OK, confidence >= min_confidence. */
+ * raw units, u = 2 binSize from the lower bound lo: (mu_bl - lo)/u,
+ * log(var_bl/u^2), (mu_a - lo)/u for the four analogs, then the five mixture
+ * weights; d_valid u8 [lanes*64] = converged, status OK, confidence >=
+ * min_confidence. */
 int32_t hwpa_mlp_targets(const hwpa_lane_hist* d_hist, const float* d_diag, uint32_t lanes,
                          float min_confidence, float* d_t, uint8_t* d_valid, void* stream);
 /* Network outputs fp32 [rows][11]. */

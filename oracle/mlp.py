@@ -72,32 +72,43 @@ def forward(p, x, round_bf16=True, dtype=np.float32):
     return {"x": x, "W1": W1, "W2": W2, "W3": W3, "h1": h1, "h2": h2, "y": y}
 
 
-def loss_terms(y, t):
-    """Per-sample loss: squared error on outputs 0-5, softmax cross-entropy on 6-10."""
-    d = y[:, :6] - t[:, :6]
+IDENTITY_MAP = np.concatenate([np.zeros(6), np.ones(6), np.ones(7)]).astype(np.float32)
+
+
+def loss_terms(y, t, omap=IDENTITY_MAP):
+    """Per-sample loss: weighted squared error of outputs 0-5 against the
+    targets mapped through (t - offset) / scale, plus the weighted softmax
+    cross-entropy of outputs 6-10.  omap = {offset[6], scale[6], weight[7]}."""
+    off, sc, w = omap[:6], omap[6:12], omap[12:]
+    d = y[:, :6] - (t[:, :6] - off) / sc
     logits = y[:, 6:]
     mx = logits.max(axis=1, keepdims=True)
     lse = mx + np.log(np.exp(logits - mx).sum(axis=1, keepdims=True))
-    return (d * d).sum(axis=1) - (t[:, 6:] * (logits - lse)).sum(axis=1)
+    return (w[:6] * d * d).sum(axis=1) - w[6] * (t[:, 6:] * (logits - lse)).sum(axis=1)
 
 
-def loss(p, x, t, round_bf16=True, dtype=np.float32):
+def loss(p, x, t, round_bf16=True, dtype=np.float32, omap=IDENTITY_MAP):
     f = forward(p, x, round_bf16, dtype)
-    return loss_terms(f["y"], np.asarray(t, dtype=dtype)).mean()
+    return loss_terms(f["y"], np.asarray(t, dtype=dtype), np.asarray(omap, dtype=dtype)).mean()
 
 
-def gradients(p, x, t, round_bf16=True, dtype=np.float32):
-    """(mean loss, flat gradient) following the GPU's rounding points."""
+def gradients(p, x, t, round_bf16=True, dtype=np.float32, omap=IDENTITY_MAP, valid=None):
+    """(mean loss, flat gradient) following the GPU's rounding points; samples
+    with valid == 0 contribute nothing (the mean stays over the batch)."""
     t = np.asarray(t, dtype=dtype)
+    omap = np.asarray(omap, dtype=dtype)
+    off, sc, w = omap[:6], omap[6:12], omap[12:]
     f = forward(p, x, round_bf16, dtype)
     y = f["y"]
     B = y.shape[0]
-    L = loss_terms(y, t).mean()
+    v = np.ones(B, dtype=dtype) if valid is None else np.asarray(valid, dtype=dtype)
+    L = (loss_terms(y, t, omap) * v).mean()
     dy = np.empty_like(y)
-    dy[:, :6] = 2 * (y[:, :6] - t[:, :6]) / B
+    dy[:, :6] = 2 * w[:6] * (y[:, :6] - (t[:, :6] - off) / sc) / B
     logits = y[:, 6:]
     e = np.exp(logits - logits.max(axis=1, keepdims=True))
-    dy[:, 6:] = (e / e.sum(axis=1, keepdims=True) - t[:, 6:]) / B
+    dy[:, 6:] = w[6] * (e / e.sum(axis=1, keepdims=True) - t[:, 6:]) / B
+    dy *= v[:, None]
     dyb = _r(dy, round_bf16)
     g = {"W3": dyb.T @ f["h2"], "b3": dy.sum(axis=0)}
     dh2 = _r((dyb @ f["W3"]) * (f["h2"] > 0), round_bf16)
@@ -110,32 +121,34 @@ def gradients(p, x, t, round_bf16=True, dtype=np.float32):
 
 
 def targets(low_bound, bin_size, diag):
-    """Training targets from the EM's raw estimate.  diag: [18, Z] rows
-    (rho[5], mu[5], var[5], conf, converged, status); returns float32 [Z, 11]."""
+    """Training targets from the EM's raw estimate, in raw units u = 2 binSize
+    from the lower bound.  diag: [18, Z] rows (rho[5], mu[5], var[5], conf,
+    converged, status); returns float32 [Z, 11]."""
     lo = np.asarray(low_bound, dtype=np.float32)
     u = np.float32(2.0) * np.asarray(bin_size, dtype=np.float32)
-    inv = np.float32(1.0) / (np.float32(32.0) * u)
     d = np.asarray(diag, dtype=np.float32)
     t = np.empty((lo.shape[0], OUT), dtype=np.float32)
-    t[:, 0] = (d[5] - lo) * inv
-    t[:, 1] = np.log(d[10] / (u * u)) * np.float32(0.25)
+    t[:, 0] = (d[5] - lo) / u
+    t[:, 1] = np.log(d[10] / (u * u))
     for a in range(4):
-        t[:, 2 + a] = (d[6 + a] - lo) * inv
+        t[:, 2 + a] = (d[6 + a] - lo) / u
     for k in range(5):
         t[:, 6 + k] = d[k]
     return t
 
 
-def decode(y, low_bound, bin_size):
+def decode(y, low_bound, bin_size, omap=IDENTITY_MAP):
     """Network outputs -> (rho[5], mu[5], baseline var) in counts."""
     y = np.asarray(y, dtype=np.float64)
+    omap = np.asarray(omap, dtype=np.float64)
+    r = omap[:6] + omap[6:12] * y[:, :6]
     u = 2.0 * np.asarray(bin_size, dtype=np.float64)
     lo = np.asarray(low_bound, dtype=np.float64)
     e = np.exp(y[:, 6:] - y[:, 6:].max(axis=1, keepdims=True))
     rho = e / e.sum(axis=1, keepdims=True)
     mu = np.empty((y.shape[0], 5))
-    mu[:, 0] = lo + 32 * u * y[:, 0]
+    mu[:, 0] = lo + u * r[:, 0]
     for a in range(4):
-        mu[:, 1 + a] = lo + 32 * u * y[:, 2 + a]
-    var0 = u * u * np.exp(4 * y[:, 1])
+        mu[:, 1 + a] = lo + u * r[:, 2 + a]
+    var0 = u * u * np.exp(r[:, 1])
     return rho, mu, var0

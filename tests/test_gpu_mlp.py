@@ -89,6 +89,9 @@ def test_gradients_match_oracle(H, gpu, batch):
     rng = np.random.default_rng(batch)
     init = H.Initializer(seed=9)
     p = init.params()
+    omap = np.concatenate([rng.normal(0, 1, 6), rng.uniform(0.5, 2, 6), rng.uniform(0.2, 3, 7)]).astype(np.float32)
+    init.set_output_map(omap[:6], omap[6:12], omap[12:])
+    assert np.array_equal(init.output_map(), omap)
     x, t = _realistic_batch(rng, batch)
     valid = (rng.uniform(size=batch) > 0.1).astype(np.uint8)
     d_x, d_t, d_v = _upload_bf16(H, x), H.dev_from_numpy(t), H.dev_from_numpy(valid)
@@ -99,12 +102,8 @@ def test_gradients_match_oracle(H, gpu, batch):
     g = H.to_numpy(d_g, np.float32, (M.NPARAM,))
     loss = H.to_numpy(d_l, np.float32, (1,))[0]
     # oracle: invalid samples contribute nothing, the mean stays over the batch
-    tm = t.copy()
-    L_ref, g_ref = M.gradients(p, x, tm)
-    f = M.forward(p, x)
-    per = M.loss_terms(f["y"], t) * valid
-    assert abs(loss - per.mean()) <= 1e-4 * abs(per.mean())
-    _, g_ref = _masked_gradients(p, x, t, valid)
+    L_ref, g_ref = M.gradients(p, x, t, omap=omap, valid=valid)
+    assert abs(loss - L_ref) <= 1e-4 * abs(L_ref)
     o = 0
     for name, shape in M.SIZES:
         n = int(np.prod(shape))
@@ -112,27 +111,6 @@ def test_gradients_match_oracle(H, gpu, batch):
         err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
         assert err < 2e-2, (name, err)
         o += n
-
-
-def _masked_gradients(p, x, t, valid):
-    """Oracle gradients with per-sample validity (dY = 0 for invalid samples)."""
-    f = M.forward(p, x)
-    y = f["y"]
-    B = y.shape[0]
-    dy = np.empty_like(y)
-    dy[:, :6] = 2 * (y[:, :6] - t[:, :6]) / B
-    e = np.exp(y[:, 6:] - y[:, 6:].max(axis=1, keepdims=True))
-    dy[:, 6:] = (e / e.sum(axis=1, keepdims=True) - t[:, 6:]) / B
-    dy *= valid[:, None]
-    dyb = M.bf16(dy.astype(np.float32))
-    g = {"W3": dyb.T @ f["h2"], "b3": dy.sum(axis=0)}
-    dh2 = M.bf16((dyb @ f["W3"]) * (f["h2"] > 0))
-    g["W2"] = dh2.T @ f["h1"]
-    g["b2"] = dh2.sum(axis=0)
-    dh1 = M.bf16((dh2 @ f["W2"]) * (f["h1"] > 0))
-    g["W1"] = dh1.T @ f["x"]
-    g["b1"] = dh1.sum(axis=0)
-    return None, M.pack(g)
 
 
 def test_training_lowers_loss(H, gpu):

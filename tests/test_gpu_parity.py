@@ -63,6 +63,38 @@ def test_baseliner_bit_exact(H, O, T, gpu, ftype):
         _eq_struct(g_stats, o_stats, "stats[c=%d]" % c)
 
 
+@pytest.mark.parametrize("variant", [1, 2])
+def test_baseliner_lane_tile_variants(H, O, T, gpu, variant):
+    """2 and 4 lanes per workgroup, with a lane count that leaves the last tile partial."""
+    L, F, C = 5, 512, 3
+    H.tune_set(H.TUNE_BASELINE, L, F, variant)
+    try:
+        cfg = H.default_config()
+        chunks = gen_traces(O, T, H, L, F, C, seed=77)
+        ob, gb = O.Baseliner(cfg, L), H.Baseliner(cfg, L)
+        d_out = H.dev_bytes(L * F * 64 * 2)
+        d_stats = H.dev_bytes(L * T.BASELINER_STATS.itemsize)
+        for tr in chunks:
+            o_out, o_stats = ob.run(tr, T.BASELINER_STATS)
+            gb.run(_dev(H, tr), F, d_out, d_stats)
+            assert np.array_equal(H.to_numpy(d_out, np.int16, (L, F, 64)), o_out)
+            _eq_struct(H.to_numpy(d_stats, T.BASELINER_STATS, (L,)), o_stats, "stats")
+    finally:
+        H.tune_set(H.TUNE_BASELINE, L, F, 0)
+
+
+def test_autotune_picks_a_timed_variant(H, gpu):
+    cfg = H.bench_config(512)
+    r = H.autotune(cfg, 64, 512, repeats=2)
+    assert len(r["histogram"]) == H.tune_variants(H.TUNE_HISTOGRAM) == 9
+    assert len(r["baseline"]) == H.tune_variants(H.TUNE_BASELINE) == 3
+    assert all(t > 0 for t in r["histogram"] + r["baseline"])
+    h, b = r["chosen"]
+    assert r["histogram"][h] == min(r["histogram"]) and r["baseline"][b] == min(r["baseline"])
+    H.tune_set(H.TUNE_HISTOGRAM, 64, 512, 0)
+    H.tune_set(H.TUNE_BASELINE, 64, 512, 0)
+
+
 def test_baseliner_sigma_ema_and_scaler(H, O, T, gpu):
     L, F, C = 3, 256, 5
     cfg = H.default_config(mean_ema_scale_strides=4.0, jump_suppression=2.0)
@@ -104,7 +136,17 @@ def test_stats_aggregate_bit_exact(H, O, T, gpu):
             H.stats_reset(d_agg, L)
 
 
-def test_histogram_bit_exact(H, O, T, gpu):
+@pytest.fixture(params=range(9))
+def hist_variant(request, H):
+    """Every autotuner variant of the histogram kernel (shape 6 lanes x 512)."""
+    H.tune_set(H.TUNE_HISTOGRAM, 6, 512, request.param)
+    H.tune_set(H.TUNE_HISTOGRAM, 3, 512, request.param)
+    yield request.param
+    H.tune_set(H.TUNE_HISTOGRAM, 6, 512, 0)
+    H.tune_set(H.TUNE_HISTOGRAM, 3, 512, 0)
+
+
+def test_histogram_bit_exact(H, O, T, gpu, hist_variant):
     L, F, C = 6, 512, 5
     cfg, chunks = _pipeline_inputs(H, O, T, L, F, C, seed=21)
     hist = O.Histogram(cfg, L)
@@ -127,7 +169,7 @@ def test_histogram_bit_exact(H, O, T, gpu):
         _eq_struct(H.to_numpy(d_hist, T.LANE_HIST, (L,)), hist.get(T.LANE_HIST), "hist[%d]" % c)
 
 
-def test_histogram_bin_edges_wrap_and_degenerate_bounds(H, O, T, gpu):
+def test_histogram_bin_edges_wrap_and_degenerate_bounds(H, O, T, gpu, hist_variant):
     """Samples exactly on and next to bin edges (1/binSize inexact), counters at
     the 16-bit wrap, and zero / NaN bin sizes, against the oracle."""
     L, F = 3, 512

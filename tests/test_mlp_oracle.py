@@ -33,13 +33,17 @@ def _rand_params(rng, scale=1.0):
     return M.pack(d)
 
 
-def test_backward_matches_finite_differences():
+@pytest.mark.parametrize("mapped", [False, True])
+def test_backward_matches_finite_differences(mapped):
     rng = np.random.default_rng(7)
     p = _rand_params(rng)
     x = rng.uniform(0, 2, size=(6, M.IN))
     t = np.concatenate([rng.normal(0, 1, size=(6, 6)), rng.dirichlet(np.ones(5), size=6)], axis=1)
-    L, g = M.gradients(p, x, t, round_bf16=False, dtype=np.float64)
-    assert np.isclose(L, M.loss(p, x, t, round_bf16=False, dtype=np.float64))
+    omap = M.IDENTITY_MAP.astype(np.float64)
+    if mapped:
+        omap = np.concatenate([rng.normal(0, 2, 6), rng.uniform(0.5, 3, 6), rng.uniform(0.1, 5, 7)])
+    L, g = M.gradients(p, x, t, round_bf16=False, dtype=np.float64, omap=omap)
+    assert np.isclose(L, M.loss(p, x, t, round_bf16=False, dtype=np.float64, omap=omap))
     idx = np.concatenate([rng.choice(np.arange(o, o + n), size=8, replace=False)
                           for o, n in _offsets()])
     eps = 1e-6
@@ -47,7 +51,7 @@ def test_backward_matches_finite_differences():
         pp, pm = p.copy(), p.copy()
         pp[i] += eps
         pm[i] -= eps
-        fd = (M.loss(pp, x, t, False, np.float64) - M.loss(pm, x, t, False, np.float64)) / (2 * eps)
+        fd = (M.loss(pp, x, t, False, np.float64, omap) - M.loss(pm, x, t, False, np.float64, omap)) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
 
@@ -94,9 +98,11 @@ def test_targets_decode_round_trip():
     diag[6:10] = np.array([180, 120, 80, 50])[:, None] * rng.uniform(0.5, 1.5, Z)
     diag[10] = rng.uniform(100, 200, Z)
     t = M.targets(lo, bs, diag)
+    omap = np.concatenate([rng.normal(0, 3, 6), rng.uniform(0.5, 4, 6), np.ones(7)]).astype(np.float32)
     y = t.copy()
+    y[:, :6] = (t[:, :6] - omap[:6]) / omap[6:12]   # outputs that map back onto the targets
     y[:, 6:] = np.log(t[:, 6:])         # logits whose softmax is the target weights
-    r, mu, v0 = M.decode(y, lo, bs)
+    r, mu, v0 = M.decode(y, lo, bs, omap)
     assert np.allclose(r, rho.T, rtol=1e-5)
     assert np.allclose(mu[:, 0], diag[5], atol=1e-3)
     assert np.allclose(mu[:, 1:], diag[6:10].T, rtol=1e-5)
