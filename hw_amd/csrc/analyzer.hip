@@ -85,7 +85,6 @@ struct hwpa_analyzer {
     uint32_t* dCounts = nullptr;
     uint32_t* dOvf = nullptr;
     hwpa_stat_acc* dPulseBl = nullptr;
-    float* dEmScratch = nullptr;
     int32_t* dIters = nullptr;
     bool ranEstimate = false;
     hwpa_initializer* mlp = nullptr;
@@ -109,7 +108,7 @@ static void FreeAnalyzer(hwpa_analyzer* a)
     PulserDestroy(a->pul);
     void* bufs[] = {a->dTraces, a->dBaselined, a->dStats,  a->dAgg,    a->dHist,      a->dEdge,
                     a->dModels, a->dLabels,    a->dLatent, a->dScore,  a->dPulses,    a->dCounts,
-                    a->dOvf,    a->dPulseBl,   a->dEmScratch, a->dIters};
+                    a->dOvf,    a->dPulseBl,   a->dIters};
     for (void* p : bufs) DeviceFree(p);
     LearnedWsFree(a->lws);
     DeviceFree(a->dDiag);
@@ -173,7 +172,6 @@ int32_t hwpa_analyzer_create(uint32_t pool_id, uint32_t lanes, uint32_t frames, 
         {(void**)&a->dCounts, (size_t)lanes * LS * 4, "counts"},
         {(void**)&a->dOvf, (size_t)lanes * LS * 4, "overflows"},
         {(void**)&a->dPulseBl, sizeof(hwpa_stat_acc) * lanes, "pulse stats"},
-        {(void**)&a->dEmScratch, DmeScratchBytes(lanes), "EM scratch"},
         {(void**)&a->dIters, sizeof(int32_t) * lanes, "EM iterations"},
     };
     for (auto& x : allocs) {
@@ -285,13 +283,12 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
             const int32_t mfi[2] = {a->modelFi.lo, a->modelFi.hi};
             const int32_t efi[2] = {a->histFi.lo, a->histFi.hi};
             if (a->mlp && a->cfg.learned_init) {
-                rc = DmeEstimateLearned(a->cfg, a->mlp, a->lws, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters,
-                                        a->dEmScratch, s);
+                rc = DmeEstimateLearned(a->cfg, a->mlp, a->lws, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, s);
                 a->ranLearned = true;
                 a->plan |= kPlanLearned;
             } else {
                 const bool cap = a->capX != nullptr;
-                rc = DmeEstimate(a->cfg, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, a->dEmScratch, s,
+                rc = DmeEstimate(a->cfg, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, s,
                                  cap ? a->dDiag : nullptr);
                 if (rc == HWPA_OK && cap) rc = MlpFeatures(a->dHist, L, a->capX, s);
                 if (rc == HWPA_OK && cap)

@@ -6,11 +6,10 @@
 // Mapping: one wave64 per lane, one thread per ZMW.  The only coupling
 // between ZMWs is the EM early exit, which the reference takes when every ZMW
 // of the lane has converged or failed (DmeEmHost.cpp:485); here that is one
-// wave vote per iteration.  Boundary densities and baseline responsibilities
-// of the E-step are kept in a per-lane HBM scratch ([boundary][2][zmw], so
-// every access is a coalesced 256-byte row) for the M-step's variance pass.
-// All arithmetic keeps the host path's operation order so results match the
-// CPU restatement bit for bit.
+// wave vote per iteration.  Each EM iteration is one pass over the 301 bin
+// boundaries in registers: the M-step's baseline-variance sum is carried in
+// the same pass (no per-boundary scratch).  All arithmetic keeps the CPU
+// restatement's operation order so results match it bit for bit.
 
 #include "internal.h"
 
@@ -267,11 +266,14 @@ struct EmExtra {
 // which still set the amplitude prior) by the network's prediction, caps the
 // iterations, uses its own log-likelihood convergence tolerance and accepts
 // the result only for converged ZMWs with status OK and enough confidence.
+// exp(t) rounds to +0 for every t below this (exp(-103.98) is 2^-150).
+constexpr float kExpZeroBelow = -105.0f;
+
 template <int MODE>
 __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __restrict__ hists,
                                                         const hwpa_baseliner_stats* __restrict__ stats,
-                                                        hwpa_lane_model* models, float* __restrict__ scratch,
-                                                        int32_t* iters, DmeDevCfg d, float evolve, EmExtra x)
+                                                        hwpa_lane_model* models, int32_t* iters, DmeDevCfg d,
+                                                        float evolve, EmExtra x)
 {
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
@@ -279,8 +281,6 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
     const hwpa_lane_hist& lh = hists[lane];
     const hwpa_baseliner_stats& bs = stats[lane];
     hwpa_lane_model& lm = models[lane];
-    // scratch rows: [b][0][z] = boundary density, [b][1][z] = baseline responsibility
-    float* sc = scratch + (size_t)lane * (NB + 1) * 2 * LS + z;
 
     HistView h{&lh, z, lh.low_bound[z], lh.bin_size[z]};
     const uint32_t lowOut = lh.outlier_low[z], highOut = lh.outlier_high[z];
@@ -433,54 +433,68 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
             varinv[k] = 1.0f / var[k];
             pref[k] = (hw_logf(rho[k]) - 0.5f * hw_logf(var[k])) - 0.5f * d.log2pi;
         }
-        float probSum = 0.0f, wps = 0.0f;
+        // Every use of a responsibility is boundaryProb * tau_i = exp(t_i)
+        // (DmeEmHost.cpp:431-432,547-548), so the boundary carries the component
+        // densities dd_i = exp(t_i) and boundaryProb = their sum.  The baseline
+        // second moment of the M-step is accumulated in the same pass about the
+        // pivot piv (this iteration's baseline mean): q1 and q2 are the first
+        // and second moments of (x - piv) under the baseline weights, shifted to
+        // the new mean after the M-step.  Waves skip exact zeros: exps of
+        // boundaries where every t_i < -105 (exp rounds to +0) and the
+        // accumulation of bins that are empty for all 64 ZMWs.
+        const float piv = mu[0];
+        float probSum = 0.0f, wps = 0.0f, q1 = 0.0f, q2 = 0.0f;
         float ci[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, mom1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-        float bpPrev, tauPrev[5];
-        // corner b: boundary density and responsibilities at h.Start(b)
-        auto corner = [&](int b, float& bpOut, float* tauOut) {
+        float bpPrev, dPrev[5], devPrev;
+        auto corner = [&](int b, float& bpOut, float* dOut, float& devOut) {
             const float x = h.Start(b);
             float t[5];
+            bool zero = true;
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
                 const float y = x - mu[i];
                 t[i] = pref[i] - 0.5f * y * y * varinv[i];
+                zero = zero && (t[i] < kExpZeroBelow);
             }
-            float tmax = t[0];
+            if (__all(zero)) {
 #pragma unroll
-            for (int i = 1; i < 5; ++i) tmax = hw_maxf(tmax, t[i]);
-            float es = hw_expf(t[0] - tmax);
+                for (int i = 0; i < 5; ++i) dOut[i] = 0.0f;
+            } else {
 #pragma unroll
-            for (int i = 1; i < 5; ++i) es = es + hw_expf(t[i] - tmax);
-            const float llb = tmax + hw_logf(es);
-#pragma unroll
-            for (int i = 0; i < 5; ++i) tauOut[i] = hw_expf(t[i] - llb);
-            bpOut = hw_expf(llb);
-            sc[(size_t)(b * 2 + 0) * LS] = bpOut;
-            sc[(size_t)(b * 2 + 1) * LS] = tauOut[0];
+                for (int i = 0; i < 5; ++i) dOut[i] = hw_expf(t[i]);
+            }
+            bpOut = Sum5Tree(dOut);
+            devOut = x - piv;
         };
-        corner(0, bpPrev, tauPrev);
+        corner(0, bpPrev, dPrev, devPrev);
         float x0 = h.Start(0);
         for (int b = 0; b < NB; ++b) {
-            float bpNext, tauNext[5];
-            corner(b + 1, bpNext, tauNext);
+            float bpNext, dNext[5], devNext;
+            corner(b + 1, bpNext, dNext, devNext);
             const float binp = hw_maxf(halfBin * (bpPrev + bpNext), binProbMin);
             probSum += binp;
-            const float nj = (float)h.Cnt(b);
-            wps += nj * hw_logf(binp);
-            const float factor = nj / binp;
+            const uint32_t cnt = h.Cnt(b);
             const float x1 = h.Start(b + 1);
+            if (!__all(cnt == 0u)) {
+                const float nj = (float)cnt;
+                wps += nj * hw_logf(binp);
+                const float factor = nj / binp;
 #pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                const float tmp1 = bpPrev * tauPrev[m];
-                const float tmp2 = bpNext * tauNext[m];
-                ci[m] += (tmp1 + tmp2) * factor;
-                mom1[m] += (tmp1 * x0 + tmp2 * x1) * factor;
+                for (int m = 0; m < 5; ++m) {
+                    ci[m] += (dPrev[m] + dNext[m]) * factor;
+                    mom1[m] += (dPrev[m] * x0 + dNext[m] * x1) * factor;
+                }
+                const float g0 = dPrev[0] * devPrev, g1 = dNext[0] * devNext;
+                q1 += (g0 + g1) * factor;
+                q2 += (g0 * devPrev + g1 * devNext) * factor;
             }
             bpPrev = bpNext;
 #pragma unroll
-            for (int m = 0; m < 5; ++m) tauPrev[m] = tauNext[m];
+            for (int m = 0; m < 5; ++m) dPrev[m] = dNext[m];
+            devPrev = devNext;
             x0 = x1;
         }
+        const float q0 = ci[0];
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             ci[m] *= halfBin;
@@ -538,21 +552,9 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
         }
 #pragma unroll
         for (int a = 0; a < NA; ++a) mu[a + 1] = s * rpa[a];
-        float mom2 = 0.0f;
-        float bpa = sc[0], taua = sc[LS];
-        for (int j = 0; j < NB; ++j) {
-            const float bpb = sc[(size_t)((j + 1) * 2 + 0) * LS];
-            const float taub = sc[(size_t)((j + 1) * 2 + 1) * LS];
-            const float xa = h.Start(j) - mu[0];
-            const float xb = h.Start(j + 1) - mu[0];
-            float tmp = bpa * xa * xa * taua;
-            tmp += bpb * xb * xb * taub;
-            const float binp = hw_maxf(halfBin * (bpa + bpb), binProbMin);
-            tmp *= (float)h.Cnt(j) / binp;
-            mom2 += tmp;
-            bpa = bpb;
-            taua = taub;
-        }
+        // Baseline second moment about the new mean, from the moments about piv.
+        const float dl = mu[0] - piv;
+        float mom2 = (q2 - (2.0f * dl) * q1) + (dl * dl) * q0;
         mom2 *= 0.5f * h.bs;
         var[0] = mom2 / c[0] + varQuant;
         var[0] = hw_clampf(var[0], d.blVarMin, d.blVarMax);
@@ -766,8 +768,6 @@ __global__ __launch_bounds__(64) void FallbackScatterKernel(const uint32_t* __re
     ColModel(cModels[blockIdx.x], models[dst / LS], (int)threadIdx.x, (int)(dst % LS));
 }
 
-size_t DmeScratchBytes(uint32_t lanes) { return (size_t)lanes * (NB + 1) * 2 * LS * sizeof(float); }
-
 int32_t DmeInitModels(const hwpa_config& cfg, const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
                       uint32_t lanes, hipStream_t s)
 {
@@ -789,28 +789,16 @@ static float EvolveFactor(const DmeDevCfg& d, const int32_t model_fi[2], const i
 
 int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
                     hwpa_lane_model* models, const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
-                    int32_t* d_iters, float* scratch, hipStream_t s, float* diag)
+                    int32_t* d_iters, hipStream_t s, float* diag)
 {
     if (!hist || !stats || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "dme_estimate: bad argument");
     const DmeDevCfg d = MakeDevCfg(cfg);
     const float evolve = EvolveFactor(d, model_fi, est_fi);
-    float* tmp = nullptr;
-    if (!scratch) {
-        // Stand-alone call: private scratch for this call only.
-        const int32_t rc = DeviceAlloc((void**)&tmp, DmeScratchBytes(lanes), "EM scratch");
-        if (rc != HWPA_OK) return rc;
-        scratch = tmp;
-    }
     EmExtra x{};
     x.diag = diag;
-    hipLaunchKernelGGL(DmeEstimateKernel<EM_PLAIN>, dim3(lanes), dim3(LS), 0, s, hist, stats, models, scratch, d_iters,
-                       d, evolve, x);
-    int32_t rc = CheckLaunch("DmeEstimateKernel");
-    if (tmp) {
-        if (hipStreamSynchronize(s) != hipSuccess && rc == HWPA_OK) rc = Fail(HWPA_ERR_HIP, "EM kernel failed");
-        DeviceFree(tmp);
-    }
-    return rc;
+    hipLaunchKernelGGL(DmeEstimateKernel<EM_PLAIN>, dim3(lanes), dim3(LS), 0, s, hist, stats, models, d_iters, d,
+                       evolve, x);
+    return CheckLaunch("DmeEstimateKernel");
 }
 
 void LearnedWsFree(LearnedWs& w)
@@ -860,9 +848,9 @@ int32_t LearnedWsAlloc(LearnedWs& w, uint32_t lanes)
 
 int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, LearnedWs& w, const hwpa_lane_hist* hist,
                            const hwpa_baseliner_stats* stats, hwpa_lane_model* models, const int32_t model_fi[2],
-                           const int32_t est_fi[2], uint32_t lanes, int32_t* d_iters, float* scratch, hipStream_t s)
+                           const int32_t est_fi[2], uint32_t lanes, int32_t* d_iters, hipStream_t s)
 {
-    if (!mlp || !hist || !stats || !models || !scratch || lanes == 0 || lanes > w.lanes)
+    if (!mlp || !hist || !stats || !models || lanes == 0 || lanes > w.lanes)
         return Fail(HWPA_ERR_INVALID_ARG, "dme_estimate_learned: bad argument");
     if (cfg.learned_em_iteration_limit == 0) return Fail(HWPA_ERR_BAD_CONFIG, "learned EM iteration limit must be positive");
     const DmeDevCfg d = MakeDevCfg(cfg);
@@ -877,8 +865,8 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
     x.acceptConf = cfg.learned_conf_threshold;
     x.convTol = cfg.learned_conv_tol;
     x.iterLimit = (int32_t)cfg.learned_em_iteration_limit;
-    hipLaunchKernelGGL(DmeEstimateKernel<EM_LEARNED>, dim3(lanes), dim3(LS), 0, s, hist, stats, models, scratch,
-                       d_iters, d, evolve, x);
+    hipLaunchKernelGGL(DmeEstimateKernel<EM_LEARNED>, dim3(lanes), dim3(LS), 0, s, hist, stats, models, d_iters, d,
+                       evolve, x);
     if ((rc = CheckLaunch("DmeEstimateKernel")) != HWPA_OK) return rc;
     hipLaunchKernelGGL(FallbackCountKernel, dim3(lanes), dim3(LS), 0, s, w.fallback, w.cnt);
     if ((rc = CheckLaunch("FallbackCountKernel")) != HWPA_OK) return rc;
@@ -891,8 +879,8 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
     if ((rc = CheckLaunch("FallbackPadKernel")) != HWPA_OK) return rc;
     EmExtra p{};
     p.active = w.total;
-    hipLaunchKernelGGL(DmeEstimateKernel<EM_PLAIN>, dim3(lanes), dim3(LS), 0, s, w.cHist, w.cStats, w.cModels, scratch,
-                       w.cIters, d, evolve, p);
+    hipLaunchKernelGGL(DmeEstimateKernel<EM_PLAIN>, dim3(lanes), dim3(LS), 0, s, w.cHist, w.cStats, w.cModels, w.cIters,
+                       d, evolve, p);
     if ((rc = CheckLaunch("DmeEstimateKernel")) != HWPA_OK) return rc;
     hipLaunchKernelGGL(FallbackScatterKernel, dim3(lanes), dim3(LS), 0, s, w.total, w.map, w.cModels, models);
     return CheckLaunch("FallbackScatterKernel");

@@ -81,79 +81,87 @@ __global__ __launch_bounds__(64) void HistResetKernel(hwpa_lane_hist* hist, cons
 
 // Bin of a sample: floor((x - lo) / bs) with the reference's IEEE divide
 // (UHistogramSimd::BinIndex, dataTypes/UHistogramSimd.h:199-229).  The
-// quotient is first formed as (x - lo) * (1 / bs), which is within
-// 3 * 2^-24 |y| of the divided value; for |y| < 2048 that is below 3.7e-4, so
-// when the product is more than 1/1024 away from an integer both share the
-// same floor.  For 2048 <= |y| < 2^30 the sample is an outlier on the same side
-// either way.  Everything else -- near a bin edge, NaN, infinities, huge
-// quotients -- takes the divide (whose float -> int conversion gives
-// 0x80000000 out of range, a low outlier, as on the host); the divide runs
-// only when some lane of the wave needs it.
-__device__ __forceinline__ int BinOf(float a, float bs, float inv)
+// quotient is first formed as q = (x - lo) * (1 / bs) with 1 / bs rounded
+// once; q is within 2 * 2^-24 |y| of y = (x - lo) / bs and the divide's
+// result within 2^-24 |y|, so both lie within 3 * 2^-24 |y| < 2^-22 |q| of
+// each other.  When the fraction of q is farther than that from 0 and 1 the
+// two share their floor (for |q| < 1 the common sign decides).  For
+// 2048 <= |q| < 2^30 the sample is an outlier on the same side either way.
+// Everything else -- near a bin edge, zero, NaN, infinities, huge quotients
+// -- is flagged for the divide (whose float -> int conversion gives
+// 0x80000000 out of range, a low outlier, as on the host).
+__device__ __forceinline__ int BinFast(float a, float inv, bool& exact)
 {
     const float yp = a * inv;
     const float ay = hw_fabsf(yp);
     const bool far = ay >= 2048.0f && ay < 1073741824.0f;
     const float fl = hw_floorf(yp);
     const float d = yp - fl;
-    const bool clear = ay < 2048.0f && d >= 0.0009765625f && d <= 0.9990234375f;
+    const float tol = __builtin_fmaxf(ay * 2.384185791015625e-07f, 9.5367431640625e-07f);  // 2^-22 |q|, >= 2^-20
+    const bool clear = ay < 2048.0f && d >= tol && d <= 1.0f - tol;
     // Branch-free: clamp into [-1, NB] in float, pick the outlier side, convert once.
     float bf = __builtin_fminf(__builtin_fmaxf(fl, -1.0f), (float)NB);
     const float side = (yp > 0.0f) ? (float)NB : -1.0f;
     bf = far ? side : bf;
-    int bin = (int)bf;
-    const bool exact = !far && !clear;
-    if (__any(exact)) {
-        const int e = hw_floor_i32(a / bs);
-        bin = exact ? e : bin;
-    }
-    return bin;
+    exact = !far && !clear;
+    return (int)bf;
 }
 
-// TraceHistogramAccumHost::AddBlock (TraceHistogramAccumHost.cpp:128-169).
+// One batch of N frames of TraceHistogramAccumHost::AddBlock
+// (TraceHistogramAccumHost.cpp:128-169): EdgeFrameClassifier::IsEdgeFrame
+// then UHistogramSimd::AddDatum per frame.  The classifier's two-entry
+// circular array is a shift register here: bl1/bl2 = baseline flags of the
+// previous two frames, pushes saturates at 2 (lane-uniform); the previous
+// frame counts when the array was full and its two neighbours agree.
 // Counters live in LDS as words holding bins 2j (low half) and 2j+1 (high
 // half) of one ZMW, so an increment is a single ds_add_u32 that nothing waits
 // on (frames that are not counted add 0).  That cannot carry into the
 // neighbouring half while every count stays below 65536 -- checked before the
 // chunk; otherwise the wave takes the WRAP variant, a 16-bit wrapping
-// read-modify-write, the reference's u16 counter semantics.
-// Bins [0, NL) are counted in LDS; the fast pass (NL < NB) counts the rest with
-// 32-bit global atomics on the lane's histogram in HBM, each word holding the
-// 16-bit counters of ZMWs z and z^1 for one bin (no carry under the same
-// 65536 guard).
-template <int B, bool WRAP, int NL>
-__device__ __forceinline__ void HistFrames(uint32_t* cnt, hwpa_lane_hist& hg, const int16_t* tr, int frames, int z,
-                                           float lo, float bs, float inv, int thr, int& prev, bool& bl1, bool& bl2,
-                                           int& pushes, uint32_t& lowOut, uint32_t& highOut)
+// read-modify-write, the reference's u16 counter semantics.  Bins [0, NL) are
+// counted in LDS; bins [NL, NB) with 32-bit global atomics on the lane's
+// histogram in HBM, each word holding the 16-bit counters of ZMWs z and z^1
+// for one bin (no carry under the same 65536 guard).  The rare divides and
+// global atomics are taken once per batch, by the lanes that need them.
+template <int N, bool WRAP, int NL>
+__device__ __forceinline__ void HistBatch(uint32_t* cnt, hwpa_lane_hist& hg, const int (&fv)[N], int z, float lo,
+                                          float bs, float inv, int thr, int& prev, bool& bl1, bool& bl2, int& pushes,
+                                          uint32_t& lowOut, uint32_t& highOut)
 {
-    // One frame: EdgeFrameClassifier::IsEdgeFrame then UHistogramSimd::AddDatum.
-    // The classifier's two-entry circular array is a shift register here:
-    // bl1/bl2 = baseline flags of the previous two frames, pushes saturates
-    // at 2 (lane-uniform); the previous frame counts when the array was full
-    // and its two neighbours agree.
-    auto frame = [&](int f) {
-        const bool isBl = f < thr;
-        const bool counted = (pushes >= 2) && (isBl == bl2);
+    int bin[N];
+    uint32_t countedMask = 0, exactMask = 0;
+#pragma clang loop unroll(full)
+    for (int u = 0; u < N; ++u) {
+        const bool isBl = fv[u] < thr;
+        const bool counted = (u >= 2 || pushes + u >= 2) && (isBl == bl2);
         bl2 = bl1;
         bl1 = isBl;
-        pushes = (pushes < 2) ? pushes + 1 : 2;
-        const int cand = prev;
-        prev = f;
-        const int bin = BinOf((float)cand - lo, bs, inv);
-        const bool low = bin < 0, high = bin >= NB;
-        lowOut = (lowOut + ((counted && low) ? 1u : 0u)) & 0xffffu;
-        highOut = (highOut + ((counted && high) ? 1u : 0u)) & 0xffffu;
-        const bool in = counted && !low && !high;
-        const int bc = low ? 0 : (high ? NB - 1 : bin);
-        if (NL < NB) {
-            const bool glob = in && bc >= NL;
-            if (__any(glob)) {
-                if (glob) {
-                    uint32_t* gw = reinterpret_cast<uint32_t*>(&hg.bin_count[bc][z & ~1]);
-                    atomicAdd(gw, 1u << ((uint32_t)(z & 1) * 16u));
-                }
+        const int cand = (u == 0) ? prev : fv[u - 1];
+        bool ex;
+        bin[u] = BinFast((float)cand - lo, inv, ex);
+        countedMask |= (counted ? 1u : 0u) << u;
+        exactMask |= (ex ? 1u : 0u) << u;
+    }
+    if (__any(exactMask != 0u)) {
+#pragma clang loop unroll(full)
+        for (int u = 0; u < N; ++u) {
+            if ((exactMask >> u) & 1u) {
+                const int cand = (u == 0) ? prev : fv[u - 1];
+                bin[u] = hw_floor_i32(((float)cand - lo) / bs);
             }
         }
+    }
+    uint32_t globMask = 0, nLow = 0, nHigh = 0;
+#pragma clang loop unroll(full)
+    for (int u = 0; u < N; ++u) {
+        const bool counted = (countedMask >> u) & 1u;
+        const int b = bin[u];
+        const bool low = b < 0, high = b >= NB;
+        nLow += (counted && low) ? 1u : 0u;
+        nHigh += (counted && high) ? 1u : 0u;
+        const bool in = counted && !low && !high;
+        const int bc = low ? 0 : (high ? NB - 1 : b);
+        if (NL < NB) globMask |= ((in && bc >= NL) ? 1u : 0u) << u;
         const int bl = (NL < NB && bc >= NL) ? NL - 1 : bc;
         uint32_t* w = &cnt[(bl >> 1) * LS + z];
         const uint32_t sh = (uint32_t)(bl & 1) * 16u;
@@ -165,7 +173,27 @@ __device__ __forceinline__ void HistFrames(uint32_t* cnt, hwpa_lane_hist& hg, co
             const uint32_t c = ((v >> sh) + 1u) & 0xffffu;
             *w = (v & ~(0xffffu << sh)) | (c << sh);
         }
-    };
+    }
+    if (NL < NB && __any(globMask != 0u)) {
+#pragma clang loop unroll(full)
+        for (int u = 0; u < N; ++u) {
+            if ((globMask >> u) & 1u) {
+                uint32_t* gw = reinterpret_cast<uint32_t*>(&hg.bin_count[bin[u]][z & ~1]);
+                atomicAdd(gw, 1u << ((uint32_t)(z & 1) * 16u));
+            }
+        }
+    }
+    lowOut = (lowOut + nLow) & 0xffffu;
+    highOut = (highOut + nHigh) & 0xffffu;
+    prev = fv[N - 1];
+    pushes = (pushes + N < 2) ? pushes + N : 2;
+}
+
+template <int B, bool WRAP, int NL>
+__device__ __forceinline__ void HistFrames(uint32_t* cnt, hwpa_lane_hist& hg, const int16_t* tr, int frames, int z,
+                                           float lo, float bs, float inv, int thr, int& prev, bool& bl1, bool& bl2,
+                                           int& pushes, uint32_t& lowOut, uint32_t& highOut)
+{
     const int full = frames / B;
     int fv[B];
     if (full > 0) {
@@ -177,12 +205,24 @@ __device__ __forceinline__ void HistFrames(uint32_t* cnt, hwpa_lane_hist& hg, co
         const int gn = (g + 1 < full) ? g + 1 : g;
 #pragma clang loop unroll(full)
         for (int u = 0; u < B; ++u) nv[u] = tr[(size_t)(gn * B + u) * LS];
-#pragma clang loop unroll(full)
-        for (int u = 0; u < B; ++u) frame(fv[u]);
+        HistBatch<B, WRAP, NL>(cnt, hg, fv, z, lo, bs, inv, thr, prev, bl1, bl2, pushes, lowOut, highOut);
 #pragma clang loop unroll(full)
         for (int u = 0; u < B; ++u) fv[u] = nv[u];
     }
-    for (int t = full * B; t < frames; ++t) frame(tr[(size_t)t * LS]);
+    for (int t = full * B; t < frames; ++t) {
+        const int f1[1] = {tr[(size_t)t * LS]};
+        HistBatch<1, WRAP, NL>(cnt, hg, f1, z, lo, bs, inv, thr, prev, bl1, bl2, pushes, lowOut, highOut);
+    }
+}
+
+// The word of ZMW pair (z & ~1) that thread z moves between HBM and the
+// packed LDS layout: bin 2j for even z, bin 2j+1 for odd z.  One dword per
+// thread and pair of bins; the two halves are exchanged with the neighbour.
+__device__ __forceinline__ uint32_t UMax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+__device__ __forceinline__ uint32_t SwapPair(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
 }
 
 // Two passes so each is compiled on its own: the fast pass runs every lane
@@ -200,12 +240,50 @@ __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLa
     hwpa_lane_hist& h = hist[lane];
     EdgeLane& e = edge[lane];
     if (WRAP && e.needWrap == 0) return;
+    const bool odd = (z & 1) != 0;
+    // LDS bins: thread z reads the dword of bin 2j + (z & 1) holding ZMWs
+    // z & ~1 and (z & ~1) + 1, 16 bin pairs per batch so the loads overlap.
+    const uint32_t* rows = reinterpret_cast<const uint32_t*>(&h.bin_count[0][0]);
+    const int col = (z >> 1) + (odd ? LS / 2 : 0);   // dword index within a bin pair (2 rows = 64 dwords)
     uint32_t mx = 0;
-    for (int j = 0; j < NB / 2; ++j) {
-        const uint32_t c0 = h.bin_count[2 * j][z], c1 = h.bin_count[2 * j + 1][z];
-        mx = c0 > mx ? c0 : mx;
-        mx = c1 > mx ? c1 : mx;
-        if (2 * j < NL) cnt[j * LS + z] = c0 | (c1 << 16);
+    constexpr int NJ = NL / 2;
+#pragma unroll 1
+    for (int j0 = 0; j0 < NJ; j0 += 16) {
+        uint32_t c[16];
+#pragma clang loop unroll(full)
+        for (int k = 0; k < 16; ++k) {
+            const int j = (j0 + k < NJ) ? j0 + k : NJ - 1;
+            c[k] = rows[j * LS + col];
+        }
+#pragma clang loop unroll(full)
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t v = c[k], n = SwapPair(v);
+            mx = UMax(mx, UMax(v & 0xffffu, v >> 16));
+            const uint32_t packed = odd ? ((n >> 16) | (v & 0xffff0000u)) : ((v & 0xffffu) | (n << 16));
+            const int j = (j0 + k < NJ) ? j0 + k : NJ - 1;
+            cnt[j * LS + z] = packed;
+        }
+    }
+    // Bins [NL, NB): only their maximum matters; read as 16-byte vectors.
+    if (NL < NB) {
+        const uint4* rest = reinterpret_cast<const uint4*>(&h.bin_count[NL][0]);
+        constexpr int NV = (NB - NL) * LS * 2 / 16;
+#pragma unroll 1
+        for (int i0 = 0; i0 < NV; i0 += 8 * LS) {
+            uint4 c[8];
+#pragma clang loop unroll(full)
+            for (int k = 0; k < 8; ++k) {
+                const int i = i0 + k * LS + z;
+                c[k] = rest[i < NV ? i : NV - 1];
+            }
+#pragma clang loop unroll(full)
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t w[4] = {c[k].x, c[k].y, c[k].z, c[k].w};
+#pragma clang loop unroll(full)
+                for (int q = 0; q < 4; ++q)
+                    mx = UMax(mx, UMax(w[q] & 0xffffu, w[q] >> 16));
+            }
+        }
     }
     const bool fast = __all(mx + (uint32_t)frames <= 65535u);
     if (!WRAP && !fast) {
@@ -233,10 +311,13 @@ __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLa
     const int16_t* tr = traces + (size_t)lane * frames * LS + z;
     HistFrames<B, WRAP, NL>(cnt, h, tr, frames, z, lo, bs, inv, thr, prev, bl1, bl2, pushes, lowOut, highOut);
 
-    for (int j = 0; j < NL / 2; ++j) {
-        const uint32_t v = cnt[j * LS + z];
-        h.bin_count[2 * j][z] = (uint16_t)(v & 0xffffu);
-        h.bin_count[2 * j + 1][z] = (uint16_t)(v >> 16);
+    // Back to HBM, one dword per thread and bin pair (the neighbour's halves
+    // swapped in).  Bins >= NL were counted in place.
+    uint32_t* wrows = reinterpret_cast<uint32_t*>(&h.bin_count[0][0]);
+#pragma unroll 4
+    for (int j = 0; j < NJ; ++j) {
+        const uint32_t v = cnt[j * LS + z], n = SwapPair(v);
+        wrows[j * LS + col] = odd ? ((n >> 16) | (v & 0xffff0000u)) : ((v & 0xffffu) | (n << 16));
     }
     h.outlier_low[z] = (uint16_t)lowOut;
     h.outlier_high[z] = (uint16_t)highOut;
@@ -282,6 +363,8 @@ int32_t HistAddVariant(int32_t variant, hwpa_lane_hist* hist, void* edge, const 
                        const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
 {
     if (!hist || !edge || !traces || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "hist_add: bad argument");
+    if (reinterpret_cast<uintptr_t>(hist) % 16 != 0)
+        return Fail(HWPA_ERR_INVALID_ARG, "hist_add: histograms must be 16-byte aligned");
     EdgeLane* e = (EdgeLane*)edge;
     switch (variant) {
     case 0: return HistAddLaunch<16, 128>(hist, e, traces, frames, models, lanes, s);

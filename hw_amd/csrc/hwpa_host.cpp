@@ -552,7 +552,7 @@ int32_t hwpa_dme_estimate(const hwpa_config* cfg, const hwpa_lane_hist* d_hist, 
                           uint32_t lanes, int32_t* d_iters, void* stream)
 {
     if (!cfg || !model_fi || !est_fi) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
-    return DmeEstimate(*cfg, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, nullptr, AsStream(stream));
+    return DmeEstimate(*cfg, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, AsStream(stream));
 }
 
 int32_t hwpa_dme_estimate_diag(const hwpa_config* cfg, const hwpa_lane_hist* d_hist,
@@ -561,7 +561,7 @@ int32_t hwpa_dme_estimate_diag(const hwpa_config* cfg, const hwpa_lane_hist* d_h
                                int32_t* d_iters, float* d_diag, void* stream)
 {
     if (!cfg || !model_fi || !est_fi || !d_diag) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
-    return DmeEstimate(*cfg, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, nullptr, AsStream(stream),
+    return DmeEstimate(*cfg, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, AsStream(stream),
                        d_diag);
 }
 
@@ -573,17 +573,13 @@ int32_t hwpa_dme_estimate_learned(const hwpa_config* cfg, hwpa_initializer* m, c
     if (!cfg || !m || !model_fi || !est_fi || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "null argument or no lanes");
     hipStream_t s = AsStream(stream);
     LearnedWs w;
-    float* scratch = nullptr;
     int32_t rc = LearnedWsAlloc(w, lanes);
-    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&scratch, DmeScratchBytes(lanes), "EM scratch");
-    if (rc == HWPA_OK)
-        rc = DmeEstimateLearned(*cfg, m, w, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, scratch, s);
+    if (rc == HWPA_OK) rc = DmeEstimateLearned(*cfg, m, w, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, s);
     if (hipStreamSynchronize(s) != hipSuccess && rc == HWPA_OK) rc = Fail(HWPA_ERR_HIP, "learned EM failed");
     if (rc == HWPA_OK && fallback_count) {
         if (hipMemcpy(fallback_count, w.total, 4, hipMemcpyDeviceToHost) != hipSuccess)
             rc = Fail(HWPA_ERR_HIP, "fallback count copy failed");
     }
-    DeviceFree(scratch);
     LearnedWsFree(w);
     return rc;
 }

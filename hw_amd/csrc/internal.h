@@ -85,8 +85,7 @@ int32_t DmeInitModels(const hwpa_config& cfg, const hwpa_baseliner_stats* stats,
                       uint32_t lanes, hipStream_t s);
 int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
                     hwpa_lane_model* models, const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
-                    int32_t* d_iters, float* scratch, hipStream_t s, float* diag = nullptr);
-size_t DmeScratchBytes(uint32_t lanes);
+                    int32_t* d_iters, hipStream_t s, float* diag = nullptr);
 
 // Learned start of the EM: network input/output and fallback compaction buffers
 // for up to `lanes` lanes.
@@ -110,7 +109,7 @@ int32_t LearnedWsAlloc(LearnedWs& w, uint32_t lanes);
 void LearnedWsFree(LearnedWs& w);
 int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, LearnedWs& w, const hwpa_lane_hist* hist,
                            const hwpa_baseliner_stats* stats, hwpa_lane_model* models, const int32_t model_fi[2],
-                           const int32_t est_fi[2], uint32_t lanes, int32_t* d_iters, float* scratch, hipStream_t s);
+                           const int32_t est_fi[2], uint32_t lanes, int32_t* d_iters, hipStream_t s);
 int32_t MlpFeatures(const hwpa_lane_hist* hist, uint32_t lanes, uint16_t* x, hipStream_t s);
 int32_t MlpForward(const hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, uint16_t* h1, uint16_t* h2,
                    hipStream_t s);

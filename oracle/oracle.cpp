@@ -752,6 +752,7 @@ struct EmZ {
     uint16_t numFrames;
     float njSum;
     float varQuant;
+    float piv, q0, q1, q2;  // baseline moments about the pivot, E-step -> M-step
 };
 
 // DmeEmHost::PrelimScaleFactor (DmeEmHost.cpp:618-654)
@@ -814,9 +815,6 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
     static thread_local std::vector<EmZ> ez;
     ez.resize(LS);
     std::vector<HistZ> hz(LS);
-    // tau[z][mode][b], boundaryProb[z][b], binProb[z][b]
-    std::vector<float> tau((size_t)LS * 5 * (NB + 1)), bp((size_t)LS * (NB + 1)), binProb((size_t)LS * NB);
-    auto TAU = [&](int z, int m, int b) -> float& { return tau[((size_t)z * 5 + m) * (NB + 1) + b]; };
 
     const float rpa[NA] = {d.relAmp[0], d.relAmp[1], d.relAmp[2], d.relAmp[3]};
     const float rpaSqSum = (rpa[0] * rpa[0] + rpa[2] * rpa[2]) + (rpa[1] * rpa[1] + rpa[3] * rpa[3]);
@@ -927,42 +925,51 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
                 varinv[k] = 1.0f / e.var[k];
                 pref[k] = (hw_logf(e.rho[k]) - 0.5f * hw_logf(e.var[k])) - 0.5f * d.log2pi;
             }
-            float* bpz = &bp[(size_t)z * (NB + 1)];
-            float* binp = &binProb[(size_t)z * NB];
-            auto corner = [&](int b) {
+            // Every use of a responsibility is boundaryProb * tau_i = exp(t_i)
+            // (DmeEmHost.cpp:431-432,547-548): a boundary carries the component
+            // densities dd[i] = exp(t_i), boundaryProb is their sum.  The
+            // baseline second moment (DmeEmHost.cpp:536-552) is accumulated in
+            // the same pass about the pivot piv = this iteration's baseline mean
+            // and shifted to the new mean in the M-step.
+            const float piv = e.mu[0];
+            float dd[2][5], bpb[2], dev[2];
+            auto corner = [&](int b, int k) {
                 const float x = h.start[b];
-                float t[5];
                 for (int i = 0; i < 5; ++i) {
                     const float y = x - e.mu[i];
-                    t[i] = pref[i] - 0.5f * y * y * varinv[i];
+                    const float t = pref[i] - 0.5f * y * y * varinv[i];
+                    dd[k][i] = hw_expf(t);
                 }
-                float tmax = t[0];
-                for (int i = 1; i < 5; ++i) tmax = hw_maxf(tmax, t[i]);
-                float es = hw_expf(t[0] - tmax);
-                for (int i = 1; i < 5; ++i) es = es + hw_expf(t[i] - tmax);
-                const float llb = tmax + hw_logf(es);
-                for (int i = 0; i < 5; ++i) TAU(z, i, b) = hw_expf(t[i] - llb);
-                bpz[b] = hw_expf(llb);
+                bpb[k] = Sum5Tree(dd[k]);
+                dev[k] = x - piv;
             };
-            float probSum = 0.0f, wps = 0.0f;
+            float probSum = 0.0f, wps = 0.0f, q1 = 0.0f, q2 = 0.0f;
             float ci[5] = {0, 0, 0, 0, 0}, mom1[5] = {0, 0, 0, 0, 0};
             const float halfBin = 0.5f * h.binSize;
-            corner(0);
+            corner(0, 0);
             for (int b = 0; b < NB; ++b) {
-                corner(b + 1);
-                binp[b] = hw_maxf(halfBin * (bpz[b] + bpz[b + 1]), e.binProbMin);
-                probSum += binp[b];
+                const int k0 = b & 1, k1 = k0 ^ 1;
+                corner(b + 1, k1);
+                const float binp = hw_maxf(halfBin * (bpb[k0] + bpb[k1]), e.binProbMin);
+                probSum += binp;
                 const float nj = (float)h.cnt[b];
-                wps += nj * hw_logf(binp[b]);
-                const float factor = nj / binp[b];
+                wps += nj * hw_logf(binp);
+                const float factor = nj / binp;
                 const float x0 = h.start[b], x1 = h.start[b + 1];
                 for (int m = 0; m < 5; ++m) {
-                    const float tmp1 = bpz[b] * TAU(z, m, b);
-                    const float tmp2 = bpz[b + 1] * TAU(z, m, b + 1);
+                    const float tmp1 = dd[k0][m];
+                    const float tmp2 = dd[k1][m];
                     ci[m] += (tmp1 + tmp2) * factor;
                     mom1[m] += (tmp1 * x0 + tmp2 * x1) * factor;
                 }
+                const float g0 = dd[k0][0] * dev[k0], g1 = dd[k1][0] * dev[k1];
+                q1 += (g0 + g1) * factor;
+                q2 += (g0 * dev[k0] + g1 * dev[k1]) * factor;
             }
+            e.q0 = ci[0];
+            e.q1 = q1;
+            e.q2 = q2;
+            e.piv = piv;
             for (int m = 0; m < 5; ++m) {
                 ci[m] *= halfBin;
                 mom1[m] *= halfBin;
@@ -1027,17 +1034,9 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
                 E.status |= HWPA_ZS_VLOW_SIGNAL;
             }
             for (int a = 0; a < NA; ++a) E.mu[a + 1] = E.s * rpa[a];
-            const float* bpz = &bp[(size_t)z * (NB + 1)];
-            const float* binpz = &binProb[(size_t)z * NB];
-            float mom2 = 0.0f;
-            for (int j = 0; j < NB; ++j) {
-                const float x0 = H.start[j] - E.mu[0];
-                const float x1 = H.start[j + 1] - E.mu[0];
-                float tmp = bpz[j] * x0 * x0 * TAU(z, 0, j);
-                tmp += bpz[j + 1] * x1 * x1 * TAU(z, 0, j + 1);
-                tmp *= (float)H.cnt[j] / binpz[j];
-                mom2 += tmp;
-            }
+            // Baseline second moment about the new mean, from the moments about the pivot.
+            const float dl = E.mu[0] - E.piv;
+            float mom2 = (E.q2 - (2.0f * dl) * E.q1) + (dl * dl) * E.q0;
             mom2 *= 0.5f * H.binSize;
             E.var[0] = mom2 / c[0] + E.varQuant;
             E.var[0] = hw_clampf(E.var[0], d.k.blVarMin, d.k.blVarMax);

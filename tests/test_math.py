@@ -39,6 +39,21 @@ def test_exp_limits(O):
     assert abs(got[5] - np.float32(np.exp(-100.0))) <= 4 * np.float32(1.4e-45)
 
 
+def test_exp_is_positive_zero_below_minus_105(O):
+    """The EM kernel skips exps of boundaries whose exponents all lie below
+    -105 and uses +0 instead; that is exact only if exp returns +0 there.
+    Every float in [-106, -105) and a log-spaced sweep down to -FLT_MAX."""
+    lo = np.float32(-106.0).view(np.uint32)
+    hi = np.float32(-105.0).view(np.uint32)
+    dense = np.arange(hi, lo + 1, dtype=np.uint32).view(np.float32)   # negative floats: bits grow with |x|
+    sweep = -np.logspace(np.log10(105.0), np.log10(3.0e38), 100001).astype(np.float32)
+    x = np.concatenate([dense, sweep, np.float32([-np.inf, -3.4028235e38])])
+    got = _eval(O, 0, x)
+    assert np.all(got == 0.0) and not np.signbit(got).any()
+    # and -104 is still above the threshold of a non-zero result being possible
+    assert _eval(O, 0, np.float32([-103.0]))[0] > 0.0
+
+
 def test_log_accuracy(O):
     x = np.concatenate([np.exp(np.linspace(-80, 80, 200001)).astype(np.float32),
                         np.float32([1.0, 2.0, 0.5, 1.5e-42, 3.4e38, 1.0000001, 0.99999994])])
