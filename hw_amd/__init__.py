@@ -69,13 +69,15 @@ def load_json(cfg, text):
     return cfg
 
 
-def bench_config(frames=512):
-    """The every-chunk-estimates configuration used for measurement: sigma
-    EMA off (baseliner start-up 1024 frames) and one estimate per chunk."""
+def bench_config(frames=512, estimate_frames=None):
+    """The measurement configuration: sigma EMA off (baseliner start-up 1024
+    frames) and one model estimate per chunk, or per `estimate_frames`
+    frames when that is longer than a chunk."""
     cfg = default_config()
+    est = max(int(frames), int(estimate_frames or 0))
     cfg.sigma_ema_scale_strides = 0.0
-    cfg.min_frames_for_estimate = frames
-    cfg.num_frames_pre_accum_stats = frames
+    cfg.min_frames_for_estimate = est
+    cfg.num_frames_pre_accum_stats = est
     check(lib.hwpa_config_validate(ctypes.byref(cfg)))
     return cfg
 

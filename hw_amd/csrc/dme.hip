@@ -449,13 +449,16 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
         auto corner = [&](int b, float& bpOut, float* dOut, float& devOut) {
             const float x = h.Start(b);
             float t[5];
-            bool zero = true;
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
                 const float y = x - mu[i];
                 t[i] = pref[i] - 0.5f * y * y * varinv[i];
-                zero = zero && (t[i] < kExpZeroBelow);
             }
+            // Every t below the threshold and none NaN (t is never +inf: the
+            // variances are clamped above zero, so a NaN shows in the sum).
+            const float tmx = __builtin_fmaxf(__builtin_fmaxf(__builtin_fmaxf(t[0], t[1]), __builtin_fmaxf(t[2], t[3])), t[4]);
+            const float tsum = ((t[0] + t[1]) + (t[2] + t[3])) + t[4];
+            const bool zero = tmx < kExpZeroBelow && tsum == tsum;
             if (__all(zero)) {
 #pragma unroll
                 for (int i = 0; i < 5; ++i) dOut[i] = 0.0f;
@@ -468,16 +471,18 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
         };
         corner(0, bpPrev, dPrev, devPrev);
         float x0 = h.Start(0);
+        uint32_t cnt = h.Cnt(0);
         for (int b = 0; b < NB; ++b) {
+            const uint32_t cntNext = h.Cnt(b + 1);   // in flight while this boundary is computed
             float bpNext, dNext[5], devNext;
             corner(b + 1, bpNext, dNext, devNext);
+            // binp >= binProbMin >= FLT_MIN and finite: the log needs no special cases
             const float binp = hw_maxf(halfBin * (bpPrev + bpNext), binProbMin);
             probSum += binp;
-            const uint32_t cnt = h.Cnt(b);
             const float x1 = h.Start(b + 1);
             if (!__all(cnt == 0u)) {
                 const float nj = (float)cnt;
-                wps += nj * hw_logf(binp);
+                wps += nj * hw_logf_normal(binp);
                 const float factor = nj / binp;
 #pragma unroll
                 for (int m = 0; m < 5; ++m) {
@@ -493,6 +498,7 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
             for (int m = 0; m < 5; ++m) dPrev[m] = dNext[m];
             devPrev = devNext;
             x0 = x1;
+            cnt = cntNext;
         }
         const float q0 = ci[0];
 #pragma unroll

@@ -4,9 +4,9 @@
 // traceAnalysis/EdgeFrameClassifier.h).
 //
 // Mapping: one wave64 per lane, one thread per ZMW.  The counters of bins
-// [0, 128) -- where baseline and pulses of ordinary signal levels land -- are
-// staged in LDS for the chunk (16 KB per lane, so about nine waves fit a CU);
-// higher bins are counted with global atomics in place.  Each thread owns one
+// [0, NL) -- where baseline and pulses of ordinary signal levels land; NL is
+// 128, 160 or 192 (16, 20 or 24 KB of LDS per lane) -- are staged in LDS for
+// the chunk; higher bins are counted with global atomics in place.  Each thread owns one
 // ZMW column, and the [bin pair][zmw] layout puts the 64 lanes of every
 // increment on distinct LDS banks whatever the bins.  Frames are read in
 // batches of 16 with the next batch in flight while one is binned.
@@ -367,15 +367,15 @@ int32_t HistAddVariant(int32_t variant, hwpa_lane_hist* hist, void* edge, const 
         return Fail(HWPA_ERR_INVALID_ARG, "hist_add: histograms must be 16-byte aligned");
     EdgeLane* e = (EdgeLane*)edge;
     switch (variant) {
-    case 0: return HistAddLaunch<16, 128>(hist, e, traces, frames, models, lanes, s);
-    case 1: return HistAddLaunch<8, 128>(hist, e, traces, frames, models, lanes, s);
-    case 2: return HistAddLaunch<32, 128>(hist, e, traces, frames, models, lanes, s);
-    case 3: return HistAddLaunch<16, 96>(hist, e, traces, frames, models, lanes, s);
-    case 4: return HistAddLaunch<16, 64>(hist, e, traces, frames, models, lanes, s);
-    case 5: return HistAddLaunch<32, 96>(hist, e, traces, frames, models, lanes, s);
-    case 6: return HistAddLaunch<32, 64>(hist, e, traces, frames, models, lanes, s);
-    case 7: return HistAddLaunch<8, 96>(hist, e, traces, frames, models, lanes, s);
-    case 8: return HistAddLaunch<8, 64>(hist, e, traces, frames, models, lanes, s);
+    case 0: return HistAddLaunch<16, 160>(hist, e, traces, frames, models, lanes, s);
+    case 1: return HistAddLaunch<16, 128>(hist, e, traces, frames, models, lanes, s);
+    case 2: return HistAddLaunch<16, 192>(hist, e, traces, frames, models, lanes, s);
+    case 3: return HistAddLaunch<8, 160>(hist, e, traces, frames, models, lanes, s);
+    case 4: return HistAddLaunch<8, 128>(hist, e, traces, frames, models, lanes, s);
+    case 5: return HistAddLaunch<8, 192>(hist, e, traces, frames, models, lanes, s);
+    case 6: return HistAddLaunch<32, 160>(hist, e, traces, frames, models, lanes, s);
+    case 7: return HistAddLaunch<32, 128>(hist, e, traces, frames, models, lanes, s);
+    case 8: return HistAddLaunch<32, 192>(hist, e, traces, frames, models, lanes, s);
     default: return Fail(HWPA_ERR_INVALID_ARG, "hist_add: unknown variant");
     }
 }

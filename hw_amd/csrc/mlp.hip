@@ -49,6 +49,10 @@ struct GemmArgs {
     float* Cf;
     uint16_t* Cb;
     int64_t c_m, c_n;
+    // Split-K (EPI_F32 only): slice ks of kSplit covers k in [ks * kSlice,
+    // (ks + 1) * kSlice) and writes its partial product at Cf + ks * sliceStride.
+    int32_t kSplit, kSlice;
+    int64_t sliceStride;
 };
 
 template <bool CONTIG>
@@ -74,9 +78,9 @@ __device__ __forceinline__ bf16x8 LoadK(const uint16_t* __restrict__ p, bool val
 __device__ __forceinline__ bool PositiveBf16(uint16_t b) { return b != 0 && (b & 0x8000u) == 0; }
 
 template <bool AK, bool BK, int EPI>
-__global__ __launch_bounds__(256) void GemmKernel(GemmArgs g, uint32_t nTiles)
+__global__ __launch_bounds__(256) void GemmKernel(GemmArgs g, uint32_t nTiles, uint32_t tiles)
 {
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = blockIdx.x % tiles, ks = blockIdx.x / tiles;
     const int mt = (int)(tile / nTiles), nt = (int)(tile % nTiles);
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int m0 = mt * 64 + (wave >> 1) * 32, n0 = nt * 64 + (wave & 1) * 32;
@@ -85,11 +89,14 @@ __global__ __launch_bounds__(256) void GemmKernel(GemmArgs g, uint32_t nTiles)
     const bool aValid = am < g.M, bValid = bn < g.N;
     const int64_t aOff = (int64_t)am * g.a_m, bOff = (int64_t)bn * g.b_n;
     f32x16 acc = {};
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
-        const bf16x8 a = LoadK<AK>(g.A, aValid, aOff, k0 + kq, g.a_k, g.K);
-        const bf16x8 b = LoadK<BK>(g.B, bValid, bOff, k0 + kq, g.b_k, g.K);
+    const int kBeg = (int)ks * g.kSlice;
+    const int kEnd = (kBeg + g.kSlice < g.K) ? kBeg + g.kSlice : g.K;
+    for (int k0 = kBeg; k0 < kEnd; k0 += 16) {
+        const bf16x8 a = LoadK<AK>(g.A, aValid, aOff, k0 + kq, g.a_k, kEnd);
+        const bf16x8 b = LoadK<BK>(g.B, bValid, bOff, k0 + kq, g.b_k, kEnd);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
+    float* const cf = g.Cf + (int64_t)ks * g.sliceStride;
     const int n = n0 + r;
     if (n >= g.N) return;
 #pragma unroll
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(256) void GemmKernel(GemmArgs g, uint32_t nTiles)
         const int64_t c = (int64_t)m * g.c_m + (int64_t)n * g.c_n;
         const float v = acc[i];
         if (EPI == EPI_F32) {
-            g.Cf[c] = v;
+            cf[c] = v;
         } else if (EPI == EPI_BIAS_F32) {
             g.Cf[c] = v + g.bias[n];
         } else if (EPI == EPI_BIAS_RELU_BF16) {
@@ -121,9 +128,54 @@ static int32_t Gemm(const GemmArgs& g, hipStream_t s)
         return Fail(HWPA_ERR_INVALID_ARG, "gemm: A not 16-byte aligned along k");
     if (BK && (((uintptr_t)g.B & 15) != 0 || (g.b_n & 7) != 0 || g.b_k != 1))
         return Fail(HWPA_ERR_INVALID_ARG, "gemm: B not 16-byte aligned along k");
-    const uint32_t mt = (uint32_t)((g.M + 63) / 64), nt = (uint32_t)((g.N + 63) / 64);
-    hipLaunchKernelGGL((GemmKernel<AK, BK, EPI>), dim3(mt * nt), dim3(256), 0, s, g, nt);
+    GemmArgs h = g;
+    if (h.kSplit <= 1) {
+        h.kSplit = 1;
+        h.kSlice = h.K;
+        h.sliceStride = 0;
+    } else if (EPI != EPI_F32 || h.kSlice % 16 != 0 || (int64_t)h.kSplit * h.kSlice < h.K) {
+        return Fail(HWPA_ERR_INVALID_ARG, "gemm: bad k split");
+    }
+    const uint32_t mt = (uint32_t)((h.M + 63) / 64), nt = (uint32_t)((h.N + 63) / 64);
+    hipLaunchKernelGGL((GemmKernel<AK, BK, EPI>), dim3(mt * nt * (uint32_t)h.kSplit), dim3(256), 0, s, h, nt, mt * nt);
     return CheckLaunch("GemmKernel");
+}
+
+// out[i] = sum over the kSplit slices of part[ks * n + i], slices in order.
+__global__ __launch_bounds__(256) void SliceSumKernel(const float* __restrict__ part, int32_t slices, int64_t n,
+                                                      float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+    for (int32_t k = 0; k < slices; ++k) acc += part[(int64_t)k * n + i];
+    out[i] = acc;
+}
+
+// Weight gradient out[M][N] (row-major) = A^T-like product over a long K
+// (the batch): K is cut into up to kMaxSplit slices of at least 128 rows
+// whose partial products are summed in a fixed order.
+constexpr int kMaxSplit = 32;
+
+template <bool AK, bool BK>
+static int32_t GemmSplitK(GemmArgs g, float* part, float* out, hipStream_t s)
+{
+    int split = (g.K + 127) / 128;
+    split = split < kMaxSplit ? split : kMaxSplit;
+    if (split < 1) split = 1;
+    const int slice = (((g.K + split - 1) / split) + 15) / 16 * 16;
+    split = (g.K + slice - 1) / slice;
+    g.kSplit = split;
+    g.kSlice = slice;
+    g.Cf = part;
+    g.c_m = g.N;
+    g.c_n = 1;
+    g.sliceStride = (int64_t)g.M * g.N;
+    int32_t rc = Gemm<AK, BK, EPI_F32>(g, s);
+    if (rc != HWPA_OK) return rc;
+    const int64_t n = (int64_t)g.M * g.N;
+    hipLaunchKernelGGL(SliceSumKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, split, n, out);
+    return CheckLaunch("SliceSumKernel");
 }
 
 static GemmArgs MakeGemm(int M, int N, int K, const uint16_t* A, int64_t a_m, int64_t a_k, const uint16_t* B,
@@ -337,6 +389,7 @@ struct hwpa_initializer {
     float* lossDev = nullptr;
     float* omap = nullptr;      // device {off[6], scale[6], w[7]}
     float omapHost[19] = {};
+    float* kPart = nullptr;     // split-K partial weight gradients, kMaxSplit x MH x MH
 };
 
 namespace hwpa {
@@ -459,10 +512,7 @@ static int32_t Backward(hwpa_initializer* m, const uint16_t* x, const float* t, 
     const int B = (int)batch;
     // dW3[o][h] = sum_b dY[b][o] H2[b][h]
     GemmArgs g = MakeGemm(MO, MH, B, m->dYb, 1, MO, m->H2, MH, 1);
-    g.Cf = m->G + OFF_W3;
-    g.c_m = MH;
-    g.c_n = 1;
-    if ((rc = Gemm<false, false, EPI_F32>(g, s)) != HWPA_OK) return rc;
+    if ((rc = GemmSplitK<false, false>(g, m->kPart, m->G + OFF_W3, s)) != HWPA_OK) return rc;
     if ((rc = ColSum(m->dY, false, batch, MO, m->colPart, m->G + OFF_B3, s)) != HWPA_OK) return rc;
     // dH2[b][h] = (sum_o dY[b][o] W3[o][h]) * [H2 > 0]
     g = MakeGemm(B, MH, MO, m->dYb, MO, 1, Wb + OFF_W3, MH, 1);
@@ -475,10 +525,7 @@ static int32_t Backward(hwpa_initializer* m, const uint16_t* x, const float* t, 
     if ((rc = Gemm<false, false, EPI_MASK_BF16>(g, s)) != HWPA_OK) return rc;
     // dW2[o][i] = sum_b dH2[b][o] H1[b][i]
     g = MakeGemm(MH, MH, B, m->dH2, 1, MH, m->H1, MH, 1);
-    g.Cf = m->G + OFF_W2;
-    g.c_m = MH;
-    g.c_n = 1;
-    if ((rc = Gemm<false, false, EPI_F32>(g, s)) != HWPA_OK) return rc;
+    if ((rc = GemmSplitK<false, false>(g, m->kPart, m->G + OFF_W2, s)) != HWPA_OK) return rc;
     if ((rc = ColSum(m->dH2, true, batch, MH, m->colPart, m->G + OFF_B2, s)) != HWPA_OK) return rc;
     // dH1[b][i] = (sum_o dH2[b][o] W2[o][i]) * [H1 > 0]
     g = MakeGemm(B, MH, MH, m->dH2, MH, 1, Wb + OFF_W2, MH, 1);
@@ -491,10 +538,7 @@ static int32_t Backward(hwpa_initializer* m, const uint16_t* x, const float* t, 
     if ((rc = Gemm<true, false, EPI_MASK_BF16>(g, s)) != HWPA_OK) return rc;
     // dW1[o][i] = sum_b dH1[b][o] X[b][i]
     g = MakeGemm(MH, MI, B, m->dH1, 1, MH, x, MI, 1);
-    g.Cf = m->G + OFF_W1;
-    g.c_m = MI;
-    g.c_n = 1;
-    if ((rc = Gemm<false, false, EPI_F32>(g, s)) != HWPA_OK) return rc;
+    if ((rc = GemmSplitK<false, false>(g, m->kPart, m->G + OFF_W1, s)) != HWPA_OK) return rc;
     return ColSum(m->dH1, true, batch, MH, m->colPart, m->G + OFF_B1, s);
 }
 
@@ -558,6 +602,7 @@ int32_t hwpa_initializer_create(uint64_t seed, hwpa_initializer** out)
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->Ve, NPARAM * 4, "mlp moments");
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->lossDev, 16, "mlp loss");
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->omap, 19 * sizeof(float), "mlp output map");
+    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->kPart, (size_t)kMaxSplit * MH * MH * 4, "mlp split-K partials");
     if (rc != HWPA_OK) {
         hwpa_initializer_destroy(m);
         return rc;
@@ -596,6 +641,7 @@ int32_t hwpa_initializer_destroy(hwpa_initializer* m)
     DeviceFree(m->Ve);
     DeviceFree(m->lossDev);
     DeviceFree(m->omap);
+    DeviceFree(m->kPart);
     delete m;
     return HWPA_OK;
 }

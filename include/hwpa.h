@@ -123,8 +123,8 @@ int32_t hwpa_batch_autotune(hwpa_batch* b, int32_t repeats);
 
 /* ---- kernel-shape autotuner (no reference counterpart) -------------------
  * Tunable kernels run in variants that give the same bits with different
- * shapes: kernel 0 = histogram (frames per prefetched batch x bins kept in
- * LDS, 9 variants), kernel 1 = baseline filter (1, 2 or 4 lanes per
+ * shapes: kernel 0 = histogram (8, 16 or 32 frames per prefetched batch x
+ * 128, 160 or 192 bins kept in LDS: 9 variants), kernel 1 = baseline filter (1, 2 or 4 lanes per
  * workgroup).  The launchers look up the variant for their (lanes, frames);
  * unset shapes use variant 0.  hwpa_autotune times every variant on
  * synthetic traces of the shape (`repeats` timed runs each, HIP events) and

@@ -128,17 +128,11 @@ HWPA_HD float hw_expf(float x)
     return hw_isnan(x) ? x : v;
 }
 
-/* Returns log(m) for m in [sqrt(1/2), sqrt(2)) and the binary exponent e. */
-HWPA_HD float hw_log_reduce(float x, int* e_out)
+/* log(m) for m in [sqrt(1/2), sqrt(2)) and the binary exponent e of the
+ * positive normal float with bits u, offset by e0. */
+HWPA_HD float hw_log_reduce_bits(uint32_t u, int e0, int* e_out)
 {
-    uint32_t u = hw_f2u(x);
-    int e = 0;
-    if (u < 0x00800000u) {           /* subnormal: scale by 2^24 */
-        x = x * 16777216.0f;
-        u = hw_f2u(x);
-        e = -24;
-    }
-    e += (int)((u >> 23) & 0xffu) - 127;
+    int e = e0 + (int)((u >> 23) & 0xffu) - 127;
     float m = hw_u2f((u & 0x007fffffu) | 0x3f800000u);
     if (m > 1.41421353816986083984375f) {
         m = m * 0.5f;
@@ -155,6 +149,31 @@ HWPA_HD float hw_log_reduce(float x, int* e_out)
     q = q * s2;
     *e_out = e;
     return hw_fmaf(s + s, q, s + s);
+}
+
+/* Returns log(m) for m in [sqrt(1/2), sqrt(2)) and the binary exponent e. */
+HWPA_HD float hw_log_reduce(float x, int* e_out)
+{
+    uint32_t u = hw_f2u(x);
+    int e0 = 0;
+    if (u < 0x00800000u) {           /* subnormal: scale by 2^24 */
+        x = x * 16777216.0f;
+        u = hw_f2u(x);
+        e0 = -24;
+    }
+    return hw_log_reduce_bits(u, e0, e_out);
+}
+
+/* hw_logf for finite x >= FLT_MIN: the same operations without the special
+ * cases, so the same bits. */
+HWPA_HD float hw_logf_normal(float x)
+{
+    int e;
+    const float lm = hw_log_reduce_bits(hw_f2u(x), 0, &e);
+    const float ef = (float)e;
+    const float ln2hi = 0.693145751953125f;
+    const float ln2lo = 1.428606765330187045e-06f;
+    return hw_fmaf(ef, ln2hi, hw_fmaf(ef, ln2lo, lm));
 }
 
 HWPA_HD float hw_logf(float x)

@@ -953,7 +953,7 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
                 const float binp = hw_maxf(halfBin * (bpb[k0] + bpb[k1]), e.binProbMin);
                 probSum += binp;
                 const float nj = (float)h.cnt[b];
-                wps += nj * hw_logf(binp);
+                wps += nj * hw_logf_normal(binp);   // binp >= binProbMin >= FLT_MIN, finite
                 const float factor = nj / binp;
                 const float x0 = h.start[b], x1 = h.start[b + 1];
                 for (int m = 0; m < 5; ++m) {
@@ -1960,6 +1960,7 @@ void orc_math(int32_t kind, const float* a, const float* b, float* out, int32_t 
         case 4: out[i] = hw_powf(a[i], b[i]); break;
         case 5: out[i] = hw_h2f(hw_f2h(a[i])); break;
         case 6: out[i] = (float)hw_round_i16(a[i]); break;
+        case 8: out[i] = hw_logf_normal(a[i]); break;
         default: out[i] = (float)hw_floor_i32(a[i]); break;
         }
     }
