@@ -203,7 +203,8 @@ def count_equal_i16(a, b, n):
     return out.value
 
 
-TUNE_HISTOGRAM, TUNE_BASELINE = 0, 1
+TUNE_HISTOGRAM, TUNE_BASELINE, TUNE_LABEL, TUNE_PULSE = 0, 1, 2, 3
+_TUNE_NAMES = ("histogram", "baseline", "labels", "pulses")
 
 
 def tune_variants(kernel):
@@ -225,13 +226,18 @@ def tune_get(kernel, lanes, frames):
 
 def autotune(cfg, lanes, frames, repeats=3):
     """Times every kernel variant on this shape and keeps the fastest; returns
-    {"histogram": [ms...], "baseline": [ms...], "chosen": (hist, baseline)}."""
-    nh, nb = tune_variants(TUNE_HISTOGRAM), tune_variants(TUNE_BASELINE)
-    ms = (ctypes.c_float * (nh + nb))()
+    {"histogram": [ms...], "baseline": [...], "labels": [...], "pulses": [...],
+    "chosen": (one variant per kernel, in that order)}."""
+    counts = [tune_variants(k) for k in range(len(_TUNE_NAMES))]
+    ms = (ctypes.c_float * sum(counts))()
     check(lib.hwpa_autotune(ctypes.byref(cfg), ctypes.c_uint32(lanes), ctypes.c_uint32(frames),
-                            ctypes.c_int32(repeats), ms, ctypes.c_int32(nh + nb)))
-    return {"histogram": list(ms)[:nh], "baseline": list(ms)[nh:],
-            "chosen": (tune_get(TUNE_HISTOGRAM, lanes, frames), tune_get(TUNE_BASELINE, lanes, frames))}
+                            ctypes.c_int32(repeats), ms, ctypes.c_int32(sum(counts))))
+    out, o = {}, 0
+    for name, n in zip(_TUNE_NAMES, counts):
+        out[name] = list(ms)[o:o + n]
+        o += n
+    out["chosen"] = tuple(tune_get(k, lanes, frames) for k in range(len(_TUNE_NAMES)))
+    return out
 
 
 def copy_bandwidth(nbytes=1 << 30, launches=10):

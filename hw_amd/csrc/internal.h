@@ -74,7 +74,7 @@ int32_t HistAdd(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_
 
 // Kernel shape variants chosen per (lanes, frames) by the autotuner; every
 // variant computes the same bits.  Variant 0 is the default.
-enum TuneKernel { kTuneHistogram = 0, kTuneBaseline = 1, kTuneKernels = 2 };
+enum TuneKernel { kTuneHistogram = 0, kTuneBaseline = 1, kTuneLabel = 2, kTunePulse = 3, kTuneKernels = 4 };
 int32_t TuneVariants(int kernel);
 int32_t TuneGet(int kernel, uint32_t lanes, uint32_t frames);
 void TuneSet(int kernel, uint32_t lanes, uint32_t frames, int32_t variant);

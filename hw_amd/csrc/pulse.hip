@@ -39,7 +39,7 @@ __device__ __forceinline__ float StdMin(float a, float b) { return (b < a) ? b :
 __device__ __forceinline__ float StdMax(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ uint16_t ToFixed(float v) { return (uint16_t)(v / 0.1f + 0.5f); }
 
-template <int UNROLL>
+template <int UNROLL, bool PREFETCH>
 __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ labels, const int16_t* __restrict__ latent,
                                                   const int16_t* __restrict__ traces, int frames, int32_t firstFrame,
                                                   const hwpa_lane_model* __restrict__ models, SegLane* seg,
@@ -62,14 +62,22 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
     const int16_t* lt = latent + (size_t)lane * NL * LS + z;
     const int16_t* tr = traces + (size_t)lane * frames * LS + z;
 
-    for (int f0 = 0; f0 < frames; f0 += UNROLL) {
-        int lv[UNROLL], sv[UNROLL];
+    // Labels and signal of frames [f0, f0 + UNROLL); with PREFETCH the next
+    // batch is loaded while one is accumulated.
+    auto load = [&](int f0, int (&lv)[UNROLL], int (&sv)[UNROLL]) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             const int rf = f0 + u;
             lv[u] = (rf < frames) ? (int)lb[(size_t)rf * LS] : 0;
             sv[u] = (rf < NL) ? (int)lt[(size_t)rf * LS] : ((rf < frames) ? (int)tr[(size_t)(rf - NL) * LS] : 0);
         }
+    };
+    int lv[UNROLL], sv[UNROLL];
+    if (PREFETCH) load(0, lv, sv);
+    for (int f0 = 0; f0 < frames; f0 += UNROLL) {
+        int ln[UNROLL], sn[UNROLL];
+        if (PREFETCH) load(f0 + UNROLL, ln, sn);
+        else load(f0, lv, sv);
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             const int rf = f0 + u;
@@ -120,6 +128,13 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
                 bm0 += 1.0f;
                 bm1 += x;
                 bm2 += x * x;
+            }
+        }
+        if (PREFETCH) {
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                lv[u] = ln[u];
+                sv[u] = sn[u];
             }
         }
     }
@@ -194,8 +209,16 @@ int32_t PulserRun(PulserDev* p, const int16_t* labels, const int16_t* latent, co
     if (!p || !labels || !latent || !traces || !models || !pulses || !counts || !overflows || !bl_stats)
         return Fail(HWPA_ERR_INVALID_ARG, "pulser_run: null argument");
     if (frames <= (uint32_t)NL) return Fail(HWPA_ERR_INVALID_ARG, "pulser_run: frames must be > 24");
-    hipLaunchKernelGGL(PulseKernel<8>, dim3(p->lanes), dim3(LS), 0, s, labels, latent, traces, (int)frames,
-                       first_frame, models, p->seg, pulses, counts, overflows, bl_stats, p->k);
+#define HWPA_PULSE_LAUNCH(U, PF)                                                                                  \
+    hipLaunchKernelGGL((PulseKernel<U, PF>), dim3(p->lanes), dim3(LS), 0, s, labels, latent, traces, (int)frames,   \
+                       first_frame, models, p->seg, pulses, counts, overflows, bl_stats, p->k)
+    // Variants: frames per batch x next batch in flight.
+    switch (TuneGet(kTunePulse, p->lanes, frames)) {
+    case 1: HWPA_PULSE_LAUNCH(8, true); break;
+    case 2: HWPA_PULSE_LAUNCH(16, true); break;
+    default: HWPA_PULSE_LAUNCH(8, false); break;
+    }
+#undef HWPA_PULSE_LAUNCH
     return CheckLaunch("PulseKernel");
 }
 

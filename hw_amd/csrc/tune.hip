@@ -1,8 +1,9 @@
 // tune.hip -- autotuner of kernel shapes.  No reference counterpart.
 //
 // Tunable kernels come in variants that compute the same bits with different
-// shapes: the baseline filter's lane tile (lanes per workgroup) and the
-// histogram's frame block (frames per prefetched batch) and LDS bin window.
+// shapes: the baseline filter's lane tile (lanes per workgroup), the
+// histogram's frame block (frames per prefetched batch) and LDS bin window,
+// and the frame batches of the Viterbi labeler and the pulse accumulator.
 // The table maps (kernel, lanes, frames) to the variant to launch; the
 // autotuner fills it by timing every variant on synthetic traces of the
 // shape with HIP events, and the launchers read it on every call.
@@ -24,6 +25,8 @@ int32_t TuneVariants(int kernel)
     switch (kernel) {
     case kTuneHistogram: return 9;
     case kTuneBaseline: return 3;
+    case kTuneLabel: return 2;
+    case kTunePulse: return 3;
     default: return 0;
     }
 }
@@ -152,6 +155,42 @@ int32_t hwpa_autotune(const hwpa_config* cfg, uint32_t lanes, uint32_t frames, i
         BaselinerDev* b;
         ~BlGuard() { BaselinerDestroy(b); }
     } blg{bl};
+    // Labeler and pulser inputs: labels, latent frames, scores and pulse slots.
+    const bool stages = frames > (uint32_t)NL;
+    LabelerDev* lab = nullptr;
+    PulserDev* pul = nullptr;
+    if (stages) {
+        if ((rc = LabelerCreate(*cfg, lanes, frames, &lab)) != HWPA_OK) return rc;
+        if ((rc = PulserCreate(*cfg, lanes, &pul)) != HWPA_OK) {
+            LabelerDestroy(lab);
+            return rc;
+        }
+    }
+    struct StageGuard {
+        LabelerDev* l;
+        PulserDev* p;
+        ~StageGuard()
+        {
+            if (l) LabelerDestroy(l);
+            if (p) PulserDestroy(p);
+        }
+    } sg{lab, pul};
+    int16_t *labels = nullptr, *latent = nullptr;
+    float* score = nullptr;
+    hwpa_pulse* pulses = nullptr;
+    uint32_t *counts = nullptr, *ovf = nullptr;
+    hwpa_stat_acc* pbl = nullptr;
+    if (stages) {
+        const uint32_t mc = PulserMaxCalls(pul);
+        if ((rc = sc.Alloc(&labels, trace * 2)) != HWPA_OK ||
+            (rc = sc.Alloc(&latent, (size_t)lanes * NL * LS * 2)) != HWPA_OK ||
+            (rc = sc.Alloc(&score, (size_t)lanes * LS * 4)) != HWPA_OK ||
+            (rc = sc.Alloc(&pulses, (size_t)lanes * LS * mc * sizeof(hwpa_pulse))) != HWPA_OK ||
+            (rc = sc.Alloc(&counts, (size_t)lanes * LS * 4)) != HWPA_OK ||
+            (rc = sc.Alloc(&ovf, (size_t)lanes * LS * 4)) != HWPA_OK ||
+            (rc = sc.Alloc(&pbl, lanes * sizeof(hwpa_stat_acc))) != HWPA_OK)
+            return rc;
+    }
 
     std::vector<float> times;
     for (int kernel = 0; kernel < kTuneKernels; ++kernel) {
@@ -163,6 +202,19 @@ int32_t hwpa_autotune(const hwpa_config* cfg, uint32_t lanes, uint32_t frames, i
             if (kernel == kTuneBaseline) {
                 TuneSet(kTuneBaseline, lanes, frames, v);
                 t = TimeIt(s, repeats, [&]() { return BaselinerRun(bl, raw, frames, bls, st, s); });
+            } else if (kernel == kTuneLabel || kernel == kTunePulse) {
+                // Labels of the baselined traces under the start-up models.
+                if (!stages) {
+                    t = 0.0f;
+                } else if (kernel == kTuneLabel) {
+                    TuneSet(kTuneLabel, lanes, frames, v);
+                    t = TimeIt(s, repeats, [&]() { return LabelerRun(lab, bls, frames, models, labels, latent, score, s); });
+                } else {
+                    TuneSet(kTunePulse, lanes, frames, v);
+                    t = TimeIt(s, repeats, [&]() {
+                        return PulserRun(pul, labels, latent, bls, frames, 0, models, pulses, counts, ovf, pbl, s);
+                    });
+                }
             } else {
                 // Histogram inputs: this shape's baselined traces, bounds and models from its stats.
                 if (v == 0) {

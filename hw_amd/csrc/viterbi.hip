@@ -175,6 +175,8 @@ __device__ __forceinline__ int Pred(uint32_t w, int state)
     return ((w >> (20 + q)) & 1u) ? 5 + i : 1 + i;
 }
 
+// PB = frames per prefetched batch (8 or 16; an ROI word holds 16 frames).
+template <int PB>
 __global__ __launch_bounds__(64) void LabelKernel(const int16_t* __restrict__ traces, int frames,
                                                   const hwpa_lane_model* __restrict__ models, LabLaneState* state,
                                                   uint32_t* __restrict__ tbAll, int tbFrames,
@@ -198,33 +200,53 @@ __global__ __launch_bounds__(64) void LabelKernel(const int16_t* __restrict__ tr
 
     const float inv1 = 1.0f / hw_sqrtf(hw_h2f(st.oldModel.baseline.vars[z]));
     const float inv2 = 1.0f / hw_sqrtf(hw_h2f(model.baseline.vars[z]));
-    auto seq = [&](int k) -> float {
-        return (k < NL) ? (float)st.prevLat[k][z] * inv1 : (float)tr[(size_t)(k - NL) * LS] * inv2;
+    // Frame k of the latent ++ chunk sequence (k < NL: the carried latent frames).
+    auto rawAt = [&](int k) -> int { return (k < NL) ? (int)st.prevLat[k][z] : (int)tr[(size_t)(k - NL) * LS]; };
+    // Frames are read in batches of PB with the next batch in flight; indices
+    // past `limit` re-read the last frame (their values are never used).
+    auto loadBatch = [&](int base, int limit, int (&dst)[PB]) {
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+            const int k = (base + u < limit) ? base + u : limit - 1;
+            dst[u] = rawAt(k);
+        }
     };
 
-    // ROI forward pass (RoiHost.h:137-193)
+    // ROI forward pass (RoiHost.h:137-193) over k < nroi, reading frame k + 2.
     {
-        float p0 = (float)st.latentTrc[z] * inv1, p1 = seq(0), p2 = seq(1);
+        float p0 = (float)st.latentTrc[z] * inv1, p1 = (float)rawAt(0) * inv1, p2 = (float)rawAt(1) * inv1;
         uint32_t prevRoi = st.roiBoundary[z];
         const float s2 = hw_sqrtf(2.0f), s3 = hw_sqrtf(3.0f);
+        const int limit = nroi + 2;
+        int cur[PB], nxt[PB];
+        loadBatch(2, limit, cur);
         uint32_t word = 0;
-        for (int k = 0; k < nroi; ++k) {
-            const float nv = seq(k + 2);
-            float mval = p1;
-            mval = hw_maxf(mval, (p0 + p1) / s2);
-            mval = hw_maxf(mval, (p0 + p1 + p2) / s3);
-            mval = hw_maxf(mval, (p0 + p1 + p2 + nv) / 2.0f);
-            const float thresh = (prevRoi & 1u) ? roiLower : roiUpper;
-            const uint32_t v = (mval >= thresh ? 1u : 0u) | (mval >= roiLower ? 2u : 0u);
-            word |= v << (2 * (k & 15));
-            if ((k & 15) == 15 || k == nroi - 1) {
-                roiWords[(k >> 4) * LS + z] = word;
-                word = 0;
+        for (int k0 = 0; k0 < nroi; k0 += PB) {
+            loadBatch(k0 + PB + 2, limit, nxt);
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const int k = k0 + u;
+                if (k < nroi) {
+                    const float nv = (float)cur[u] * ((k + 2 < NL) ? inv1 : inv2);
+                    float mval = p1;
+                    mval = hw_maxf(mval, (p0 + p1) / s2);
+                    mval = hw_maxf(mval, (p0 + p1 + p2) / s3);
+                    mval = hw_maxf(mval, (p0 + p1 + p2 + nv) / 2.0f);
+                    const float thresh = (prevRoi & 1u) ? roiLower : roiUpper;
+                    const uint32_t v = (mval >= thresh ? 1u : 0u) | (mval >= roiLower ? 2u : 0u);
+                    word |= v << (2 * (k & 15));
+                    if ((k & 15) == 15 || k == nroi - 1) {
+                        roiWords[(k >> 4) * LS + z] = word;
+                        word = 0;
+                    }
+                    prevRoi = v;
+                    p0 = p1;
+                    p1 = p2;
+                    p2 = nv;
+                }
             }
-            prevRoi = v;
-            p0 = p1;
-            p1 = p2;
-            p2 = nv;
+#pragma unroll
+            for (int u = 0; u < PB; ++u) cur[u] = nxt[u];
         }
     }
     // ROI backward pass (RoiHost.h:195-243): final ROI bit into bit 0 of each field
@@ -250,11 +272,9 @@ __global__ __launch_bounds__(64) void LabelKernel(const int16_t* __restrict__ tr
 #pragma unroll
     for (int s = 0; s < NS; ++s) ll[s] = (lb == s) ? 0.0f : hw_u2f(0xff800000u);
     ScorerZ sc;
-    ScorerSetup(sc, st.oldModel, z);
     float anchorLL[NS];
-    for (int f = 0; f < nlab; ++f) {
-        if (f == NL) ScorerSetup(sc, model, z);
-        const float d = (f < NL) ? (float)st.prevLat[f][z] : (float)tr[(size_t)(f - NL) * LS];
+    auto step = [&](int f, int raw) {
+        const float d = (float)raw;
         const bool isRoi = ((roiWords[(f >> 4) * LS + z] >> (2 * (f & 15))) & 1u) != 0;
         float s[NS];
         Scores(sc, d, s);
@@ -262,6 +282,24 @@ __global__ __launch_bounds__(64) void LabelKernel(const int16_t* __restrict__ tr
         if (f == frames - 1) {
 #pragma unroll
             for (int r = 0; r < NS; ++r) anchorLL[r] = ll[r];
+        }
+    };
+    // The latent frames are scored with the previous chunk's model, the chunk
+    // and stitch frames with the current one.
+    ScorerSetup(sc, st.oldModel, z);
+    for (int f = 0; f < NL; ++f) step(f, (int)st.prevLat[f][z]);
+    ScorerSetup(sc, model, z);
+    {
+        int cur[PB], nxt[PB];
+        loadBatch(NL, nlab, cur);
+        for (int f0 = NL; f0 < nlab; f0 += PB) {
+            loadBatch(f0 + PB, nlab, nxt);
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                if (f0 + u < nlab) step(f0 + u, cur[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < PB; ++u) cur[u] = nxt[u];
         }
     }
 
@@ -308,12 +346,32 @@ __global__ __launch_bounds__(64) void LabelKernel(const int16_t* __restrict__ tr
     for (int i = 1; i < NS; ++i) sv = (anchorState == i) ? anchorLL[i] : sv;
     score[(size_t)lane * LS + z] = sv;
 
-    // Traceback
+    // Traceback, PB predecessor words per batch with the next batch in flight.
     int16_t* lo = labels + (size_t)lane * frames * LS + z;
     int stt = anchorState;
-    for (int f = frames - 1; f >= 0; --f) {
-        lo[(size_t)f * LS] = (int16_t)stt;
-        stt = Pred(tb[(size_t)f * LS], stt);
+    {
+        auto loadTb = [&](int top, uint32_t (&dst)[PB]) {
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const int f = (top - u >= 0) ? top - u : 0;
+                dst[u] = tb[(size_t)f * LS];
+            }
+        };
+        uint32_t cur[PB], nxt[PB];
+        loadTb(frames - 1, cur);
+        for (int top = frames - 1; top >= 0; top -= PB) {
+            loadTb(top - PB, nxt);
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const int f = top - u;
+                if (f >= 0) {
+                    lo[(size_t)f * LS] = (int16_t)stt;
+                    stt = Pred(cur[u], stt);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PB; ++u) cur[u] = nxt[u];
+        }
     }
 
     // Latent update
@@ -502,9 +560,15 @@ int32_t LabelerRun(LabelerDev* l, const int16_t* traces, uint32_t frames, const 
         return Fail(HWPA_ERR_INVALID_ARG, "labeler_run: frames outside (24, max_frames]");
     const int nroi = (int)frames + HWPA_VITERBI_LOOKBACK + HWPA_ROI_STITCH;
     const size_t lds = (size_t)((nroi + 15) / 16) * LS * sizeof(uint32_t);
-    hipLaunchKernelGGL(LabelKernel, dim3(l->lanes), dim3(LS), lds, s, traces, (int)frames, models, l->state, l->tb,
-                       (int)(l->maxFrames + HWPA_VITERBI_LOOKBACK), labels, latent_out, score, l->T, l->roiUpper,
-                       l->roiLower);
+    // Variants: frames per prefetched batch.
+    if (TuneGet(kTuneLabel, l->lanes, frames) == 1)
+        hipLaunchKernelGGL(LabelKernel<8>, dim3(l->lanes), dim3(LS), lds, s, traces, (int)frames, models, l->state,
+                           l->tb, (int)(l->maxFrames + HWPA_VITERBI_LOOKBACK), labels, latent_out, score, l->T,
+                           l->roiUpper, l->roiLower);
+    else
+        hipLaunchKernelGGL(LabelKernel<16>, dim3(l->lanes), dim3(LS), lds, s, traces, (int)frames, models, l->state,
+                           l->tb, (int)(l->maxFrames + HWPA_VITERBI_LOOKBACK), labels, latent_out, score, l->T,
+                           l->roiUpper, l->roiLower);
     return CheckLaunch("LabelKernel");
 }
 

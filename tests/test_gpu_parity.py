@@ -86,13 +86,12 @@ def test_baseliner_lane_tile_variants(H, O, T, gpu, variant):
 def test_autotune_picks_a_timed_variant(H, gpu):
     cfg = H.bench_config(512)
     r = H.autotune(cfg, 64, 512, repeats=2)
-    assert len(r["histogram"]) == H.tune_variants(H.TUNE_HISTOGRAM) == 9
-    assert len(r["baseline"]) == H.tune_variants(H.TUNE_BASELINE) == 3
-    assert all(t > 0 for t in r["histogram"] + r["baseline"])
-    h, b = r["chosen"]
-    assert r["histogram"][h] == min(r["histogram"]) and r["baseline"][b] == min(r["baseline"])
-    H.tune_set(H.TUNE_HISTOGRAM, 64, 512, 0)
-    H.tune_set(H.TUNE_BASELINE, 64, 512, 0)
+    counts = {"histogram": 9, "baseline": 3, "labels": 2, "pulses": 3}
+    for i, (name, n) in enumerate(counts.items()):
+        assert len(r[name]) == H.tune_variants(i) == n
+        assert all(t > 0 for t in r[name])
+        assert r[name][r["chosen"][i]] == min(r[name])
+        H.tune_set(i, 64, 512, 0)
 
 
 def test_baseliner_sigma_ema_and_scaler(H, O, T, gpu):
@@ -247,9 +246,25 @@ def test_dme_estimate_bit_exact(H, O, T, gpu, seed):
     _eq_struct(g_models, o_models, "models")
 
 
-def test_labeler_bit_exact(H, O, T, gpu):
-    L, F, C = 4, 512, 4
-    cfg, chunks = _pipeline_inputs(H, O, T, L, F, C, seed=33)
+@pytest.fixture(params=[0, 1])
+def label_variant(request, H):
+    """Both autotuner variants of the labeler (16 or 8 frames per batch)."""
+    for F in (512, 200):
+        H.tune_set(H.TUNE_LABEL, 4, F, request.param)
+    yield request.param
+    for F in (512, 200):
+        H.tune_set(H.TUNE_LABEL, 4, F, 0)
+
+
+@pytest.mark.parametrize("F", [512, 200])
+def test_labeler_bit_exact(H, O, T, gpu, label_variant, F):
+    """Labels, latent frames and scores bit for bit; 200-frame chunks (the
+    first 200 frames of baselined 208-frame chunks) end in partial batches."""
+    L, C = 4, 4
+    cfg, chunks = _pipeline_inputs(H, O, T, L, 512 if F == 512 else 208, C, seed=33)
+    if F != 512:
+        cfg = H.bench_config(F)
+        chunks = [(np.ascontiguousarray(bl[:, :F, :]), st) for bl, st in chunks]
     agg = O.Aggregator(L)
     ol = O.Labeler(cfg, L)
     gl = H.Labeler(cfg, L, F)
@@ -268,8 +283,10 @@ def test_labeler_bit_exact(H, O, T, gpu):
         assert np.array_equal(H.to_numpy(d_score, np.float32, (L, 64)).view(np.uint32), o_score.view(np.uint32))
 
 
-def test_pulses_bit_exact(H, O, T, gpu):
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_pulses_bit_exact(H, O, T, gpu, variant):
     L, F, C = 4, 512, 4
+    H.tune_set(H.TUNE_PULSE, L, F, variant)
     cfg, chunks = _pipeline_inputs(H, O, T, L, F, C, seed=44)
     agg = O.Aggregator(L)
     ol = O.Labeler(cfg, L)
@@ -298,6 +315,7 @@ def test_pulses_bit_exact(H, O, T, gpu):
                 b = np.ascontiguousarray(o_p[z, :k]).view(np.uint8)
                 assert np.array_equal(a, b), "chunk %d zmw %d pulses differ" % (c, z)
         _eq_struct(H.to_numpy(d_bl, T.STAT_ACC, (L,)), o_bl, "pulse baseline stats")
+    H.tune_set(H.TUNE_PULSE, L, F, 0)
 
 
 def test_analyzer_end_to_end(H, O, T, gpu):
