@@ -288,7 +288,7 @@ int32_t hwpa_config_default(hwpa_config* c)
     c->learned_init = 0;
     c->learned_conf_threshold = 0.0f;
     c->learned_em_iteration_limit = 3;
-    c->learned_conv_tol = 1e-3f;
+    c->learned_conv_tol = 2e-3f;
     // Movie description of the synthetic workload: 100 fps, 1 e-/DN, four analogs
     // with amplitudes 180/120/80/50 over a 12-count baseline sigma.
     c->movie.frame_rate = 100.0f;
