@@ -158,7 +158,8 @@ __device__ __forceinline__ void RingRefill(uint32_t* ring, uint32_t last)
 }
 
 // LPB lanes (waves) per workgroup -- the autotuned lane tile.
-template <class T, int LPB>
+// LPB lanes per workgroup; PD groups of raw frames in flight (1 or 2).
+template <class T, int LPB, int PD>
 __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const int16_t* __restrict__ raw, int frames, int lanes,
                                                            int16_t* __restrict__ out,
                                                            hwpa_baseliner_stats* __restrict__ stats,
@@ -210,9 +211,15 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const int16_t* __rest
     const int groups = frames / S;
     const float minSigma = .288675135f;
 
-    int rv[S];
+    static_assert(PD == 1 || PD == 2, "prefetch depth");
+    int rv[S], pv[S];
 #pragma unroll
     for (int j = 0; j < S; ++j) rv[j] = rl[(size_t)j * LS];
+    if (PD == 2) {
+        const int g1 = (groups > 1) ? 1 : 0;
+#pragma unroll
+        for (int j = 0; j < S; ++j) pv[j] = rl[(size_t)(g1 * S + j) * LS];
+    }
 
     for (int g = 0; g < groups; ++g) {
         // Final-stage ring: refill on wrap, then read the fronts early.
@@ -222,9 +229,9 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const int16_t* __rest
         }
         const uint32_t frontA = ringA[ctr * LS];
         const uint32_t frontB = ringB[ctr * LS];
-        // Prefetch the next group's raw frames.
+        // Prefetch the raw frames of group g + PD (the last group re-read past the end).
         int nv[S];
-        const int gn = (g + 1 < groups) ? g + 1 : g;
+        const int gn = (g + PD < groups) ? g + PD : groups - 1;
 #pragma unroll
         for (int j = 0; j < S; ++j) nv[j] = rl[(size_t)(gn * S + j) * LS];
 
@@ -309,7 +316,10 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const int16_t* __rest
         gm1 += bgv;
         gm2 += bgv * bgv;
 #pragma unroll
-        for (int j = 0; j < S; ++j) rv[j] = nv[j];
+        for (int j = 0; j < S; ++j) {
+            rv[j] = (PD == 2) ? pv[j] : nv[j];
+            pv[j] = nv[j];
+        }
     }
 
     // Persist carried state.
@@ -400,18 +410,22 @@ static int32_t Launch(BaselinerDev* b, const int16_t* raw, uint32_t frames, int1
 {
     if (frames % T::S != 0) return Fail(HWPA_ERR_INVALID_ARG, "frames must be a multiple of the aggregate stride");
     if ((size_t)T::winPerLane != b->winPerLane) return Fail(HWPA_ERR_UNSUPPORTED, "baseliner: window layout mismatch");
+    // Variants: lanes per workgroup (1, 2, 4) x raw-frame groups in flight (1, 2).
     const int v = TuneGet(kTuneBaseline, b->lanes, frames);
-    const int lpb = (v == 1) ? 2 : (v == 2 ? 4 : 1);
+    const int lpb = (v % 3 == 1) ? 2 : (v % 3 == 2 ? 4 : 1);
+    const bool deep = v >= 3;
     const unsigned grid = (b->lanes + (unsigned)lpb - 1) / (unsigned)lpb;
-    if (lpb == 1)
-        hipLaunchKernelGGL((BaselineKernel<T, 1>), dim3(grid), dim3(64), 0, s, raw, (int)frames, (int)b->lanes, out,
-                           stats, b->state, b->win, b->k);
-    else if (lpb == 2)
-        hipLaunchKernelGGL((BaselineKernel<T, 2>), dim3(grid), dim3(128), 0, s, raw, (int)frames, (int)b->lanes, out,
-                           stats, b->state, b->win, b->k);
-    else
-        hipLaunchKernelGGL((BaselineKernel<T, 4>), dim3(grid), dim3(256), 0, s, raw, (int)frames, (int)b->lanes, out,
-                           stats, b->state, b->win, b->k);
+#define HWPA_BL_LAUNCH(L, P)                                                                                         \
+    hipLaunchKernelGGL((BaselineKernel<T, L, P>), dim3(grid), dim3(64 * L), 0, s, raw, (int)frames, (int)b->lanes,   \
+                       out, stats, b->state, b->win, b->k)
+    if (lpb == 1) {
+        if (deep) HWPA_BL_LAUNCH(1, 2); else HWPA_BL_LAUNCH(1, 1);
+    } else if (lpb == 2) {
+        if (deep) HWPA_BL_LAUNCH(2, 2); else HWPA_BL_LAUNCH(2, 1);
+    } else {
+        if (deep) HWPA_BL_LAUNCH(4, 2); else HWPA_BL_LAUNCH(4, 1);
+    }
+#undef HWPA_BL_LAUNCH
     return CheckLaunch("BaselineKernel");
 }
 

@@ -24,7 +24,7 @@ int32_t TuneVariants(int kernel)
 {
     switch (kernel) {
     case kTuneHistogram: return 9;
-    case kTuneBaseline: return 3;
+    case kTuneBaseline: return 6;
     case kTuneLabel: return 2;
     case kTunePulse: return 3;
     default: return 0;

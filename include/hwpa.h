@@ -125,7 +125,7 @@ int32_t hwpa_batch_autotune(hwpa_batch* b, int32_t repeats);
  * Tunable kernels run in variants that give the same bits with different
  * shapes: kernel 0 = histogram (8, 16 or 32 frames per prefetched batch x
  * 128, 160 or 192 bins kept in LDS: 9 variants), kernel 1 = baseline filter (1, 2 or 4 lanes per
- * workgroup), kernel 2 = Viterbi labeler (16 or 8 frames per prefetched
+ * workgroup, then the same with two groups of raw frames in flight), kernel 2 = Viterbi labeler (16 or 8 frames per prefetched
  * batch), kernel 3 = pulse accumulator (8 frames, 8 frames with the next
  * batch in flight, 16 with the next in flight).  The launchers look up the variant for their (lanes, frames);
  * unset shapes use variant 0.  hwpa_autotune times every variant on

@@ -63,10 +63,12 @@ def test_baseliner_bit_exact(H, O, T, gpu, ftype):
         _eq_struct(g_stats, o_stats, "stats[c=%d]" % c)
 
 
-@pytest.mark.parametrize("variant", [1, 2])
-def test_baseliner_lane_tile_variants(H, O, T, gpu, variant):
-    """2 and 4 lanes per workgroup, with a lane count that leaves the last tile partial."""
-    L, F, C = 5, 512, 3
+@pytest.mark.parametrize("F", [512, 32, 16])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+def test_baseliner_lane_tile_variants(H, O, T, gpu, variant, F):
+    """2 and 4 lanes per workgroup (a lane count that leaves the last tile
+    partial), one or two frame groups in flight, chunks of 1, 2 or 32 groups."""
+    L, C = 5, 3
     H.tune_set(H.TUNE_BASELINE, L, F, variant)
     try:
         cfg = H.default_config()
@@ -86,7 +88,7 @@ def test_baseliner_lane_tile_variants(H, O, T, gpu, variant):
 def test_autotune_picks_a_timed_variant(H, gpu):
     cfg = H.bench_config(512)
     r = H.autotune(cfg, 64, 512, repeats=2)
-    counts = {"histogram": 9, "baseline": 3, "labels": 2, "pulses": 3}
+    counts = {"histogram": 9, "baseline": 6, "labels": 2, "pulses": 3}
     for i, (name, n) in enumerate(counts.items()):
         assert len(r[name]) == H.tune_variants(i) == n
         assert all(t > 0 for t in r[name])
