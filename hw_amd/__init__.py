@@ -433,6 +433,48 @@ def tracegen(params, lanes, frames, first_frame, d_state, init, d_out, lane_offs
                                 ctypes.c_int32(1 if init else 0), ptr(d_out), _vp(stream)))
 
 
+def activity_tree(children_left, children_right, feature, threshold, value, hsw_curve, max_acceptable_hsw_rate):
+    """A trained activity-labeler tree as the C-ABI takes it; returns
+    (ActivityTree, arrays) -- keep the tuple alive while the tree is in use."""
+    arrs = (np.ascontiguousarray(children_left, dtype=np.int16), np.ascontiguousarray(children_right, dtype=np.int16),
+            np.ascontiguousarray(feature, dtype=np.int8), np.ascontiguousarray(threshold, dtype=np.float32),
+            np.ascontiguousarray(value, dtype=np.int8))
+    t = T.ActivityTree()
+    t.num_nodes = len(arrs[0])
+    t.max_acceptable_hsw_rate = float(max_acceptable_hsw_rate)
+    t.hsw_curve[0], t.hsw_curve[1] = float(hsw_curve[0]), float(hsw_curve[1])
+    t.children_left, t.children_right, t.feature, t.threshold, t.value = [a.ctypes.data for a in arrs]
+    return t, arrs
+
+
+class Metrics:
+    """HFMetricsFilter counterpart (traceAnalysis/HFMetricsFilterHost.cpp): one
+    process call per chunk of pulses; returns True when a block ended and its
+    BasecallingMetrics were written to `out`."""
+
+    def __init__(self, cfg, lanes):
+        self.h = _vp()
+        self.lanes = lanes
+        check(lib.hwpa_metrics_create(ctypes.byref(cfg), ctypes.c_uint32(lanes), ctypes.byref(self.h)))
+
+    def set_activity_tree(self, tree):
+        check(lib.hwpa_metrics_set_activity_tree(self.h, ctypes.byref(tree[0]) if tree is not None else None))
+
+    def process(self, d_pulses, d_counts, max_calls, frames, last_frame, d_bl_stats, d_score, d_pd_stats, d_models,
+                d_out, stream=None):
+        produced = ctypes.c_int32(0)
+        check(lib.hwpa_metrics_process(self.h, ptr(d_pulses), ptr(d_counts), ctypes.c_uint32(max_calls),
+                                       ctypes.c_uint32(frames), ctypes.c_int32(last_frame), ptr(d_bl_stats),
+                                       ptr(d_score), ptr(d_pd_stats), ptr(d_models), ptr(d_out),
+                                       ctypes.byref(produced), _vp(stream)))
+        return produced.value != 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.hwpa_metrics_destroy(self.h)
+            self.h = None
+
+
 class BatchAnalyzer:
     """Per-pool analyzer with the reference's BatchAnalyzer shape
     (analyzer/BatchAnalyzer.h:51-106): construct once per pool, call once per
@@ -514,6 +556,14 @@ class BatchAnalyzer:
         return self._fetch(lib.hwpa_result_pulse_baseline_stats, self.lanes * STAT_ACC.itemsize, STAT_ACC,
                            (self.lanes,))
 
+    def metrics(self):
+        """BasecallingMetrics of the block that ended with the last chunk (flags()[3])."""
+        return self._fetch(lib.hwpa_result_metrics, self.lanes * T.METRICS.itemsize, T.METRICS, (self.lanes,))
+
+    def set_activity_tree(self, tree):
+        """tree: an activity_tree(...) result or None."""
+        check(lib.hwpa_analyzer_set_activity_tree(self.h, ctypes.byref(tree[0]) if tree is not None else None))
+
     def pulses(self):
         """Returns (pulses [lanes*64, max_calls], counts [lanes*64], overflows [lanes*64])."""
         pp, pc, po = _vp(), _vp(), _vp()
@@ -533,9 +583,10 @@ class BatchAnalyzer:
         check(lib.hwpa_analyzer_set_profiling(self.h, ctypes.c_int32(1 if on else 0)))
 
     def stage_ms(self):
+        """[baseline, aggregate, histogram, dme, labels, pulses, metrics] ms of the last call."""
         ms = (ctypes.c_float * 8)()
         check(lib.hwpa_analyzer_stage_ms(self.h, ms))
-        return list(ms)[:6]
+        return list(ms)[:7]
 
     def em_iterations(self):
         it = ctypes.c_int32()

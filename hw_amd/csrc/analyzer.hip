@@ -40,13 +40,13 @@ struct Interval {
 };
 
 enum { kHistInit = 0, kDmeInit = 1, kSequencing = 2 };
-enum { kStBaseline = 0, kStAggregate, kStHistogram, kStDme, kStLabels, kStPulses, kNumStages };
+enum { kStBaseline = 0, kStAggregate, kStHistogram, kStDme, kStLabels, kStPulses, kStMetrics, kNumStages };
 // Kernel groups one process call enqueued; equal plans enqueue the same launch sequence.
 enum : uint32_t {
     kPlanCopyIn = 1u << 0, kPlanEarlyZero = 1u << 1, kPlanAggregate = 1u << 2, kPlanHistAdd = 1u << 3,
     kPlanDmeInit = 1u << 4, kPlanEstimate = 1u << 5, kPlanLearned = 1u << 6, kPlanCapture = 1u << 7,
     kPlanReset = 1u << 8, kPlanLabels = 1u << 9, kPlanPulses = 1u << 10, kPlanLateZero = 1u << 11,
-    kPlanEvents = 1u << 12
+    kPlanEvents = 1u << 12, kPlanMetrics = 1u << 13, kPlanMetricsEnd = 1u << 14
 };
 
 }  // namespace hwpa
@@ -71,6 +71,8 @@ struct hwpa_analyzer {
     BaselinerDev* bl = nullptr;
     LabelerDev* lab = nullptr;
     PulserDev* pul = nullptr;
+    hwpa_metrics* met = nullptr;
+    hwpa_basecalling_metrics* dMetrics = nullptr;
     int16_t* dTraces = nullptr;
     int16_t* dBaselined = nullptr;
     hwpa_baseliner_stats* dStats = nullptr;
@@ -106,9 +108,10 @@ static void FreeAnalyzer(hwpa_analyzer* a)
     BaselinerDestroy(a->bl);
     LabelerDestroy(a->lab);
     PulserDestroy(a->pul);
+    MetricsDestroy(a->met);
     void* bufs[] = {a->dTraces, a->dBaselined, a->dStats,  a->dAgg,    a->dHist,      a->dEdge,
                     a->dModels, a->dLabels,    a->dLatent, a->dScore,  a->dPulses,    a->dCounts,
-                    a->dOvf,    a->dPulseBl,   a->dIters};
+                    a->dOvf,    a->dPulseBl,   a->dIters,  a->dMetrics};
     for (void* p : bufs) DeviceFree(p);
     LearnedWsFree(a->lws);
     DeviceFree(a->dDiag);
@@ -172,6 +175,7 @@ int32_t hwpa_analyzer_create(uint32_t pool_id, uint32_t lanes, uint32_t frames, 
         {(void**)&a->dCounts, (size_t)lanes * LS * 4, "counts"},
         {(void**)&a->dOvf, (size_t)lanes * LS * 4, "overflows"},
         {(void**)&a->dPulseBl, sizeof(hwpa_stat_acc) * lanes, "pulse stats"},
+        {(void**)&a->dMetrics, sizeof(hwpa_basecalling_metrics) * lanes, "basecalling metrics"},
         {(void**)&a->dIters, sizeof(int32_t) * lanes, "EM iterations"},
     };
     for (auto& x : allocs) {
@@ -182,7 +186,7 @@ int32_t hwpa_analyzer_create(uint32_t pool_id, uint32_t lanes, uint32_t frames, 
         }
     }
     if ((rc = BaselinerCreate(*cfg, lanes, &a->bl)) != HWPA_OK || (rc = LabelerCreate(*cfg, lanes, frames, &a->lab)) != HWPA_OK ||
-        (rc = PulserCreate(*cfg, lanes, &a->pul)) != HWPA_OK) {
+        (rc = PulserCreate(*cfg, lanes, &a->pul)) != HWPA_OK || (rc = MetricsCreate(*cfg, lanes, &a->met)) != HWPA_OK) {
         FreeAnalyzer(a);
         return rc;
     }
@@ -322,6 +326,16 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
         return rc;
     if (a->fullEstimation) {
         a->flags[0] = 1;
+        // HF metrics of this pulse batch, which ends NL latent frames before the
+        // chunk (DynamicEstimateBatchAnalyzer::AnalyzeImpl, BatchAnalyzer.cpp:366-372).
+        HWPA_STAGE_BEGIN(a, kStMetrics);
+        int32_t produced = 0;
+        if ((rc = MetricsProcess(a->met, a->dPulses, a->dCounts, a->cfg.max_calls_per_zmw, F,
+                                 first_frame + (int32_t)F - NL, a->dStats, a->dScore, a->dPulseBl, a->dModels,
+                                 a->dMetrics, &produced, s)) != HWPA_OK)
+            return rc;
+        a->flags[3] = produced;
+        a->plan |= kPlanMetrics | (produced ? kPlanMetricsEnd : 0u);
     } else {
         a->plan |= kPlanLateZero;
         if ((rc = zeroCounts()) != HWPA_OK) return rc;
@@ -394,6 +408,19 @@ int32_t hwpa_result_viterbi_score(const hwpa_analyzer* a, const float** d)
     if (!a || !d) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
     *d = a->dScore;
     return HWPA_OK;
+}
+
+int32_t hwpa_result_metrics(const hwpa_analyzer* a, const hwpa_basecalling_metrics** d)
+{
+    if (!a || !d) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *d = a->dMetrics;
+    return HWPA_OK;
+}
+
+int32_t hwpa_analyzer_set_activity_tree(hwpa_analyzer* a, const hwpa_activity_tree* tree)
+{
+    if (!a) return Fail(HWPA_ERR_INVALID_ARG, "null analyzer");
+    return MetricsSetTree(a->met, tree);
 }
 
 int32_t hwpa_analyzer_set_profiling(hwpa_analyzer* a, int32_t enable)

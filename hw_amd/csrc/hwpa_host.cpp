@@ -285,6 +285,10 @@ int32_t hwpa_config_default(hwpa_config* c)
     c->max_calls_per_zmw = 48;
     c->xsp_amp_thresh = 0.70;
     c->xsp_width_thresh = 3.5f;
+    // BasecallerMetricsConfig (BasecallerMetricsConfig.h:57-60)
+    c->sandwich_tolerance = 0;
+    c->frames_per_hf_metric_block = 4096;
+    c->realtime_activity_labels = 1;
     c->learned_init = 0;
     c->learned_conf_threshold = 0.0f;
     c->learned_em_iteration_limit = 3;
@@ -364,6 +368,8 @@ int32_t hwpa_config_load_json(hwpa_config* c, const char* json)
         {"dmeConfig.MinFramesForEstimate", &c->min_frames_for_estimate},
         {"dmeConfig.NumFramesPreAccumStats", &c->num_frames_pre_accum_stats},
         {"pulseAccumConfig.maxCallsPerZmw", &c->max_calls_per_zmw},
+        {"metricsConfig.sandwichTolerance", &c->sandwich_tolerance},
+        {"metricsConfig.framesPerHFMetricBlock", &c->frames_per_hf_metric_block},
         {"learnedInit.EmIterationLimit", &c->learned_em_iteration_limit},
     };
     for (const auto& kv : jf.out) {
@@ -388,10 +394,12 @@ int32_t hwpa_config_load_json(hwpa_config* c, const char* json)
             const int t = FilterByName(v);
             if (t < 0) return Fail(HWPA_ERR_BAD_CONFIG, "unknown filter type: " + v);
             c->filter_type = t;
-        } else if (k == "dmeConfig.IterateToLimit" || k == "learnedInit.Enable") {
+        } else if (k == "dmeConfig.IterateToLimit" || k == "learnedInit.Enable" ||
+                   k == "metricsConfig.realtimeActivityLabels") {
             if (!ToF(v, &d)) return Fail(HWPA_ERR_BAD_CONFIG, "not a bool: " + k);
             if (k == "dmeConfig.IterateToLimit") c->iterate_to_limit = d != 0;
-            else c->learned_init = d != 0;
+            else if (k == "learnedInit.Enable") c->learned_init = d != 0;
+            else c->realtime_activity_labels = d != 0;
         } else if (k == "pulseAccumConfig.XspAmpThresh") {
             if (!ToF(v, &d)) return Fail(HWPA_ERR_BAD_CONFIG, "not a number: " + k);
             c->xsp_amp_thresh = d;
@@ -443,6 +451,7 @@ int32_t hwpa_config_validate(const hwpa_config* c)
     if (c->xsp_width_thresh < 0) return Fail(HWPA_ERR_BAD_CONFIG, "Bad XspWidthThresh: must be non-negative");
     if (c->xsp_amp_thresh < 0.0 || c->xsp_amp_thresh > 1.0) return Fail(HWPA_ERR_BAD_CONFIG, "Bad XspAmpThresh: valid range is [0, 1]");
     if (c->max_calls_per_zmw == 0) return Fail(HWPA_ERR_BAD_CONFIG, "maxCallsPerZmw must be positive");
+    if (c->frames_per_hf_metric_block == 0) return Fail(HWPA_ERR_BAD_CONFIG, "framesPerHFMetricBlock must be positive");
     if (!(c->movie.photoelectron_sensitivity > 0)) return Fail(HWPA_ERR_BAD_CONFIG, "photoelectronSensitivity must be positive");
     // DmeEmHost::EstimateLaneDetModel relative amplitude check (DmeEmHost.cpp:301-305)
     for (int a = 0; a < 4; ++a)
@@ -462,7 +471,7 @@ int32_t hwpa_layout_sizes(int64_t* out, int32_t n)
                           (int64_t)sizeof(hwpa_baseliner_stats), (int64_t)sizeof(hwpa_lane_hist),
                           (int64_t)sizeof(hwpa_lane_model),      (int64_t)sizeof(hwpa_pulse),
                           (int64_t)sizeof(hwpa_tracegen_params), (int64_t)sizeof(hwpa_tracegen_state),
-                          (int64_t)sizeof(hwpa_transition)};
+                          (int64_t)sizeof(hwpa_transition),      (int64_t)sizeof(hwpa_basecalling_metrics)};
     const int32_t m = (int32_t)(sizeof(sz) / sizeof(sz[0]));
     if (!out) return m;
     for (int32_t i = 0; i < n && i < m; ++i) out[i] = sz[i];

@@ -117,6 +117,14 @@ const float* MlpOutputMap(const hwpa_initializer* m);  // device {off[6], scale[
 int32_t MlpTargets(const hwpa_lane_hist* hist, const float* diag, uint32_t lanes, float minConf, float* t,
                    uint8_t* valid, hipStream_t s);
 
+int32_t MetricsCreate(const hwpa_config& cfg, uint32_t lanes, hwpa_metrics** out);
+void MetricsDestroy(hwpa_metrics* m);
+int32_t MetricsSetTree(hwpa_metrics* m, const hwpa_activity_tree* t);
+int32_t MetricsProcess(hwpa_metrics* m, const hwpa_pulse* pulses, const uint32_t* counts, uint32_t maxCalls,
+                       uint32_t frames, int32_t lastFrame, const hwpa_baseliner_stats* blStats, const float* score,
+                       const hwpa_stat_acc* pdStats, const hwpa_lane_model* models, hwpa_basecalling_metrics* out,
+                       int32_t* produced, hipStream_t s);
+
 struct LabelerDev;
 int32_t LabelerCreate(const hwpa_config& cfg, uint32_t lanes, uint32_t max_frames, LabelerDev** out);
 void LabelerDestroy(LabelerDev* l);

@@ -56,7 +56,9 @@ class Config(ctypes.Structure):
         ("viterbi_alpha", ctypes.c_float), ("viterbi_beta", ctypes.c_float), ("viterbi_gamma", ctypes.c_float),
         ("roi_upper_threshold", ctypes.c_float), ("roi_lower_threshold", ctypes.c_float),
         ("max_calls_per_zmw", ctypes.c_uint32), ("xsp_amp_thresh", ctypes.c_double),
-        ("xsp_width_thresh", ctypes.c_float), ("learned_init", ctypes.c_int32),
+        ("xsp_width_thresh", ctypes.c_float), ("sandwich_tolerance", ctypes.c_uint32),
+        ("frames_per_hf_metric_block", ctypes.c_uint32), ("realtime_activity_labels", ctypes.c_int32),
+        ("learned_init", ctypes.c_int32),
         ("learned_conf_threshold", ctypes.c_float), ("learned_em_iteration_limit", ctypes.c_uint32),
         ("learned_conv_tol", ctypes.c_float), ("movie", MovieInfo)]
 
@@ -106,12 +108,36 @@ TRACEGEN_STATE = np.dtype([("slope", np.float32, (LANE,)), ("step_frame", np.flo
                            ("in_pulse", np.uint8, (LANE,)), ("analog", np.uint8, (LANE,)),
                            ("empty", np.uint8, (LANE,)), ("pad", np.uint8, (LANE,))], align=True)
 
+_U16 = (np.uint16, (LANE,))
+_A16 = (np.uint16, (NUM_ANALOGS, LANE))
+_AF = (np.float32, (NUM_ANALOGS, LANE))
+METRICS = np.dtype([("activity_label", np.uint8, (LANE,)), ("num_pulse_frames",) + _U16,
+                    ("num_base_frames",) + _U16, ("num_sandwiches",) + _U16, ("num_half_sandwiches",) + _U16,
+                    ("num_pulse_label_stutters",) + _U16, ("num_bases",) + _U16, ("num_pulses",) + _U16,
+                    ("pk_mid_signal",) + _AF, ("bp_zvar",) + _AF, ("pk_zvar",) + _AF, ("pk_max",) + _AF,
+                    ("num_pk_mid_frames",) + _A16, ("num_pk_mid_bases_by_analog",) + _A16,
+                    ("num_bases_by_analog",) + _A16, ("num_pulses_by_analog",) + _A16,
+                    ("frame_baseline_dws", np.float32, (LANE,)), ("frame_baseline_variance_dws", np.float32, (LANE,)),
+                    ("num_frames_baseline",) + _U16, ("start_frame", np.uint32, (LANE,)), ("num_frames",) + _U16,
+                    ("autocorrelation", np.float32, (LANE,)), ("pulse_detection_score", np.float32, (LANE,)),
+                    ("pixel_checksum", np.int16, (LANE,))], align=True)
+assert METRICS.itemsize == 137 * LANE
+
+
+class ActivityTree(ctypes.Structure):
+    _fields_ = [("num_nodes", ctypes.c_int32), ("max_acceptable_hsw_rate", ctypes.c_float),
+                ("hsw_curve", ctypes.c_float * 2), ("children_left", ctypes.c_void_p),
+                ("children_right", ctypes.c_void_p), ("feature", ctypes.c_void_p), ("threshold", ctypes.c_void_p),
+                ("value", ctypes.c_void_p)]
+
+
 LAYOUT_ORDER = [("hwpa_config", ctypes.sizeof(Config)), ("hwpa_movie_info", ctypes.sizeof(MovieInfo)),
                 ("hwpa_stat_acc", STAT_ACC.itemsize), ("hwpa_autocorr_acc", AUTOCORR.itemsize),
                 ("hwpa_baseliner_stats", BASELINER_STATS.itemsize), ("hwpa_lane_hist", LANE_HIST.itemsize),
                 ("hwpa_lane_model", LANE_MODEL.itemsize), ("hwpa_pulse", PULSE.itemsize),
                 ("hwpa_tracegen_params", ctypes.sizeof(TracegenParams)),
-                ("hwpa_tracegen_state", TRACEGEN_STATE.itemsize), ("hwpa_transition", ctypes.sizeof(Transition))]
+                ("hwpa_tracegen_state", TRACEGEN_STATE.itemsize), ("hwpa_transition", ctypes.sizeof(Transition)),
+                ("hwpa_basecalling_metrics", METRICS.itemsize)]
 
 
 def half_to_float(a):

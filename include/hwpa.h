@@ -57,6 +57,7 @@ int32_t hwpa_transition_matrix(const hwpa_config* cfg, hwpa_transition* out);
 
 typedef struct hwpa_analyzer hwpa_analyzer;
 typedef struct hwpa_initializer hwpa_initializer;  /* learned EM initialiser, below */
+typedef struct hwpa_metrics hwpa_metrics;          /* HF metrics stage, below */
 
 /* BatchAnalyzer ctor + DynamicEstimateBatchAnalyzer (analyzer/BatchAnalyzer.cpp:71-118). */
 int32_t hwpa_analyzer_create(uint32_t pool_id, uint32_t lanes, uint32_t frames_per_chunk,
@@ -85,10 +86,17 @@ int32_t hwpa_result_models(const hwpa_analyzer* a, const hwpa_lane_model** d_mod
 int32_t hwpa_result_baseliner_stats(const hwpa_analyzer* a, const hwpa_baseliner_stats** d_stats);
 int32_t hwpa_result_pulse_baseline_stats(const hwpa_analyzer* a, const hwpa_stat_acc** d_stats);
 int32_t hwpa_result_viterbi_score(const hwpa_analyzer* a, const float** d_score);
+/* BasecallingMetrics of the block that ended with the last chunk (flags[3]);
+ * BatchResult::metrics of the reference (AN/BatchAnalyzer.cpp:366-372). */
+int32_t hwpa_result_metrics(const hwpa_analyzer* a, const hwpa_basecalling_metrics** d_metrics);
+/* Trained activity-labeler tree for the analyzer's metrics (copied; NULL clears it:
+ * activity labels then stay EMPTY, as with realtime_activity_labels = 0). */
+int32_t hwpa_analyzer_set_activity_tree(hwpa_analyzer* a, const hwpa_activity_tree* tree);
 
 /* Per-stage device time of the last process call in milliseconds, measured
  * with HIP events on the analyzer's stream when profiling is enabled:
- * [0] baseline [1] aggregate [2] histogram [3] dme [4] labels [5] pulses. */
+ * [0] baseline [1] aggregate [2] histogram [3] dme [4] labels [5] pulses
+ * [6] metrics. */
 int32_t hwpa_analyzer_set_profiling(hwpa_analyzer* a, int32_t enable);
 int32_t hwpa_analyzer_stage_ms(const hwpa_analyzer* a, float ms[8]);
 /* Number of EM iterations the last estimation executed (max over lanes). */
@@ -137,6 +145,22 @@ int32_t hwpa_tune_set(int32_t kernel, uint32_t lanes, uint32_t frames, int32_t v
 int32_t hwpa_tune_get(int32_t kernel, uint32_t lanes, uint32_t frames, int32_t* variant);
 int32_t hwpa_autotune(const hwpa_config* cfg, uint32_t lanes, uint32_t frames, int32_t repeats, float* ms,
                       int32_t ms_len);
+
+/* ---- HF metrics (HFMetricsFilterHost, traceAnalysis/HFMetricsFilterHost.cpp:37-116) ----
+ * Per-ZMW basecalling metrics over blocks of frames_per_hf_metric_block
+ * frames.  hwpa_metrics_process takes one chunk's pulse batch (device
+ * pointers; last_frame = the batch's LastFrame, i.e. the chunk's end minus
+ * the 24 latent frames), the chunk's baseliner statistics, Viterbi scores,
+ * pulse-detector baseline statistics and current models; when the block ends
+ * it writes d_out[lanes] and sets *produced. */
+int32_t hwpa_metrics_create(const hwpa_config* cfg, uint32_t lanes, hwpa_metrics** out);
+int32_t hwpa_metrics_destroy(hwpa_metrics* m);
+int32_t hwpa_metrics_set_activity_tree(hwpa_metrics* m, const hwpa_activity_tree* tree);
+int32_t hwpa_metrics_process(hwpa_metrics* m, const hwpa_pulse* d_pulses, const uint32_t* d_counts,
+                             uint32_t max_calls, uint32_t frames, int32_t last_frame,
+                             const hwpa_baseliner_stats* d_bl_stats, const float* d_viterbi_score,
+                             const hwpa_stat_acc* d_pd_stats, const hwpa_lane_model* d_models,
+                             hwpa_basecalling_metrics* d_out, int32_t* produced, void* stream);
 
 /* ---- stage entry points (Device* strategies) ----------------------------- */
 

@@ -89,6 +89,54 @@ typedef struct hwpa_baseliner_stats {
     hwpa_stat_acc background;
 } hwpa_baseliner_stats;
 
+/* BasecallingMetrics of one lane (dataTypes/BasecallingMetrics.h:53-80): 137
+ * bytes per ZMW, same field order and types.  activity_label holds
+ * HQRFPhysicalStates (0 EMPTY, 1 SINGLE, 2 MULTI, 3 LOW). */
+typedef struct hwpa_basecalling_metrics {
+    uint8_t activity_label[HWPA_LANE_SIZE];
+    uint16_t num_pulse_frames[HWPA_LANE_SIZE];
+    uint16_t num_base_frames[HWPA_LANE_SIZE];
+    uint16_t num_sandwiches[HWPA_LANE_SIZE];
+    uint16_t num_half_sandwiches[HWPA_LANE_SIZE];
+    uint16_t num_pulse_label_stutters[HWPA_LANE_SIZE];
+    uint16_t num_bases[HWPA_LANE_SIZE];
+    uint16_t num_pulses[HWPA_LANE_SIZE];
+    float pk_mid_signal[HWPA_NUM_ANALOGS][HWPA_LANE_SIZE];
+    float bp_zvar[HWPA_NUM_ANALOGS][HWPA_LANE_SIZE];
+    float pk_zvar[HWPA_NUM_ANALOGS][HWPA_LANE_SIZE];
+    float pk_max[HWPA_NUM_ANALOGS][HWPA_LANE_SIZE];
+    uint16_t num_pk_mid_frames[HWPA_NUM_ANALOGS][HWPA_LANE_SIZE];
+    uint16_t num_pk_mid_bases_by_analog[HWPA_NUM_ANALOGS][HWPA_LANE_SIZE];
+    uint16_t num_bases_by_analog[HWPA_NUM_ANALOGS][HWPA_LANE_SIZE];
+    uint16_t num_pulses_by_analog[HWPA_NUM_ANALOGS][HWPA_LANE_SIZE];
+    float frame_baseline_dws[HWPA_LANE_SIZE];
+    float frame_baseline_variance_dws[HWPA_LANE_SIZE];
+    uint16_t num_frames_baseline[HWPA_LANE_SIZE];
+    uint32_t start_frame[HWPA_LANE_SIZE];
+    uint16_t num_frames[HWPA_LANE_SIZE];
+    float autocorrelation[HWPA_LANE_SIZE];
+    float pulse_detection_score[HWPA_LANE_SIZE];
+    int16_t pixel_checksum[HWPA_LANE_SIZE];
+} hwpa_basecalling_metrics;
+
+/* A trained activity-labeler decision tree (the shape of
+ * ActivityLabeler::TrainedCart, dataTypes/TrainedCartParams.h:45-54).  Node
+ * i tests feature[i] of the 11 block features (< 0: leaf of class value[i])
+ * against threshold[i] and goes to children_left[i] if <=, else to
+ * children_right[i]; children must have larger indices than their parent.
+ * The trained values are the caller's data: pass them with
+ * hwpa_metrics_set_activity_tree / hwpa_analyzer_set_activity_tree. */
+typedef struct hwpa_activity_tree {
+    int32_t num_nodes;
+    float max_acceptable_hsw_rate;
+    float hsw_curve[2];
+    const int16_t* children_left;
+    const int16_t* children_right;
+    const int8_t* feature;
+    const float* threshold;
+    const int8_t* value;
+} hwpa_activity_tree;
+
 typedef struct hwpa_lane_hist {
     float low_bound[HWPA_LANE_SIZE];
     float bin_size[HWPA_LANE_SIZE];
@@ -185,6 +233,10 @@ typedef struct hwpa_config {
     uint32_t max_calls_per_zmw;
     double xsp_amp_thresh;
     float xsp_width_thresh;
+    /* BasecallerMetricsConfig */
+    uint32_t sandwich_tolerance;
+    uint32_t frames_per_hf_metric_block;
+    int32_t realtime_activity_labels;
     /* Learned initialiser (no reference counterpart): 0 = plain EM. */
     int32_t learned_init;
     float learned_conf_threshold;

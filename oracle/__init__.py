@@ -205,7 +205,7 @@ class Pulser:
 class Result(ctypes.Structure):
     _fields_ = [("pulses", _vp), ("pulse_counts", _vp), ("pulse_overflows", _vp), ("pulse_bl_stats", _vp),
                 ("baselined", _vp), ("bl_stats", _vp), ("labels", _vp), ("viterbi_score", _vp), ("models", _vp),
-                ("flags", ctypes.c_int32 * 4)]
+                ("metrics", _vp), ("flags", ctypes.c_int32 * 4)]
 
 
 class Analyzer:
@@ -245,6 +245,10 @@ class Analyzer:
         if "models" in want:
             out["models"] = np.zeros(L, dtype=model_dt)
             r.models = _p(out["models"])
+        if "metrics" in want:
+            from hw_amd import types as T
+            out["metrics"] = np.zeros(L, dtype=T.METRICS)
+            r.metrics = _p(out["metrics"])
         traces = np.ascontiguousarray(traces, dtype=np.int16)
         rc = lib.orc_analyzer_process(self.h, _p(traces), ctypes.c_uint32(self.pool_id if pool_id is None else pool_id),
                                       ctypes.c_int32(first_frame), ctypes.byref(r))
@@ -252,7 +256,48 @@ class Analyzer:
         out["flags"] = list(r.flags)
         return out
 
+    def set_activity_tree(self, tree):
+        """tree: ActivityTree (hw_amd.types) or None."""
+        lib.orc_analyzer_set_activity_tree(self.h, ctypes.byref(tree) if tree is not None else None)
+
     def __del__(self):
         if getattr(self, "h", None):
             lib.orc_analyzer_destroy(self.h)
             self.h = None
+
+
+class Metrics:
+    """HFMetricsFilterHost restated on the CPU: one call per chunk of pulses."""
+
+    def __init__(self, cfg, lanes):
+        lib.orc_metrics_create.restype = _vp
+        self.h = _vp(lib.orc_metrics_create(ctypes.byref(cfg), ctypes.c_int32(lanes)))
+        self.lanes = lanes
+
+    def set_activity_tree(self, tree):
+        lib.orc_metrics_set_tree(self.h, ctypes.byref(tree) if tree is not None else None)
+
+    def process(self, pulses, counts, max_calls, frames, last_frame, bl_stats, score, pd_stats, models, metrics_dt):
+        out = np.zeros(self.lanes, dtype=metrics_dt)
+        done = lib.orc_metrics_process(self.h, _p(np.ascontiguousarray(pulses)), _p(np.ascontiguousarray(counts)),
+                                       ctypes.c_uint32(max_calls), ctypes.c_int32(frames), ctypes.c_int32(last_frame),
+                                       _p(np.ascontiguousarray(bl_stats)),
+                                       _p(np.ascontiguousarray(score, dtype=np.float32)),
+                                       _p(np.ascontiguousarray(pd_stats)), _p(np.ascontiguousarray(models)), _p(out))
+        return done, out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_metrics_destroy(self.h)
+            self.h = None
+
+
+def autocorr(x, pieces=1):
+    """AutocorrAccumulator (lag 4) autocorrelation of a series, built in merged parts:
+    `pieces` equal parts, or a list of part sizes, merged left to right."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    lib.orc_autocorr.restype = ctypes.c_float
+    if isinstance(pieces, int):
+        return float(lib.orc_autocorr(_p(x), ctypes.c_int32(len(x)), None, ctypes.c_int32(pieces)))
+    sizes = np.ascontiguousarray(pieces, dtype=np.int32)
+    return float(lib.orc_autocorr(_p(x), ctypes.c_int32(len(x)), _p(sizes), ctypes.c_int32(len(sizes))))

@@ -149,13 +149,106 @@ struct StatZ {
     }
 };
 
-struct LaneStats {
-    // AutocorrAccumulator (lag 4) over all frames
-    StatZ corr[LS];
-    float lagM2[LS];
-    float fbuf[4][LS];
-    float bbuf[4][LS];
+// AutocorrAccumulator<LaneArray<float>> with lag 4 (common/AutocorrAccumulator.{h,cpp}):
+// per-ZMW moments and lag product, lane-wide buffer indices.
+struct AcLane {
+    StatZ s[LS];
+    float m2[LS];
+    float f[4][LS];
+    float b[4][LS];
     int fbi = 0, bbi = 0;
+
+    AcLane() { Reset(); }
+    // AutocorrAccumulator::Reset (common/AutocorrAccumulator.h:164-171): the offset stays.
+    void Reset()
+    {
+        for (int z = 0; z < LS; ++z) {
+            s[z].m0 = s[z].m1 = s[z].m2 = 0;
+            m2[z] = 0;
+            for (int k = 0; k < 4; ++k) f[k][z] = b[k][z] = 0;
+        }
+        fbi = bbi = 0;
+    }
+    // AutocorrAccumulator::AddSample (common/AutocorrAccumulator.cpp:29-38)
+    void AddSample(const float* v)
+    {
+        const bool fillFront = fbi < 4;
+        const int bslot = bbi % 4;
+        for (int z = 0; z < LS; ++z) {
+            const float x = v[z] - s[z].off;
+            if (fillFront) f[fbi][z] = x;
+            m2[z] += b[bslot][z] * x;
+            b[bslot][z] = x;
+            s[z].Add(v[z]);
+        }
+        if (fillFront) ++fbi;
+        bbi = (bbi + 1) % 4;
+    }
+    // AutocorrAccumulator::Merge (common/AutocorrAccumulator.cpp:72-102), "this" on the left.
+    void Merge(const AcLane& o)
+    {
+        const int n1 = 4 - o.fbi;
+        const int n2 = 4 - fbi;
+        for (int z = 0; z < LS; ++z) {
+            s[z].m0 += o.s[z].m0;
+            s[z].m1 += o.s[z].m1;
+            s[z].m2 += o.s[z].m2;
+            m2[z] += o.m2[z];
+            for (int k = 0; k < 4 - n1; ++k) {
+                m2[z] += b[(bbi + k) % 4][z] * o.f[k][z];
+                b[(bbi + k) % 4][z] = o.b[(o.bbi + n1 + k) % 4][z];
+            }
+            for (int k = 0; k < n2; ++k) f[fbi + k][z] = o.f[k][z];
+        }
+        fbi += n2;
+        bbi = (bbi + (4 - n1)) % 4;
+    }
+    // AutocorrAccumulator::Autocorrelation (common/AutocorrAccumulator.cpp:42-67): the
+    // clamp keeps a NaN quotient as -1 (max(NaN, -1) = -1), only nmk < 1 gives NaN.
+    float Autocorrelation(int z) const
+    {
+        const float nmk = s[z].m0 - 4.0f;
+        const float mu = s[z].Mean() - s[z].off;
+        float m1x2 = 2.0f * s[z].m1;
+        for (int k = 0; k < 4; ++k) m1x2 -= f[k][z] + b[k][z];
+        float ac = mu * (m1x2 - nmk * mu);
+        ac = (m2[z] - ac) / ((nmk - 1.0f) * s[z].Variance());
+        ac = hw_minf(hw_maxf(ac, -1.0f), 1.0f);
+        return (nmk < 1.0f || std::isnan(ac)) ? NAN : ac;
+    }
+    void ToState(hwpa_autocorr_acc* a) const
+    {
+        for (int z = 0; z < LS; ++z) {
+            a->basic.offset[z] = s[z].off;
+            a->basic.m0[z] = s[z].m0;
+            a->basic.m1[z] = s[z].m1;
+            a->basic.m2[z] = s[z].m2;
+            a->lag_m2[z] = m2[z];
+            for (int k = 0; k < 4; ++k) {
+                a->fbuf[k][z] = f[k][z];
+                a->bbuf[k][z] = b[k][z];
+            }
+            a->bidx[0][z] = (uint8_t)fbi;
+            a->bidx[1][z] = (uint8_t)bbi;
+        }
+    }
+    void FromState(const hwpa_autocorr_acc& a)
+    {
+        for (int z = 0; z < LS; ++z) {
+            s[z] = StatZ{a.basic.offset[z], a.basic.m0[z], a.basic.m1[z], a.basic.m2[z]};
+            m2[z] = a.lag_m2[z];
+            for (int k = 0; k < 4; ++k) {
+                f[k][z] = a.fbuf[k][z];
+                b[k][z] = a.bbuf[k][z];
+            }
+        }
+        fbi = a.bidx[0][0];
+        bbi = a.bidx[1][0];
+    }
+};
+
+struct LaneStats {
+    AcLane ac;  // over all frames
     int16_t tmin[LS], tmax[LS];
     StatZ bl[LS];
     float rawBlSum[LS];
@@ -164,39 +257,28 @@ struct LaneStats {
     LaneStats() { Reset(); }
     void Reset()
     {
+        ac = AcLane{};
         for (int z = 0; z < LS; ++z) {
-            corr[z] = StatZ{};
-            lagM2[z] = 0;
-            for (int k = 0; k < 4; ++k) fbuf[k][z] = bbuf[k][z] = 0;
             tmin[z] = 32767;
             tmax[z] = -32768;
             bl[z] = StatZ{};
             rawBlSum[z] = 0;
             bg[z] = StatZ{};
         }
-        fbi = bbi = 0;
     }
 
     // BaselinerStatAccumulator::AddSample (dataTypes/BaselinerStatAccumulator.cpp:38-53)
     void AddSample(const int16_t* raw, const int16_t* bls, const bool* mask)
     {
-        const bool fillFront = fbi < 4;
-        const int bslot = bbi % 4;
+        float v[LS];
+        for (int z = 0; z < LS; ++z) v[z] = (float)bls[z];
+        ac.AddSample(v);
         for (int z = 0; z < LS; ++z) {
-            const float bs = (float)bls[z];
-            // AutocorrAccumulator::AddSample (common/AutocorrAccumulator.cpp:29-38)
-            const float v = bs - corr[z].off;
-            if (fillFront) fbuf[fbi][z] = v;
-            lagM2[z] += bbuf[bslot][z] * v;
-            bbuf[bslot][z] = v;
-            corr[z].Add(bs);
             tmin[z] = std::min(bls[z], tmin[z]);
             tmax[z] = std::max(bls[z], tmax[z]);
-            bl[z].AddMasked(bs, mask[z]);
+            bl[z].AddMasked(v[z], mask[z]);
             rawBlSum[z] += (float)(mask[z] ? raw[z] : (int16_t)0);
         }
-        if (fillFront) ++fbi;
-        bbi = (bbi + 1) % 4;
     }
 
     void AddBackground(const int16_t* v)
@@ -206,18 +288,8 @@ struct LaneStats {
 
     void ToState(hwpa_baseliner_stats* s) const
     {
+        ac.ToState(&s->full);
         for (int z = 0; z < LS; ++z) {
-            s->full.basic.offset[z] = corr[z].off;
-            s->full.basic.m0[z] = corr[z].m0;
-            s->full.basic.m1[z] = corr[z].m1;
-            s->full.basic.m2[z] = corr[z].m2;
-            s->full.lag_m2[z] = lagM2[z];
-            for (int k = 0; k < 4; ++k) {
-                s->full.fbuf[k][z] = fbuf[k][z];
-                s->full.bbuf[k][z] = bbuf[k][z];
-            }
-            s->full.bidx[0][z] = (uint8_t)fbi;
-            s->full.bidx[1][z] = (uint8_t)bbi;
             s->trace_min[z] = tmin[z];
             s->trace_max[z] = tmax[z];
             s->baseline.offset[z] = bl[z].off;
@@ -234,40 +306,22 @@ struct LaneStats {
 
     void FromState(const hwpa_baseliner_stats* s)
     {
+        ac.FromState(s->full);
         for (int z = 0; z < LS; ++z) {
-            corr[z] = StatZ{s->full.basic.offset[z], s->full.basic.m0[z], s->full.basic.m1[z], s->full.basic.m2[z]};
-            lagM2[z] = s->full.lag_m2[z];
-            for (int k = 0; k < 4; ++k) {
-                fbuf[k][z] = s->full.fbuf[k][z];
-                bbuf[k][z] = s->full.bbuf[k][z];
-            }
             tmin[z] = s->trace_min[z];
             tmax[z] = s->trace_max[z];
             bl[z] = StatZ{s->baseline.offset[z], s->baseline.m0[z], s->baseline.m1[z], s->baseline.m2[z]};
             rawBlSum[z] = s->raw_baseline_sum[z];
             bg[z] = StatZ{s->background.offset[z], s->background.m0[z], s->background.m1[z], s->background.m2[z]};
         }
-        fbi = s->full.bidx[0][0];
-        bbi = s->full.bidx[1][0];
     }
 
     // BaselinerStatAccumulator::Merge (dataTypes/BaselinerStatAccumulator.cpp:64-75) with
-    // AutocorrAccumulator::Merge (common/AutocorrAccumulator.cpp:72-104) and
-    // StatAccumulator::Merge (common/StatAccumulator.h:183-191).
+    // AutocorrAccumulator::Merge and StatAccumulator::Merge (common/StatAccumulator.h:201-210).
     void Merge(const LaneStats& o)
     {
-        const int n1 = 4 - o.fbi;
-        const int n2 = 4 - fbi;
+        ac.Merge(o.ac);
         for (int z = 0; z < LS; ++z) {
-            corr[z].m0 += o.corr[z].m0;
-            corr[z].m1 += o.corr[z].m1;
-            corr[z].m2 += o.corr[z].m2;
-            lagM2[z] += o.lagM2[z];
-            for (int k = 0; k < 4 - n1; ++k) {
-                lagM2[z] += bbuf[(bbi + k) % 4][z] * o.fbuf[k][z];
-                bbuf[(bbi + k) % 4][z] = o.bbuf[(o.bbi + n1 + k) % 4][z];
-            }
-            for (int k = 0; k < n2; ++k) fbuf[fbi + k][z] = o.fbuf[k][z];
             tmin[z] = std::min(tmin[z], o.tmin[z]);
             tmax[z] = std::max(tmax[z], o.tmax[z]);
             bl[z].m0 += o.bl[z].m0;
@@ -278,8 +332,6 @@ struct LaneStats {
             bg[z].m1 += o.bg[z].m1;
             bg[z].m2 += o.bg[z].m2;
         }
-        fbi += n2;
-        bbi = (bbi + (4 - n1)) % 4;
     }
 };
 
@@ -1698,6 +1750,367 @@ struct Interval {
     }
 };
 
+// ---------------------------------------------------------------------------
+// HF metrics (traceAnalysis/HFMetricsFilterHost.cpp:37-116; the accumulator
+// dataTypes/BasecallingMetricsAccumulator.cpp:62-504 and its trace metrics,
+// dataTypes/TraceAnalysisMetrics.h:47-190)
+// ---------------------------------------------------------------------------
+
+constexpr uint8_t kLabelNone = 5;  // Pulse::NucleotideLabel::NONE (dataTypes/Pulse.h:52-57)
+
+struct HfmTree {
+    bool present = false;
+    float maxHsw = 0, hsw[2] = {0, 0};
+    std::vector<int16_t> left, right;
+    std::vector<int8_t> feature, value;
+    std::vector<float> threshold;
+};
+
+// StatAccumulator::Offset(value) (common/StatAccumulator.h:234-246), lane-wide early return.
+static void StatSetOffset(StatZ* a, const float* value)
+{
+    bool same = true;
+    for (int z = 0; z < LS; ++z) same = same && (value[z] == a[z].off);
+    if (same) return;
+    for (int z = 0; z < LS; ++z) {
+        const float m1new = a[z].m1 + a[z].m0 * (a[z].off - value[z]);
+        const float m2new = a[z].m2 + (m1new * m1new - a[z].m1 * a[z].m1) / a[z].m0;
+        a[z].off = value[z];
+        a[z].m1 = m1new;
+        a[z].m2 = (a[z].m0 == 0.0f) ? 0.0f : m2new;
+    }
+}
+
+// StatAccumulator::operator+= (common/StatAccumulator.h:145-170).
+static void StatAddAssign(StatZ* a, StatZ* o)
+{
+    bool allZero = true, anyNan = false;
+    for (int z = 0; z < LS; ++z) {
+        allZero = allZero && (o[z].m0 == 0.0f);
+        anyNan = anyNan || std::isnan(a[z].off);
+    }
+    if (allZero) return;
+    if (anyNan) {
+        for (int z = 0; z < LS; ++z)
+            if (std::isnan(a[z].off)) a[z] = StatZ{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    float offNew[LS];
+    for (int z = 0; z < LS; ++z) {
+        const float w = o[z].m0 / (a[z].m0 + o[z].m0);
+        const float on = (1.0f - w) * a[z].off + w * o[z].off;
+        offNew[z] = std::isnan(on) ? a[z].off : on;
+    }
+    StatSetOffset(o, offNew);
+    StatSetOffset(a, offNew);
+    for (int z = 0; z < LS; ++z) {
+        a[z].m0 += o[z].m0;
+        a[z].m1 += o[z].m1;
+        a[z].m2 += o[z].m2;
+    }
+}
+
+// LaneArray sqrt (common/simd/m512f_SSE.h:406-415): 0 maps to 0.
+static float LaneSqrt(float x) { return (x == 0.0f) ? 0.0f : hw_sqrtf(x); }
+
+struct HfmLane {
+    uint16_t numPulseFrames[LS], numBaseFrames[LS], numSandwiches[LS], numHalfSandwiches[LS], numStutters[LS];
+    uint8_t activity[LS];
+    float pkMidSignal[NA][LS], bpZvar[NA][LS], pkZvar[NA][LS], pkMax[NA][LS], modelVar[NA][LS], modelMean[NA][LS];
+    uint16_t numPkMidFrames[NA][LS], numPkMidBases[NA][LS], numBasesBy[NA][LS], numPulsesBy[NA][LS];
+    // TraceAnalysisMetrics
+    uint32_t startFrame[LS], numFrames[LS];
+    float pdScore[LS];
+    StatZ blStat[LS];   // baselineStatAccum_: Reset keeps the offset
+    AcLane ac;          // autocorrAccum_
+    // basecall caches: [0] previous pulse, [1] the one before
+    uint32_t cStart[2][LS];
+    uint16_t cWidth[2][LS];
+    uint8_t cLabel[2][LS];
+
+    HfmLane()
+    {
+        for (int z = 0; z < LS; ++z) {
+            startFrame[z] = numFrames[z] = 0;
+            pdScore[z] = 0.0f;
+            blStat[z] = StatZ{};
+            for (int a = 0; a < NA; ++a) modelVar[a][z] = modelMean[a][z] = 0.0f;
+        }
+        Reset();
+    }
+
+    // BasecallingMetricsAccumulator::Reset (BasecallingMetricsAccumulator.cpp:254-277) and
+    // TraceAnalysisMetrics::Reset (TraceAnalysisMetrics.h:71-84).
+    void Reset()
+    {
+        for (int z = 0; z < LS; ++z) {
+            numPulseFrames[z] = numBaseFrames[z] = numSandwiches[z] = numHalfSandwiches[z] = numStutters[z] = 0;
+            activity[z] = 0;
+            for (int c = 0; c < 2; ++c) {
+                cStart[c][z] = 0;
+                cWidth[c][z] = 0;
+                cLabel[c][z] = kLabelNone;
+            }
+            for (int a = 0; a < NA; ++a) {
+                pkMidSignal[a][z] = bpZvar[a][z] = pkZvar[a][z] = pkMax[a][z] = 0.0f;
+                numPkMidFrames[a][z] = numPkMidBases[a][z] = numBasesBy[a][z] = numPulsesBy[a][z] = 0;
+            }
+            blStat[z].m0 = blStat[z].m1 = blStat[z].m2 = 0.0f;
+            startFrame[z] = startFrame[z] + numFrames[z];
+            numFrames[z] = 0;
+            pdScore[z] = 0.0f;
+        }
+        ac.Reset();
+    }
+
+    // BasecallingMetricsAccumulator::Count (BasecallingMetricsAccumulator.cpp:333-410).  Labels
+    // beyond T (never emitted by the pulse accumulator) are not counted per analog.
+    void Count(const hwpa_pulse* pulses, const uint32_t* counts, uint32_t maxCalls, uint32_t frames)
+    {
+        for (int z = 0; z < LS; ++z) {
+            numFrames[z] += frames;
+            uint32_t pS = cStart[0][z], ppS = cStart[1][z];
+            uint16_t pW = cWidth[0][z], ppW = cWidth[1][z];
+            uint8_t pL = cLabel[0][z], ppL = cLabel[1][z];
+            const uint32_t n = std::min(counts[z], maxCalls);
+            for (uint32_t bi = 0; bi < n; ++bi) {
+                const hwpa_pulse& p = pulses[(size_t)z * maxCalls + bi];
+                const uint8_t label = p.label;
+                numPulseFrames[z] = (uint16_t)(numPulseFrames[z] + p.width);
+                if (label < NA) {
+                    numPulsesBy[label][z] = (uint16_t)(numPulsesBy[label][z] + 1);
+                    const float mx = (float)p.max_signal * 0.1f;
+                    pkMax[label][z] = (pkMax[label][z] < mx) ? mx : pkMax[label][z];
+                }
+                if (pL != kLabelNone) {
+                    if (pL == label) numStutters[z] = (uint16_t)(numStutters[z] + 1);
+                    const bool abutted = p.start == pS + pW;
+                    if (abutted && pL != label) numHalfSandwiches[z] = (uint16_t)(numHalfSandwiches[z] + 1);
+                    if (ppL != kLabelNone) {
+                        const bool prevAbutted = pS == ppS + ppW;
+                        if (prevAbutted && abutted && ppL == label && ppL != pL)
+                            numSandwiches[z] = (uint16_t)(numSandwiches[z] + 1);
+                    }
+                }
+                if (!p.is_reject) {
+                    numBaseFrames[z] = (uint16_t)(numBaseFrames[z] + p.width);
+                    if (label < NA) {
+                        numBasesBy[label][z] = (uint16_t)(numBasesBy[label][z] + 1);
+                        if (p.width >= 3) {   // MidSignal() is NaN below 3 frames
+                            numPkMidBases[label][z] = (uint16_t)(numPkMidBases[label][z] + 1);
+                            const uint16_t midWidth = (uint16_t)(p.width - 2);
+                            numPkMidFrames[label][z] = (uint16_t)(numPkMidFrames[label][z] + midWidth);
+                            const float mid = (float)p.mid_signal * 0.1f;
+                            pkMidSignal[label][z] += mid * (float)midWidth;
+                            bpZvar[label][z] += mid * mid * (float)midWidth;
+                            pkZvar[label][z] += p.signal_m2;
+                        }
+                    }
+                }
+                ppS = pS;
+                ppW = pW;
+                ppL = pL;
+                pS = p.start;
+                pW = p.width;
+                pL = label;
+            }
+            cStart[0][z] = pS;
+            cWidth[0][z] = pW;
+            cLabel[0][z] = pL;
+            cStart[1][z] = ppS;
+            cWidth[1][z] = ppW;
+            cLabel[1][z] = ppL;
+        }
+    }
+
+    // BasecallingMetricsAccumulator::AddModels (BasecallingMetricsAccumulator.cpp:302-313)
+    void AddModels(const hwpa_lane_model& m)
+    {
+        for (int a = 0; a < NA; ++a)
+            for (int z = 0; z < LS; ++z) {
+                modelVar[a][z] = hw_h2f(m.analogs[a].vars[z]);
+                modelMean[a][z] = hw_h2f(m.analogs[a].means[z]);
+            }
+    }
+
+    // BasecallingMetricsAccumulator::AddBatchMetrics (BasecallingMetricsAccumulator.cpp:315-331)
+    void AddBatchMetrics(const hwpa_baseliner_stats& bs, const float* score, const hwpa_stat_acc& pd)
+    {
+        AcLane o;
+        o.FromState(bs.full);
+        ac.Merge(o);
+        StatZ p[LS];
+        for (int z = 0; z < LS; ++z) {
+            pdScore[z] += score[z];
+            const float bgMean = bs.background.m1[z] / bs.background.m0[z] + bs.background.offset[z];
+            p[z] = StatZ{pd.offset[z] + bgMean, pd.m0[z], pd.m1[z], pd.m2[z]};
+        }
+        StatAddAssign(blStat, p);
+    }
+
+    uint16_t NumPulses(int z) const
+    {
+        uint16_t r = 0;
+        for (int a = 0; a < NA; ++a) r = (uint16_t)(r + numPulsesBy[a][z]);
+        return r;
+    }
+    uint16_t NumBases(int z) const
+    {
+        uint16_t r = 0;
+        for (int a = 0; a < NA; ++a) r = (uint16_t)(r + numBasesBy[a][z]);
+        return r;
+    }
+
+    // BasecallingMetricsAccumulator::FinalizeMetrics (BasecallingMetricsAccumulator.cpp:412-500)
+    void Finalize(bool realtime, float frameRate, const HfmTree& tree)
+    {
+        for (int a = 0; a < NA; ++a) {
+            bool allZero = true, allFewBases = true, allFewFrames = true;
+            for (int z = 0; z < LS; ++z) {
+                allZero = allZero && ((int16_t)numPkMidBases[a][z] == 0);
+                allFewBases = allFewBases && ((float)numPkMidBases[a][z] < 2.0f);
+                allFewFrames = allFewFrames && ((float)numPkMidFrames[a][z] < 2.0f);
+            }
+            if (allZero) {
+                for (int z = 0; z < LS; ++z) pkMidSignal[a][z] = bpZvar[a][z] = pkZvar[a][z] = NAN;
+            } else if (allFewBases || allFewFrames) {
+                for (int z = 0; z < LS; ++z) bpZvar[a][z] = pkZvar[a][z] = NAN;
+            } else {
+                for (int z = 0; z < LS; ++z) {
+                    float bpZ = bpZvar[a][z], pkZ = pkZvar[a][z], pkM = pkMidSignal[a][z];
+                    const float nf = (float)numPkMidFrames[a][z];
+                    const float nb = (float)numPkMidBases[a][z];
+                    bpZ = (bpZ - (pkM * pkM / nf)) / nf;
+                    bpZ = bpZ * nb / (nb - 1.0f);
+                    const float blVar = blStat[z].Variance();
+                    bpZ -= blVar / (nf / nb);
+                    pkZ = (pkZ - (pkM * pkM / nf)) / (nf - 1.0f);
+                    pkZ -= bpZ + blVar;
+                    const float miv = modelVar[a][z];
+                    bool mask = miv < blVar;
+                    pkZ /= miv - blVar;
+                    mask = mask || (pkZ < 0.0f);
+                    pkZ = mask ? 0.0f : pkZ;
+                    const float pkMid = pkM / nf;
+                    mask = pkMid < 0.0f;
+                    bpZ /= (pkMid * pkMid);
+                    mask = mask || (bpZ < 0.0f);
+                    bpZ = mask ? NAN : bpZ;
+                    const int16_t nbi = (int16_t)numPkMidBases[a][z];
+                    mask = nbi == 0;
+                    pkM = mask ? NAN : pkM;
+                    mask = mask || (nbi < 2);
+                    mask = mask || ((int16_t)numPkMidFrames[a][z] < 2);
+                    bpZ = mask ? NAN : bpZ;
+                    pkZ = mask ? NAN : pkZ;
+                    pkMidSignal[a][z] = pkM;
+                    bpZvar[a][z] = bpZ;
+                    pkZvar[a][z] = pkZ;
+                }
+            }
+        }
+        for (int z = 0; z < LS; ++z) {
+            const float pd = pdScore[z] / (float)numFrames[z];
+            pdScore[z] = std::isnan(pd) ? 0.0f : pd;
+        }
+        if (realtime && tree.present) LabelBlock(frameRate, tree);
+    }
+
+    // BasecallingMetricsAccumulator::LabelBlock (BasecallingMetricsAccumulator.cpp:62-200)
+    void LabelBlock(float frameRate, const HfmTree& t)
+    {
+        auto z0 = [](float v) { return std::isnan(v) ? 0.0f : v; };
+        for (int z = 0; z < LS; ++z) {
+            float feat[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            const float stdDev = LaneSqrt(blStat[z].Variance());
+            const float nBases = (float)NumBases(z);
+            const float nPulses = (float)NumPulses(z);
+            const float pulseWidth = z0((float)numPulseFrames[z] / nPulses);
+            float pkmid[NA];
+            for (int a = 0; a < NA; ++a) pkmid[a] = pkMidSignal[a][z] / (float)numPkMidFrames[a][z];
+            const float seconds = (float)numFrames[z] / frameRate;
+            feat[0] = nPulses / seconds;
+            feat[1] = z0((float)numSandwiches[z] / nPulses);
+            const float hswr = z0((float)numHalfSandwiches[z] / nPulses);
+            float hswrExp = t.hsw[0] * feat[0] + t.hsw[1];
+            hswrExp = (hswrExp > t.maxHsw) ? t.maxHsw : hswrExp;
+            feat[2] = hswr - hswrExp;
+            feat[3] = pdScore[z];
+            feat[4] = pulseWidth;
+            feat[5] = z0((float)numStutters[z] / nPulses);
+            float rel[NA];
+            for (int a = 0; a < NA; ++a) rel[a] = modelMean[a][z];
+            float maxamp = rel[0];
+            for (int a = 1; a < NA; ++a) maxamp = hw_maxf(rel[a], maxamp);
+            for (int a = 0; a < NA; ++a) rel[a] /= maxamp;
+            int lowIdx = 0;
+            float minamp = rel[0];
+            for (int a = 1; a < NA; ++a) {
+                lowIdx = (rel[a] < minamp) ? a : lowIdx;
+                minamp = hw_minf(rel[a], minamp);
+            }
+            for (int a = 0; a < NA; ++a)
+                feat[6] += pkmid[a] / stdDev * (float)numBasesBy[a][z] / nBases * minamp / rel[a];
+            feat[6] = z0(feat[6]);
+            for (int a = 0; a < NA; ++a) feat[7] = hw_maxf(feat[7], (pkMax[a][z] - pkmid[a]) / stdDev);
+            feat[7] = z0(feat[7]);
+            feat[8] = ac.Autocorrelation(z);
+            for (int a = 0; a < NA; ++a) {
+                const float b = bpZvar[a][z], k = pkZvar[a][z];
+                feat[9] += (std::isnan(b) || lowIdx == a) ? 0.0f : b;
+                feat[10] += (std::isnan(k) || lowIdx == a) ? 0.0f : k;
+            }
+            feat[9] /= 3.0f;
+            feat[10] /= 3.0f;
+            int cur = 0;
+            for (size_t guard = 0; guard < t.feature.size() && t.feature[cur] >= 0; ++guard)
+                cur = (feat[t.feature[cur]] <= t.threshold[cur]) ? t.left[cur] : t.right[cur];
+            activity[z] = (uint8_t)t.value[cur];
+        }
+    }
+
+    // BasecallingMetricsAccumulator::PopulateBasecallingMetrics (BasecallingMetricsAccumulator.cpp:202-252)
+    void Populate(hwpa_basecalling_metrics& m) const
+    {
+        for (int z = 0; z < LS; ++z) {
+            const float acr = ac.Autocorrelation(z);
+            m.activity_label[z] = activity[z];
+            m.num_pulse_frames[z] = numPulseFrames[z];
+            m.num_base_frames[z] = numBaseFrames[z];
+            m.num_sandwiches[z] = numSandwiches[z];
+            m.num_half_sandwiches[z] = numHalfSandwiches[z];
+            m.num_pulse_label_stutters[z] = numStutters[z];
+            m.num_bases[z] = NumBases(z);
+            m.num_pulses[z] = NumPulses(z);
+            for (int a = 0; a < NA; ++a) {
+                m.pk_mid_signal[a][z] = pkMidSignal[a][z];
+                m.bp_zvar[a][z] = bpZvar[a][z];
+                m.pk_zvar[a][z] = pkZvar[a][z];
+                m.pk_max[a][z] = pkMax[a][z];
+                m.num_pk_mid_frames[a][z] = numPkMidFrames[a][z];
+                m.num_pk_mid_bases_by_analog[a][z] = numPkMidBases[a][z];
+                m.num_bases_by_analog[a][z] = numBasesBy[a][z];
+                m.num_pulses_by_analog[a][z] = numPulsesBy[a][z];
+            }
+            m.frame_baseline_dws[z] = blStat[z].Mean();
+            m.frame_baseline_variance_dws[z] = blStat[z].Variance();
+            m.num_frames_baseline[z] = (uint16_t)hw_round_i32(blStat[z].m0);
+            m.start_frame[z] = startFrame[z];
+            m.num_frames[z] = (uint16_t)numFrames[z];
+            m.autocorrelation[z] = std::isnan(acr) ? 0.0f : acr;
+            m.pulse_detection_score[z] = pdScore[z];
+            m.pixel_checksum[z] = 0;
+        }
+    }
+};
+
+struct OrcMetrics {
+    hwpa_config cfg;
+    int lanes = 0;
+    uint32_t framesSeen = 0;
+    std::vector<HfmLane> lane;
+    HfmTree tree;
+};
+
 enum { ST_HIST_INIT = 0, ST_DME_INIT = 1, ST_SEQUENCING = 2 };
 
 }  // namespace
@@ -1716,6 +2129,7 @@ struct OrcAnalyzer {
     void* hist;
     void* lab;
     void* pulse;
+    void* metrics;
     Interval aggFi, histFi, modelFi;
     std::vector<hwpa_lane_model> models;
     std::vector<int16_t> baselined, labels, latent;
@@ -1817,6 +2231,84 @@ void orc_tracegen(const hwpa_tracegen_params* p, int32_t lanes, int32_t lane_off
     });
 }
 
+void* orc_metrics_create(const hwpa_config* cfg, int32_t lanes)
+{
+    auto* m = new OrcMetrics;
+    m->cfg = *cfg;
+    m->lanes = lanes;
+    m->lane.resize(lanes);
+    return m;
+}
+
+void orc_metrics_destroy(void* h) { delete static_cast<OrcMetrics*>(h); }
+
+void orc_metrics_set_tree(void* h, const hwpa_activity_tree* t)
+{
+    auto* m = static_cast<OrcMetrics*>(h);
+    HfmTree& r = m->tree;
+    r = HfmTree{};
+    if (!t || t->num_nodes <= 0) return;
+    const int n = t->num_nodes;
+    r.present = true;
+    r.maxHsw = t->max_acceptable_hsw_rate;
+    r.hsw[0] = t->hsw_curve[0];
+    r.hsw[1] = t->hsw_curve[1];
+    r.left.assign(t->children_left, t->children_left + n);
+    r.right.assign(t->children_right, t->children_right + n);
+    r.feature.assign(t->feature, t->feature + n);
+    r.threshold.assign(t->threshold, t->threshold + n);
+    r.value.assign(t->value, t->value + n);
+}
+
+// HFMetricsFilterHost::Process (HFMetricsFilterHost.cpp:86-116) for one chunk of pulses whose
+// batch ends at last_frame; returns 1 and fills out[lanes] when a metrics block ends.
+int32_t orc_metrics_process(void* h, const hwpa_pulse* pulses, const uint32_t* counts, uint32_t max_calls,
+                            int32_t frames, int32_t last_frame, const hwpa_baseliner_stats* bl_stats,
+                            const float* viterbi_score, const hwpa_stat_acc* pd_stats,
+                            const hwpa_lane_model* models, hwpa_basecalling_metrics* out)
+{
+    auto* m = static_cast<OrcMetrics*>(h);
+    if (m->framesSeen == 0)
+        for (auto& l : m->lane) l.Reset();
+    for (int l = 0; l < m->lanes; ++l) {
+        HfmLane& a = m->lane[l];
+        a.Count(pulses + (size_t)l * LS * max_calls, counts + (size_t)l * LS, max_calls, (uint32_t)frames);
+        a.AddModels(models[l]);
+        a.AddBatchMetrics(bl_stats[l], viterbi_score + (size_t)l * LS, pd_stats[l]);
+    }
+    m->framesSeen += (uint32_t)frames;
+    if ((uint32_t)last_frame % m->cfg.frames_per_hf_metric_block >= (uint32_t)frames) return 0;
+    for (int l = 0; l < m->lanes; ++l) {
+        m->lane[l].Finalize(m->cfg.realtime_activity_labels != 0, m->cfg.movie.frame_rate, m->tree);
+        if (out) m->lane[l].Populate(out[l]);
+    }
+    m->framesSeen = 0;
+    return 1;
+}
+
+// Autocorrelation of the series x[0..n) added one value at a time (every ZMW the same) in
+// `pieces` consecutive accumulators of sizes[p] values (equal sizes when sizes is null)
+// merged left to right.
+float orc_autocorr(const float* x, int32_t n, const int32_t* sizes, int32_t pieces)
+{
+    AcLane total;
+    const int per = (n + pieces - 1) / pieces;
+    int begin = 0;
+    for (int p = 0; p < pieces; ++p) {
+        AcLane part;
+        const int end = sizes ? std::min(n, begin + sizes[p]) : std::min(n, (p + 1) * per);
+        for (int i = begin; i < end; ++i) {
+            float v[LS];
+            for (int z = 0; z < LS; ++z) v[z] = x[i];
+            part.AddSample(v);
+        }
+        begin = end;
+        if (p == 0) total = part;
+        else total.Merge(part);
+    }
+    return total.Autocorrelation(0);
+}
+
 void* orc_analyzer_create(const hwpa_config* cfg, uint32_t pool_id, int32_t lanes, int32_t frames)
 {
     auto* a = new OrcAnalyzer;
@@ -1840,6 +2332,7 @@ void* orc_analyzer_create(const hwpa_config* cfg, uint32_t pool_id, int32_t lane
     a->hist = orc_hist_create(cfg, lanes);
     a->lab = orc_labeler_create(cfg, lanes);
     a->pulse = orc_pulse_create(cfg, lanes);
+    a->metrics = orc_metrics_create(cfg, lanes);
     a->models.resize(lanes);
     std::memset(a->models.data(), 0, sizeof(hwpa_lane_model) * lanes);
     a->baselined.resize((size_t)lanes * frames * LS);
@@ -1860,6 +2353,7 @@ void orc_analyzer_destroy(void* h)
     orc_hist_destroy(a->hist);
     orc_labeler_destroy(a->lab);
     orc_pulse_destroy(a->pulse);
+    orc_metrics_destroy(a->metrics);
     delete a;
 }
 
@@ -1935,8 +2429,20 @@ int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id, i
         if (res->pulse_counts) std::memcpy(res->pulse_counts, counts.data(), counts.size() * 4);
         if (res->pulse_overflows) std::memcpy(res->pulse_overflows, ovf.data(), ovf.size() * 4);
         if (res->pulse_bl_stats) std::memcpy(res->pulse_bl_stats, pbl.data(), sizeof(hwpa_stat_acc) * L);
+        // HF metrics of the pulse batch, which ends 24 latent frames before the chunk
+        // (DynamicEstimateBatchAnalyzer::AnalyzeImpl, BatchAnalyzer.cpp:366-372).
+        std::vector<hwpa_basecalling_metrics> met(L);
+        res->flags[3] = orc_metrics_process(a->metrics, pulses.data(), counts.data(), mc, F,
+                                            first_frame + F - HWPA_LATENT_FRAMES, a->stats.data(), a->score.data(),
+                                            pbl.data(), a->models.data(), met.data());
+        if (res->flags[3] && res->metrics) std::memcpy(res->metrics, met.data(), sizeof(hwpa_basecalling_metrics) * L);
     }
     return HWPA_OK;
+}
+
+void orc_analyzer_set_activity_tree(void* h, const hwpa_activity_tree* t)
+{
+    orc_metrics_set_tree(static_cast<OrcAnalyzer*>(h)->metrics, t);
 }
 
 }  // extern "C"

@@ -41,7 +41,9 @@ typedef struct orc_result {
     int16_t* labels;                 /* [L][F][64] */
     float* viterbi_score;            /* [L*64] */
     hwpa_lane_model* models;         /* [L] */
-    int32_t flags[4];                /* [0] pulses emitted, [1] estimation ran, [2] labels ran */
+    hwpa_basecalling_metrics* metrics; /* [L], written when flags[3] */
+    int32_t flags[4];                /* [0] pulses emitted, [1] estimation ran, [2] labels ran,
+                                        [3] a metrics block ended */
 } orc_result;
 
 int orc_threads(int n);              /* set worker thread count, returns previous */
@@ -86,10 +88,20 @@ void orc_pulse_process(void* h, const int16_t* labels, const int16_t* latent_tra
                        const hwpa_lane_model* models, hwpa_pulse* pulses, uint32_t* counts,
                        uint32_t* overflows, hwpa_stat_acc* bl_stats);
 
+void* orc_metrics_create(const hwpa_config* cfg, int32_t lanes);
+void orc_metrics_destroy(void* h);
+void orc_metrics_set_tree(void* h, const hwpa_activity_tree* t);
+int32_t orc_metrics_process(void* h, const hwpa_pulse* pulses, const uint32_t* counts, uint32_t max_calls,
+                            int32_t frames, int32_t last_frame, const hwpa_baseliner_stats* bl_stats,
+                            const float* viterbi_score, const hwpa_stat_acc* pd_stats,
+                            const hwpa_lane_model* models, hwpa_basecalling_metrics* out);
+float orc_autocorr(const float* x, int32_t n, const int32_t* sizes, int32_t pieces);
+
 void* orc_analyzer_create(const hwpa_config* cfg, uint32_t pool_id, int32_t lanes, int32_t frames);
 void orc_analyzer_destroy(void* h);
 int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id,
                              int32_t first_frame, orc_result* res);
+void orc_analyzer_set_activity_tree(void* h, const hwpa_activity_tree* t);
 
 void orc_tracegen(const hwpa_tracegen_params* p, int32_t lanes, int32_t lane_offset,
                   int32_t frames, uint32_t first_frame, hwpa_tracegen_state* state,

@@ -113,5 +113,5 @@ def test_learned_start_gains_over_ablations(H, T, trained):
           "label agreement with plain %s" % (acc_capped, acc_untrained, acc_learned, ["%.5f" % x for x in agree]))
     assert acc_learned > acc_capped + 0.1
     assert acc_learned > acc_untrained + 0.1
-    # measured 0.993-0.995 with the default gate (cap 3, tolerance 1e-3)
+    # measured 0.993-0.996 with the default gate (cap 3, convergence tolerance 2e-3)
     assert min(agree) > 0.99
