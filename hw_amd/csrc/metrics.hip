@@ -124,6 +124,8 @@ __global__ __launch_bounds__(64) void HfmChunkKernel(HfmLaneDev* state, const hw
     }
 
     // Count (BasecallingMetricsAccumulator.cpp:333-410); labels beyond T are not counted per analog.
+    // The per-analog accumulators live in registers for the chunk (selected by compare, so the
+    // label never indexes memory) and each keeps the reference's order of additions.
     {
         S.numFrames[z] += frames;
         uint32_t pS = S.cStart[0][z], ppS = S.cStart[1][z];
@@ -131,50 +133,89 @@ __global__ __launch_bounds__(64) void HfmChunkKernel(HfmLaneDev* state, const hw
         uint32_t pL = S.cLabel[0][z], ppL = S.cLabel[1][z];
         uint32_t nPF = S.numPulseFrames[z], nBF = S.numBaseFrames[z], nSw = S.numSandwiches[z];
         uint32_t nHs = S.numHalfSandwiches[z], nSt = S.numStutters[z];
+        uint32_t cP[NA], cB[NA], cMB[NA], cMF[NA];
+        float aMid[NA], aBp[NA], aPk[NA], aMax[NA];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            cP[a] = S.numPulsesBy[a][z];
+            cB[a] = S.numBasesBy[a][z];
+            cMB[a] = S.numPkMidBases[a][z];
+            cMF[a] = S.numPkMidFrames[a][z];
+            aMid[a] = S.pkMidSignal[a][z];
+            aBp[a] = S.bpZvar[a][z];
+            aPk[a] = S.pkZvar[a][z];
+            aMax[a] = S.pkMax[a][z];
+        }
         const size_t zg = (size_t)lane * LS + z;
         const uint32_t n = counts[zg] < maxCalls ? counts[zg] : maxCalls;
-        const hwpa_pulse* zp = pulses + zg * maxCalls;
+        // The 20-byte record as five dwords (records are 4-byte aligned).
+        const uint32_t* zp = reinterpret_cast<const uint32_t*>(pulses + zg * maxCalls);
+        uint32_t r[5] = {0u, 0u, 0u, 0u, 0u};
+        if (n > 0)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) r[k] = zp[k];
         for (uint32_t bi = 0; bi < n; ++bi) {
-            const hwpa_pulse p = zp[bi];
-            const uint32_t label = p.label;
-            const uint32_t w = p.width;
+            uint32_t nx[5] = {0u, 0u, 0u, 0u, 0u};
+            if (bi + 1 < n)
+#pragma unroll
+                for (int k = 0; k < 5; ++k) nx[k] = zp[5 * (bi + 1) + k];
+            const uint32_t start = r[0];
+            const uint32_t w = r[1] & 0xffffu;
+            const uint32_t midS = r[2] & 0xffffu;
+            const uint32_t maxS = r[2] >> 16;
+            const float m2 = __uint_as_float(r[3]);
+            const uint32_t label = r[4] & 0xffu;
+            const bool reject = ((r[4] >> 8) & 0xffu) != 0u;
             nPF = (nPF + w) & 0xffffu;
-            if (label < (uint32_t)NA) {
-                S.numPulsesBy[label][z] = (uint16_t)(S.numPulsesBy[label][z] + 1);
-                const float mx = (float)p.max_signal * 0.1f;
-                const float cur = S.pkMax[label][z];
-                S.pkMax[label][z] = (cur < mx) ? mx : cur;
-            }
+            const float mx = (float)maxS * 0.1f;
             if (pL != kLabelNone) {
                 if (pL == label) nSt = (nSt + 1) & 0xffffu;
-                const bool abutted = p.start == pS + pW;
+                const bool abutted = start == pS + pW;
                 if (abutted && pL != label) nHs = (nHs + 1) & 0xffffu;
                 if (ppL != kLabelNone) {
                     const bool prevAbutted = pS == ppS + ppW;
                     if (prevAbutted && abutted && ppL == label && ppL != pL) nSw = (nSw + 1) & 0xffffu;
                 }
             }
-            if (!p.is_reject) {
-                nBF = (nBF + w) & 0xffffu;
-                if (label < (uint32_t)NA) {
-                    S.numBasesBy[label][z] = (uint16_t)(S.numBasesBy[label][z] + 1);
-                    if (w >= 3u) {  // MidSignal() is NaN below 3 frames
-                        S.numPkMidBases[label][z] = (uint16_t)(S.numPkMidBases[label][z] + 1);
-                        const uint16_t midWidth = (uint16_t)(w - 2u);
-                        S.numPkMidFrames[label][z] = (uint16_t)(S.numPkMidFrames[label][z] + midWidth);
-                        const float mid = (float)p.mid_signal * 0.1f;
-                        S.pkMidSignal[label][z] = S.pkMidSignal[label][z] + mid * (float)midWidth;
-                        S.bpZvar[label][z] = S.bpZvar[label][z] + mid * mid * (float)midWidth;
-                        S.pkZvar[label][z] = S.pkZvar[label][z] + p.signal_m2;
+            if (!reject) nBF = (nBF + w) & 0xffffu;
+            const bool mid3 = !reject && w >= 3u;  // MidSignal() is NaN below 3 frames
+            const uint32_t midWidth = (w - 2u) & 0xffffu;
+            const float mid = (float)midS * 0.1f;
+            const float fMidW = (float)midWidth;
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                if (label == (uint32_t)a) {
+                    cP[a] = (cP[a] + 1u) & 0xffffu;
+                    aMax[a] = (aMax[a] < mx) ? mx : aMax[a];
+                    if (!reject) cB[a] = (cB[a] + 1u) & 0xffffu;
+                    if (mid3) {
+                        cMB[a] = (cMB[a] + 1u) & 0xffffu;
+                        cMF[a] = (cMF[a] + midWidth) & 0xffffu;
+                        aMid[a] = aMid[a] + mid * fMidW;
+                        aBp[a] = aBp[a] + mid * mid * fMidW;
+                        aPk[a] = aPk[a] + m2;
                     }
                 }
             }
             ppS = pS;
             ppW = pW;
             ppL = pL;
-            pS = p.start;
+            pS = start;
             pW = w;
             pL = label;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) r[k] = nx[k];
+        }
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            S.numPulsesBy[a][z] = (uint16_t)cP[a];
+            S.numBasesBy[a][z] = (uint16_t)cB[a];
+            S.numPkMidBases[a][z] = (uint16_t)cMB[a];
+            S.numPkMidFrames[a][z] = (uint16_t)cMF[a];
+            S.pkMidSignal[a][z] = aMid[a];
+            S.bpZvar[a][z] = aBp[a];
+            S.pkZvar[a][z] = aPk[a];
+            S.pkMax[a][z] = aMax[a];
         }
         S.numPulseFrames[z] = (uint16_t)nPF;
         S.numBaseFrames[z] = (uint16_t)nBF;
