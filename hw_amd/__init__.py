@@ -260,9 +260,9 @@ class Baseliner:
         self.lanes = lanes
         check(lib.hwpa_baseliner_create(ctypes.byref(cfg), ctypes.c_uint32(lanes), ctypes.byref(self.h)))
 
-    def run(self, d_raw, frames, d_out, d_stats, stream=None):
-        check(lib.hwpa_baseliner_run(self.h, ptr(d_raw), ctypes.c_uint32(frames), ptr(d_out), ptr(d_stats),
-                                     _vp(stream)))
+    def run(self, d_raw, frames, d_out, d_stats, stream=None, encoding=T.ENCODING_INT16):
+        check(lib.hwpa_baseliner_run_encoded(self.h, ptr(d_raw), ctypes.c_int32(encoding), ctypes.c_uint32(frames),
+                                             ptr(d_out), ptr(d_stats), _vp(stream)))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -492,19 +492,23 @@ class BatchAnalyzer:
     def pool_id_(self):
         return self.pool_id
 
-    def __call__(self, traces, first_frame, pool_id=None, on_device=None):
-        """traces: numpy int16 [lanes, frames, 64] (host) or a device buffer
-        (anything with data_ptr()) already in HBM."""
+    def __call__(self, traces, first_frame, pool_id=None, on_device=None, encoding=None):
+        """traces: numpy int16 or uint8 [lanes, frames, 64] (host) or a device
+        buffer (anything with data_ptr()) already in HBM; `encoding` (ENCODING_*)
+        is taken from a numpy array's dtype, and defaults to int16 for device buffers."""
         pid = self.pool_id if pool_id is None else pool_id
         if isinstance(traces, np.ndarray):
-            a = np.ascontiguousarray(traces, dtype=np.int16)
+            enc = T.ENCODING_UINT8 if traces.dtype == np.uint8 else T.ENCODING_INT16
+            a = np.ascontiguousarray(traces, dtype=np.uint8 if enc == T.ENCODING_UINT8 else np.int16)
             if a.size != self.lanes * self.frames * T.LANE:
                 raise HwpaError(1, "trace batch shape does not match the analyzer")
-            rc = lib.hwpa_analyzer_process(self.h, a.ctypes.data_as(_vp), ctypes.c_int32(0), ctypes.c_uint32(pid),
-                                           ctypes.c_int32(first_frame))
+            rc = lib.hwpa_analyzer_process_encoded(self.h, a.ctypes.data_as(_vp), ctypes.c_int32(enc),
+                                                   ctypes.c_int32(0), ctypes.c_uint32(pid),
+                                                   ctypes.c_int32(first_frame))
         else:
-            rc = lib.hwpa_analyzer_process(self.h, ptr(traces), ctypes.c_int32(1), ctypes.c_uint32(pid),
-                                           ctypes.c_int32(first_frame))
+            enc = T.ENCODING_INT16 if encoding is None else encoding
+            rc = lib.hwpa_analyzer_process_encoded(self.h, ptr(traces), ctypes.c_int32(enc), ctypes.c_int32(1),
+                                                   ctypes.c_uint32(pid), ctypes.c_int32(first_frame))
         check(rc)
         return self
 
@@ -648,11 +652,16 @@ class Batch:
         self._init = init
         check(lib.hwpa_batch_set_initializer(self.h, init.h if init is not None else None))
 
-    def process(self, chunks, first_frames):
+    def process(self, chunks, first_frames, encodings=None):
+        """chunks: one device buffer per pool; encodings: ENCODING_* per pool (int16 if None)."""
         n = len(self.specs)
         ptrs = (ctypes.c_void_p * n)(*[c.data_ptr() for c in chunks])
         ff = (ctypes.c_int32 * n)(*first_frames)
-        check(lib.hwpa_batch_process(self.h, ptrs, ff))
+        if encodings is None:
+            check(lib.hwpa_batch_process(self.h, ptrs, ff))
+        else:
+            enc = (ctypes.c_int32 * n)(*encodings)
+            check(lib.hwpa_batch_process_encoded(self.h, ptrs, enc, ff))
 
     def sync(self):
         check(lib.hwpa_batch_sync(self.h))

@@ -46,8 +46,33 @@ enum : uint32_t {
     kPlanCopyIn = 1u << 0, kPlanEarlyZero = 1u << 1, kPlanAggregate = 1u << 2, kPlanHistAdd = 1u << 3,
     kPlanDmeInit = 1u << 4, kPlanEstimate = 1u << 5, kPlanLearned = 1u << 6, kPlanCapture = 1u << 7,
     kPlanReset = 1u << 8, kPlanLabels = 1u << 9, kPlanPulses = 1u << 10, kPlanLateZero = 1u << 11,
-    kPlanEvents = 1u << 12, kPlanMetrics = 1u << 13, kPlanMetricsEnd = 1u << 14
+    kPlanEvents = 1u << 12, kPlanMetrics = 1u << 13, kPlanMetricsEnd = 1u << 14, kPlanUint8 = 1u << 15,
+    kPlanEmptyPulses = 1u << 16
 };
+
+// FixedModelBatchAnalyzer ctor (analyzer/BatchAnalyzer.cpp:120-145) with
+// StaticDetModelConfig::SetupAnalogs (dataTypes/configs/StaticDetModelConfig.h:53-68):
+// refSignal = refSnr sqrt(blVar); mean_a = blMean + relAmp_a refSignal;
+// var_a = blVar + mean_a + (excessNoiseCV_a mean_a)^2 (the square as one product).
+static void StaticModel(const hwpa_config& c, hwpa_lane_model* m)
+{
+    std::memset(m, 0, sizeof(*m));
+    const float blMean = c.static_baseline_mean, blVar = c.static_baseline_variance;
+    const float refSignal = c.movie.ref_snr * std::sqrt(blVar);
+    for (int z = 0; z < LS; ++z) {
+        m->baseline.means[z] = hw_f2h(blMean);
+        m->baseline.vars[z] = hw_f2h(blVar);
+    }
+    for (int a = 0; a < NA; ++a) {
+        const float mean = blMean + c.movie.analogs[a].rel_amplitude * refSignal;
+        const float cvm = c.movie.analogs[a].excess_noise_cv * mean;
+        const float var = blVar + mean + cvm * cvm;
+        for (int z = 0; z < LS; ++z) {
+            m->analogs[a].means[z] = hw_f2h(mean);
+            m->analogs[a].vars[z] = hw_f2h(var);
+        }
+    }
+}
 
 }  // namespace hwpa
 
@@ -58,7 +83,8 @@ struct hwpa_analyzer {
     uint32_t poolId, lanes, frames;
     int32_t nextFrame = 0;
     uint32_t baselinerLatency = 0, dmeDelay = 0;
-    bool fullEstimation = false;
+    bool fullEstimation = false;   // DynamicEstimate: a full estimation has occurred
+    bool modelInitialized = false; // SingleEstimate: the first estimate is in place
     int status = kHistInit;
     int32_t framesRemaining = 0;
     Interval aggFi, histFi, modelFi;
@@ -119,6 +145,75 @@ static void FreeAnalyzer(hwpa_analyzer* a)
         if (e) (void)hipEventDestroy(e);
     if (a->stream) (void)hipStreamDestroy(a->stream);
     delete a;
+}
+
+#define HWPA_STAGE_BEGIN(a, k)                                          \
+    do {                                                                \
+        if ((a)->profiling) HWPA_HIP(hipEventRecord((a)->ev[k], (a)->stream)); \
+        (a)->stageRan[k] = true;                                        \
+    } while (0)
+
+// DetectionModelEstimator::AddBatch (traceAnalysis/DetectionModelEstimator.cpp:93-160) on the
+// chunk just baselined: aggregate statistics, histogram, model init before sequencing, and
+// every MinFramesForEstimate frames an estimate followed by the histogram / aggregate reset.
+// Sets a->ranEstimate when an estimate ran.
+static int32_t DmeAddBatch(hwpa_analyzer* a, int32_t first_frame)
+{
+    const uint32_t L = a->lanes, F = a->frames;
+    hipStream_t s = a->stream;
+    int32_t rc;
+    HWPA_STAGE_BEGIN(a, kStAggregate);
+    a->plan |= kPlanAggregate;
+    if ((rc = StatsMerge(a->dAgg, a->dStats, L, s)) != HWPA_OK) return rc;
+    a->aggFi.Hull(first_frame, first_frame + (int32_t)F);
+    if (a->status != kHistInit) {
+        HWPA_STAGE_BEGIN(a, kStHistogram);
+        a->plan |= kPlanHistAdd;
+        if ((rc = HistAdd(a->dHist, a->dEdge, a->dBaselined, F, a->dModels, L, s)) != HWPA_OK) return rc;
+        a->histFi.Hull(first_frame, first_frame + (int32_t)F);
+    }
+    HWPA_STAGE_BEGIN(a, kStDme);
+    if (a->status != kSequencing) {
+        a->plan |= kPlanDmeInit;
+        if ((rc = DmeInitModels(a->cfg, a->dAgg, a->dModels, L, s)) != HWPA_OK) return rc;
+        a->modelFi = a->aggFi;
+    }
+    a->framesRemaining -= (int32_t)F;
+    if (a->framesRemaining <= 0) {
+        if (a->status == kDmeInit) a->status = kSequencing;
+        if (a->status == kSequencing) {
+            const int32_t mfi[2] = {a->modelFi.lo, a->modelFi.hi};
+            const int32_t efi[2] = {a->histFi.lo, a->histFi.hi};
+            if (a->mlp && a->cfg.learned_init) {
+                rc = DmeEstimateLearned(a->cfg, a->mlp, a->lws, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, s);
+                a->ranLearned = true;
+                a->plan |= kPlanLearned;
+            } else {
+                const bool cap = a->capX != nullptr;
+                rc = DmeEstimate(a->cfg, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, s,
+                                 cap ? a->dDiag : nullptr);
+                if (rc == HWPA_OK && cap) rc = MlpFeatures(a->dHist, L, a->capX, s);
+                if (rc == HWPA_OK && cap)
+                    rc = MlpTargets(a->dHist, a->dDiag, L, a->capMinConf, a->capT, a->capValid, s);
+                a->ranLearned = false;
+                a->plan |= kPlanEstimate | (cap ? kPlanCapture : 0u);
+            }
+            if (rc != HWPA_OK) return rc;
+            a->modelFi = a->histFi;
+            a->flags[1] = 1;
+            a->ranEstimate = true;
+            a->fullEstimation = true;
+        }
+        a->plan |= kPlanReset;
+        if ((rc = HistReset(a->cfg, a->dHist, a->dAgg, L, s)) != HWPA_OK) return rc;
+        a->histFi = Interval{};
+        if ((rc = StatsReset(a->dAgg, L, s)) != HWPA_OK) return rc;
+        a->aggFi = Interval{};
+        if (a->status == kHistInit) a->status = kDmeInit;
+        a->framesRemaining = (int32_t)a->cfg.min_frames_for_estimate;
+    }
+
+    return HWPA_OK;
 }
 
 extern "C" {
@@ -198,6 +293,17 @@ int32_t hwpa_analyzer_create(uint32_t pool_id, uint32_t lanes, uint32_t frames, 
         FreeAnalyzer(a);
         return rc;
     }
+    if (cfg->model_estimation_mode == HWPA_MODEL_FIXED) {
+        std::vector<hwpa_lane_model> h(lanes);
+        StaticModel(*cfg, &h[0]);
+        for (uint32_t l = 1; l < lanes; ++l) h[l] = h[0];
+        if (hipMemcpyAsync(a->dModels, h.data(), sizeof(hwpa_lane_model) * lanes, hipMemcpyHostToDevice, a->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(a->stream) != hipSuccess) {
+            FreeAnalyzer(a);
+            return Fail(HWPA_ERR_HIP, "analyzer: static model upload failed");
+        }
+    }
     if (hipStreamSynchronize(a->stream) != hipSuccess) {
         FreeAnalyzer(a);
         return Fail(HWPA_ERR_HIP, "analyzer init failed");
@@ -212,36 +318,37 @@ int32_t hwpa_analyzer_destroy(hwpa_analyzer* a)
     return HWPA_OK;
 }
 
-#define HWPA_STAGE_BEGIN(a, k)                                          \
-    do {                                                                \
-        if ((a)->profiling) HWPA_HIP(hipEventRecord((a)->ev[k], (a)->stream)); \
-        (a)->stageRan[k] = true;                                        \
-    } while (0)
 
-int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t on_device, uint32_t pool_id,
-                              int32_t first_frame)
+// BatchAnalyzer::operator()(const TraceBatchVariant&) (analyzer/BatchAnalyzer.cpp:149-170) and
+// the AnalyzeImpl of the configured mode: FixedModel (:172-205), SingleEstimate (:207-283),
+// DynamicEstimate (:288-373).
+static int32_t AnalyzerProcess(hwpa_analyzer* a, const void* traces, int32_t encoding, int32_t on_device,
+                               uint32_t pool_id, int32_t first_frame)
 {
     if (!a || !traces) return Fail(HWPA_ERR_INVALID_ARG, "analyzer_process: null argument");
+    if (encoding != HWPA_ENCODING_INT16 && encoding != HWPA_ENCODING_UINT8)
+        return Fail(HWPA_ERR_INVALID_ARG, "Unexpected data encoding in analyzer input");
     // BatchAnalyzer::operator() checks (BatchAnalyzer.cpp:151-152)
     if (pool_id != a->poolId) return Fail(HWPA_ERR_BAD_POOL, "Bad pool ID.");
     if (first_frame != a->nextFrame) return Fail(HWPA_ERR_BAD_FRAME, "Bad frame ID.");
     const uint32_t L = a->lanes, F = a->frames;
-    const size_t traceBytes = (size_t)L * F * LS * 2;
+    const size_t traceBytes = (size_t)L * F * LS * (encoding == HWPA_ENCODING_UINT8 ? 1 : 2);
     hipStream_t s = a->stream;
     int32_t rc;
     std::memset(a->flags, 0, sizeof(a->flags));
     std::fill(std::begin(a->stageRan), std::end(a->stageRan), false);
     a->ranEstimate = false;
     a->plan = a->profiling ? kPlanEvents : 0u;
+    if (encoding == HWPA_ENCODING_UINT8) a->plan |= kPlanUint8;
 
-    const int16_t* src = traces;
+    const void* src = traces;
     if (!on_device) {
         HWPA_HIP(hipMemcpyAsync(a->dTraces, traces, traceBytes, hipMemcpyHostToDevice, s));
         src = a->dTraces;
         a->plan |= kPlanCopyIn;
     }
     HWPA_STAGE_BEGIN(a, kStBaseline);
-    if ((rc = BaselinerRun(a->bl, src, F, a->dBaselined, a->dStats, s)) != HWPA_OK) return rc;
+    if ((rc = BaselinerRun(a->bl, src, F, a->dBaselined, a->dStats, s, encoding)) != HWPA_OK) return rc;
     a->nextFrame = first_frame + (int32_t)F;
 
     auto finish = [&](int32_t r) -> int32_t {
@@ -256,91 +363,87 @@ int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t o
         hipLaunchKernelGGL(ZeroCountsKernel, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, a->dCounts, a->dOvf, nz);
         return CheckLaunch("ZeroCountsKernel");
     };
-    // Early empty result during the baseliner start-up and the pool stagger (BatchAnalyzer.cpp:331-337)
+    auto runMetrics = [&]() -> int32_t {
+        // HF metrics of this pulse batch, which ends NL latent frames before the chunk.
+        HWPA_STAGE_BEGIN(a, kStMetrics);
+        int32_t produced = 0;
+        const int32_t r = MetricsProcess(a->met, a->dPulses, a->dCounts, a->cfg.max_calls_per_zmw, F,
+                                         first_frame + (int32_t)F - NL, a->dStats, a->dScore, a->dPulseBl,
+                                         a->dModels, a->dMetrics, &produced, s);
+        if (r != HWPA_OK) return r;
+        a->flags[3] = produced;
+        a->plan |= kPlanMetrics | (produced ? kPlanMetricsEnd : 0u);
+        return HWPA_OK;
+    };
+    auto labelsAndPulses = [&]() -> int32_t {
+        HWPA_STAGE_BEGIN(a, kStLabels);
+        a->plan |= kPlanLabels | kPlanPulses;
+        int32_t r = LabelerRun(a->lab, a->dBaselined, F, a->dModels, a->dLabels, a->dLatent, a->dScore, s);
+        if (r != HWPA_OK) return r;
+        a->flags[2] = 1;
+        HWPA_STAGE_BEGIN(a, kStPulses);
+        return PulserRun(a->pul, a->dLabels, a->dLatent, a->dBaselined, F, first_frame - NL, a->dModels, a->dPulses,
+                         a->dCounts, a->dOvf, a->dPulseBl, s);
+    };
+
+    if (a->cfg.model_estimation_mode == HWPA_MODEL_FIXED) {
+        // FixedModelBatchAnalyzer: the static model, every stage every chunk.
+        if ((rc = labelsAndPulses()) != HWPA_OK) return rc;
+        a->flags[0] = 1;
+        if ((rc = runMetrics()) != HWPA_OK) return rc;
+        return finish(HWPA_OK);
+    }
+
+    if (a->cfg.model_estimation_mode == HWPA_MODEL_INITIAL) {
+        // SingleEstimateBatchAnalyzer: feed the DME after the baseliner start-up until its first
+        // estimate, then keep that model; empty pulses (and empty labeler / pulse-detector
+        // metrics: score 0, no baseline frames) until then.  Metrics run every chunk.
+        if (!a->modelInitialized && first_frame > (int32_t)a->baselinerLatency) {
+            if ((rc = DmeAddBatch(a, first_frame)) != HWPA_OK) return rc;
+            a->modelInitialized = a->ranEstimate;
+        }
+        if (a->modelInitialized) {
+            if ((rc = labelsAndPulses()) != HWPA_OK) return rc;
+            a->flags[0] = 1;
+        } else {
+            a->plan |= kPlanEmptyPulses;
+            if ((rc = zeroCounts()) != HWPA_OK) return rc;
+            if ((rc = Fill32(a->dScore, 0, nz, s)) != HWPA_OK) return rc;
+            if ((rc = Fill32(a->dPulseBl, 0, sizeof(hwpa_stat_acc) * L / 4, s)) != HWPA_OK) return rc;
+        }
+        if ((rc = runMetrics()) != HWPA_OK) return rc;
+        return finish(HWPA_OK);
+    }
+
+    // DynamicEstimateBatchAnalyzer.  Early empty result during the baseliner start-up and the
+    // pool stagger (BatchAnalyzer.cpp:331-337).
     if (first_frame < (int32_t)(a->baselinerLatency + a->dmeDelay)) {
         a->plan |= kPlanEarlyZero;
         if ((rc = zeroCounts()) != HWPA_OK) return rc;
         return finish(HWPA_OK);
     }
-
-    // DetectionModelEstimator::AddBatch
-    HWPA_STAGE_BEGIN(a, kStAggregate);
-    a->plan |= kPlanAggregate;
-    if ((rc = StatsMerge(a->dAgg, a->dStats, L, s)) != HWPA_OK) return rc;
-    a->aggFi.Hull(first_frame, first_frame + (int32_t)F);
-    if (a->status != kHistInit) {
-        HWPA_STAGE_BEGIN(a, kStHistogram);
-        a->plan |= kPlanHistAdd;
-        if ((rc = HistAdd(a->dHist, a->dEdge, a->dBaselined, F, a->dModels, L, s)) != HWPA_OK) return rc;
-        a->histFi.Hull(first_frame, first_frame + (int32_t)F);
-    }
-    HWPA_STAGE_BEGIN(a, kStDme);
-    if (a->status != kSequencing) {
-        a->plan |= kPlanDmeInit;
-        if ((rc = DmeInitModels(a->cfg, a->dAgg, a->dModels, L, s)) != HWPA_OK) return rc;
-        a->modelFi = a->aggFi;
-    }
-    a->framesRemaining -= (int32_t)F;
-    if (a->framesRemaining <= 0) {
-        if (a->status == kDmeInit) a->status = kSequencing;
-        if (a->status == kSequencing) {
-            const int32_t mfi[2] = {a->modelFi.lo, a->modelFi.hi};
-            const int32_t efi[2] = {a->histFi.lo, a->histFi.hi};
-            if (a->mlp && a->cfg.learned_init) {
-                rc = DmeEstimateLearned(a->cfg, a->mlp, a->lws, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, s);
-                a->ranLearned = true;
-                a->plan |= kPlanLearned;
-            } else {
-                const bool cap = a->capX != nullptr;
-                rc = DmeEstimate(a->cfg, a->dHist, a->dAgg, a->dModels, mfi, efi, L, a->dIters, s,
-                                 cap ? a->dDiag : nullptr);
-                if (rc == HWPA_OK && cap) rc = MlpFeatures(a->dHist, L, a->capX, s);
-                if (rc == HWPA_OK && cap)
-                    rc = MlpTargets(a->dHist, a->dDiag, L, a->capMinConf, a->capT, a->capValid, s);
-                a->ranLearned = false;
-                a->plan |= kPlanEstimate | (cap ? kPlanCapture : 0u);
-            }
-            if (rc != HWPA_OK) return rc;
-            a->modelFi = a->histFi;
-            a->flags[1] = 1;
-            a->ranEstimate = true;
-            a->fullEstimation = true;
-        }
-        a->plan |= kPlanReset;
-        if ((rc = HistReset(a->cfg, a->dHist, a->dAgg, L, s)) != HWPA_OK) return rc;
-        a->histFi = Interval{};
-        if ((rc = StatsReset(a->dAgg, L, s)) != HWPA_OK) return rc;
-        a->aggFi = Interval{};
-        if (a->status == kHistInit) a->status = kDmeInit;
-        a->framesRemaining = (int32_t)a->cfg.min_frames_for_estimate;
-    }
-
-    HWPA_STAGE_BEGIN(a, kStLabels);
-    a->plan |= kPlanLabels | kPlanPulses;
-    if ((rc = LabelerRun(a->lab, a->dBaselined, F, a->dModels, a->dLabels, a->dLatent, a->dScore, s)) != HWPA_OK)
-        return rc;
-    a->flags[2] = 1;
-    HWPA_STAGE_BEGIN(a, kStPulses);
-    if ((rc = PulserRun(a->pul, a->dLabels, a->dLatent, a->dBaselined, F, first_frame - NL, a->dModels, a->dPulses,
-                        a->dCounts, a->dOvf, a->dPulseBl, s)) != HWPA_OK)
-        return rc;
+    if ((rc = DmeAddBatch(a, first_frame)) != HWPA_OK) return rc;
+    if ((rc = labelsAndPulses()) != HWPA_OK) return rc;
     if (a->fullEstimation) {
         a->flags[0] = 1;
-        // HF metrics of this pulse batch, which ends NL latent frames before the
-        // chunk (DynamicEstimateBatchAnalyzer::AnalyzeImpl, BatchAnalyzer.cpp:366-372).
-        HWPA_STAGE_BEGIN(a, kStMetrics);
-        int32_t produced = 0;
-        if ((rc = MetricsProcess(a->met, a->dPulses, a->dCounts, a->cfg.max_calls_per_zmw, F,
-                                 first_frame + (int32_t)F - NL, a->dStats, a->dScore, a->dPulseBl, a->dModels,
-                                 a->dMetrics, &produced, s)) != HWPA_OK)
-            return rc;
-        a->flags[3] = produced;
-        a->plan |= kPlanMetrics | (produced ? kPlanMetricsEnd : 0u);
+        if ((rc = runMetrics()) != HWPA_OK) return rc;
     } else {
         a->plan |= kPlanLateZero;
         if ((rc = zeroCounts()) != HWPA_OK) return rc;
     }
     return finish(HWPA_OK);
+}
+
+int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t on_device, uint32_t pool_id,
+                              int32_t first_frame)
+{
+    return AnalyzerProcess(a, traces, HWPA_ENCODING_INT16, on_device, pool_id, first_frame);
+}
+
+int32_t hwpa_analyzer_process_encoded(hwpa_analyzer* a, const void* traces, int32_t encoding, int32_t on_device,
+                                      uint32_t pool_id, int32_t first_frame)
+{
+    return AnalyzerProcess(a, traces, encoding, on_device, pool_id, first_frame);
 }
 
 int32_t hwpa_analyzer_sync(hwpa_analyzer* a)
@@ -615,14 +718,16 @@ int32_t hwpa_batch_set_initializer(hwpa_batch* b, hwpa_initializer* m)
     return HWPA_OK;
 }
 
-int32_t hwpa_batch_process(hwpa_batch* b, const int16_t* const* d_traces, const int32_t* first_frames)
+static int32_t BatchProcess(hwpa_batch* b, const void* const* d_traces, const int32_t* encodings,
+                            const int32_t* first_frames)
 {
     if (!b || !d_traces || !first_frames) return Fail(HWPA_ERR_INVALID_ARG, "batch_process: null argument");
     const size_t n = b->pools.size();
+    auto enc = [&](size_t i) { return encodings ? encodings[i] : (int32_t)HWPA_ENCODING_INT16; };
     if (!b->useGraph) {
         for (size_t i = 0; i < n; ++i) {
             hwpa_analyzer* a = b->pools[i];
-            const int32_t rc = hwpa_analyzer_process(a, d_traces[i], 1, a->poolId, first_frames[i]);
+            const int32_t rc = AnalyzerProcess(a, d_traces[i], enc(i), 1, a->poolId, first_frames[i]);
             if (rc != HWPA_OK) return rc;
         }
         b->steps += 1;
@@ -639,7 +744,7 @@ int32_t hwpa_batch_process(hwpa_batch* b, const int16_t* const* d_traces, const 
             rc = Fail(HWPA_ERR_HIP, "batch: fork wait failed");
             break;
         }
-        rc = hwpa_analyzer_process(a, d_traces[i], 1, a->poolId, first_frames[i]);
+        rc = AnalyzerProcess(a, d_traces[i], enc(i), 1, a->poolId, first_frames[i]);
         if (rc != HWPA_OK) break;
         if (hipEventRecord(b->joins[i], a->stream) != hipSuccess ||
             hipStreamWaitEvent(b->master, b->joins[i], 0) != hipSuccess)
@@ -694,6 +799,19 @@ int32_t hwpa_batch_process(hwpa_batch* b, const int16_t* const* d_traces, const 
     for (auto* a : b->pools) HWPA_HIP(hipStreamWaitEvent(a->stream, b->done, 0));
     b->steps += 1;
     return HWPA_OK;
+}
+
+int32_t hwpa_batch_process(hwpa_batch* b, const int16_t* const* d_traces, const int32_t* first_frames)
+{
+    if (!b || !d_traces) return Fail(HWPA_ERR_INVALID_ARG, "batch_process: null argument");
+    std::vector<const void*> p(d_traces, d_traces + b->pools.size());
+    return BatchProcess(b, p.data(), nullptr, first_frames);
+}
+
+int32_t hwpa_batch_process_encoded(hwpa_batch* b, const void* const* d_traces, const int32_t* encodings,
+                                   const int32_t* first_frames)
+{
+    return BatchProcess(b, d_traces, encodings, first_frames);
 }
 
 int32_t hwpa_batch_sync(hwpa_batch* b)

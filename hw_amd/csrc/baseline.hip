@@ -157,10 +157,11 @@ __device__ __forceinline__ void RingRefill(uint32_t* ring, uint32_t last)
     }
 }
 
-// LPB lanes (waves) per workgroup -- the autotuned lane tile.
-// LPB lanes per workgroup; PD groups of raw frames in flight (1 or 2).
-template <class T, int LPB, int PD>
-__global__ __launch_bounds__(64 * LPB) void BaselineKernel(const int16_t* __restrict__ raw, int frames, int lanes,
+// E: raw sample type (int16_t, or uint8_t zero-extended as BatchData::Extract does,
+// dataTypes/BatchData.h:160-182); LPB lanes (waves) per workgroup -- the autotuned lane
+// tile; PD groups of raw frames in flight (1 or 2).
+template <class T, class E, int LPB, int PD>
+__global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__ raw, int frames, int lanes,
                                                            int16_t* __restrict__ out,
                                                            hwpa_baseliner_stats* __restrict__ stats,
                                                            BlLaneState* __restrict__ state,
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const int16_t* __rest
     float bm0 = 0.f, bm1 = 0.f, bm2 = 0.f, rawBl = 0.f;
     float gm0 = 0.f, gm1 = 0.f, gm2 = 0.f;
 
-    const int16_t* rl = raw + (size_t)lane * frames * LS + z;
+    const E* rl = raw + (size_t)lane * frames * LS + z;
     int16_t* ol = out + (size_t)lane * frames * LS + z;
     const int groups = frames / S;
     const float minSigma = .288675135f;
@@ -405,7 +406,7 @@ using TTSM = Table<2, 2, 8, 1, 9, 31, 1>;
 using TTSS = Table<2, 2, 8, 1, 7, 17, 1>;
 
 template <class T>
-static int32_t Launch(BaselinerDev* b, const int16_t* raw, uint32_t frames, int16_t* out,
+static int32_t Launch(BaselinerDev* b, const void* raw, int32_t encoding, uint32_t frames, int16_t* out,
                       hwpa_baseliner_stats* stats, hipStream_t s)
 {
     if (frames % T::S != 0) return Fail(HWPA_ERR_INVALID_ARG, "frames must be a multiple of the aggregate stride");
@@ -415,9 +416,15 @@ static int32_t Launch(BaselinerDev* b, const int16_t* raw, uint32_t frames, int1
     const int lpb = (v % 3 == 1) ? 2 : (v % 3 == 2 ? 4 : 1);
     const bool deep = v >= 3;
     const unsigned grid = (b->lanes + (unsigned)lpb - 1) / (unsigned)lpb;
-#define HWPA_BL_LAUNCH(L, P)                                                                                         \
-    hipLaunchKernelGGL((BaselineKernel<T, L, P>), dim3(grid), dim3(64 * L), 0, s, raw, (int)frames, (int)b->lanes,   \
-                       out, stats, b->state, b->win, b->k)
+#define HWPA_BL_LAUNCH(L, P)                                                                                       \
+    do {                                                                                                           \
+        if (encoding == HWPA_ENCODING_UINT8)                                                                       \
+            hipLaunchKernelGGL((BaselineKernel<T, uint8_t, L, P>), dim3(grid), dim3(64 * L), 0, s,                 \
+                               (const uint8_t*)raw, (int)frames, (int)b->lanes, out, stats, b->state, b->win, b->k); \
+        else                                                                                                       \
+            hipLaunchKernelGGL((BaselineKernel<T, int16_t, L, P>), dim3(grid), dim3(64 * L), 0, s,                 \
+                               (const int16_t*)raw, (int)frames, (int)b->lanes, out, stats, b->state, b->win, b->k); \
+    } while (0)
     if (lpb == 1) {
         if (deep) HWPA_BL_LAUNCH(1, 2); else HWPA_BL_LAUNCH(1, 1);
     } else if (lpb == 2) {
@@ -484,17 +491,20 @@ void BaselinerDestroy(BaselinerDev* b)
     delete b;
 }
 
-int32_t BaselinerRun(BaselinerDev* b, const int16_t* raw, uint32_t frames, int16_t* out,
-                     hwpa_baseliner_stats* stats, hipStream_t s)
+int32_t BaselinerRun(BaselinerDev* b, const void* raw, uint32_t frames, int16_t* out,
+                     hwpa_baseliner_stats* stats, hipStream_t s, int32_t encoding)
 {
     if (!b || !raw || !out || !stats || frames == 0) return Fail(HWPA_ERR_INVALID_ARG, "baseliner: bad argument");
+    if (encoding != HWPA_ENCODING_INT16 && encoding != HWPA_ENCODING_UINT8)
+        return Fail(HWPA_ERR_INVALID_ARG, "Unexpected data encoding in baseline filter");
+    const int32_t e = encoding;
     switch (b->fp.num_stages == 3 ? b->fp.widths[0] * 100 + b->fp.widths[2] : 1000 + b->fp.widths[0] * 100 + b->fp.widths[1]) {
-    case 1161: return Launch<TMSL>(b, raw, frames, out, stats, s);
-    case 931: return Launch<TMSM>(b, raw, frames, out, stats, s);
-    case 717: return Launch<TMSS>(b, raw, frames, out, stats, s);
-    case 2161: return Launch<TTSL>(b, raw, frames, out, stats, s);
-    case 1931: return Launch<TTSM>(b, raw, frames, out, stats, s);
-    case 1717: return Launch<TTSS>(b, raw, frames, out, stats, s);
+    case 1161: return Launch<TMSL>(b, raw, e, frames, out, stats, s);
+    case 931: return Launch<TMSM>(b, raw, e, frames, out, stats, s);
+    case 717: return Launch<TMSS>(b, raw, e, frames, out, stats, s);
+    case 2161: return Launch<TTSL>(b, raw, e, frames, out, stats, s);
+    case 1931: return Launch<TTSM>(b, raw, e, frames, out, stats, s);
+    case 1717: return Launch<TTSS>(b, raw, e, frames, out, stats, s);
     default: return Fail(HWPA_ERR_UNSUPPORTED, "baseliner: unsupported filter table");
     }
 }

@@ -289,6 +289,10 @@ int32_t hwpa_config_default(hwpa_config* c)
     c->sandwich_tolerance = 0;
     c->frames_per_hf_metric_block = 4096;
     c->realtime_activity_labels = 1;
+    // BasecallerAlgorithmConfig.h:65, StaticDetModelConfig.h:43-44
+    c->model_estimation_mode = HWPA_MODEL_DYNAMIC;
+    c->static_baseline_mean = 0.0f;
+    c->static_baseline_variance = 33.0f;
     c->learned_init = 0;
     c->learned_conf_threshold = 0.0f;
     c->learned_em_iteration_limit = 3;
@@ -356,6 +360,8 @@ int32_t hwpa_config_load_json(hwpa_config* c, const char* json)
         {"pulseAccumConfig.XspWidthThresh", &c->xsp_width_thresh},
         {"learnedInit.ConfidenceThreshold", &c->learned_conf_threshold},
         {"learnedInit.ConvergenceTolerance", &c->learned_conv_tol},
+        {"staticDetModelConfig.baselineMean", &c->static_baseline_mean},
+        {"staticDetModelConfig.baselineVariance", &c->static_baseline_variance},
         {"movie.frameRate", &c->movie.frame_rate},
         {"movie.photoelectronSensitivity", &c->movie.photoelectron_sensitivity},
         {"movie.refSnr", &c->movie.ref_snr},
@@ -390,7 +396,14 @@ int32_t hwpa_config_load_json(hwpa_config* c, const char* json)
                 done = true;
             }
         if (done) continue;
-        if (k == "baselinerConfig.Filter") {
+        if (k == "modelEstimationMode") {
+            static const char* modes[] = {"FixedEstimations", "InitialEstimations", "DynamicEstimations"};
+            int m = -1;
+            for (int i = 0; i < 3; ++i)
+                if (v == modes[i]) m = i;
+            if (m < 0) return Fail(HWPA_ERR_BAD_CONFIG, "unknown model estimation mode: " + v);
+            c->model_estimation_mode = m;
+        } else if (k == "baselinerConfig.Filter") {
             const int t = FilterByName(v);
             if (t < 0) return Fail(HWPA_ERR_BAD_CONFIG, "unknown filter type: " + v);
             c->filter_type = t;
@@ -452,6 +465,11 @@ int32_t hwpa_config_validate(const hwpa_config* c)
     if (c->xsp_amp_thresh < 0.0 || c->xsp_amp_thresh > 1.0) return Fail(HWPA_ERR_BAD_CONFIG, "Bad XspAmpThresh: valid range is [0, 1]");
     if (c->max_calls_per_zmw == 0) return Fail(HWPA_ERR_BAD_CONFIG, "maxCallsPerZmw must be positive");
     if (c->frames_per_hf_metric_block == 0) return Fail(HWPA_ERR_BAD_CONFIG, "framesPerHFMetricBlock must be positive");
+    if (c->model_estimation_mode < HWPA_MODEL_FIXED || c->model_estimation_mode > HWPA_MODEL_DYNAMIC)
+        return Fail(HWPA_ERR_BAD_CONFIG, "unknown model estimation mode");
+    if (!std::isfinite(c->static_baseline_mean) || !(c->static_baseline_variance >= 0.0f) ||
+        !std::isfinite(c->static_baseline_variance))
+        return Fail(HWPA_ERR_BAD_CONFIG, "static detection model: mean must be finite, variance finite and >= 0");
     if (!(c->movie.photoelectron_sensitivity > 0)) return Fail(HWPA_ERR_BAD_CONFIG, "photoelectronSensitivity must be positive");
     // DmeEmHost::EstimateLaneDetModel relative amplitude check (DmeEmHost.cpp:301-305)
     for (int a = 0; a < 4; ++a)
@@ -516,6 +534,13 @@ int32_t hwpa_baseliner_run(hwpa_baseliner* b, const int16_t* d_raw, uint32_t fra
 {
     if (!b) return Fail(HWPA_ERR_INVALID_ARG, "null baseliner");
     return BaselinerRun(b->dev, d_raw, frames, d_out, d_stats, AsStream(stream));
+}
+
+int32_t hwpa_baseliner_run_encoded(hwpa_baseliner* b, const void* d_raw, int32_t encoding, uint32_t frames,
+                                   int16_t* d_out, hwpa_baseliner_stats* d_stats, void* stream)
+{
+    if (!b) return Fail(HWPA_ERR_INVALID_ARG, "null baseliner");
+    return BaselinerRun(b->dev, d_raw, frames, d_out, d_stats, AsStream(stream), encoding);
 }
 
 int32_t hwpa_stats_reset(hwpa_baseliner_stats* d_agg, uint32_t lanes, void* stream)

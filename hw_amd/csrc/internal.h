@@ -57,8 +57,9 @@ DmeConsts MakeDmeConsts(const hwpa_config& c);
 struct BaselinerDev;
 int32_t BaselinerCreate(const hwpa_config& cfg, uint32_t lanes, BaselinerDev** out);
 void BaselinerDestroy(BaselinerDev* b);
-int32_t BaselinerRun(BaselinerDev* b, const int16_t* raw, uint32_t frames, int16_t* out,
-                     hwpa_baseliner_stats* stats, hipStream_t s);
+// raw: [lane][frame][64] samples of the given HWPA_ENCODING_* type.
+int32_t BaselinerRun(BaselinerDev* b, const void* raw, uint32_t frames, int16_t* out,
+                     hwpa_baseliner_stats* stats, hipStream_t s, int32_t encoding = HWPA_ENCODING_INT16);
 uint32_t BaselinerStartupLatency(const hwpa_config& cfg);
 
 int32_t StatsReset(hwpa_baseliner_stats* agg, uint32_t lanes, hipStream_t s);

@@ -22,6 +22,9 @@ ERR_BAD_POOL = 3
 ERR_BAD_FRAME = 4
 ERR_BAD_CONFIG = 2
 
+MODEL_FIXED, MODEL_INITIAL, MODEL_DYNAMIC = 0, 1, 2
+ENCODING_INT16, ENCODING_UINT8 = 0, 1
+
 FILTER_TYPES = {"MultiScaleLarge": 0, "MultiScaleMedium": 1, "MultiScaleSmall": 2,
                 "TwoScaleLarge": 3, "TwoScaleMedium": 4, "TwoScaleSmall": 5}
 
@@ -58,6 +61,8 @@ class Config(ctypes.Structure):
         ("max_calls_per_zmw", ctypes.c_uint32), ("xsp_amp_thresh", ctypes.c_double),
         ("xsp_width_thresh", ctypes.c_float), ("sandwich_tolerance", ctypes.c_uint32),
         ("frames_per_hf_metric_block", ctypes.c_uint32), ("realtime_activity_labels", ctypes.c_int32),
+        ("model_estimation_mode", ctypes.c_int32), ("static_baseline_mean", ctypes.c_float),
+        ("static_baseline_variance", ctypes.c_float),
         ("learned_init", ctypes.c_int32),
         ("learned_conf_threshold", ctypes.c_float), ("learned_em_iteration_limit", ctypes.c_uint32),
         ("learned_conv_tol", ctypes.c_float), ("movie", MovieInfo)]

@@ -44,7 +44,8 @@ int32_t hwpa_config_validate(const hwpa_config* cfg);
 /* sizeof() of the public structures, in this order: hwpa_config,
  * hwpa_movie_info, hwpa_stat_acc, hwpa_autocorr_acc, hwpa_baseliner_stats,
  * hwpa_lane_hist, hwpa_lane_model, hwpa_pulse, hwpa_tracegen_params,
- * hwpa_tracegen_state, hwpa_transition.  Returns how many were written. */
+ * hwpa_tracegen_state, hwpa_transition, hwpa_basecalling_metrics.  Returns how
+ * many were written. */
 int32_t hwpa_layout_sizes(int64_t* out, int32_t n);
 
 /* Filter table lookup (traceAnalysis/BaselinerParams.cpp:10-72). */
@@ -59,7 +60,9 @@ typedef struct hwpa_analyzer hwpa_analyzer;
 typedef struct hwpa_initializer hwpa_initializer;  /* learned EM initialiser, below */
 typedef struct hwpa_metrics hwpa_metrics;          /* HF metrics stage, below */
 
-/* BatchAnalyzer ctor + DynamicEstimateBatchAnalyzer (analyzer/BatchAnalyzer.cpp:71-118). */
+/* BatchAnalyzer ctor (analyzer/BatchAnalyzer.cpp:71-118) for the analyzer that
+ * cfg->model_estimation_mode selects (appModules/Basecaller.h:103-116): Fixed (static model),
+ * Initial (single estimate) or Dynamic (default). */
 int32_t hwpa_analyzer_create(uint32_t pool_id, uint32_t lanes, uint32_t frames_per_chunk,
                              const hwpa_config* cfg, hwpa_analyzer** out);
 int32_t hwpa_analyzer_destroy(hwpa_analyzer* a);
@@ -69,13 +72,18 @@ int32_t hwpa_analyzer_destroy(hwpa_analyzer* a);
  * Errors: HWPA_ERR_BAD_POOL (wrong pool), HWPA_ERR_BAD_FRAME (out of order). */
 int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t traces_on_device,
                               uint32_t pool_id, int32_t first_frame);
+/* Same for either TraceBatchVariant alternative (dataTypes/TraceBatch.h:97-128): traces are
+ * [lanes][frames][64] samples of HWPA_ENCODING_INT16 or HWPA_ENCODING_UINT8 (zero-extended,
+ * dataTypes/BatchData.h:160-182; the movie pedestal is subtracted as for int16). */
+int32_t hwpa_analyzer_process_encoded(hwpa_analyzer* a, const void* traces, int32_t encoding,
+                                      int32_t traces_on_device, uint32_t pool_id, int32_t first_frame);
 
 /* Blocks until the analyzer's stream is idle. */
 int32_t hwpa_analyzer_sync(hwpa_analyzer* a);
 
 /* Result accessors for the last process call (device pointers owned by the
  * analyzer, valid until the next process call).  flags: [0] pulses emitted,
- * [1] full estimation ran, [2] frame labels ran. */
+ * [1] full estimation ran, [2] frame labels ran, [3] a metrics block ended. */
 int32_t hwpa_result_flags(const hwpa_analyzer* a, int32_t flags[4]);
 int32_t hwpa_result_pulses(const hwpa_analyzer* a, const hwpa_pulse** d_pulses,
                            const uint32_t** d_counts, const uint32_t** d_overflows,
@@ -122,6 +130,9 @@ int32_t hwpa_batch_pool(hwpa_batch* b, uint32_t index, hwpa_analyzer** a);
 int32_t hwpa_batch_set_graph(hwpa_batch* b, int32_t enable);
 int32_t hwpa_batch_set_initializer(hwpa_batch* b, hwpa_initializer* m);
 int32_t hwpa_batch_process(hwpa_batch* b, const int16_t* const* d_traces, const int32_t* first_frames);
+/* Per-pool encodings (HWPA_ENCODING_*); NULL means int16 for every pool. */
+int32_t hwpa_batch_process_encoded(hwpa_batch* b, const void* const* d_traces, const int32_t* encodings,
+                                   const int32_t* first_frames);
 int32_t hwpa_batch_sync(hwpa_batch* b);
 /* [0] steps run, [1] graph instantiations, [2] in-place graph updates. */
 int32_t hwpa_batch_graph_stats(const hwpa_batch* b, uint64_t stats[3]);
@@ -171,6 +182,10 @@ int32_t hwpa_baseliner_create(const hwpa_config* cfg, uint32_t lanes, hwpa_basel
 int32_t hwpa_baseliner_destroy(hwpa_baseliner* b);
 int32_t hwpa_baseliner_run(hwpa_baseliner* b, const int16_t* d_raw, uint32_t frames,
                            int16_t* d_out, hwpa_baseliner_stats* d_stats, void* stream);
+/* The same on HWPA_ENCODING_INT16 or HWPA_ENCODING_UINT8 samples (DeviceMultiScaleBaseliner's
+ * uint8 / int16 filters, traceAnalysis/DeviceMultiScaleBaseliner.cu:84-101). */
+int32_t hwpa_baseliner_run_encoded(hwpa_baseliner* b, const void* d_raw, int32_t encoding, uint32_t frames,
+                                   int16_t* d_out, hwpa_baseliner_stats* d_stats, void* stream);
 
 /* Stats aggregator: BaselineStatsAggregatorDevice::AddMetricsImpl / ResetImpl
  * (traceAnalysis/BaselineStatsAggregatorHost.cpp:40-71). */

@@ -65,6 +65,21 @@ enum hwpa_filter_type {
     HWPA_FILTER_TWOSCALE_SMALL = 5
 };
 
+/* BasecallerAlgorithmConfig::ModelEstimationMode (dataTypes/configs/BasecallerAlgorithmConfig.h:64-65):
+ * which BatchAnalyzer the analyzer follows (analyzer/BatchAnalyzer.cpp:120-373). */
+enum hwpa_model_estimation_mode {
+    HWPA_MODEL_FIXED = 0,    /* FixedModelBatchAnalyzer: StaticDetModelConfig model, no DME */
+    HWPA_MODEL_INITIAL = 1,  /* SingleEstimateBatchAnalyzer: first estimate kept forever */
+    HWPA_MODEL_DYNAMIC = 2   /* DynamicEstimateBatchAnalyzer (default) */
+};
+
+/* Raw trace encodings of TraceBatchVariant (dataTypes/TraceBatch.h:97-128); uint8 samples are
+ * zero-extended to int16 before any arithmetic (dataTypes/BatchData.h:160-182). */
+enum hwpa_trace_encoding {
+    HWPA_ENCODING_INT16 = 0,
+    HWPA_ENCODING_UINT8 = 1
+};
+
 typedef struct hwpa_stat_acc {
     float offset[HWPA_LANE_SIZE];
     float m0[HWPA_LANE_SIZE];
@@ -237,6 +252,11 @@ typedef struct hwpa_config {
     uint32_t sandwich_tolerance;
     uint32_t frames_per_hf_metric_block;
     int32_t realtime_activity_labels;
+    /* BasecallerAlgorithmConfig::modelEstimationMode and StaticDetModelConfig
+     * (dataTypes/configs/StaticDetModelConfig.h:43-44) */
+    int32_t model_estimation_mode;
+    float static_baseline_mean;
+    float static_baseline_variance;
     /* Learned initialiser (no reference counterpart): 0 = plain EM. */
     int32_t learned_init;
     float learned_conf_threshold;

@@ -249,9 +249,11 @@ class Analyzer:
             from hw_amd import types as T
             out["metrics"] = np.zeros(L, dtype=T.METRICS)
             r.metrics = _p(out["metrics"])
-        traces = np.ascontiguousarray(traces, dtype=np.int16)
-        rc = lib.orc_analyzer_process(self.h, _p(traces), ctypes.c_uint32(self.pool_id if pool_id is None else pool_id),
-                                      ctypes.c_int32(first_frame), ctypes.byref(r))
+        enc = 1 if np.asarray(traces).dtype == np.uint8 else 0      # HWPA_ENCODING_UINT8 / _INT16
+        traces = np.ascontiguousarray(traces, dtype=np.uint8 if enc else np.int16)
+        rc = lib.orc_analyzer_process_encoded(self.h, _p(traces), ctypes.c_int32(enc),
+                                              ctypes.c_uint32(self.pool_id if pool_id is None else pool_id),
+                                              ctypes.c_int32(first_frame), ctypes.byref(r))
         out["rc"] = rc
         out["flags"] = list(r.flags)
         return out

@@ -2122,6 +2122,7 @@ struct OrcAnalyzer {
     int32_t nextFrame = 0;
     uint32_t baselinerLatency, dmeDelay;
     bool fullEstimation = false;
+    bool modelInitialized = false;
     int status = ST_HIST_INIT;
     int32_t framesRemaining;
     void* bl;
@@ -2335,6 +2336,27 @@ void* orc_analyzer_create(const hwpa_config* cfg, uint32_t pool_id, int32_t lane
     a->metrics = orc_metrics_create(cfg, lanes);
     a->models.resize(lanes);
     std::memset(a->models.data(), 0, sizeof(hwpa_lane_model) * lanes);
+    if (cfg->model_estimation_mode == HWPA_MODEL_FIXED) {
+        // FixedModelBatchAnalyzer ctor (analyzer/BatchAnalyzer.cpp:120-145) with
+        // StaticDetModelConfig::SetupAnalogs (dataTypes/configs/StaticDetModelConfig.h:53-68).
+        hwpa_lane_model& m = a->models[0];
+        const float blMean = cfg->static_baseline_mean, blVar = cfg->static_baseline_variance;
+        const float refSignal = cfg->movie.ref_snr * std::sqrt(blVar);
+        for (int z = 0; z < LS; ++z) {
+            m.baseline.means[z] = hw_f2h(blMean);
+            m.baseline.vars[z] = hw_f2h(blVar);
+        }
+        for (int an = 0; an < NA; ++an) {
+            const float mean = blMean + cfg->movie.analogs[an].rel_amplitude * refSignal;
+            const float cvm = cfg->movie.analogs[an].excess_noise_cv * mean;
+            const float var = blVar + mean + cvm * cvm;   // std::pow(cv * mean, 2.f)
+            for (int z = 0; z < LS; ++z) {
+                m.analogs[an].means[z] = hw_f2h(mean);
+                m.analogs[an].vars[z] = hw_f2h(var);
+            }
+        }
+        for (int l = 1; l < lanes; ++l) a->models[l] = m;
+    }
     a->baselined.resize((size_t)lanes * frames * LS);
     a->labels.resize((size_t)lanes * frames * LS);
     a->latent.resize((size_t)lanes * HWPA_LATENT_FRAMES * LS);
@@ -2357,24 +2379,12 @@ void orc_analyzer_destroy(void* h)
     delete a;
 }
 
-int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id, int32_t first_frame,
-                             orc_result* res)
+// DetectionModelEstimator::AddBatch (traceAnalysis/DetectionModelEstimator.cpp:93-160);
+// *ran tells whether an estimate ran.
+static int32_t OrcDmeAddBatch(OrcAnalyzer* a, int32_t first_frame, bool* ran)
 {
-    auto* a = static_cast<OrcAnalyzer*>(h);
-    if (pool_id != a->poolId) return HWPA_ERR_BAD_POOL;
-    if (first_frame != a->nextFrame) return HWPA_ERR_BAD_FRAME;
     const int L = a->lanes, F = a->frames;
-    res->flags[0] = res->flags[1] = res->flags[2] = res->flags[3] = 0;
-    int32_t rc = orc_baseliner_process(a->bl, traces, F, a->baselined.data(), a->stats.data());
-    if (rc != HWPA_OK) return rc;
-    a->nextFrame = first_frame + F;
-    if (res->baselined) std::memcpy(res->baselined, a->baselined.data(), a->baselined.size() * 2);
-    if (res->bl_stats) std::memcpy(res->bl_stats, a->stats.data(), sizeof(hwpa_baseliner_stats) * L);
-    if (res->pulse_counts) std::memset(res->pulse_counts, 0, sizeof(uint32_t) * L * LS);
-    if (res->pulse_overflows) std::memset(res->pulse_overflows, 0, sizeof(uint32_t) * L * LS);
-    if (first_frame < (int32_t)(a->baselinerLatency + a->dmeDelay)) return HWPA_OK;
-
-    // DetectionModelEstimator::AddBatch
+    *ran = false;
     orc_agg_add(a->agg, a->stats.data());
     a->aggFi.Hull(first_frame, first_frame + F);
     if (a->status != ST_HIST_INIT) {
@@ -2390,7 +2400,7 @@ int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id, i
     if (a->framesRemaining <= 0) {
         if (a->status == ST_DME_INIT) a->status = ST_SEQUENCING;
         if (a->status == ST_SEQUENCING) {
-            res->flags[1] = 1;
+            *ran = true;
             orc_hist_get(a->hist, a->hists.data());
             orc_agg_get(a->agg, a->aggStats.data());
             const int32_t mfi[2] = {a->modelFi.lo, a->modelFi.hi};
@@ -2399,45 +2409,107 @@ int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id, i
             a->modelFi = a->histFi;
         }
         orc_agg_get(a->agg, a->aggStats.data());
-        rc = orc_hist_reset(a->hist, a->aggStats.data());
+        const int32_t rc = orc_hist_reset(a->hist, a->aggStats.data());
         if (rc != HWPA_OK) return rc;
         a->histFi = Interval{};
         orc_agg_reset(a->agg);
         a->aggFi = Interval{};
         if (a->status == ST_HIST_INIT) a->status = ST_DME_INIT;
         a->framesRemaining = (int32_t)a->cfg.min_frames_for_estimate;
-        if (res->flags[1]) a->fullEstimation = true;
+    }
+    return HWPA_OK;
+}
+
+// BatchAnalyzer::operator() (analyzer/BatchAnalyzer.cpp:149-170) and the AnalyzeImpl of the
+// configured mode: FixedModel (:172-205), SingleEstimate (:207-283), DynamicEstimate (:288-373).
+// uint8 samples are zero-extended to int16 first (dataTypes/BatchData.h:160-182).
+int32_t orc_analyzer_process_encoded(void* h, const void* traces, int32_t encoding, uint32_t pool_id,
+                                     int32_t first_frame, orc_result* res)
+{
+    auto* a = static_cast<OrcAnalyzer*>(h);
+    if (encoding != HWPA_ENCODING_INT16 && encoding != HWPA_ENCODING_UINT8) return HWPA_ERR_INVALID_ARG;
+    if (pool_id != a->poolId) return HWPA_ERR_BAD_POOL;
+    if (first_frame != a->nextFrame) return HWPA_ERR_BAD_FRAME;
+    const int L = a->lanes, F = a->frames;
+    const size_t n = (size_t)L * F * LS;
+    std::vector<int16_t> widened;
+    const int16_t* tr = static_cast<const int16_t*>(traces);
+    if (encoding == HWPA_ENCODING_UINT8) {
+        widened.resize(n);
+        const uint8_t* u = static_cast<const uint8_t*>(traces);
+        for (size_t i = 0; i < n; ++i) widened[i] = (int16_t)u[i];
+        tr = widened.data();
+    }
+    res->flags[0] = res->flags[1] = res->flags[2] = res->flags[3] = 0;
+    int32_t rc = orc_baseliner_process(a->bl, tr, F, a->baselined.data(), a->stats.data());
+    if (rc != HWPA_OK) return rc;
+    a->nextFrame = first_frame + F;
+    if (res->baselined) std::memcpy(res->baselined, a->baselined.data(), a->baselined.size() * 2);
+    if (res->bl_stats) std::memcpy(res->bl_stats, a->stats.data(), sizeof(hwpa_baseliner_stats) * L);
+    if (res->pulse_counts) std::memset(res->pulse_counts, 0, sizeof(uint32_t) * L * LS);
+    if (res->pulse_overflows) std::memset(res->pulse_overflows, 0, sizeof(uint32_t) * L * LS);
+
+    const int mode = a->cfg.model_estimation_mode;
+    bool label = true, metrics = true;
+    if (mode == HWPA_MODEL_INITIAL) {
+        if (!a->modelInitialized && first_frame > (int32_t)a->baselinerLatency) {
+            bool ran = false;
+            if ((rc = OrcDmeAddBatch(a, first_frame, &ran)) != HWPA_OK) return rc;
+            res->flags[1] = ran ? 1 : 0;
+            a->modelInitialized = ran;
+        }
+        label = a->modelInitialized;
+    } else if (mode == HWPA_MODEL_DYNAMIC) {
+        if (first_frame < (int32_t)(a->baselinerLatency + a->dmeDelay)) return HWPA_OK;
+        bool ran = false;
+        if ((rc = OrcDmeAddBatch(a, first_frame, &ran)) != HWPA_OK) return rc;
+        res->flags[1] = ran ? 1 : 0;
+        if (ran) a->fullEstimation = true;
+        metrics = a->fullEstimation;
     }
     if (res->models) std::memcpy(res->models, a->models.data(), sizeof(hwpa_lane_model) * L);
 
-    orc_labeler_process(a->lab, a->baselined.data(), F, a->models.data(), a->labels.data(), a->latent.data(),
-                        a->score.data());
-    res->flags[2] = 1;
-    if (res->labels) std::memcpy(res->labels, a->labels.data(), a->labels.size() * 2);
-    if (res->viterbi_score) std::memcpy(res->viterbi_score, a->score.data(), a->score.size() * 4);
-
     const uint32_t mc = a->cfg.max_calls_per_zmw;
     std::vector<hwpa_pulse> pulses((size_t)L * LS * mc);
-    std::vector<uint32_t> counts((size_t)L * LS), ovf((size_t)L * LS);
+    std::vector<uint32_t> counts((size_t)L * LS, 0u), ovf((size_t)L * LS, 0u);
     std::vector<hwpa_stat_acc> pbl(L);
-    orc_pulse_process(a->pulse, a->labels.data(), a->latent.data(), a->baselined.data(), F,
-                      first_frame - HWPA_LATENT_FRAMES, a->models.data(), pulses.data(), counts.data(),
-                      ovf.data(), pbl.data());
-    if (a->fullEstimation) {
-        res->flags[0] = 1;
+    std::memset(pbl.data(), 0, sizeof(hwpa_stat_acc) * L);
+    if (label) {
+        orc_labeler_process(a->lab, a->baselined.data(), F, a->models.data(), a->labels.data(), a->latent.data(),
+                            a->score.data());
+        res->flags[2] = 1;
+        if (res->labels) std::memcpy(res->labels, a->labels.data(), a->labels.size() * 2);
+        if (res->viterbi_score) std::memcpy(res->viterbi_score, a->score.data(), a->score.size() * 4);
+        orc_pulse_process(a->pulse, a->labels.data(), a->latent.data(), a->baselined.data(), F,
+                          first_frame - HWPA_LATENT_FRAMES, a->models.data(), pulses.data(), counts.data(),
+                          ovf.data(), pbl.data());
+    } else {
+        // Empty labeler and pulse-detector metrics: Viterbi score 0 (FrameLabeler::EmptyMetrics),
+        // no baseline frames.
+        std::fill(a->score.begin(), a->score.end(), 0.0f);
+    }
+    if (!metrics) return HWPA_OK;
+    res->flags[0] = label ? 1 : 0;
+    if (label) {
         if (res->pulses) std::memcpy(res->pulses, pulses.data(), pulses.size() * sizeof(hwpa_pulse));
         if (res->pulse_counts) std::memcpy(res->pulse_counts, counts.data(), counts.size() * 4);
         if (res->pulse_overflows) std::memcpy(res->pulse_overflows, ovf.data(), ovf.size() * 4);
         if (res->pulse_bl_stats) std::memcpy(res->pulse_bl_stats, pbl.data(), sizeof(hwpa_stat_acc) * L);
-        // HF metrics of the pulse batch, which ends 24 latent frames before the chunk
-        // (DynamicEstimateBatchAnalyzer::AnalyzeImpl, BatchAnalyzer.cpp:366-372).
-        std::vector<hwpa_basecalling_metrics> met(L);
-        res->flags[3] = orc_metrics_process(a->metrics, pulses.data(), counts.data(), mc, F,
-                                            first_frame + F - HWPA_LATENT_FRAMES, a->stats.data(), a->score.data(),
-                                            pbl.data(), a->models.data(), met.data());
-        if (res->flags[3] && res->metrics) std::memcpy(res->metrics, met.data(), sizeof(hwpa_basecalling_metrics) * L);
     }
+    // HF metrics of the pulse batch, which ends 24 latent frames before the chunk
+    // (BatchAnalyzer.cpp:196-201, 275-280, 366-372).
+    std::vector<hwpa_basecalling_metrics> met(L);
+    res->flags[3] = orc_metrics_process(a->metrics, pulses.data(), counts.data(), mc, F,
+                                        first_frame + F - HWPA_LATENT_FRAMES, a->stats.data(), a->score.data(),
+                                        pbl.data(), a->models.data(), met.data());
+    if (res->flags[3] && res->metrics) std::memcpy(res->metrics, met.data(), sizeof(hwpa_basecalling_metrics) * L);
     return HWPA_OK;
+}
+
+int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id, int32_t first_frame,
+                             orc_result* res)
+{
+    return orc_analyzer_process_encoded(h, traces, HWPA_ENCODING_INT16, pool_id, first_frame, res);
 }
 
 void orc_analyzer_set_activity_tree(void* h, const hwpa_activity_tree* t)

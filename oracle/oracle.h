@@ -99,6 +99,8 @@ float orc_autocorr(const float* x, int32_t n, const int32_t* sizes, int32_t piec
 
 void* orc_analyzer_create(const hwpa_config* cfg, uint32_t pool_id, int32_t lanes, int32_t frames);
 void orc_analyzer_destroy(void* h);
+int32_t orc_analyzer_process_encoded(void* h, const void* traces, int32_t encoding, uint32_t pool_id,
+                                     int32_t first_frame, orc_result* res);
 int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id,
                              int32_t first_frame, orc_result* res);
 void orc_analyzer_set_activity_tree(void* h, const hwpa_activity_tree* t);
