@@ -475,6 +475,30 @@ class Metrics:
             self.h = None
 
 
+class BazEncoder:
+    """BAZ packet encoder of a set of ZMWs (ProductionPulses, or InternalPulses when
+    internal=True) with the per-ZMW StartFrame delta state kept across chunks."""
+
+    def __init__(self, zmws, internal=False):
+        self.h = _vp()
+        self.zmws = zmws
+        check(lib.hwpa_baz_encoder_create(ctypes.c_uint32(zmws), ctypes.c_int32(1 if internal else 0),
+                                          ctypes.byref(self.h)))
+
+    def encode(self, d_pulses, d_counts, max_calls, d_out, capacity, d_offsets, d_events, stream=None):
+        check(lib.hwpa_baz_encode(self.h, ptr(d_pulses), ptr(d_counts), ctypes.c_uint32(max_calls),
+                                  ptr(d_out) if d_out is not None else None, ctypes.c_uint64(capacity),
+                                  ptr(d_offsets), ptr(d_events), _vp(stream)))
+
+    def reset(self, stream=None):
+        check(lib.hwpa_baz_encoder_reset(self.h, _vp(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.hwpa_baz_encoder_destroy(self.h)
+            self.h = None
+
+
 class BatchAnalyzer:
     """Per-pool analyzer with the reference's BatchAnalyzer shape
     (analyzer/BatchAnalyzer.h:51-106): construct once per pool, call once per

@@ -173,6 +173,25 @@ int32_t hwpa_metrics_process(hwpa_metrics* m, const hwpa_pulse* d_pulses, const 
                              const hwpa_stat_acc* d_pd_stats, const hwpa_lane_model* d_models,
                              hwpa_basecalling_metrics* d_out, int32_t* produced, void* stream);
 
+/* ---- BAZ packet encoding of pulses (bazio/encoding/ObjectToBaz.h, FieldTransforms.h,
+ * FieldSerializers.h; layouts basecaller/dataTypes/PulseGroups.h:41-94) --------------
+ * One encoder per set of ZMWs keeps each ZMW's StartFrame delta state across chunks, as the
+ * reference keeps one serializer per ZMW (appModules/PrelimHQFilter.cpp:148-160).
+ * internal_mode 0 = ProductionPulses (Label, PulseWidth, StartFrame; bases only), 1 =
+ * InternalPulses (plus Pkmax, Pkmid, Pkmean, Pkvar, IsBase; every pulse).
+ * hwpa_baz_encode serialises one chunk (d_pulses [zmws][max_calls], d_counts [zmws], as the
+ * analyzer produces them) into one stream ordered by ZMW: d_offsets [zmws + 1] receives each
+ * ZMW's byte offset and the total at [zmws], d_events [zmws] the pulses written per ZMW.  If
+ * the total exceeds `capacity`, nothing is written to d_out and the delta state is unchanged
+ * (a size query: capacity 0). */
+typedef struct hwpa_baz_encoder hwpa_baz_encoder;
+int32_t hwpa_baz_encoder_create(uint32_t zmws, int32_t internal_mode, hwpa_baz_encoder** out);
+int32_t hwpa_baz_encoder_destroy(hwpa_baz_encoder* e);
+int32_t hwpa_baz_encoder_reset(hwpa_baz_encoder* e, void* stream);
+int32_t hwpa_baz_encode(hwpa_baz_encoder* e, const hwpa_pulse* d_pulses, const uint32_t* d_counts,
+                        uint32_t max_calls, uint8_t* d_out, uint64_t capacity, uint64_t* d_offsets,
+                        uint32_t* d_events, void* stream);
+
 /* ---- stage entry points (Device* strategies) ----------------------------- */
 
 /* Baseliner: DeviceMultiScaleBaseliner / HostMultiScaleBaseliner::FilterBaseline

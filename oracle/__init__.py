@@ -303,3 +303,89 @@ def autocorr(x, pieces=1):
         return float(lib.orc_autocorr(_p(x), ctypes.c_int32(len(x)), None, ctypes.c_int32(pieces)))
     sizes = np.ascontiguousarray(pieces, dtype=np.int32)
     return float(lib.orc_autocorr(_p(x), ctypes.c_int32(len(x)), _p(sizes), ctypes.c_int32(len(sizes))))
+
+
+# ---------------------------------------------------------------------------
+# BAZ packet encoding (layouts: 0 ProductionPulses, 1 InternalPulses, 2 lossy
+# truncate, 3 simple overflow, 4 compact overflow)
+# ---------------------------------------------------------------------------
+
+BAZ_TRUNC, BAZ_SIMPLE, BAZ_COMPACT = 0, 1, 2
+_BAZ_FIELDS = (("label", np.uint8), ("start", np.uint32), ("width", np.uint16), ("isbase", np.uint8),
+               ("pkmax", np.float32), ("pkmid", np.float32), ("pkmean", np.float32), ("pkvar", np.float32))
+
+
+def baz_encode_fields(layout, fields, last=0):
+    """fields: dict of per-pulse arrays (label, start, width; optional isbase and the four
+    float signals).  Returns (bytes, last start frame)."""
+    n = len(fields["label"])
+    arrs = {k: (np.ascontiguousarray(fields[k], dtype=dt) if k in fields else None) for k, dt in _BAZ_FIELDS}
+    lst = ctypes.c_uint64(last)
+    ptrs = [(_p(arrs[k]) if arrs[k] is not None else None) for k, _ in _BAZ_FIELDS]
+    lib.orc_baz_encode_fields.restype = ctypes.c_int64
+    size = lib.orc_baz_encode_fields(ctypes.c_int32(layout), ctypes.c_int32(n), *ptrs, ctypes.byref(ctypes.c_uint64(last)),
+                                     None)
+    out = np.zeros(max(size, 1), dtype=np.uint8)
+    lib.orc_baz_encode_fields(ctypes.c_int32(layout), ctypes.c_int32(n), *ptrs, ctypes.byref(lst), _p(out))
+    return out[:size].tobytes(), lst.value
+
+
+def baz_decode_fields(layout, data, n, last=0):
+    """Returns (fields dict, bytes consumed, last start frame)."""
+    buf = np.frombuffer(bytes(data) + b"\0" * 16, dtype=np.uint8).copy()
+    out = {k: np.zeros(n, dtype=dt) for k, dt in _BAZ_FIELDS}
+    lst = ctypes.c_uint64(last)
+    lib.orc_baz_decode_fields.restype = ctypes.c_int64
+    used = lib.orc_baz_decode_fields(ctypes.c_int32(layout), ctypes.c_int32(n), _p(buf), ctypes.byref(lst),
+                                     *[_p(out[k]) for k, _ in _BAZ_FIELDS])
+    return out, int(used), lst.value
+
+
+def baz_layout_groups(layout):
+    bits = np.zeros(16, dtype=np.int32)
+    nf = np.zeros(16, dtype=np.int32)
+    ng = lib.orc_baz_layout_groups(ctypes.c_int32(layout), _p(bits), _p(nf), ctypes.c_int32(16))
+    return list(bits[:ng]), list(nf[:ng])
+
+
+def baz_serial_roundtrip(ser, bits, obytes, signed, v):
+    """(decoded value as uint64, overflow bytes) of one serialiser round trip."""
+    ov = ctypes.c_int32(0)
+    lib.orc_baz_serial_roundtrip.restype = ctypes.c_uint64
+    r = lib.orc_baz_serial_roundtrip(ctypes.c_int32(ser), ctypes.c_int32(bits), ctypes.c_int32(obytes),
+                                     ctypes.c_int32(1 if signed else 0), ctypes.c_uint64(v & (2 ** 64 - 1)),
+                                     ctypes.byref(ov))
+    return r, ov.value
+
+
+def baz_floatfixed(f, signed, scale, nbytes):
+    lib.orc_baz_floatfixed_apply.restype = ctypes.c_uint64
+    lib.orc_baz_floatfixed_revert.restype = ctypes.c_float
+    code = lib.orc_baz_floatfixed_apply(ctypes.c_float(f), ctypes.c_int32(1 if signed else 0), ctypes.c_uint32(scale),
+                                        ctypes.c_int32(nbytes))
+    back = lib.orc_baz_floatfixed_revert(ctypes.c_uint64(code), ctypes.c_int32(1 if signed else 0),
+                                         ctypes.c_uint32(scale))
+    return code, back
+
+
+def baz_encode_batch(internal, pulses, counts, max_calls, last, cap=None):
+    """A chunk of pulse records [zmws * max_calls] into one stream ordered by ZMW.
+    `last` (uint64 [zmws]) is the delta state, updated in place when the stream fits.
+    Returns (stream bytes or None, offsets [zmws + 1], events [zmws])."""
+    zmws = len(counts)
+    offsets = np.zeros(zmws + 1, dtype=np.uint64)
+    events = np.zeros(zmws, dtype=np.uint32)
+    lib.orc_baz_encode_batch.restype = ctypes.c_int64
+    pulses = np.ascontiguousarray(pulses)
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    assert last.dtype == np.uint64 and last.flags.c_contiguous
+    if cap is None:
+        tmp = last.copy()
+        cap = int(lib.orc_baz_encode_batch(ctypes.c_int32(internal), _p(pulses), _p(counts), ctypes.c_uint32(max_calls),
+                                           ctypes.c_int32(zmws), _p(tmp), None, ctypes.c_uint64(0), _p(offsets),
+                                           _p(events)))
+    out = np.zeros(max(cap, 1), dtype=np.uint8)
+    total = int(lib.orc_baz_encode_batch(ctypes.c_int32(internal), _p(pulses), _p(counts), ctypes.c_uint32(max_calls),
+                                         ctypes.c_int32(zmws), _p(last), _p(out), ctypes.c_uint64(cap), _p(offsets),
+                                         _p(events)))
+    return (out[:total].tobytes() if total <= cap else None), offsets, events

@@ -2520,6 +2520,422 @@ void orc_analyzer_set_activity_tree(void* h, const hwpa_activity_tree* t)
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// BAZ packet encoding, restated from the reference's templates at run time
+// (bazio/encoding/ObjectToBaz.h, FieldTransforms.h, FieldSerializers.h): a
+// layout is a list of fields, each with a transform and a serialiser; fields
+// are grouped as GenerateGroups does (ObjectToBaz.h:318-345) and every group is
+// written as ceil(bits/8) little-endian bytes followed by its fields' overflow
+// bytes in field order.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+enum { BZ_LABEL, BZ_START, BZ_WIDTH, BZ_ISBASE, BZ_PKMAX, BZ_PKMID, BZ_PKMEAN, BZ_PKVAR };
+enum { BT_NOOP, BT_DELTA, BT_FLOATFIXED };
+enum { BS_TRUNC, BS_SIMPLE, BS_COMPACT };
+
+struct BzField {
+    int name;
+    bool sgn;
+    int trans;
+    uint32_t scale;
+    int tbytes;
+    int ser;
+    int bits;
+    int obytes;
+};
+
+struct BzLayout {
+    std::vector<BzField> f;
+    std::vector<int> groupStart;  // first field of each group; groupStart.back() == f.size()
+    std::vector<int> groupBytes;
+};
+
+static BzLayout MakeLayout(std::vector<BzField> fields)
+{
+    BzLayout L;
+    L.f = std::move(fields);
+    int cur = 0;
+    L.groupStart.push_back(0);
+    for (size_t k = 0; k < L.f.size(); ++k) {
+        const int next = cur + L.f[k].bits;
+        const bool evenBytes = cur % 8 == 0 && cur > 0;
+        if (evenBytes || next > 64) {
+            L.groupBytes.push_back((cur + 7) / 8);
+            L.groupStart.push_back((int)k);
+            cur = L.f[k].bits;
+        } else {
+            cur = next;
+        }
+    }
+    L.groupBytes.push_back((cur + 7) / 8);
+    L.groupStart.push_back((int)L.f.size());
+    return L;
+}
+
+// Layouts: 0 ProductionPulses, 1 InternalPulses (basecaller/dataTypes/PulseGroups.h:41-94);
+// 2-4 the lossy-truncate, simple-overflow and compact-overflow layouts of the reference's
+// PulseToBaz tests (bazio/encoding/test/TestPulseToBaz.cu:110-300).
+static const BzLayout& Layout(int id)
+{
+    static const BzLayout ls[5] = {
+        MakeLayout({{BZ_LABEL, false, BT_NOOP, 0, 0, BS_TRUNC, 2, 0},
+                    {BZ_WIDTH, false, BT_NOOP, 0, 0, BS_COMPACT, 7, 0},
+                    {BZ_START, false, BT_DELTA, 0, 0, BS_COMPACT, 7, 0}}),
+        MakeLayout({{BZ_LABEL, false, BT_NOOP, 0, 0, BS_TRUNC, 2, 0},
+                    {BZ_WIDTH, false, BT_NOOP, 0, 0, BS_COMPACT, 7, 0},
+                    {BZ_START, false, BT_DELTA, 0, 0, BS_COMPACT, 7, 0},
+                    {BZ_PKMAX, true, BT_FLOATFIXED, 10, 2, BS_SIMPLE, 8, 2},
+                    {BZ_PKMID, true, BT_FLOATFIXED, 10, 2, BS_SIMPLE, 8, 2},
+                    {BZ_PKMEAN, true, BT_FLOATFIXED, 10, 2, BS_SIMPLE, 8, 2},
+                    {BZ_PKVAR, false, BT_FLOATFIXED, 10, 2, BS_SIMPLE, 7, 2},
+                    {BZ_ISBASE, false, BT_NOOP, 0, 0, BS_TRUNC, 1, 0}}),
+        MakeLayout({{BZ_LABEL, false, BT_NOOP, 0, 0, BS_TRUNC, 2, 0},
+                    {BZ_WIDTH, false, BT_NOOP, 0, 0, BS_TRUNC, 7, 0},
+                    {BZ_START, false, BT_DELTA, 0, 0, BS_TRUNC, 7, 0}}),
+        MakeLayout({{BZ_LABEL, false, BT_NOOP, 0, 0, BS_TRUNC, 2, 0},
+                    {BZ_WIDTH, false, BT_NOOP, 0, 0, BS_SIMPLE, 7, 4},
+                    {BZ_START, false, BT_DELTA, 0, 0, BS_SIMPLE, 7, 4}}),
+        MakeLayout({{BZ_LABEL, false, BT_NOOP, 0, 0, BS_TRUNC, 2, 0},
+                    {BZ_WIDTH, false, BT_NOOP, 0, 0, BS_COMPACT, 7, 0},
+                    {BZ_START, false, BT_DELTA, 0, 0, BS_COMPACT, 7, 0}}),
+    };
+    return ls[id];
+}
+
+// FloatFixedCodec special codes and offset (FieldTransforms.h:201-207).
+constexpr uint64_t kFfNan = 0, kFfInf = 1, kFfNinf = 2, kFfOffset = 3;
+
+static uint64_t FloatFixedApply(float f, bool sgn, uint32_t scale, int bytes)
+{
+    float scaled = std::round(f * (float)scale);
+    if (sgn) {
+        const int64_t mx = (int64_t)((1ull << (8 * bytes - 1)) - 1 - kFfOffset);
+        if ((float)(-mx) <= scaled && (float)mx >= scaled) {
+            if (scaled >= 0) scaled += (float)kFfOffset;
+            return (uint64_t)(int64_t)scaled;
+        }
+    } else {
+        const uint64_t mx = (bytes >= 8 ? ~0ull : (1ull << (8 * bytes)) - 1) - kFfOffset;
+        if (0 <= scaled && (float)mx >= scaled) {
+            scaled += (float)kFfOffset;
+            return (uint64_t)scaled;
+        }
+    }
+    if (std::isnan(f)) return kFfNan;
+    return scaled < 0 ? kFfNinf : kFfInf;
+}
+
+static float FloatFixedRevert(uint64_t v, bool sgn, uint32_t scale)
+{
+    if (v == kFfNan) return NAN;
+    if (v == kFfInf) return INFINITY;
+    if (v == kFfNinf) return -INFINITY;
+    if (sgn) {
+        int64_t s = (int64_t)v;
+        if (s >= 0) s -= (int64_t)kFfOffset;
+        return (float)s / (float)scale;
+    }
+    return (float)(v - kFfOffset) / (float)scale;
+}
+
+// Overflow bytes a serialiser needs for v (FieldSerializers.h:57-263).
+static int OverflowBytes(const BzField& f, uint64_t v)
+{
+    if (f.ser == BS_TRUNC) return 0;
+    if (f.ser == BS_SIMPLE) {
+        uint64_t t = v;
+        if (f.sgn) t += (1ull << (f.bits - 1)) - 1;
+        return t >= (1ull << f.bits) - 1 ? f.obytes : 0;
+    }
+    if (f.sgn) {
+        const int discard = __builtin_clrsbl((long)v);
+        const int keep = 64 - discard - (f.bits - 1);
+        return (keep + 6) / 7;
+    }
+    uint64_t t = v >> (f.bits - 1);
+    int n = 0;
+    while (t) {
+        ++n;
+        t >>= 7;
+    }
+    return n;
+}
+
+static uint64_t ToBinary(const BzField& f, uint64_t v, uint8_t*& p, bool write)
+{
+    if (f.ser == BS_TRUNC) return v;
+    if (f.ser == BS_SIMPLE) {
+        if (OverflowBytes(f, v)) {
+            if (write) std::memcpy(p, &v, (size_t)f.obytes);
+            p += f.obytes;
+            return (1ull << f.bits) - 1;
+        }
+        return f.sgn ? v + ((1ull << (f.bits - 1)) - 1) : v;
+    }
+    const uint64_t contMask = 1ull << (f.bits - 1);
+    int n = OverflowBytes(f, v);
+    uint64_t main = v & (contMask - 1);
+    v >>= (f.bits - 1);
+    if (n) main |= contMask;
+    while (n) {
+        uint8_t b = (uint8_t)(v & 0x7F);
+        v >>= 7;
+        --n;
+        if (n) b |= 0x80;
+        if (write) *p = b;
+        ++p;
+    }
+    return main;
+}
+
+static uint64_t FromBinary(const BzField& f, uint64_t v, const uint8_t*& p)
+{
+    if (f.ser == BS_TRUNC) {
+        if (f.sgn && (v & (1ull << (f.bits - 1)))) v |= ~0ull << f.bits;
+        return v;
+    }
+    if (f.ser == BS_SIMPLE) {
+        if (v == (1ull << f.bits) - 1) {
+            uint64_t r = 0;
+            std::memcpy(&r, p, (size_t)f.obytes);
+            p += f.obytes;
+            const int signBit = 8 * f.obytes;
+            if (f.sgn && (r & (1ull << (signBit - 1)))) r |= ~0ull << (signBit - 1);
+            return r;
+        }
+        return f.sgn ? v - ((1ull << (f.bits - 1)) - 1) : v;
+    }
+    const uint64_t contMask = 1ull << (f.bits - 1);
+    uint64_t r = v & (contMask - 1);
+    bool cont = (v & contMask) != 0;
+    int cur = f.bits - 1;
+    while (cont) {
+        const uint64_t b = *p++;
+        cont = (b & 0x80) != 0;
+        r |= (b & 0x7F) << cur;
+        cur += 7;
+    }
+    if (f.sgn && (r & (1ull << (cur - 1)))) r |= ~0ull << cur;
+    return r;
+}
+
+struct BzValues {
+    uint64_t label, start, width, isbase;
+    float pkmax, pkmid, pkmean, pkvar;
+};
+
+static uint64_t Apply(const BzField& f, const BzValues& v, uint64_t& last)
+{
+    uint64_t x = 0;
+    float fx = 0.0f;
+    switch (f.name) {
+    case BZ_LABEL: x = v.label; break;
+    case BZ_START: x = v.start; break;
+    case BZ_WIDTH: x = v.width; break;
+    case BZ_ISBASE: x = v.isbase; break;
+    case BZ_PKMAX: fx = v.pkmax; break;
+    case BZ_PKMID: fx = v.pkmid; break;
+    case BZ_PKMEAN: fx = v.pkmean; break;
+    default: fx = v.pkvar; break;
+    }
+    if (f.trans == BT_FLOATFIXED) return FloatFixedApply(fx, f.sgn, f.scale, f.tbytes);
+    if (f.trans == BT_DELTA) {
+        const uint64_t d = x - last;
+        last = x;
+        return d;
+    }
+    return x;
+}
+
+// ObjectEncoder::Serialize (ObjectToBaz.h:259-265) of one object; write = false only counts.
+static size_t EncodeOne(const BzLayout& L, const BzValues& v, uint64_t& last, uint8_t* out, bool write)
+{
+    uint8_t* p = out;
+    for (size_t g = 0; g + 1 < L.groupStart.size(); ++g) {
+        const int nb = L.groupBytes[g];
+        uint8_t* p2 = p + nb;
+        uint64_t data = 0;
+        int pos = 0;
+        for (int k = L.groupStart[g]; k < L.groupStart[g + 1]; ++k) {
+            const BzField& f = L.f[k];
+            const uint64_t main = ToBinary(f, Apply(f, v, last), p2, write);
+            data |= (main & ((1ull << f.bits) - 1)) << pos;
+            pos += f.bits;
+        }
+        if (write) std::memcpy(p, &data, (size_t)nb);
+        p = p2;
+    }
+    return (size_t)(p - out);
+}
+
+static size_t DecodeOne(const BzLayout& L, const uint8_t* in, uint64_t& last, BzValues& v)
+{
+    const uint8_t* p = in;
+    for (size_t g = 0; g + 1 < L.groupStart.size(); ++g) {
+        const int nb = L.groupBytes[g];
+        uint64_t data = 0;
+        std::memcpy(&data, p, (size_t)nb);
+        p += nb;
+        for (int k = L.groupStart[g]; k < L.groupStart[g + 1]; ++k) {
+            const BzField& f = L.f[k];
+            const uint64_t raw = data & ((1ull << f.bits) - 1);
+            data >>= f.bits;
+            uint64_t x = FromBinary(f, raw, p);
+            if (f.trans == BT_DELTA) {
+                last += x;
+                x = last;
+            }
+            switch (f.name) {
+            case BZ_LABEL: v.label = x; break;
+            case BZ_START: v.start = x; break;
+            case BZ_WIDTH: v.width = x; break;
+            case BZ_ISBASE: v.isbase = x; break;
+            default: {
+                const float fv = f.trans == BT_FLOATFIXED ? FloatFixedRevert(x, f.sgn, f.scale) : (float)x;
+                if (f.name == BZ_PKMAX) v.pkmax = fv;
+                else if (f.name == BZ_PKMID) v.pkmid = fv;
+                else if (f.name == BZ_PKMEAN) v.pkmean = fv;
+                else v.pkvar = fv;
+            }
+            }
+        }
+    }
+    return (size_t)(p - in);
+}
+
+// FieldAccessor<Pulse, PacketFieldName>::Get (basecaller/dataTypes/PulseFieldAccessors.h:40-70)
+// on the fixed-point pulse record (dataTypes/Pulse.h:104-126).
+static BzValues PulseValues(const hwpa_pulse& p)
+{
+    BzValues v;
+    v.label = p.label;
+    v.start = p.start;
+    v.width = p.width;
+    v.isbase = p.is_reject ? 0 : 1;
+    v.pkmax = (float)p.max_signal * 0.1f;
+    v.pkmean = (float)p.mean_signal * 0.1f;
+    v.pkmid = p.width < 3 ? NAN : (float)p.mid_signal * 0.1f;
+    v.pkvar = p.width < 3 ? NAN : p.signal_m2;
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t orc_baz_encode_fields(int32_t layout, int32_t n, const uint8_t* label, const uint32_t* start,
+                              const uint16_t* width, const uint8_t* isbase, const float* pkmax, const float* pkmid,
+                              const float* pkmean, const float* pkvar, uint64_t* last, uint8_t* out)
+{
+    const BzLayout& L = Layout(layout);
+    size_t off = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        BzValues v{label[i], start[i], width[i], isbase ? isbase[i] : 1u,
+                   pkmax ? pkmax[i] : 0.0f, pkmid ? pkmid[i] : 0.0f, pkmean ? pkmean[i] : 0.0f, pkvar ? pkvar[i] : 0.0f};
+        off += EncodeOne(L, v, *last, out ? out + off : nullptr, out != nullptr);
+    }
+    return (int64_t)off;
+}
+
+int64_t orc_baz_decode_fields(int32_t layout, int32_t n, const uint8_t* in, uint64_t* last, uint8_t* label,
+                              uint32_t* start, uint16_t* width, uint8_t* isbase, float* pkmax, float* pkmid,
+                              float* pkmean, float* pkvar)
+{
+    const BzLayout& L = Layout(layout);
+    size_t off = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        BzValues v{0, 0, 0, 1, 0.0f, 0.0f, 0.0f, 0.0f};
+        off += DecodeOne(L, in + off, *last, v);
+        label[i] = (uint8_t)v.label;
+        start[i] = (uint32_t)v.start;
+        width[i] = (uint16_t)v.width;
+        if (isbase) isbase[i] = (uint8_t)v.isbase;
+        if (pkmax) pkmax[i] = v.pkmax;
+        if (pkmid) pkmid[i] = v.pkmid;
+        if (pkmean) pkmean[i] = v.pkmean;
+        if (pkvar) pkvar[i] = v.pkvar;
+    }
+    return (int64_t)off;
+}
+
+// Group bit widths of a layout (GroupParams::numBits / totalBits); returns the group count.
+int32_t orc_baz_layout_groups(int32_t layout, int32_t* group_bits, int32_t* group_fields, int32_t cap)
+{
+    const BzLayout& L = Layout(layout);
+    const int ng = (int)L.groupBytes.size();
+    for (int g = 0; g < ng && g < cap; ++g) {
+        int bits = 0;
+        for (int k = L.groupStart[g]; k < L.groupStart[g + 1]; ++k) bits += L.f[k].bits;
+        group_bits[g] = bits;
+        group_fields[g] = L.groupStart[g + 1] - L.groupStart[g];
+    }
+    return ng;
+}
+
+// One serialiser on its own: ToBinary, the main portion masked to `bits`, FromBinary
+// (the round trips of bazio/encoding/test/TestFieldSerializers.cpp).
+uint64_t orc_baz_serial_roundtrip(int32_t ser, int32_t bits, int32_t obytes, int32_t sgn, uint64_t v,
+                                  int32_t* overflow_bytes)
+{
+    const BzField f{BZ_START, sgn != 0, BT_NOOP, 0, 0, ser, bits, obytes};
+    uint8_t buf[16] = {0};
+    uint8_t* p = buf;
+    if (overflow_bytes) *overflow_bytes = OverflowBytes(f, v);
+    uint64_t main = ToBinary(f, v, p, true) & (bits >= 64 ? ~0ull : (1ull << bits) - 1);
+    const uint8_t* q = buf;
+    return FromBinary(f, main, q);
+}
+
+uint64_t orc_baz_floatfixed_apply(float f, int32_t sgn, uint32_t scale, int32_t bytes)
+{
+    return FloatFixedApply(f, sgn != 0, scale, bytes);
+}
+
+float orc_baz_floatfixed_revert(uint64_t v, int32_t sgn, uint32_t scale)
+{
+    return FloatFixedRevert(v, sgn != 0, scale);
+}
+
+// A chunk of pulses [zmws][max_calls] into one stream ordered by ZMW, as hwpa_baz_encode
+// (production: bases only, PrelimHQFilter.cpp:148-160).  Returns the total; writes nothing
+// and keeps the delta state when it exceeds cap.
+int64_t orc_baz_encode_batch(int32_t internal, const hwpa_pulse* pulses, const uint32_t* counts, uint32_t max_calls,
+                             int32_t zmws, uint64_t* last, uint8_t* out, uint64_t cap, uint64_t* offsets,
+                             uint32_t* events)
+{
+    const BzLayout& L = Layout(internal ? 1 : 0);
+    uint64_t total = 0;
+    for (int32_t z = 0; z < zmws; ++z) {
+        offsets[z] = total;
+        uint64_t prev = last[z];
+        uint32_t ev = 0;
+        const uint32_t n = std::min(counts[z], max_calls);
+        for (uint32_t i = 0; i < n; ++i) {
+            const hwpa_pulse& p = pulses[(size_t)z * max_calls + i];
+            if (!internal && p.is_reject) continue;
+            total += EncodeOne(L, PulseValues(p), prev, nullptr, false);
+            ++ev;
+        }
+        events[z] = ev;
+    }
+    offsets[zmws] = total;
+    if (total > cap) return (int64_t)total;
+    for (int32_t z = 0; z < zmws; ++z) {
+        uint8_t* o = out + offsets[z];
+        const uint32_t n = std::min(counts[z], max_calls);
+        for (uint32_t i = 0; i < n; ++i) {
+            const hwpa_pulse& p = pulses[(size_t)z * max_calls + i];
+            if (!internal && p.is_reject) continue;
+            o += EncodeOne(L, PulseValues(p), last[z], o, true);
+        }
+    }
+    return (int64_t)total;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
 // Hooks used by the known-answer tests
 // ---------------------------------------------------------------------------
 

@@ -109,6 +109,22 @@ void orc_tracegen(const hwpa_tracegen_params* p, int32_t lanes, int32_t lane_off
                   int32_t frames, uint32_t first_frame, hwpa_tracegen_state* state,
                   int32_t init, int16_t* out, int16_t* truth /* nullable: 0 = baseline, a+1 = analog a */);
 
+/* BAZ packet encoding (layouts: 0 ProductionPulses, 1 InternalPulses, 2-4 reference test layouts). */
+int64_t orc_baz_encode_fields(int32_t layout, int32_t n, const uint8_t* label, const uint32_t* start,
+                              const uint16_t* width, const uint8_t* isbase, const float* pkmax, const float* pkmid,
+                              const float* pkmean, const float* pkvar, uint64_t* last, uint8_t* out);
+int64_t orc_baz_decode_fields(int32_t layout, int32_t n, const uint8_t* in, uint64_t* last, uint8_t* label,
+                              uint32_t* start, uint16_t* width, uint8_t* isbase, float* pkmax, float* pkmid,
+                              float* pkmean, float* pkvar);
+int32_t orc_baz_layout_groups(int32_t layout, int32_t* group_bits, int32_t* group_fields, int32_t cap);
+uint64_t orc_baz_serial_roundtrip(int32_t ser, int32_t bits, int32_t obytes, int32_t sgn, uint64_t v,
+                                  int32_t* overflow_bytes);
+uint64_t orc_baz_floatfixed_apply(float f, int32_t sgn, uint32_t scale, int32_t bytes);
+float orc_baz_floatfixed_revert(uint64_t v, int32_t sgn, uint32_t scale);
+int64_t orc_baz_encode_batch(int32_t internal, const hwpa_pulse* pulses, const uint32_t* counts, uint32_t max_calls,
+                             int32_t zmws, uint64_t* last, uint8_t* out, uint64_t cap, uint64_t* offsets,
+                             uint32_t* events);
+
 #ifdef __cplusplus
 }
 #endif
