@@ -188,6 +188,13 @@ def to_numpy(buf, dtype, shape=None):
     return buf.download(dtype, shape)
 
 
+def fetch(dev, dtype, shape):
+    """Copies `shape` elements of `dtype` from a device buffer or borrowed pointer."""
+    sync()
+    n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+    return _dtoh(dev.data_ptr(), n).view(dtype).reshape(shape)
+
+
 def ptr(buf):
     return _vp(buf.data_ptr())
 
@@ -554,6 +561,13 @@ class BatchAnalyzer:
         check(getter(self.h, ctypes.byref(p)))
         self.sync()
         return _dtoh(p.value, nbytes).view(dtype).reshape(shape)
+
+    def pulses_device(self):
+        """(pulses, counts, max_calls) of the last chunk as borrowed device pointers."""
+        pp, pc, po = _vp(), _vp(), _vp()
+        mc = ctypes.c_uint32()
+        check(lib.hwpa_result_pulses(self.h, ctypes.byref(pp), ctypes.byref(pc), ctypes.byref(po), ctypes.byref(mc)))
+        return _Ptr(pp.value), _Ptr(pc.value), mc.value
 
     def labels_device(self):
         """Device pointer of the last labels (valid until the next call)."""

@@ -17,9 +17,9 @@
 // followed by the overflow bytes of its fields in field order.
 //
 // Here the whole chunk becomes one contiguous stream ordered by ZMW: a size
-// pass (one thread per ZMW), an exclusive scan of the sizes, and an encode
-// pass in which every thread writes its ZMW's bytes at its offset and keeps
-// the ZMW's delta state (last start frame) for the next chunk.  When the
+// pass (one wave per lane of 64 ZMWs), an exclusive scan of the lane sizes,
+// and an encode pass in which each wave writes its lane's ZMWs in order and
+// keeps every ZMW's delta state (last start frame) for the next chunk.  When the
 // stream would not fit the caller's buffer nothing is written and the state
 // is left unchanged, so the call can be repeated with a larger buffer.
 
@@ -188,29 +188,115 @@ __device__ __forceinline__ void BazPutPulse(const BazPulseFields& f, uint64_t de
     }
 }
 
-// Size pass: bytes and events per ZMW of this chunk (production: bases only).
-template <bool INTERNAL>
-__global__ __launch_bounds__(256) void BazSizeKernel(const hwpa_pulse* __restrict__ pulses,
+// One lane (64 ZMWs) per wave.  The wave walks the lane's ZMWs in order; for each ZMW
+// thread t handles pulse slot t of a window of 64 slots: the records load coalesced (20 B
+// apart), the StartFrame delta takes the previous selected pulse's start from the lane that
+// holds it (the carried state for the first), and a wave prefix sum of the pulse sizes gives
+// every pulse its byte position, so the ZMW's bytes are written by the wave side by side.
+// ENCODE = false only sums the lane's bytes (size pass).
+template <bool INTERNAL, bool ENCODE>
+__global__ __launch_bounds__(256) void BazLaneKernel(const hwpa_pulse* __restrict__ pulses,
                                                      const uint32_t* __restrict__ counts, uint32_t maxCalls,
-                                                     uint32_t zmws, const uint64_t* __restrict__ last,
-                                                     uint32_t* __restrict__ sizes, uint32_t* __restrict__ events)
+                                                     uint32_t lanes, uint64_t* __restrict__ last,
+                                                     uint32_t* __restrict__ laneBytes,
+                                                     const uint32_t* __restrict__ local,
+                                                     const uint64_t* __restrict__ blockOff, uint8_t* __restrict__ out,
+                                                     uint64_t capacity, uint64_t* __restrict__ offsets,
+                                                     uint32_t* __restrict__ events)
 {
-    const uint32_t z = blockIdx.x * blockDim.x + threadIdx.x;
-    if (z >= zmws) return;
-    const uint32_t n = counts[z] < maxCalls ? counts[z] : maxCalls;
-    const uint32_t* zp = reinterpret_cast<const uint32_t*>(pulses + (size_t)z * maxCalls);
-    uint64_t prev = last[z];
-    uint32_t bytes = 0, ev = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        const BazPulseFields f = BazLoad(zp + 5 * i);
-        if (!INTERNAL && f.reject) continue;
-        const uint64_t s = f.start;
-        bytes += BazPulseBytes<INTERNAL>(f, s - prev);
-        prev = s;
-        ++ev;
+    const uint32_t lane = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t t = threadIdx.x & 63u;
+    if (lane >= lanes) return;
+    uint64_t total = 0, run = 0;
+    if (ENCODE) {
+        total = blockOff[(lanes + kBazScanBlock - 1) / kBazScanBlock];
+        run = blockOff[lane / kBazScanBlock] + local[lane];
+        if (lane == 0 && t == 0) offsets[(size_t)lanes * LS] = total;
     }
-    sizes[z] = bytes;
-    events[z] = ev;
+    const bool write = ENCODE && total <= capacity;
+    const uint64_t below = (t == 0) ? 0ull : (~0ull >> (64u - t));
+    // The lane's counts and delta states, one per thread, broadcast per ZMW below.
+    const size_t zt = (size_t)lane * LS + t;
+    const uint32_t myN = counts[zt] < maxCalls ? counts[zt] : maxCalls;
+    const uint64_t myPrev = last[zt];
+    const uint32_t* lp = reinterpret_cast<const uint32_t*>(pulses + (size_t)lane * LS * maxCalls);
+    const size_t zStride = (size_t)maxCalls * 5u;  // dwords per ZMW
+    auto loadWindow = [&](uint32_t zi, uint32_t base, uint32_t n, uint32_t* r) {
+        const uint32_t i = base + t;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) r[k] = 0u;
+        if (i < n) {
+            const uint32_t* q = lp + zi * zStride + 5u * i;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) r[k] = q[k];
+        }
+    };
+    uint32_t cur[5];
+    uint32_t nCur = __shfl(myN, 0, 64);
+    loadWindow(0, 0, nCur, cur);
+    uint64_t myOff = 0, myLast = 0;
+    uint32_t myEv = 0;
+    for (uint32_t zi = 0; zi < (uint32_t)LS; ++zi) {
+        const uint32_t n = nCur;
+        const uint32_t prevLo = __shfl((uint32_t)myPrev, (int)zi, 64);
+        const uint32_t prevHi = __shfl((uint32_t)(myPrev >> 32), (int)zi, 64);
+        uint64_t prev = ((uint64_t)prevHi << 32) | prevLo;
+        // Next ZMW's first window in flight while this one is encoded.
+        uint32_t nxt[5];
+        const uint32_t zn = zi + 1 < (uint32_t)LS ? zi + 1 : zi;
+        const uint32_t nNext = __shfl(myN, (int)zn, 64);
+        loadWindow(zn, 0, zi + 1 < (uint32_t)LS ? nNext : 0u, nxt);
+        const uint64_t zStart = run;
+        uint32_t ev = 0;
+        for (uint32_t base = 0; base < n; base += 64u) {
+            uint32_t r[5];
+            if (base == 0) {
+#pragma unroll
+                for (int k = 0; k < 5; ++k) r[k] = cur[k];
+            } else {
+                loadWindow(zi, base, n, r);
+            }
+            const uint32_t i = base + t;
+            const BazPulseFields f = BazLoad(r);
+            const bool sel = i < n && (INTERNAL || !f.reject);
+            const uint64_t mask = __ballot(sel);
+            const uint64_t mb = mask & below;
+            const int src = mb ? 63 - __clzll((long long)mb) : 0;
+            const uint32_t srcStart = __shfl(f.start, src, 64);
+            const uint64_t pstart = mb ? (uint64_t)srcStart : prev;
+            const uint64_t delta = (uint64_t)f.start - pstart;
+            const uint32_t bytes = sel ? BazPulseBytes<INTERNAL>(f, delta) : 0u;
+            uint32_t incl = bytes;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(incl, d, 64);
+                if (t >= (uint32_t)d) incl += y;
+            }
+            if (write && sel) {
+                uint8_t* p = out + run + (incl - bytes);
+                BazPutPulse<INTERNAL>(f, delta, p);
+            }
+            if (mask) prev = (uint64_t)__shfl(f.start, 63 - __clzll((long long)mask), 64);
+            run += __shfl(incl, 63, 64);
+            ev += (uint32_t)__popcll(mask);
+        }
+        if (t == zi) {
+            myOff = zStart;
+            myEv = ev;
+            myLast = prev;
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) cur[k] = nxt[k];
+        nCur = nNext;
+    }
+    if (!ENCODE) {
+        if (t == 0) laneBytes[lane] = (uint32_t)run;
+        return;
+    }
+    const size_t z = (size_t)lane * LS + t;
+    offsets[z] = myOff;
+    events[z] = myEv;
+    if (write) last[z] = myLast;
 }
 
 // Block-local exclusive scan of the sizes (1024 per block) and the block totals.
@@ -284,37 +370,6 @@ __global__ __launch_bounds__(kBazScanBlock) void BazScanTotalsKernel(uint64_t* _
     if (threadIdx.x == 0) blockSums[nb] = carry;
 }
 
-// Encode pass: every ZMW's bytes at its offset; offsets[z] (and offsets[zmws] = total)
-// for the caller; the delta state advances only when the whole stream fits.
-template <bool INTERNAL>
-__global__ __launch_bounds__(256) void BazEncodeKernel(const hwpa_pulse* __restrict__ pulses,
-                                                       const uint32_t* __restrict__ counts, uint32_t maxCalls,
-                                                       uint32_t zmws, uint64_t* __restrict__ last,
-                                                       const uint32_t* __restrict__ local,
-                                                       const uint64_t* __restrict__ blockOff, uint8_t* __restrict__ out,
-                                                       uint64_t capacity, uint64_t* __restrict__ offsets)
-{
-    const uint32_t z = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t total = blockOff[(zmws + kBazScanBlock - 1) / kBazScanBlock];
-    if (z == 0) offsets[zmws] = total;
-    if (z >= zmws) return;
-    const uint64_t off = blockOff[z / kBazScanBlock] + local[z];
-    offsets[z] = off;
-    if (total > capacity) return;
-    const uint32_t n = counts[z] < maxCalls ? counts[z] : maxCalls;
-    const uint32_t* zp = reinterpret_cast<const uint32_t*>(pulses + (size_t)z * maxCalls);
-    uint8_t* p = out + off;
-    uint64_t prev = last[z];
-    for (uint32_t i = 0; i < n; ++i) {
-        const BazPulseFields f = BazLoad(zp + 5 * i);
-        if (!INTERNAL && f.reject) continue;
-        const uint64_t s = f.start;
-        BazPutPulse<INTERNAL>(f, s - prev, p);
-        prev = s;
-    }
-    last[z] = prev;
-}
-
 __global__ void BazZeroKernel(uint64_t* p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -329,8 +384,8 @@ struct hwpa_baz_encoder {
     uint32_t zmws = 0;
     int32_t internal = 0;
     uint64_t* last = nullptr;       // per ZMW: last start frame serialised (DeltaCompression state)
-    uint32_t* sizes = nullptr;      // per ZMW bytes of the current chunk
-    uint32_t* local = nullptr;      // block-local exclusive offsets
+    uint32_t* sizes = nullptr;      // per lane bytes of the current chunk
+    uint32_t* local = nullptr;      // block-local exclusive lane offsets
     uint64_t* blockSums = nullptr;  // block offsets, then the total
 };
 
@@ -338,16 +393,18 @@ extern "C" {
 
 int32_t hwpa_baz_encoder_create(uint32_t zmws, int32_t internal_mode, hwpa_baz_encoder** out)
 {
-    if (!out || zmws == 0) return Fail(HWPA_ERR_INVALID_ARG, "baz_encoder_create: bad argument");
+    if (!out || zmws == 0 || zmws % LS != 0)
+        return Fail(HWPA_ERR_INVALID_ARG, "baz_encoder_create: zmws must be a positive multiple of 64");
     *out = nullptr;
     auto* e = new (std::nothrow) hwpa_baz_encoder();
     if (!e) return Fail(HWPA_ERR_OOM, "out of host memory");
     e->zmws = zmws;
     e->internal = internal_mode != 0;
-    const uint32_t nb = (zmws + kBazScanBlock - 1) / kBazScanBlock;
+    const uint32_t lanes = zmws / LS;
+    const uint32_t nb = (lanes + kBazScanBlock - 1) / kBazScanBlock;
     int32_t rc = DeviceAlloc((void**)&e->last, sizeof(uint64_t) * zmws, "baz state");
-    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&e->sizes, sizeof(uint32_t) * zmws, "baz sizes");
-    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&e->local, sizeof(uint32_t) * zmws, "baz offsets");
+    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&e->sizes, sizeof(uint32_t) * lanes, "baz lane sizes");
+    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&e->local, sizeof(uint32_t) * lanes, "baz lane offsets");
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&e->blockSums, sizeof(uint64_t) * (nb + 1), "baz block sums");
     if (rc == HWPA_OK) {
         hipLaunchKernelGGL(BazZeroKernel, dim3((zmws + 255) / 256), dim3(256), 0, nullptr, e->last, zmws);
@@ -386,29 +443,31 @@ int32_t hwpa_baz_encode(hwpa_baz_encoder* e, const hwpa_pulse* d_pulses, const u
     if (!e || !d_pulses || !d_counts || !d_offsets || !d_events || max_calls == 0 || (!d_out && capacity != 0))
         return Fail(HWPA_ERR_INVALID_ARG, "baz_encode: bad argument");
     if (((uintptr_t)d_pulses & 3u) != 0) return Fail(HWPA_ERR_INVALID_ARG, "baz_encode: pulses must be 4-byte aligned");
+    // A lane's bytes are counted in 32 bits: at most 64 x max_calls x 30 bytes.
+    if (max_calls > (1u << 20)) return Fail(HWPA_ERR_INVALID_ARG, "baz_encode: max_calls above 2^20");
     hipStream_t s = AsStream(stream);
-    const uint32_t Z = e->zmws;
-    const uint32_t nb = (Z + kBazScanBlock - 1) / kBazScanBlock;
-    const dim3 g256((Z + 255) / 256);
+    const uint32_t L = e->zmws / (uint32_t)LS;
+    const uint32_t nb = (L + kBazScanBlock - 1) / kBazScanBlock;
+    const dim3 grid((L + 3) / 4), block(256);
     if (e->internal)
-        hipLaunchKernelGGL(BazSizeKernel<true>, g256, dim3(256), 0, s, d_pulses, d_counts, max_calls, Z, e->last,
-                           e->sizes, d_events);
+        hipLaunchKernelGGL((BazLaneKernel<true, false>), grid, block, 0, s, d_pulses, d_counts, max_calls, L, e->last,
+                           e->sizes, nullptr, nullptr, nullptr, 0ull, nullptr, nullptr);
     else
-        hipLaunchKernelGGL(BazSizeKernel<false>, g256, dim3(256), 0, s, d_pulses, d_counts, max_calls, Z, e->last,
-                           e->sizes, d_events);
-    int32_t rc = CheckLaunch("BazSizeKernel");
+        hipLaunchKernelGGL((BazLaneKernel<false, false>), grid, block, 0, s, d_pulses, d_counts, max_calls, L, e->last,
+                           e->sizes, nullptr, nullptr, nullptr, 0ull, nullptr, nullptr);
+    int32_t rc = CheckLaunch("BazLaneKernel (sizes)");
     if (rc != HWPA_OK) return rc;
-    hipLaunchKernelGGL(BazScanBlocksKernel, dim3(nb), dim3(kBazScanBlock), 0, s, e->sizes, Z, e->local, e->blockSums);
+    hipLaunchKernelGGL(BazScanBlocksKernel, dim3(nb), dim3(kBazScanBlock), 0, s, e->sizes, L, e->local, e->blockSums);
     if ((rc = CheckLaunch("BazScanBlocksKernel")) != HWPA_OK) return rc;
     hipLaunchKernelGGL(BazScanTotalsKernel, dim3(1), dim3(kBazScanBlock), 0, s, e->blockSums, nb);
     if ((rc = CheckLaunch("BazScanTotalsKernel")) != HWPA_OK) return rc;
     if (e->internal)
-        hipLaunchKernelGGL(BazEncodeKernel<true>, g256, dim3(256), 0, s, d_pulses, d_counts, max_calls, Z, e->last,
-                           e->local, e->blockSums, d_out, capacity, d_offsets);
+        hipLaunchKernelGGL((BazLaneKernel<true, true>), grid, block, 0, s, d_pulses, d_counts, max_calls, L, e->last,
+                           nullptr, e->local, e->blockSums, d_out, capacity, d_offsets, d_events);
     else
-        hipLaunchKernelGGL(BazEncodeKernel<false>, g256, dim3(256), 0, s, d_pulses, d_counts, max_calls, Z, e->last,
-                           e->local, e->blockSums, d_out, capacity, d_offsets);
-    return CheckLaunch("BazEncodeKernel");
+        hipLaunchKernelGGL((BazLaneKernel<false, true>), grid, block, 0, s, d_pulses, d_counts, max_calls, L, e->last,
+                           nullptr, e->local, e->blockSums, d_out, capacity, d_offsets, d_events);
+    return CheckLaunch("BazLaneKernel (encode)");
 }
 
 }  // extern "C"

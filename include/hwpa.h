@@ -185,6 +185,7 @@ int32_t hwpa_metrics_process(hwpa_metrics* m, const hwpa_pulse* d_pulses, const 
  * the total exceeds `capacity`, nothing is written to d_out and the delta state is unchanged
  * (a size query: capacity 0). */
 typedef struct hwpa_baz_encoder hwpa_baz_encoder;
+/* zmws: a multiple of 64 (whole lanes); max_calls at most 2^20. */
 int32_t hwpa_baz_encoder_create(uint32_t zmws, int32_t internal_mode, hwpa_baz_encoder** out);
 int32_t hwpa_baz_encoder_destroy(hwpa_baz_encoder* e);
 int32_t hwpa_baz_encoder_reset(hwpa_baz_encoder* e, void* stream);

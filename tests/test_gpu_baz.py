@@ -25,9 +25,8 @@ def _run(H, enc, p, counts, mc, cap):
     return data, off, ev
 
 
-@pytest.mark.parametrize("internal", [0, 1])
-def test_baz_random_chunks_bit_exact(H, O, T, gpu, internal):
-    zmws, mc = 300, 16
+@pytest.mark.parametrize("internal,zmws,mc", [(0, 320, 16), (1, 320, 16), (0, 128, 150), (1, 64, 100)])
+def test_baz_random_chunks_bit_exact(H, O, T, gpu, internal, zmws, mc):
     enc = H.BazEncoder(zmws, internal=bool(internal))
     last = np.zeros(zmws, dtype=np.uint64)
     for c, (p, counts) in enumerate(random_chunks(T, zmws, mc, 3, seed=21 + internal)):
@@ -39,7 +38,7 @@ def test_baz_random_chunks_bit_exact(H, O, T, gpu, internal):
 
 
 def test_baz_small_buffer_keeps_state(H, O, T, gpu):
-    zmws, mc = 200, 12
+    zmws, mc = 192, 12
     enc = H.BazEncoder(zmws)
     last = np.zeros(zmws, dtype=np.uint64)
     chunks = random_chunks(T, zmws, mc, 2, seed=5)
