@@ -482,6 +482,29 @@ class Metrics:
             self.h = None
 
 
+class Repacker:
+    """BlockRepacker counterpart: dense sensor packets (device buffers) into a lane-major batch."""
+
+    def __init__(self, max_packets=4096):
+        self.h = _vp()
+        check(lib.hwpa_repacker_create(ctypes.c_uint32(max_packets), ctypes.byref(self.h)))
+
+    def run(self, packets, encoding, d_batch, first_zmw, first_frame, lanes, frames, stream=None):
+        """packets: list of (device buffer, start_zmw, start_frame, blocks, frames, block_width)."""
+        n = len(packets)
+        arr = (T.PacketDesc * n)()
+        for i, (buf, z0, f0, nb, nf, bw) in enumerate(packets):
+            arr[i] = T.PacketDesc(buf.data_ptr(), z0, f0, nb, nf, bw, 0)
+        check(lib.hwpa_repack(self.h, arr, ctypes.c_uint32(n), ctypes.c_int32(encoding), ptr(d_batch),
+                              ctypes.c_uint32(first_zmw), ctypes.c_uint32(first_frame), ctypes.c_uint32(lanes),
+                              ctypes.c_uint32(frames), _vp(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.hwpa_repacker_destroy(self.h)
+            self.h = None
+
+
 class BazEncoder:
     """BAZ packet encoder of a set of ZMWs (ProductionPulses, or InternalPulses when
     internal=True) with the per-ZMW StartFrame delta state kept across chunks."""

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libhwpa.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["hwpa_host.cpp", "baseline.hip", "stats.hip", "histogram.hip", "dme.hip",
            "viterbi.hip", "pulse.hip", "tracegen.hip", "analyzer.hip", "mlp.hip", "tune.hip", "metrics.hip",
-           "baz.hip"]
+           "baz.hip", "repack.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-Wno-unused-result"]

@@ -129,6 +129,12 @@ METRICS = np.dtype([("activity_label", np.uint8, (LANE,)), ("num_pulse_frames",)
 assert METRICS.itemsize == 137 * LANE
 
 
+class PacketDesc(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("start_zmw", ctypes.c_uint32), ("start_frame", ctypes.c_uint32),
+                ("blocks", ctypes.c_uint32), ("frames", ctypes.c_uint32), ("block_width", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
 class ActivityTree(ctypes.Structure):
     _fields_ = [("num_nodes", ctypes.c_int32), ("max_acceptable_hsw_rate", ctypes.c_float),
                 ("hsw_curve", ctypes.c_float * 2), ("children_left", ctypes.c_void_p),

@@ -193,6 +193,20 @@ int32_t hwpa_baz_encode(hwpa_baz_encoder* e, const hwpa_pulse* d_pulses, const u
                         uint32_t max_calls, uint8_t* d_out, uint64_t capacity, uint64_t* d_offsets,
                         uint32_t* d_events, void* stream);
 
+/* ---- block repacker (appModules/BlockRepacker.cpp:200-330, BatchFiller::Process) -----
+ * Copies the parts of n dense packets that overlap a batch [batch_lanes][batch_frames][64]
+ * starting at (batch_first_zmw, batch_first_frame) into it, samples of HWPA_ENCODING_INT16
+ * or HWPA_ENCODING_UINT8 kept as they are.  Packets need 16-byte aligned data and block
+ * widths / start ZMWs of whole 16-byte pieces; the batch starts on a lane.  One launch per
+ * call; calls on one repacker are ordered on one stream, and descriptor slots rotate over 4
+ * calls, so at most 3 earlier calls may still be in flight. */
+typedef struct hwpa_repacker hwpa_repacker;
+int32_t hwpa_repacker_create(uint32_t max_packets, hwpa_repacker** out);
+int32_t hwpa_repacker_destroy(hwpa_repacker* r);
+int32_t hwpa_repack(hwpa_repacker* r, const hwpa_packet_desc* packets, uint32_t n, int32_t encoding, void* d_batch,
+                    uint32_t batch_first_zmw, uint32_t batch_first_frame, uint32_t batch_lanes,
+                    uint32_t batch_frames, void* stream);
+
 /* ---- stage entry points (Device* strategies) ----------------------------- */
 
 /* Baseliner: DeviceMultiScaleBaseliner / HostMultiScaleBaseliner::FilterBaseline

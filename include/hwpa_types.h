@@ -80,6 +80,20 @@ enum hwpa_trace_encoding {
     HWPA_ENCODING_UINT8 = 1
 };
 
+/* A dense SensorPacket (pa-common DataSource::SensorPacket with BLOCK_LAYOUT_DENSE, as the
+ * reference's BlockRepacker takes it, appModules/BlockRepacker.h:41-62): `blocks` blocks of
+ * `frames` x `block_width` samples, [block][frame][zmw], ZMWs start_zmw .. and frames
+ * start_frame .. of the chip.  data is a device pointer. */
+typedef struct hwpa_packet_desc {
+    const void* data;
+    uint32_t start_zmw;
+    uint32_t start_frame;
+    uint32_t blocks;
+    uint32_t frames;
+    uint32_t block_width;
+    uint32_t pad;
+} hwpa_packet_desc;
+
 typedef struct hwpa_stat_acc {
     float offset[HWPA_LANE_SIZE];
     float m0[HWPA_LANE_SIZE];

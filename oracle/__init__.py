@@ -389,3 +389,19 @@ def baz_encode_batch(internal, pulses, counts, max_calls, last, cap=None):
                                          ctypes.c_int32(zmws), _p(last), _p(out), ctypes.c_uint64(cap), _p(offsets),
                                          _p(events)))
     return (out[:total].tobytes() if total <= cap else None), offsets, events
+
+
+def repack(packets, first_zmw, first_frame, lanes, frames, dtype):
+    """BlockRepacker's BatchFiller::Process (appModules/BlockRepacker.cpp:255-330) restated:
+    packets = [(samples [blocks, packet_frames, block_width], start_zmw, start_frame)];
+    returns the batch [lanes, frames, 64] with the overlapping samples copied (zeros elsewhere)."""
+    out = np.zeros((frames, lanes * LANE), dtype=dtype)
+    for data, z0, f0 in packets:
+        nb, pf, bw = data.shape
+        rows = np.ascontiguousarray(data.transpose(1, 0, 2)).reshape(pf, nb * bw)   # [frame][zmw]
+        za, zb = max(z0, first_zmw), min(z0 + nb * bw, first_zmw + lanes * LANE)
+        fa, fb = max(f0, first_frame), min(f0 + pf, first_frame + frames)
+        if za >= zb or fa >= fb:
+            continue
+        out[fa - first_frame:fb - first_frame, za - first_zmw:zb - first_zmw] = rows[fa - f0:fb - f0, za - z0:zb - z0]
+    return np.ascontiguousarray(out.reshape(frames, lanes, LANE).transpose(1, 0, 2))
