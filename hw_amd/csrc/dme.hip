@@ -811,8 +811,6 @@ void LearnedWsFree(LearnedWs& w)
 {
     DeviceFree(w.x);
     DeviceFree(w.y);
-    DeviceFree(w.h1);
-    DeviceFree(w.h2);
     DeviceFree(w.fallback);
     DeviceFree(w.cnt);
     DeviceFree(w.off);
@@ -834,7 +832,6 @@ int32_t LearnedWsAlloc(LearnedWs& w, uint32_t lanes)
         size_t n;
     } req[] = {
         {(void**)&w.x, Z * HWPA_MLP_IN * 2},      {(void**)&w.y, Z * HWPA_MLP_OUT * 4},
-        {(void**)&w.h1, Z * HWPA_MLP_HIDDEN * 2}, {(void**)&w.h2, Z * HWPA_MLP_HIDDEN * 2},
         {(void**)&w.fallback, Z},                 {(void**)&w.cnt, (size_t)lanes * 4},
         {(void**)&w.off, (size_t)lanes * 4},      {(void**)&w.total, 16},
         {(void**)&w.map, Z * 4},                  {(void**)&w.cHist, (size_t)lanes * sizeof(hwpa_lane_hist)},
@@ -862,7 +859,7 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
     const DmeDevCfg d = MakeDevCfg(cfg);
     const float evolve = EvolveFactor(d, model_fi, est_fi);
     int32_t rc = MlpFeatures(hist, lanes, w.x, s);
-    if (rc == HWPA_OK) rc = MlpForward(mlp, w.x, lanes * LS, w.y, w.h1, w.h2, s);
+    if (rc == HWPA_OK) rc = MlpForward(mlp, w.x, lanes * LS, w.y, nullptr, nullptr, s);
     if (rc != HWPA_OK) return rc;
     EmExtra x{};
     x.init = w.y;

@@ -94,8 +94,6 @@ struct LearnedWs {
     uint32_t lanes = 0;
     uint16_t* x = nullptr;            // [Z][128] bf16 network input
     float* y = nullptr;               // [Z][11] network output
-    uint16_t* h1 = nullptr;           // [Z][256] bf16 hidden activations
-    uint16_t* h2 = nullptr;
     uint8_t* fallback = nullptr;      // [Z]
     uint32_t* cnt = nullptr;          // [lanes] fallback ZMWs per lane
     uint32_t* off = nullptr;          // [lanes] exclusive prefix

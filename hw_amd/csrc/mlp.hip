@@ -470,31 +470,117 @@ static int32_t ColSum(const void* d, bool bf16, uint32_t rows, uint32_t cols, fl
     return CheckLaunch("ColSumFinalKernel");
 }
 
+// Fused forward pass: 32 rows per wave, 4 waves per workgroup.  Each layer is the
+// 32x32x16 bf16 MFMA over 32-column tiles with k ascending -- the arithmetic of
+// GemmKernel, so outputs equal the three-GEMM pass bit for bit -- but the hidden
+// activations stay in LDS (written transposed into row-major tiles, read back as the
+// next layer's A operand) instead of HBM; h1 / h2 are written out only when given
+// (training keeps them for the backward pass).  Weights come from L2.
+constexpr int kFusedRows = 32, kFusedWaves = 4, kHsStride = MH + 8;
+
+__device__ __forceinline__ bf16x8 LoadRow8(const uint16_t* __restrict__ p)
+{
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+
+__global__ __launch_bounds__(256) void MlpFusedForwardKernel(const uint16_t* __restrict__ x, uint32_t rows,
+                                                             const uint16_t* __restrict__ Wb,
+                                                             const float* __restrict__ P, float* __restrict__ y,
+                                                             uint16_t* __restrict__ h1out, uint16_t* __restrict__ h2out)
+{
+    __shared__ __align__(16) uint16_t hs[kFusedWaves][kFusedRows][kHsStride];
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int r = l & 31, kq = (l >> 5) * 8;
+    const int64_t m0 = ((int64_t)blockIdx.x * kFusedWaves + wave) * kFusedRows;
+    const bool rowValid = m0 + r < (int64_t)rows;
+    uint16_t (*h)[kHsStride] = hs[wave];
+    const bf16x8 zero8 = __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
+
+    // Output row of accumulator element i of this lane (f32x16 layout of the 32x32 MFMA).
+    auto outRow = [&](int i) { return (i >> 2) * 8 + (l >> 5) * 4 + (i & 3); };
+
+    // ---- layer 1: [32 x MI] x [MI x MH]
+    f32x16 acc[MH / 32];
+#pragma unroll
+    for (int j = 0; j < MH / 32; ++j) acc[j] = f32x16{};
+    const uint16_t* xr = x + (m0 + r) * MI;
+    for (int k0 = 0; k0 < MI; k0 += 16) {
+        const bf16x8 a = rowValid ? LoadRow8(xr + k0 + kq) : zero8;
+        bf16x8 b[MH / 32];
+#pragma unroll
+        for (int j = 0; j < MH / 32; ++j) b[j] = LoadRow8(Wb + OFF_W1 + (size_t)(j * 32 + r) * MI + k0 + kq);
+#pragma unroll
+        for (int j = 0; j < MH / 32; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < MH / 32; ++j) {
+        const int n = j * 32 + r;
+        const float bias = P[OFF_B1 + n];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const float zv = acc[j][i] + bias;
+            const uint16_t hv = hw_f2bf(zv > 0.0f ? zv : 0.0f);
+            const int mr = outRow(i);
+            h[mr][n] = hv;
+            if (h1out && m0 + mr < (int64_t)rows) h1out[(m0 + mr) * MH + n] = hv;
+        }
+    }
+    __syncthreads();
+
+    // ---- layer 2: [32 x MH] x [MH x MH]
+#pragma unroll
+    for (int j = 0; j < MH / 32; ++j) acc[j] = f32x16{};
+    for (int k0 = 0; k0 < MH; k0 += 16) {
+        const bf16x8 a = LoadRow8(&h[r][k0 + kq]);
+        bf16x8 b[MH / 32];
+#pragma unroll
+        for (int j = 0; j < MH / 32; ++j) b[j] = LoadRow8(Wb + OFF_W2 + (size_t)(j * 32 + r) * MH + k0 + kq);
+#pragma unroll
+        for (int j = 0; j < MH / 32; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < MH / 32; ++j) {
+        const int n = j * 32 + r;
+        const float bias = P[OFF_B2 + n];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const float zv = acc[j][i] + bias;
+            const uint16_t hv = hw_f2bf(zv > 0.0f ? zv : 0.0f);
+            const int mr = outRow(i);
+            h[mr][n] = hv;
+            if (h2out && m0 + mr < (int64_t)rows) h2out[(m0 + mr) * MH + n] = hv;
+        }
+    }
+    __syncthreads();
+
+    // ---- layer 3: [32 x MH] x [MH x MO], one 32-column tile of which MO columns are real
+    f32x16 out = f32x16{};
+    for (int k0 = 0; k0 < MH; k0 += 16) {
+        const bf16x8 a = LoadRow8(&h[r][k0 + kq]);
+        const bf16x8 b = r < MO ? LoadRow8(Wb + OFF_W3 + (size_t)r * MH + k0 + kq) : zero8;
+        out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, out, 0, 0, 0);
+    }
+    if (r < MO) {
+        const float bias = P[OFF_B3 + r];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int64_t m = m0 + outRow(i);
+            if (m < (int64_t)rows) y[m * MO + r] = out[i] + bias;
+        }
+    }
+}
+
 // Forward pass over `rows` inputs through the activation buffers h1, h2 into y.
 static int32_t Forward(const hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, uint16_t* h1,
                        uint16_t* h2, hipStream_t s)
 {
-    const uint16_t* Wb = m->Pb;
-    GemmArgs g = MakeGemm((int)rows, MH, MI, x, MI, 1, Wb + OFF_W1, 1, MI);
-    g.bias = m->P + OFF_B1;
-    g.Cb = h1;
-    g.c_m = MH;
-    g.c_n = 1;
-    int32_t rc = Gemm<true, true, EPI_BIAS_RELU_BF16>(g, s);
-    if (rc != HWPA_OK) return rc;
-    g = MakeGemm((int)rows, MH, MH, h1, MH, 1, Wb + OFF_W2, 1, MH);
-    g.bias = m->P + OFF_B2;
-    g.Cb = h2;
-    g.c_m = MH;
-    g.c_n = 1;
-    rc = Gemm<true, true, EPI_BIAS_RELU_BF16>(g, s);
-    if (rc != HWPA_OK) return rc;
-    g = MakeGemm((int)rows, MO, MH, h2, MH, 1, Wb + OFF_W3, 1, MH);
-    g.bias = m->P + OFF_B3;
-    g.Cf = y;
-    g.c_m = MO;
-    g.c_n = 1;
-    return Gemm<true, true, EPI_BIAS_F32>(g, s);
+    if (rows == 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: no rows");
+    if (((uintptr_t)x & 15) != 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: input not 16-byte aligned");
+    const uint32_t per = (uint32_t)(kFusedRows * kFusedWaves);
+    hipLaunchKernelGGL(MlpFusedForwardKernel, dim3((rows + per - 1) / per), dim3(256), 0, s, x, rows, m->Pb, m->P, y,
+                       h1, h2);
+    return CheckLaunch("MlpFusedForwardKernel");
 }
 
 // Loss and gradients of the mean loss over a batch; gradients into m->G.
@@ -718,9 +804,8 @@ int32_t hwpa_mlp_forward(hwpa_initializer* m, const uint16_t* d_x, uint32_t rows
 {
     if (!m || !d_x || !d_y || rows == 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: null argument or no rows");
     if (((uintptr_t)d_x & 15) != 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: input must be 16-byte aligned");
-    int32_t rc = EnsureRows(m, rows);
-    if (rc != HWPA_OK) return rc;
-    return Forward(m, d_x, rows, d_y, m->H1, m->H2, (hipStream_t)stream);
+    // Inference keeps the hidden activations on chip (the fused kernel writes them only for training).
+    return Forward(m, d_x, rows, d_y, nullptr, nullptr, (hipStream_t)stream);
 }
 
 int32_t hwpa_mlp_gradients(hwpa_initializer* m, const uint16_t* d_x, const float* d_t, const uint8_t* d_valid,
