@@ -370,6 +370,9 @@ class Initializer:
     def forward(self, d_x, rows, d_y, stream=None):
         check(lib.hwpa_mlp_forward(self.h, ptr(d_x), ctypes.c_uint32(rows), ptr(d_y), _vp(stream)))
 
+    def forward_hist(self, d_hist, lanes, d_y, stream=None):
+        check(lib.hwpa_mlp_forward_hist(self.h, ptr(d_hist), ctypes.c_uint32(lanes), ptr(d_y), _vp(stream)))
+
     def gradients(self, d_x, d_t, batch, d_grads, d_loss, d_valid=None, stream=None):
         check(lib.hwpa_mlp_gradients(self.h, ptr(d_x), ptr(d_t), _opt(d_valid), ctypes.c_uint32(batch), ptr(d_grads),
                                      ptr(d_loss), _vp(stream)))

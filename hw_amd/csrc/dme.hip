@@ -809,7 +809,6 @@ int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hw
 
 void LearnedWsFree(LearnedWs& w)
 {
-    DeviceFree(w.x);
     DeviceFree(w.y);
     DeviceFree(w.fallback);
     DeviceFree(w.cnt);
@@ -831,7 +830,7 @@ int32_t LearnedWsAlloc(LearnedWs& w, uint32_t lanes)
         void** p;
         size_t n;
     } req[] = {
-        {(void**)&w.x, Z * HWPA_MLP_IN * 2},      {(void**)&w.y, Z * HWPA_MLP_OUT * 4},
+        {(void**)&w.y, Z * HWPA_MLP_OUT * 4},
         {(void**)&w.fallback, Z},                 {(void**)&w.cnt, (size_t)lanes * 4},
         {(void**)&w.off, (size_t)lanes * 4},      {(void**)&w.total, 16},
         {(void**)&w.map, Z * 4},                  {(void**)&w.cHist, (size_t)lanes * sizeof(hwpa_lane_hist)},
@@ -858,8 +857,7 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
     if (cfg.learned_em_iteration_limit == 0) return Fail(HWPA_ERR_BAD_CONFIG, "learned EM iteration limit must be positive");
     const DmeDevCfg d = MakeDevCfg(cfg);
     const float evolve = EvolveFactor(d, model_fi, est_fi);
-    int32_t rc = MlpFeatures(hist, lanes, w.x, s);
-    if (rc == HWPA_OK) rc = MlpForward(mlp, w.x, lanes * LS, w.y, nullptr, nullptr, s);
+    int32_t rc = MlpForwardFromHist(mlp, hist, lanes, w.y, s);
     if (rc != HWPA_OK) return rc;
     EmExtra x{};
     x.init = w.y;

@@ -92,7 +92,6 @@ int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hw
 // for up to `lanes` lanes.
 struct LearnedWs {
     uint32_t lanes = 0;
-    uint16_t* x = nullptr;            // [Z][128] bf16 network input
     float* y = nullptr;               // [Z][11] network output
     uint8_t* fallback = nullptr;      // [Z]
     uint32_t* cnt = nullptr;          // [lanes] fallback ZMWs per lane
@@ -110,6 +109,9 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
                            const hwpa_baseliner_stats* stats, hwpa_lane_model* models, const int32_t model_fi[2],
                            const int32_t est_fi[2], uint32_t lanes, int32_t* d_iters, hipStream_t s);
 int32_t MlpFeatures(const hwpa_lane_hist* hist, uint32_t lanes, uint16_t* x, hipStream_t s);
+// Network outputs for every ZMW of `lanes` lane histograms (features computed in the kernel).
+int32_t MlpForwardFromHist(const hwpa_initializer* m, const hwpa_lane_hist* hist, uint32_t lanes, float* y,
+                           hipStream_t s);
 int32_t MlpForward(const hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, uint16_t* h1, uint16_t* h2,
                    hipStream_t s);
 const float* MlpOutputMap(const hwpa_initializer* m);  // device {off[6], scale[6], w[7]}

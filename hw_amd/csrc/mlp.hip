@@ -483,7 +483,14 @@ __device__ __forceinline__ bf16x8 LoadRow8(const uint16_t* __restrict__ p)
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
 }
 
-__global__ __launch_bounds__(256) void MlpFusedForwardKernel(const uint16_t* __restrict__ x, uint32_t rows,
+// HIST: the inputs are computed from the lane histograms as FeaturesKernel does (row =
+// lane * 64 + ZMW), so inference reads the counters once and writes no input matrix.  Each
+// half-wave needs half of the 128 re-binned inputs of its row (k-chunks kq = 0 / 8 of every
+// 16-wide k-step): it sums exactly those bin pairs, the two halves' partial totals give the
+// row total, and the scaled bf16 inputs stay in registers.
+template <bool HIST>
+__global__ __launch_bounds__(256) void MlpFusedForwardKernel(const uint16_t* __restrict__ x,
+                                                             const hwpa_lane_hist* __restrict__ hists, uint32_t rows,
                                                              const uint16_t* __restrict__ Wb,
                                                              const float* __restrict__ P, float* __restrict__ y,
                                                              uint16_t* __restrict__ h1out, uint16_t* __restrict__ h2out)
@@ -499,13 +506,45 @@ __global__ __launch_bounds__(256) void MlpFusedForwardKernel(const uint16_t* __r
     // Output row of accumulator element i of this lane (f32x16 layout of the 32x32 MFMA).
     auto outRow = [&](int i) { return (i >> 2) * 8 + (l >> 5) * 4 + (i & 3); };
 
+    // ---- inputs from the histogram (HIST)
+    uint32_t xin[MI / 16][4];
+    if (HIST) {
+        uint32_t pair[MI / 16][8];
+        uint32_t part = 0;
+        const int64_t row = rowValid ? m0 + r : 0;
+        const uint16_t* bc = &hists[row / LS].bin_count[0][row % LS];
+#pragma unroll
+        for (int s = 0; s < MI / 16; ++s)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int i = 16 * s + kq + j;
+                const uint32_t v = rowValid ? (uint32_t)bc[(2 * i) * LS] + (uint32_t)bc[(2 * i + 1) * LS] : 0u;
+                pair[s][j] = v;
+                part += v;
+            }
+        const uint32_t total = part + (uint32_t)__shfl_xor((int)part, 32, 64);
+        const float scale = total ? 16.0f / (float)total : 0.0f;
+#pragma unroll
+        for (int s = 0; s < MI / 16; ++s)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                xin[s][q] = (uint32_t)hw_f2bf((float)pair[s][2 * q] * scale) |
+                            ((uint32_t)hw_f2bf((float)pair[s][2 * q + 1] * scale) << 16);
+    }
+
     // ---- layer 1: [32 x MI] x [MI x MH]
     f32x16 acc[MH / 32];
 #pragma unroll
     for (int j = 0; j < MH / 32; ++j) acc[j] = f32x16{};
-    const uint16_t* xr = x + (m0 + r) * MI;
+    const uint16_t* xr = HIST ? nullptr : x + (m0 + r) * MI;
+#pragma unroll
     for (int k0 = 0; k0 < MI; k0 += 16) {
-        const bf16x8 a = rowValid ? LoadRow8(xr + k0 + kq) : zero8;
+        bf16x8 a;
+        if (HIST)
+            a = __builtin_bit_cast(bf16x8, make_uint4(xin[k0 / 16][0], xin[k0 / 16][1], xin[k0 / 16][2],
+                                                      xin[k0 / 16][3]));
+        else
+            a = rowValid ? LoadRow8(xr + k0 + kq) : zero8;
         bf16x8 b[MH / 32];
 #pragma unroll
         for (int j = 0; j < MH / 32; ++j) b[j] = LoadRow8(Wb + OFF_W1 + (size_t)(j * 32 + r) * MI + k0 + kq);
@@ -578,8 +617,20 @@ static int32_t Forward(const hwpa_initializer* m, const uint16_t* x, uint32_t ro
     if (rows == 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: no rows");
     if (((uintptr_t)x & 15) != 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: input not 16-byte aligned");
     const uint32_t per = (uint32_t)(kFusedRows * kFusedWaves);
-    hipLaunchKernelGGL(MlpFusedForwardKernel, dim3((rows + per - 1) / per), dim3(256), 0, s, x, rows, m->Pb, m->P, y,
-                       h1, h2);
+    hipLaunchKernelGGL(MlpFusedForwardKernel<false>, dim3((rows + per - 1) / per), dim3(256), 0, s, x, nullptr, rows,
+                       m->Pb, m->P, y, h1, h2);
+    return CheckLaunch("MlpFusedForwardKernel");
+}
+
+// Inference straight from lane histograms: FeaturesKernel + Forward in one launch.
+static int32_t ForwardFromHist(const hwpa_initializer* m, const hwpa_lane_hist* hist, uint32_t lanes, float* y,
+                               hipStream_t s)
+{
+    if (lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: no lanes");
+    const uint32_t rows = lanes * (uint32_t)LS;
+    const uint32_t per = (uint32_t)(kFusedRows * kFusedWaves);
+    hipLaunchKernelGGL(MlpFusedForwardKernel<true>, dim3((rows + per - 1) / per), dim3(256), 0, s, nullptr, hist, rows,
+                       m->Pb, m->P, y, nullptr, nullptr);
     return CheckLaunch("MlpFusedForwardKernel");
 }
 
@@ -669,6 +720,12 @@ int32_t MlpForward(const hwpa_initializer* m, const uint16_t* x, uint32_t rows, 
                    hipStream_t s)
 {
     return Forward(m, x, rows, y, h1, h2, s);
+}
+
+int32_t MlpForwardFromHist(const hwpa_initializer* m, const hwpa_lane_hist* hist, uint32_t lanes, float* y,
+                           hipStream_t s)
+{
+    return ForwardFromHist(m, hist, lanes, y, s);
 }
 
 }  // namespace hwpa
@@ -806,6 +863,13 @@ int32_t hwpa_mlp_forward(hwpa_initializer* m, const uint16_t* d_x, uint32_t rows
     if (((uintptr_t)d_x & 15) != 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: input must be 16-byte aligned");
     // Inference keeps the hidden activations on chip (the fused kernel writes them only for training).
     return Forward(m, d_x, rows, d_y, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int32_t hwpa_mlp_forward_hist(hwpa_initializer* m, const hwpa_lane_hist* d_hist, uint32_t lanes, float* d_y,
+                              void* stream)
+{
+    if (!m || !d_hist || !d_y || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: null argument or no lanes");
+    return ForwardFromHist(m, d_hist, lanes, d_y, (hipStream_t)stream);
 }
 
 int32_t hwpa_mlp_gradients(hwpa_initializer* m, const uint16_t* d_x, const float* d_t, const uint8_t* d_valid,

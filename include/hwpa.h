@@ -325,6 +325,10 @@ int32_t hwpa_mlp_targets(const hwpa_lane_hist* d_hist, const float* d_diag, uint
                          float min_confidence, float* d_t, uint8_t* d_valid, void* stream);
 /* Network outputs fp32 [rows][11]. */
 int32_t hwpa_mlp_forward(hwpa_initializer* m, const uint16_t* d_x, uint32_t rows, float* d_y, void* stream);
+/* Network outputs fp32 [lanes*64][11] straight from lane histograms: equal to
+ * hwpa_mlp_features followed by hwpa_mlp_forward, without the input matrix. */
+int32_t hwpa_mlp_forward_hist(hwpa_initializer* m, const hwpa_lane_hist* d_hist, uint32_t lanes, float* d_y,
+                              void* stream);
 /* Mean loss over a batch (squared error on outputs 0-5, softmax cross-entropy
  * on 6-10) and its gradient, fp32 [HWPA_MLP_PARAMS] in parameter order.
  * Samples with d_valid[i] == 0 (d_valid may be NULL) contribute nothing; the

@@ -14,8 +14,11 @@ STEPS = 9
 def _pools(H):
     cfg_def = H.default_config()
     cfg_def.sigma_ema_scale_strides = 0.0          # short start-up, estimates every 4000 frames
-    return [(7, 5, 256, H.bench_config(256)), (8, 3, 512, H.bench_config(512)),
-            (9, 4, 1024, H.bench_config(1024)), (10, 2, 512, cfg_def)]
+    pools = [(7, 5, 256, H.bench_config(256)), (8, 3, 512, H.bench_config(512)),
+             (9, 4, 1024, H.bench_config(1024)), (10, 2, 512, cfg_def)]
+    for p in pools:
+        p[3].frames_per_hf_metric_block = 1024    # metrics blocks end inside the run (graph replays with resets)
+    return pools
 
 
 def _chunks(H, T, pools):
@@ -35,8 +38,9 @@ def _chunks(H, T, pools):
 
 def _snapshot(a):
     counts = a.pulses()[1]
-    return {"flags": a.flags(), "labels": a.labels(), "models": a.models().tobytes(),
-            "baselined": a.baselined(), "counts": counts}
+    f = a.flags()
+    return {"flags": f, "labels": a.labels(), "models": a.models().tobytes(),
+            "baselined": a.baselined(), "counts": counts, "metrics": a.metrics().tobytes() if f[3] else None}
 
 
 def _alone(H, pools, chunks, init=None):
@@ -73,7 +77,9 @@ def _same(x, y):
             if a["flags"][2]:      # labels (and so models) exist after the start-up window
                 assert a["models"] == b["models"], (i, c)
                 assert np.array_equal(a["labels"], b["labels"]), (i, c)
+            assert a["metrics"] == b["metrics"], (i, c)
     assert sum(x[i][c]["flags"][2] for i in range(len(x)) for c in range(STEPS)) >= len(x) * 3
+    assert sum(x[i][c]["flags"][3] for i in range(len(x)) for c in range(STEPS)) >= 4
 
 
 @pytest.mark.parametrize("graph", [True, False])

@@ -70,6 +70,30 @@ def test_features_match_oracle(H, T, gpu):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_forward_from_histograms_equals_features_then_forward(H, T, gpu, seed):
+    # The inference kernel computes the inputs from the lane histograms itself; its
+    # outputs equal features + forward bit for bit (random, not integer, weights).
+    rng = np.random.default_rng(10 + seed)
+    lanes = 5
+    hist = np.zeros(lanes, dtype=T.LANE_HIST)
+    hist["bin_count"] = rng.integers(0, 90, size=(lanes, 300, 64)).astype(np.uint16)
+    hist["bin_count"][0, :, 7] = 0           # an empty ZMW
+    hist["bin_count"][3, :, :] //= 7
+    init = H.Initializer(seed=3 + seed)
+    d_h = H.dev_from_numpy(hist)
+    d_x = H.dev_bytes(lanes * 64 * M.IN * 2)
+    d_y1, d_y2 = H.dev_bytes(lanes * 64 * M.OUT * 4), H.dev_bytes(lanes * 64 * M.OUT * 4)
+    H.mlp_features(d_h, lanes, d_x)
+    init.forward(d_x, lanes * 64, d_y1)
+    init.forward_hist(d_h, lanes, d_y2)
+    H.sync()
+    y1 = H.to_numpy(d_y1, np.float32, (lanes * 64, M.OUT))
+    y2 = H.to_numpy(d_y2, np.float32, (lanes * 64, M.OUT))
+    assert np.abs(y1).max() > 0
+    assert np.array_equal(y1.view(np.uint32), y2.view(np.uint32))
+
+
 def _realistic_batch(rng, B):
     # histogram-like non-negative inputs and plausible targets
     x = np.zeros((B, M.IN), dtype=np.float32)
