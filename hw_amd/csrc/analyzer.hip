@@ -46,7 +46,7 @@ enum : uint32_t {
     kPlanCopyIn = 1u << 0, kPlanEarlyZero = 1u << 1, kPlanAggregate = 1u << 2, kPlanHistAdd = 1u << 3,
     kPlanDmeInit = 1u << 4, kPlanEstimate = 1u << 5, kPlanLearned = 1u << 6, kPlanCapture = 1u << 7,
     kPlanReset = 1u << 8, kPlanLabels = 1u << 9, kPlanPulses = 1u << 10, kPlanLateZero = 1u << 11,
-    kPlanEvents = 1u << 12, kPlanMetrics = 1u << 13, kPlanMetricsEnd = 1u << 14, kPlanUint8 = 1u << 15,
+    kPlanEvents = 1u << 12, kPlanMetrics = 1u << 13, kPlanUint8 = 1u << 15,
     kPlanEmptyPulses = 1u << 16
 };
 
@@ -372,7 +372,7 @@ static int32_t AnalyzerProcess(hwpa_analyzer* a, const void* traces, int32_t enc
                                          a->dModels, a->dMetrics, &produced, s);
         if (r != HWPA_OK) return r;
         a->flags[3] = produced;
-        a->plan |= kPlanMetrics | (produced ? kPlanMetricsEnd : 0u);
+        a->plan |= kPlanMetrics;  // the finalize kernel is enqueued either way (same graph topology)
         return HWPA_OK;
     };
     auto labelsAndPulses = [&]() -> int32_t {

@@ -323,9 +323,12 @@ __device__ __forceinline__ float LaneSqrt(float x) { return (x == 0.0f) ? 0.0f :
 __device__ __forceinline__ float NanToZero(float x) { return IsNan(x) ? 0.0f : x; }
 
 // FinalizeMetrics (:412-500), LabelBlock (:62-200) and PopulateBasecallingMetrics (:202-252).
+// `end` = 0 makes the launch a no-op: the kernel is enqueued every chunk so that a
+// captured chunk step has the same launch sequence whether or not a block ends.
 __global__ __launch_bounds__(64) void HfmFinalizeKernel(HfmLaneDev* state, TreeDev tree, int32_t realtime,
-                                                        float frameRate, hwpa_basecalling_metrics* out)
+                                                        float frameRate, hwpa_basecalling_metrics* out, int32_t end)
 {
+    if (!end) return;
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
     HfmLaneDev& S = state[lane];
@@ -610,10 +613,9 @@ int32_t MetricsProcess(hwpa_metrics* m, const hwpa_pulse* pulses, const uint32_t
     m->framesSeen += frames;
     const bool end = (uint32_t)lastFrame % m->cfg.frames_per_hf_metric_block < frames;
     if (produced) *produced = end ? 1 : 0;
-    if (!end) return HWPA_OK;
     hipLaunchKernelGGL(HfmFinalizeKernel, dim3(m->lanes), dim3(LS), 0, s, m->state, m->tree,
-                       m->cfg.realtime_activity_labels, m->cfg.movie.frame_rate, out);
-    m->framesSeen = 0;
+                       m->cfg.realtime_activity_labels, m->cfg.movie.frame_rate, out, end ? 1 : 0);
+    if (end) m->framesSeen = 0;
     return CheckLaunch("HfmFinalizeKernel");
 }
 
