@@ -454,17 +454,14 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
                 const float y = x - mu[i];
                 t[i] = pref[i] - 0.5f * y * y * varinv[i];
             }
-            // Every t below the threshold and none NaN (t is never +inf: the
-            // variances are clamped above zero, so a NaN shows in the sum).
-            const float tmx = __builtin_fmaxf(__builtin_fmaxf(__builtin_fmaxf(t[0], t[1]), __builtin_fmaxf(t[2], t[3])), t[4]);
-            const float tsum = ((t[0] + t[1]) + (t[2] + t[3])) + t[4];
-            const bool zero = tmx < kExpZeroBelow && tsum == tsum;
-            if (__all(zero)) {
+            // Per component: when t is below the threshold for every ZMW of the wave
+            // (a NaN compares false and is computed), exp(t) is +0 for all of them.
+            // The histogram spans about 60 baseline sigmas, so at most boundaries
+            // two or three components are that far away for the whole lane.
 #pragma unroll
-                for (int i = 0; i < 5; ++i) dOut[i] = 0.0f;
-            } else {
-#pragma unroll
-                for (int i = 0; i < 5; ++i) dOut[i] = hw_expf(t[i]);
+            for (int i = 0; i < 5; ++i) {
+                if (__all(t[i] < kExpZeroBelow)) dOut[i] = 0.0f;
+                else dOut[i] = hw_expf(t[i]);
             }
             bpOut = Sum5Tree(dOut);
             devOut = x - piv;
