@@ -469,6 +469,7 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
         corner(0, bpPrev, dPrev, devPrev);
         float x0 = h.Start(0);
         uint32_t cnt = h.Cnt(0);
+#pragma unroll 2
         for (int b = 0; b < NB; ++b) {
             const uint32_t cntNext = h.Cnt(b + 1);   // in flight while this boundary is computed
             float bpNext, dNext[5], devNext;
