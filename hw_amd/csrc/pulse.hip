@@ -39,7 +39,20 @@ __device__ __forceinline__ float StdMin(float a, float b) { return (b < a) ? b :
 __device__ __forceinline__ float StdMax(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ uint16_t ToFixed(float v) { return (uint16_t)(v / 0.1f + 0.5f); }
 
-template <int UNROLL, bool PREFETCH>
+// A finished pulse: the values the record is computed from (HostPulseAccumulator's
+// LabelsSegment::ToPulse), kept until the record is written.
+struct PulseEnd {
+    uint32_t start, stop;
+    float meanNum, total, m2;
+    int smax, label;
+};
+
+// One thread per ZMW.  QUEUE: a finished segment's values go to a per-thread queue in LDS
+// (6 words, conflict-free [slot][word][ZMW] layout); when some ZMW's queue could overflow
+// during the next batch, the wave writes the queued records with one loop over the longest
+// queue.  The record's divisions and fixed-point conversions then run about once per
+// pulse of the busiest ZMW instead of at every frame where any ZMW of the wave ends one.
+template <int UNROLL, bool PREFETCH, bool QUEUE>
 __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ labels, const int16_t* __restrict__ latent,
                                                   const int16_t* __restrict__ traces, int frames, int32_t firstFrame,
                                                   const hwpa_lane_model* __restrict__ models, SegLane* seg,
@@ -47,6 +60,10 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
                                                   uint32_t* __restrict__ overflows, hwpa_stat_acc* __restrict__ blStats,
                                                   PulseConsts k)
 {
+    constexpr int QCAP = QUEUE ? 12 : 1;
+    static_assert(!QUEUE || QCAP >= 2 * UNROLL - UNROLL / 2, "queue holds the fill limit plus one batch");
+    constexpr int QFILL = QCAP - UNROLL;  // flush when any queue is above this after a batch
+    __shared__ uint32_t queue[QCAP * 6 * LS];
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
     SegLane& S = seg[lane];
@@ -56,11 +73,52 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
     const float minMean = hw_h2f(models[lane].analogs[NA - 1].means[z]);
     const float maxSignal = 65535 * 0.1f;
     hwpa_pulse* out = pulses + ((size_t)lane * LS + z) * k.maxCalls;
-    uint32_t n = 0, ovf = 0;
+    uint32_t n = 0, ovf = 0, q = 0;
     float bm0 = 0.f, bm1 = 0.f, bm2 = 0.f;
     const int16_t* lb = labels + (size_t)lane * frames * LS + z;
     const int16_t* lt = latent + (size_t)lane * NL * LS + z;
     const int16_t* tr = traces + (size_t)lane * frames * LS + z;
+
+    // LabelsSegment::ToPulse (HostPulseAccumulator.h:64-173) of a finished segment.
+    auto emit = [&](const PulseEnd& e) {
+        const int width = (int)(e.stop - e.start);
+        const float rawMean = e.meanNum / width;
+        const float rawMid = e.total / (width - 2);
+        const bool keep = (width >= k.widthThresh) || (rawMean * width >= k.ampThresh * minMean);
+        int l = e.label;
+        if (l > 2 * NA) l -= 2 * NA;
+        if (l > NA) l -= NA;
+        hwpa_pulse p;
+        p.start = e.start;
+        p.width = (uint16_t)width;
+        p.mean_signal = ToFixed(StdMin(maxSignal, StdMax(0.0f, rawMean)));
+        p.mid_signal = ToFixed(width < 3 ? 0.0f : StdMin(maxSignal, StdMax(0.0f, rawMid)));
+        p.max_signal = ToFixed(StdMin(maxSignal, StdMax(0.0f, (float)e.smax)));
+        p.signal_m2 = e.m2;
+        p.label = (l >= 1 && l <= NA) ? k.analogMap[l - 1] : (uint8_t)4;
+        p.is_reject = keep ? 0 : 1;
+        p.pad[0] = p.pad[1] = 0;
+        if (n < k.maxCalls) out[n++] = p;
+        else ++ovf;
+    };
+    auto qword = [&](uint32_t slot, int w) -> uint32_t& { return queue[(slot * 6 + (uint32_t)w) * LS + z]; };
+    auto flush = [&]() {
+        for (uint32_t j = 0; __any(j < q); ++j) {
+            if (j < q) {
+                PulseEnd e;
+                e.start = qword(j, 0);
+                e.stop = qword(j, 1);
+                e.meanNum = __uint_as_float(qword(j, 2));
+                e.total = __uint_as_float(qword(j, 3));
+                e.m2 = __uint_as_float(qword(j, 4));
+                const uint32_t sl = qword(j, 5);
+                e.smax = (int)(int16_t)(sl & 0xffffu);
+                e.label = (int)(int16_t)(sl >> 16);
+                emit(e);
+            }
+        }
+        q = 0;
+    };
 
     // Labels and signal of frames [f0, f0 + UNROLL); with PREFETCH the next
     // batch is loaded while one is accumulated.
@@ -89,25 +147,19 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
             const bool boundary = isUp || (lab == 0 && label != 0);
             const bool isPulse = label != 0;
             if (boundary && isPulse) {
-                const int width = (int)(absf - start);
-                const float rawMean = (float)(total + last + first) / width;
-                const float rawMid = (float)total / (width - 2);
-                const bool keep = (width >= k.widthThresh) || (rawMean * width >= k.ampThresh * minMean);
-                int l = label;
-                if (l > 2 * NA) l -= 2 * NA;
-                if (l > NA) l -= NA;
-                hwpa_pulse p;
-                p.start = start;
-                p.width = (uint16_t)width;
-                p.mean_signal = ToFixed(StdMin(maxSignal, StdMax(0.0f, rawMean)));
-                p.mid_signal = ToFixed(width < 3 ? 0.0f : StdMin(maxSignal, StdMax(0.0f, rawMid)));
-                p.max_signal = ToFixed(StdMin(maxSignal, StdMax(0.0f, (float)smax)));
-                p.signal_m2 = m2;
-                p.label = (l >= 1 && l <= NA) ? k.analogMap[l - 1] : (uint8_t)4;
-                p.is_reject = keep ? 0 : 1;
-                p.pad[0] = p.pad[1] = 0;
-                if (n < k.maxCalls) out[n++] = p;
-                else ++ovf;
+                // (float)(total + last + first) of ToPulse, in the same order.
+                const float meanNum = (total + last) + first;
+                if (QUEUE) {
+                    qword(q, 0) = start;
+                    qword(q, 1) = absf;
+                    qword(q, 2) = __float_as_uint(meanNum);
+                    qword(q, 3) = __float_as_uint(total);
+                    qword(q, 4) = __float_as_uint(m2);
+                    qword(q, 5) = ((uint32_t)smax & 0xffffu) | ((uint32_t)label << 16);
+                    ++q;
+                } else {
+                    emit(PulseEnd{start, absf, meanNum, total, m2, smax, label});
+                }
             }
             if (boundary) {
                 start = absf;
@@ -130,6 +182,7 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
                 bm2 += x * x;
             }
         }
+        if (QUEUE && __any(q > (uint32_t)QFILL)) flush();
         if (PREFETCH) {
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
@@ -138,6 +191,7 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
             }
         }
     }
+    if (QUEUE) flush();
     S.start[z] = start;
     S.first[z] = (int16_t)first;
     S.last[z] = (int16_t)last;
@@ -209,14 +263,16 @@ int32_t PulserRun(PulserDev* p, const int16_t* labels, const int16_t* latent, co
     if (!p || !labels || !latent || !traces || !models || !pulses || !counts || !overflows || !bl_stats)
         return Fail(HWPA_ERR_INVALID_ARG, "pulser_run: null argument");
     if (frames <= (uint32_t)NL) return Fail(HWPA_ERR_INVALID_ARG, "pulser_run: frames must be > 24");
-#define HWPA_PULSE_LAUNCH(U, PF)                                                                                  \
-    hipLaunchKernelGGL((PulseKernel<U, PF>), dim3(p->lanes), dim3(LS), 0, s, labels, latent, traces, (int)frames,   \
+#define HWPA_PULSE_LAUNCH(U, PF, Q)                                                                               \
+    hipLaunchKernelGGL((PulseKernel<U, PF, Q>), dim3(p->lanes), dim3(LS), 0, s, labels, latent, traces, (int)frames, \
                        first_frame, models, p->seg, pulses, counts, overflows, bl_stats, p->k)
-    // Variants: frames per batch x next batch in flight.
+    // Variants: frames per batch x next batch in flight x queued record writing.
     switch (TuneGet(kTunePulse, p->lanes, frames)) {
-    case 1: HWPA_PULSE_LAUNCH(8, true); break;
-    case 2: HWPA_PULSE_LAUNCH(16, true); break;
-    default: HWPA_PULSE_LAUNCH(8, false); break;
+    case 1: HWPA_PULSE_LAUNCH(8, true, false); break;
+    case 2: HWPA_PULSE_LAUNCH(16, true, false); break;
+    case 3: HWPA_PULSE_LAUNCH(8, false, true); break;
+    case 4: HWPA_PULSE_LAUNCH(8, true, true); break;
+    default: HWPA_PULSE_LAUNCH(8, false, false); break;
     }
 #undef HWPA_PULSE_LAUNCH
     return CheckLaunch("PulseKernel");

@@ -26,7 +26,7 @@ int32_t TuneVariants(int kernel)
     case kTuneHistogram: return 9;
     case kTuneBaseline: return 6;
     case kTuneLabel: return 2;
-    case kTunePulse: return 3;
+    case kTunePulse: return 5;
     default: return 0;
     }
 }

@@ -88,7 +88,7 @@ def test_baseliner_lane_tile_variants(H, O, T, gpu, variant, F):
 def test_autotune_picks_a_timed_variant(H, gpu):
     cfg = H.bench_config(512)
     r = H.autotune(cfg, 64, 512, repeats=2)
-    counts = {"histogram": 9, "baseline": 6, "labels": 2, "pulses": 3}
+    counts = {"histogram": 9, "baseline": 6, "labels": 2, "pulses": 5}
     for i, (name, n) in enumerate(counts.items()):
         assert len(r[name]) == H.tune_variants(i) == n
         assert all(t > 0 for t in r[name])
@@ -285,11 +285,16 @@ def test_labeler_bit_exact(H, O, T, gpu, label_variant, F):
         assert np.array_equal(H.to_numpy(d_score, np.float32, (L, 64)).view(np.uint32), o_score.view(np.uint32))
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
-def test_pulses_bit_exact(H, O, T, gpu, variant):
+@pytest.mark.parametrize("variant,max_calls", [(0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (0, 5), (3, 5), (4, 5)])
+def test_pulses_bit_exact(H, O, T, gpu, variant, max_calls):
+    """Pulse records, counts, overflow counts and baseline stats of every pulse-kernel
+    variant; max_calls 5 makes most ZMWs overflow their record capacity."""
     L, F, C = 4, 512, 4
     H.tune_set(H.TUNE_PULSE, L, F, variant)
-    cfg, chunks = _pipeline_inputs(H, O, T, L, F, C, seed=44)
+    cfg = H.bench_config(F)
+    if max_calls:
+        cfg.max_calls_per_zmw = max_calls
+    cfg, chunks = _pipeline_inputs(H, O, T, L, F, C, seed=44, cfg=cfg)
     agg = O.Aggregator(L)
     ol = O.Labeler(cfg, L)
     op = O.Pulser(cfg, L)
@@ -309,6 +314,8 @@ def test_pulses_bit_exact(H, O, T, gpu, variant):
         g_c = H.to_numpy(d_c, np.uint32, (n,))
         assert np.array_equal(g_c, o_c), "chunk %d counts" % c
         assert np.array_equal(H.to_numpy(d_o, np.uint32, (n,)), o_o)
+        if max_calls:
+            assert o_o.sum() > 0
         g_p = H.to_numpy(d_p, T.PULSE, (n, mc))
         for z in range(n):
             k = o_c[z]
