@@ -427,32 +427,35 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
     int it = 0;
     for (; it < iterLimit; ++it) {
         // ---- E-step
-        float varinv[5], pref[5];
+        float varinv[5], pref[5], nhv[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             varinv[k] = 1.0f / var[k];
             pref[k] = (hw_logf(rho[k]) - 0.5f * hw_logf(var[k])) - 0.5f * d.log2pi;
+            nhv[k] = -0.5f * varinv[k];
         }
         // Every use of a responsibility is boundaryProb * tau_i = exp(t_i)
         // (DmeEmHost.cpp:431-432,547-548), so the boundary carries the component
-        // densities dd_i = exp(t_i) and boundaryProb = their sum.  The baseline
-        // second moment of the M-step is accumulated in the same pass about the
-        // pivot piv (this iteration's baseline mean): q1 and q2 are the first
-        // and second moments of (x - piv) under the baseline weights, shifted to
-        // the new mean after the M-step.  Waves skip exact zeros: exps of
-        // boundaries where every t_i < -105 (exp rounds to +0) and the
-        // accumulation of bins that are empty for all 64 ZMWs.
+        // densities dd_i = exp(t_i) and boundaryProb is their sum.  The bin sums of
+        // the reference, sum_b (dd_i(b) + dd_i(b+1)) n_b / binProb_b, are taken per
+        // boundary: sum_c dd_i(c) (f_{c-1} + f_c) with f_b = n_b / binProb_b, one fused
+        // multiply-add per sum.  The baseline second moment of the M-step is
+        // accumulated in the same pass about the pivot piv (this iteration's baseline
+        // mean): q1 and q2 are the first and second moments of (x - piv) under the
+        // baseline weights, shifted to the new mean after the M-step.  Waves skip
+        // exact zeros: exps of boundaries where every t_i < -105 (exp rounds to +0),
+        // and the sums of boundaries whose two bins are empty for all 64 ZMWs.
         const float piv = mu[0];
         float probSum = 0.0f, wps = 0.0f, q1 = 0.0f, q2 = 0.0f;
         float ci[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, mom1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-        float bpPrev, dPrev[5], devPrev;
-        auto corner = [&](int b, float& bpOut, float* dOut, float& devOut) {
+        float bpPrev, dPrev[5];
+        auto corner = [&](int b, float& bpOut, float* dOut) {
             const float x = h.Start(b);
             float t[5];
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
                 const float y = x - mu[i];
-                t[i] = pref[i] - 0.5f * y * y * varinv[i];
+                t[i] = hw_fmaf(y * y, nhv[i], pref[i]);
             }
             // Per component: when t is below the threshold for every ZMW of the wave
             // (a NaN compares false and is computed), exp(t) is +0 for all of them.
@@ -464,40 +467,47 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
                 else dOut[i] = hw_expf(t[i]);
             }
             bpOut = Sum5Tree(dOut);
-            devOut = x - piv;
         };
-        corner(0, bpPrev, dPrev, devPrev);
-        float x0 = h.Start(0);
+        // Sums of boundary c with weight F = f_{c-1} + f_c; skipped when F is 0 for the
+        // whole wave (every density is finite, so the terms are exact zeros).
+        auto gather = [&](const float* dd, float x, float F) {
+            if (__all(F == 0.0f)) return;
+            const float G = x * F;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                ci[m] = hw_fmaf(dd[m], F, ci[m]);
+                mom1[m] = hw_fmaf(dd[m], G, mom1[m]);
+            }
+            const float dev = x - piv;
+            const float w = dd[0] * F;
+            q1 = hw_fmaf(w, dev, q1);
+            q2 = hw_fmaf(w * dev, dev, q2);
+        };
+        corner(0, bpPrev, dPrev);
+        float fPrev = 0.0f;
         uint32_t cnt = h.Cnt(0);
 #pragma unroll 2
         for (int b = 0; b < NB; ++b) {
             const uint32_t cntNext = h.Cnt(b + 1);   // in flight while this boundary is computed
-            float bpNext, dNext[5], devNext;
-            corner(b + 1, bpNext, dNext, devNext);
+            float bpNext, dNext[5];
+            corner(b + 1, bpNext, dNext);
             // binp >= binProbMin >= FLT_MIN and finite: the log needs no special cases
             const float binp = hw_maxf(halfBin * (bpPrev + bpNext), binProbMin);
             probSum += binp;
-            const float x1 = h.Start(b + 1);
+            float f = 0.0f;   // n_b / binp_b, +0 for an empty bin
             if (!__all(cnt == 0u)) {
                 const float nj = (float)cnt;
-                wps += nj * hw_logf_normal(binp);
-                const float factor = nj / binp;
-#pragma unroll
-                for (int m = 0; m < 5; ++m) {
-                    ci[m] += (dPrev[m] + dNext[m]) * factor;
-                    mom1[m] += (dPrev[m] * x0 + dNext[m] * x1) * factor;
-                }
-                const float g0 = dPrev[0] * devPrev, g1 = dNext[0] * devNext;
-                q1 += (g0 + g1) * factor;
-                q2 += (g0 * devPrev + g1 * devNext) * factor;
+                wps = hw_fmaf(nj, hw_logf_normal(binp), wps);
+                f = nj / binp;
             }
+            gather(dPrev, h.Start(b), fPrev + f);
             bpPrev = bpNext;
 #pragma unroll
             for (int m = 0; m < 5; ++m) dPrev[m] = dNext[m];
-            devPrev = devNext;
-            x0 = x1;
+            fPrev = f;
             cnt = cntNext;
         }
+        gather(dPrev, h.Start(NB), fPrev);
         const float q0 = ci[0];
 #pragma unroll
         for (int m = 0; m < 5; ++m) {

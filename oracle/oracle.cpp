@@ -972,52 +972,60 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
         for (int z = 0; z < LS; ++z) {
             EmZ& e = ez[z];
             const HistZ& h = hz[z];
-            float varinv[5], pref[5];
+            float varinv[5], pref[5], nhv[5];
             for (int k = 0; k < 5; ++k) {
                 varinv[k] = 1.0f / e.var[k];
                 pref[k] = (hw_logf(e.rho[k]) - 0.5f * hw_logf(e.var[k])) - 0.5f * d.log2pi;
+                nhv[k] = -0.5f * varinv[k];
             }
             // Every use of a responsibility is boundaryProb * tau_i = exp(t_i)
             // (DmeEmHost.cpp:431-432,547-548): a boundary carries the component
-            // densities dd[i] = exp(t_i), boundaryProb is their sum.  The
+            // densities dd[i] = exp(t_i), boundaryProb is their sum.  The bin sums
+            // sum_b (dd_i(b) + dd_i(b+1)) n_b / binProb_b of DmeEmHost.cpp:425-445 are
+            // regrouped per boundary, sum_c dd_i(c) (f_{c-1} + f_c) with
+            // f_b = n_b / binProb_b, each term one fused multiply-add.  The
             // baseline second moment (DmeEmHost.cpp:536-552) is accumulated in
             // the same pass about the pivot piv = this iteration's baseline mean
             // and shifted to the new mean in the M-step.
             const float piv = e.mu[0];
-            float dd[2][5], bpb[2], dev[2];
+            float dd[2][5], bpb[2];
             auto corner = [&](int b, int k) {
                 const float x = h.start[b];
                 for (int i = 0; i < 5; ++i) {
                     const float y = x - e.mu[i];
-                    const float t = pref[i] - 0.5f * y * y * varinv[i];
+                    const float t = hw_fmaf(y * y, nhv[i], pref[i]);
                     dd[k][i] = hw_expf(t);
                 }
                 bpb[k] = Sum5Tree(dd[k]);
-                dev[k] = x - piv;
             };
             float probSum = 0.0f, wps = 0.0f, q1 = 0.0f, q2 = 0.0f;
             float ci[5] = {0, 0, 0, 0, 0}, mom1[5] = {0, 0, 0, 0, 0};
             const float halfBin = 0.5f * h.binSize;
+            auto gather = [&](const float* dc, float x, float F) {
+                const float G = x * F;
+                for (int m = 0; m < 5; ++m) {
+                    ci[m] = hw_fmaf(dc[m], F, ci[m]);
+                    mom1[m] = hw_fmaf(dc[m], G, mom1[m]);
+                }
+                const float dev = x - piv;
+                const float w = dc[0] * F;
+                q1 = hw_fmaf(w, dev, q1);
+                q2 = hw_fmaf(w * dev, dev, q2);
+            };
             corner(0, 0);
+            float fPrev = 0.0f;
             for (int b = 0; b < NB; ++b) {
                 const int k0 = b & 1, k1 = k0 ^ 1;
                 corner(b + 1, k1);
                 const float binp = hw_maxf(halfBin * (bpb[k0] + bpb[k1]), e.binProbMin);
                 probSum += binp;
                 const float nj = (float)h.cnt[b];
-                wps += nj * hw_logf_normal(binp);   // binp >= binProbMin >= FLT_MIN, finite
-                const float factor = nj / binp;
-                const float x0 = h.start[b], x1 = h.start[b + 1];
-                for (int m = 0; m < 5; ++m) {
-                    const float tmp1 = dd[k0][m];
-                    const float tmp2 = dd[k1][m];
-                    ci[m] += (tmp1 + tmp2) * factor;
-                    mom1[m] += (tmp1 * x0 + tmp2 * x1) * factor;
-                }
-                const float g0 = dd[k0][0] * dev[k0], g1 = dd[k1][0] * dev[k1];
-                q1 += (g0 + g1) * factor;
-                q2 += (g0 * dev[k0] + g1 * dev[k1]) * factor;
+                wps = hw_fmaf(nj, hw_logf_normal(binp), wps);   // binp >= binProbMin >= FLT_MIN, finite
+                const float f = nj / binp;
+                gather(dd[k0], h.start[b], fPrev + f);
+                fPrev = f;
             }
+            gather(dd[NB & 1], h.start[NB], fPrev);
             e.q0 = ci[0];
             e.q1 = q1;
             e.q2 = q2;
