@@ -332,6 +332,133 @@ __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLa
     }
 }
 
+// W waves per lane, sharing the lane's LDS counters: wave w bins frames [w len, (w + 1) len)
+// of the chunk (len = frames / W rounded up to whole batches), so W times as many trace
+// rows are in flight per lane.  The edge classifier decides a frame from the frame and
+// its two predecessors only, so every wave but the first starts from the two trace frames
+// before its range (a full two-entry history); the first continues the carried state and
+// the wave holding the last frame stores it.  LDS increments from different waves are
+// atomic; the outlier counts are partial sums per wave, added mod 2^16 at the end.  Lanes
+// whose counts could pass 65535 are left to the one-wave wrapping pass, as above.
+template <int B, int NL, int W>
+__global__ __launch_bounds__(64 * W) void HistAddWavesKernel(hwpa_lane_hist* hist, EdgeLane* edge,
+                                                             const int16_t* __restrict__ traces, int frames,
+                                                             const hwpa_lane_model* __restrict__ models)
+{
+    static_assert(NL % 2 == 0 && NL <= NB && W >= 2, "LDS bins, waves");
+    __shared__ uint32_t cnt[(NL / 2) * LS];
+    __shared__ uint32_t outl[2 * LS];  // this chunk's low / high outlier counts
+    const int lane = blockIdx.x;
+    const int z = threadIdx.x % LS;
+    const int w = threadIdx.x / LS;
+    hwpa_lane_hist& h = hist[lane];
+    EdgeLane& e = edge[lane];
+    for (int i = threadIdx.x; i < 2 * LS; i += W * LS) outl[i] = 0;
+    const bool odd = (z & 1) != 0;
+    const uint32_t* rows = reinterpret_cast<const uint32_t*>(&h.bin_count[0][0]);
+    const int col = (z >> 1) + (odd ? LS / 2 : 0);
+    uint32_t mx = 0;
+    constexpr int NJ = NL / 2;
+#pragma unroll 1
+    for (int j0 = 16 * w; j0 < NJ; j0 += 16 * W) {
+        uint32_t c[16];
+#pragma clang loop unroll(full)
+        for (int k = 0; k < 16; ++k) {
+            const int j = (j0 + k < NJ) ? j0 + k : NJ - 1;
+            c[k] = rows[j * LS + col];
+        }
+#pragma clang loop unroll(full)
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t v = c[k], n = SwapPair(v);
+            mx = UMax(mx, UMax(v & 0xffffu, v >> 16));
+            const uint32_t packed = odd ? ((n >> 16) | (v & 0xffff0000u)) : ((v & 0xffffu) | (n << 16));
+            const int j = (j0 + k < NJ) ? j0 + k : NJ - 1;
+            cnt[j * LS + z] = packed;   // a clamped duplicate writes the same value
+        }
+    }
+    if (NL < NB) {
+        const uint4* rest = reinterpret_cast<const uint4*>(&h.bin_count[NL][0]);
+        constexpr int NV = (NB - NL) * LS * 2 / 16;
+#pragma unroll 1
+        for (int i0 = 8 * LS * w; i0 < NV; i0 += 8 * LS * W) {
+            uint4 c[8];
+#pragma clang loop unroll(full)
+            for (int k = 0; k < 8; ++k) {
+                const int i = i0 + k * LS + z;
+                c[k] = rest[i < NV ? i : NV - 1];
+            }
+#pragma clang loop unroll(full)
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t wd[4] = {c[k].x, c[k].y, c[k].z, c[k].w};
+#pragma clang loop unroll(full)
+                for (int q = 0; q < 4; ++q)
+                    mx = UMax(mx, UMax(wd[q] & 0xffffu, wd[q] >> 16));
+            }
+        }
+    }
+    // Barrier for the counters and the outlier sums; any wave with a count that could wrap
+    // sends the whole lane to the wrapping pass.
+    if (__syncthreads_or(mx + (uint32_t)frames > 65535u ? 1 : 0)) {
+        if (threadIdx.x == 0) e.needWrap = 1;
+        return;
+    }
+    const float lo = h.low_bound[z];
+    const float bs = h.bin_size[z];
+    const float inv = 1.0f / bs;
+    const hwpa_lane_model& m = models[lane];
+    const float bvar = hw_h2f(m.baseline.vars[z]);
+    const float bmean = hw_h2f(m.baseline.means[z]);
+    const int thr = hw_wrap16(hw_round_i32(2.0f * hw_sqrtf(bvar) + bmean));
+
+    const int len = ((frames + W - 1) / W + B - 1) / B * B;   // >= B >= 2: the launcher needs frames >= W B
+    const int f0 = w * len;
+    const int n = (frames - f0 < len) ? frames - f0 : len;   // <= 0 for waves past the end
+    const int16_t* tr = traces + (size_t)lane * frames * LS + z;
+    int prev, pushes;
+    bool bl1, bl2;
+    if (w == 0) {
+        prev = e.prev[z];
+        const int i0 = e.i0, i1 = e.i1;
+        pushes = (i1 < 0) ? 0 : ((2 + i1 - i0) & 1) + 1;
+        bl1 = (i1 >= 0) && e.hist[i1 & 1][z] != 0;
+        bl2 = (pushes == 2) && e.hist[i0 & 1][z] != 0;
+    } else {
+        const int fs = (n > 0) ? f0 : 2;   // waves past the end read in-range frames they do not use
+        prev = tr[(size_t)(fs - 1) * LS];
+        bl1 = prev < thr;
+        bl2 = tr[(size_t)(fs - 2) * LS] < thr;
+        pushes = 2;
+    }
+    uint32_t lowP = 0, highP = 0;
+    if (n > 0)
+        HistFrames<B, false, NL>(cnt, h, tr + (size_t)f0 * LS, n, z, lo, bs, inv, thr, prev, bl1, bl2, pushes, lowP,
+                                 highP);
+    atomicAdd(&outl[z], lowP);
+    atomicAdd(&outl[LS + z], highP);
+    __syncthreads();
+
+    uint32_t* wrows = reinterpret_cast<uint32_t*>(&h.bin_count[0][0]);
+#pragma unroll 4
+    for (int j = w; j < NJ; j += W) {
+        const uint32_t v = cnt[j * LS + z], nb = SwapPair(v);
+        wrows[j * LS + col] = odd ? ((nb >> 16) | (v & 0xffff0000u)) : ((v & 0xffffu) | (nb << 16));
+    }
+    if (w == 0) {
+        h.outlier_low[z] = (uint16_t)((uint32_t)h.outlier_low[z] + outl[z]);
+        h.outlier_high[z] = (uint16_t)((uint32_t)h.outlier_high[z] + outl[LS + z]);
+    }
+    if (w == (frames - 1) / len) {
+        e.prev[z] = (int16_t)prev;
+        e.hist[0][z] = (pushes == 2 ? bl2 : bl1) ? 1 : 0;
+        e.hist[1][z] = bl1 ? 1 : 0;
+        if (z == 0) {
+            e.i0 = 0;
+            e.i1 = (pushes == 0) ? -1 : (pushes == 1 ? 0 : 1);
+            e.needWrap = 0;
+        }
+    }
+}
+
 int32_t HistReset(const hwpa_config& cfg, hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats, uint32_t lanes,
                   hipStream_t s)
 {
@@ -358,7 +485,21 @@ static int32_t HistAddLaunch(hwpa_lane_hist* hist, EdgeLane* edge, const int16_t
     return CheckLaunch("HistAddKernel");
 }
 
-// Histogram variants: frames per prefetched batch x bins kept in LDS.
+template <int B, int NL, int W>
+static int32_t HistAddWavesLaunch(hwpa_lane_hist* hist, EdgeLane* edge, const int16_t* traces, uint32_t frames,
+                                  const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
+{
+    if (frames < (uint32_t)(W * B)) return HistAddLaunch<B, NL>(hist, edge, traces, frames, models, lanes, s);
+    hipLaunchKernelGGL((HistAddWavesKernel<B, NL, W>), dim3(lanes), dim3(W * LS), 0, s, hist, edge, traces,
+                       (int)frames, models);
+    const int32_t rc = CheckLaunch("HistAddWavesKernel");
+    if (rc != HWPA_OK) return rc;
+    hipLaunchKernelGGL((HistAddKernel<16, true, NB>), dim3(lanes), dim3(LS), 0, s, hist, edge, traces, (int)frames,
+                       models);
+    return CheckLaunch("HistAddKernel");
+}
+
+// Histogram variants: frames per prefetched batch x bins kept in LDS (x waves per lane).
 int32_t HistAddVariant(int32_t variant, hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
                        const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
 {
@@ -376,6 +517,13 @@ int32_t HistAddVariant(int32_t variant, hwpa_lane_hist* hist, void* edge, const 
     case 6: return HistAddLaunch<32, 160>(hist, e, traces, frames, models, lanes, s);
     case 7: return HistAddLaunch<32, 128>(hist, e, traces, frames, models, lanes, s);
     case 8: return HistAddLaunch<32, 192>(hist, e, traces, frames, models, lanes, s);
+    case 9: return HistAddWavesLaunch<8, 160, 4>(hist, e, traces, frames, models, lanes, s);
+    case 10: return HistAddWavesLaunch<8, 128, 4>(hist, e, traces, frames, models, lanes, s);
+    case 11: return HistAddWavesLaunch<16, 160, 4>(hist, e, traces, frames, models, lanes, s);
+    case 12: return HistAddWavesLaunch<8, 160, 2>(hist, e, traces, frames, models, lanes, s);
+    case 13: return HistAddWavesLaunch<8, 192, 4>(hist, e, traces, frames, models, lanes, s);
+    case 14: return HistAddWavesLaunch<8, 192, 8>(hist, e, traces, frames, models, lanes, s);
+    case 15: return HistAddWavesLaunch<8, 160, 8>(hist, e, traces, frames, models, lanes, s);
     default: return Fail(HWPA_ERR_INVALID_ARG, "hist_add: unknown variant");
     }
 }

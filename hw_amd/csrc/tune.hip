@@ -23,7 +23,7 @@ static std::map<std::tuple<int, uint32_t, uint32_t>, int32_t> g_tune;
 int32_t TuneVariants(int kernel)
 {
     switch (kernel) {
-    case kTuneHistogram: return 9;
+    case kTuneHistogram: return 16;
     case kTuneBaseline: return 6;
     case kTuneLabel: return 2;
     case kTunePulse: return 5;

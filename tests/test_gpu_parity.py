@@ -88,7 +88,7 @@ def test_baseliner_lane_tile_variants(H, O, T, gpu, variant, F):
 def test_autotune_picks_a_timed_variant(H, gpu):
     cfg = H.bench_config(512)
     r = H.autotune(cfg, 64, 512, repeats=2)
-    counts = {"histogram": 9, "baseline": 6, "labels": 2, "pulses": 5}
+    counts = {"histogram": 16, "baseline": 6, "labels": 2, "pulses": 5}
     for i, (name, n) in enumerate(counts.items()):
         assert len(r[name]) == H.tune_variants(i) == n
         assert all(t > 0 for t in r[name])
@@ -137,7 +137,7 @@ def test_stats_aggregate_bit_exact(H, O, T, gpu):
             H.stats_reset(d_agg, L)
 
 
-@pytest.fixture(params=range(9))
+@pytest.fixture(params=range(16))
 def hist_variant(request, H):
     """Every autotuner variant of the histogram kernel (shape 6 lanes x 512)."""
     H.tune_set(H.TUNE_HISTOGRAM, 6, 512, request.param)
@@ -168,6 +168,37 @@ def test_histogram_bit_exact(H, O, T, gpu, hist_variant):
         hist.add(bl, models)
         H.hist_add(d_hist, d_edge, _dev(H, bl), F, _dev(H, models), L)
         _eq_struct(H.to_numpy(d_hist, T.LANE_HIST, (L,)), hist.get(T.LANE_HIST), "hist[%d]" % c)
+
+
+@pytest.mark.parametrize("variant", [0, 9, 11, 12, 14])
+@pytest.mark.parametrize("frames", [144, 48, 208])
+def test_histogram_uneven_frames(H, O, T, gpu, variant, frames):
+    """Chunk lengths that do not split evenly over a lane's waves (144 frames in 4 waves of
+    16-frame batches: 48, 48, 48 and none; of 8-frame batches: 40, 40, 40, 24; 48 frames:
+    too short for 4 waves of 16, one wave runs), over chunks with the edge-classifier
+    state carried."""
+    L, C = 3, 4
+    H.tune_set(H.TUNE_HISTOGRAM, L, frames, variant)
+    try:
+        cfg, chunks = _pipeline_inputs(H, O, T, L, frames, C, seed=8)
+        hist = O.Histogram(cfg, L)
+        d_hist = H.dev_bytes(L * T.LANE_HIST.itemsize)
+        d_edge = H.dev_bytes(L * H.edge_state_bytes())
+        H.edge_state_init(d_edge, L)
+        agg = O.Aggregator(L)
+        for c, (bl, st) in enumerate(chunks):
+            agg.add(st)
+            s = agg.get(T.BASELINER_STATS)
+            models = O.dme_init_models(cfg, s, T.LANE_MODEL)
+            if c == 0:
+                assert hist.reset(s) == 0
+                H.hist_reset(cfg, d_hist, _dev(H, s), L)
+            hist.add(bl, models)
+            H.hist_add(d_hist, d_edge, _dev(H, bl), frames, _dev(H, models), L)
+            _eq_struct(H.to_numpy(d_hist, T.LANE_HIST, (L,)), hist.get(T.LANE_HIST), "hist[%d]" % c)
+        assert hist.get(T.LANE_HIST)["bin_count"].sum() > 0
+    finally:
+        H.tune_set(H.TUNE_HISTOGRAM, L, frames, 0)
 
 
 def test_histogram_bin_edges_wrap_and_degenerate_bounds(H, O, T, gpu, hist_variant):
