@@ -268,6 +268,8 @@ struct EmExtra {
 // the result only for converged ZMWs with status OK and enough confidence.
 // exp(t) rounds to +0 for every t below this (exp(-103.98) is 2^-150).
 constexpr float kExpZeroBelow = -105.0f;
+// Boundaries per exact restart of the density recurrence of the E-step.
+constexpr int kRecurBlock = 10;
 
 template <int MODE>
 __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __restrict__ hists,
@@ -448,64 +450,103 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
         const float piv = mu[0];
         float probSum = 0.0f, wps = 0.0f, q1 = 0.0f, q2 = 0.0f;
         float ci[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, mom1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-        float bpPrev, dPrev[5];
-        auto corner = [&](int b, float& bpOut, float* dOut) {
-            const float x = h.Start(b);
-            float t[5];
+        // Densities along the boundaries x_c = lb + c bs: t_i is quadratic in c, so
+        // dd_i(c+1) = dd_i(c) R_i(c) with R_i(c) = exp(t_i(c+1) - t_i(c)) and
+        // R_i(c+1) = R_i(c) Q_i, Q_i = exp(-bs^2 / var_i): two multiplies per density.
+        // Every kRecurBlock boundaries dd_i and R_i are recomputed from exp.  The products
+        // follow the ideal grid x_c + j bs; against exp at the rounded boundaries they
+        // differ by |x - mu| / var times the boundary's rounding (below 1e-4 relative
+        // within 14 sigma of the mean, 1e-5 near it).  A ZMW whose density at the
+        // restart is below FLT_MIN while rising, or whose ratio exceeds e^80, takes exp
+        // at every boundary of the block (a density coming out of underflow cannot be
+        // continued from +0).
+        float dd[5], rr[5], qq[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) qq[k] = hw_expf((2.0f * nhv[k]) * (h.bs * h.bs));
+        uint32_t exMask = 0;
+        auto tAt = [&](float x, int i) {
+            const float y = x - mu[i];
+            return hw_fmaf(y * y, nhv[i], pref[i]);
+        };
+        float nbs[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) nbs[k] = nhv[k] * h.bs;
+        auto restart = [&](int c) {
+            const float x0 = h.Start(c);
+            exMask = 0;
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
-                const float y = x - mu[i];
-                t[i] = hw_fmaf(y * y, nhv[i], pref[i]);
+                const float y0 = x0 - mu[i];
+                const float t0 = hw_fmaf(y0 * y0, nhv[i], pref[i]);
+                // exp(t) is +0 below the threshold: waves skip the exp when all ZMWs are.
+                if (__all(t0 < kExpZeroBelow)) dd[i] = 0.0f;
+                else dd[i] = hw_expf(t0);
+                // t(x0 + bs) - t(x0) on the ideal grid (no cancellation of rounded t)
+                const float dl = nbs[i] * hw_fmaf(2.0f, y0, h.bs);
+                rr[i] = hw_expf(dl);
+                const bool rec = (dd[i] >= FLT_MIN || dl <= 0.0f) && dl <= 80.0f;
+                exMask |= (rec ? 0u : 1u) << i;
             }
-            // Per component: when t is below the threshold for every ZMW of the wave
-            // (a NaN compares false and is computed), exp(t) is +0 for all of them.
-            // The histogram spans about 60 baseline sigmas, so at most boundaries
-            // two or three components are that far away for the whole lane.
+        };
+        auto advance = [&](int c) {
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
-                if (__all(t[i] < kExpZeroBelow)) dOut[i] = 0.0f;
-                else dOut[i] = hw_expf(t[i]);
+                dd[i] = dd[i] * rr[i];
+                rr[i] = rr[i] * qq[i];
             }
-            bpOut = Sum5Tree(dOut);
+            if (__any(exMask != 0u)) {
+                const float x = h.Start(c);
+#pragma unroll
+                for (int i = 0; i < 5; ++i)
+                    if ((exMask >> i) & 1u) dd[i] = hw_expf(tAt(x, i));
+            }
         };
         // Sums of boundary c with weight F = f_{c-1} + f_c; skipped when F is 0 for the
         // whole wave (every density is finite, so the terms are exact zeros).
-        auto gather = [&](const float* dd, float x, float F) {
+        auto gather = [&](const float* dc, float x, float F) {
             if (__all(F == 0.0f)) return;
             const float G = x * F;
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
-                ci[m] = hw_fmaf(dd[m], F, ci[m]);
-                mom1[m] = hw_fmaf(dd[m], G, mom1[m]);
+                ci[m] = hw_fmaf(dc[m], F, ci[m]);
+                mom1[m] = hw_fmaf(dc[m], G, mom1[m]);
             }
             const float dev = x - piv;
-            const float w = dd[0] * F;
+            const float w = dc[0] * F;
             q1 = hw_fmaf(w, dev, q1);
             q2 = hw_fmaf(w * dev, dev, q2);
         };
-        corner(0, bpPrev, dPrev);
+        restart(0);
+        float bpPrev = Sum5Tree(dd), dPrev[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) dPrev[m] = dd[m];
         float fPrev = 0.0f;
         uint32_t cnt = h.Cnt(0);
-#pragma unroll 2
-        for (int b = 0; b < NB; ++b) {
-            const uint32_t cntNext = h.Cnt(b + 1);   // in flight while this boundary is computed
-            float bpNext, dNext[5];
-            corner(b + 1, bpNext, dNext);
-            // binp >= binProbMin >= FLT_MIN and finite: the log needs no special cases
-            const float binp = hw_maxf(halfBin * (bpPrev + bpNext), binProbMin);
-            probSum += binp;
-            float f = 0.0f;   // n_b / binp_b, +0 for an empty bin
-            if (!__all(cnt == 0u)) {
-                const float nj = (float)cnt;
-                wps = hw_fmaf(nj, hw_logf_normal(binp), wps);
-                f = nj / binp;
-            }
-            gather(dPrev, h.Start(b), fPrev + f);
-            bpPrev = bpNext;
+        static_assert(NB % kRecurBlock == 0, "whole blocks of boundaries");
+        for (int b0 = 0; b0 < NB; b0 += kRecurBlock) {
 #pragma unroll
-            for (int m = 0; m < 5; ++m) dPrev[m] = dNext[m];
-            fPrev = f;
-            cnt = cntNext;
+            for (int j = 0; j < kRecurBlock; ++j) {
+                const int b = b0 + j;
+                const uint32_t cntNext = h.Cnt(b + 1);   // in flight while this boundary is computed
+                if (j == kRecurBlock - 1) restart(b + 1);
+                else advance(b + 1);
+                const float bpNext = Sum5Tree(dd);
+                // binp >= binProbMin >= FLT_MIN and finite: the log needs no special cases
+                const float binp = hw_maxf(halfBin * (bpPrev + bpNext), binProbMin);
+                probSum += binp;
+                float f = 0.0f;   // n_b / binp_b, +0 for an empty bin
+                if (!__all(cnt == 0u)) {
+                    const float nj = (float)cnt;
+                    wps = hw_fmaf(nj, hw_logf_normal(binp), wps);
+                    f = nj / binp;
+                }
+                gather(dPrev, h.Start(b), fPrev + f);
+                bpPrev = bpNext;
+#pragma unroll
+                for (int m = 0; m < 5; ++m) dPrev[m] = dd[m];
+                fPrev = f;
+                cnt = cntNext;
+            }
         }
         gather(dPrev, h.Start(NB), fPrev);
         const float q0 = ci[0];

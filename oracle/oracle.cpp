@@ -31,6 +31,8 @@ constexpr int LS = HWPA_LANE_SIZE;
 constexpr int NA = HWPA_NUM_ANALOGS;
 constexpr int NB = HWPA_NUM_BINS;
 constexpr int NS = HWPA_NUM_STATES;
+// Boundaries per exact restart of the E-step density recurrence (same as the device).
+constexpr int kRecurBlock = 10;
 
 int g_threads = 1;
 
@@ -988,15 +990,37 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
             // the same pass about the pivot piv = this iteration's baseline mean
             // and shifted to the new mean in the M-step.
             const float piv = e.mu[0];
-            float dd[2][5], bpb[2];
-            auto corner = [&](int b, int k) {
-                const float x = h.start[b];
+            // Densities along the boundaries: dd_i(c+1) = dd_i(c) R_i(c),
+            // R_i(c+1) = R_i(c) Q_i with Q_i = exp(-bs^2 / var_i) (t_i is quadratic in
+            // c), recomputed from exp every kRecurBlock boundaries; a density below
+            // FLT_MIN while rising, or a ratio above e^80, takes exp at every boundary
+            // of the block.  The same operations as the device E-step.
+            float cur[5], rr[5], qq[5];
+            for (int k = 0; k < 5; ++k) qq[k] = hw_expf((2.0f * nhv[k]) * (h.binSize * h.binSize));
+            bool ex[5] = {false, false, false, false, false};
+            auto tAt = [&](float x, int i) {
+                const float y = x - e.mu[i];
+                return hw_fmaf(y * y, nhv[i], pref[i]);
+            };
+            float nbs[5];
+            for (int k = 0; k < 5; ++k) nbs[k] = nhv[k] * h.binSize;
+            auto restart = [&](int c) {
+                const float x0 = h.start[c];
                 for (int i = 0; i < 5; ++i) {
-                    const float y = x - e.mu[i];
-                    const float t = hw_fmaf(y * y, nhv[i], pref[i]);
-                    dd[k][i] = hw_expf(t);
+                    const float y0 = x0 - e.mu[i];
+                    const float t0 = hw_fmaf(y0 * y0, nhv[i], pref[i]);
+                    cur[i] = hw_expf(t0);
+                    const float dl = nbs[i] * hw_fmaf(2.0f, y0, h.binSize);   // t(x0 + bs) - t(x0)
+                    rr[i] = hw_expf(dl);
+                    ex[i] = !((cur[i] >= FLT_MIN || dl <= 0.0f) && dl <= 80.0f);
                 }
-                bpb[k] = Sum5Tree(dd[k]);
+            };
+            auto advance = [&](int c) {
+                for (int i = 0; i < 5; ++i) {
+                    cur[i] = cur[i] * rr[i];
+                    rr[i] = rr[i] * qq[i];
+                    if (ex[i]) cur[i] = hw_expf(tAt(h.start[c], i));
+                }
             };
             float probSum = 0.0f, wps = 0.0f, q1 = 0.0f, q2 = 0.0f;
             float ci[5] = {0, 0, 0, 0, 0}, mom1[5] = {0, 0, 0, 0, 0};
@@ -1012,20 +1036,26 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
                 q1 = hw_fmaf(w, dev, q1);
                 q2 = hw_fmaf(w * dev, dev, q2);
             };
-            corner(0, 0);
+            restart(0);
+            float prevD[5], bpPrev = Sum5Tree(cur);
+            for (int m = 0; m < 5; ++m) prevD[m] = cur[m];
             float fPrev = 0.0f;
+            static_assert(NB % kRecurBlock == 0, "whole blocks of boundaries");
             for (int b = 0; b < NB; ++b) {
-                const int k0 = b & 1, k1 = k0 ^ 1;
-                corner(b + 1, k1);
-                const float binp = hw_maxf(halfBin * (bpb[k0] + bpb[k1]), e.binProbMin);
+                if ((b + 1) % kRecurBlock == 0) restart(b + 1);
+                else advance(b + 1);
+                const float bpNext = Sum5Tree(cur);
+                const float binp = hw_maxf(halfBin * (bpPrev + bpNext), e.binProbMin);
                 probSum += binp;
                 const float nj = (float)h.cnt[b];
                 wps = hw_fmaf(nj, hw_logf_normal(binp), wps);   // binp >= binProbMin >= FLT_MIN, finite
                 const float f = nj / binp;
-                gather(dd[k0], h.start[b], fPrev + f);
+                gather(prevD, h.start[b], fPrev + f);
+                bpPrev = bpNext;
+                for (int m = 0; m < 5; ++m) prevD[m] = cur[m];
                 fPrev = f;
             }
-            gather(dd[NB & 1], h.start[NB], fPrev);
+            gather(prevD, h.start[NB], fPrev);
             e.q0 = ci[0];
             e.q1 = q1;
             e.q2 = q2;

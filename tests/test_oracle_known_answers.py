@@ -410,3 +410,48 @@ def test_generator_distribution(H, O, T):
         assert abs(xa.mean() - amp) < 0.02 * amp
         assert abs(np.sqrt(xa.var() - 144) - 0.1 * amp) < 0.02 * amp
         assert abs((truth == a + 1).mean() / frac - 0.25) < 0.02
+
+
+def test_em_density_recurrence_error_bound():
+    """The E-step evaluates each component density along the equally spaced bin
+    boundaries by the exact identity exp(t(c+1)) = exp(t(c)) R(c), R(c+1) = R(c) Q
+    (t quadratic in c), restarted from exp every 10 boundaries, with exp taken at every
+    boundary of a block where the density rises out of the subnormal range.  Against
+    float64 densities at the float32 boundaries the float32 products stay within 2e-4
+    relative (the boundaries' own rounding, |x - mu| / var ulp(x), dominates), 1e-4
+    within e^-20 of the peak, over component sigmas 5..40 and bin sizes 0.5..4."""
+    f32 = np.float32
+    rng = np.random.default_rng(11)
+    nb, block = 300, 10
+    worst = worst_peak = 0.0
+    for _ in range(300):
+        sigma = f32(rng.uniform(5.0, 40.0))
+        bs = f32(rng.uniform(0.5, 4.0))
+        lb = f32(rng.uniform(-60.0, 0.0))
+        mu = f32(lb + rng.uniform(0.0, nb) * bs)
+        var = f32(sigma * sigma)
+        nhv = f32(f32(-0.5) * (f32(1.0) / var))
+        nbs = f32(nhv * bs)
+        pref = f32(-0.5 * np.log(var) - 0.9189385)
+        x = (lb + np.arange(nb + 1, dtype=f32) * bs).astype(f32)
+        y = (x - mu).astype(f32)
+        t = (pref + nhv * (y * y)).astype(f32)
+        q = f32(np.exp(f32(2.0) * nhv * (bs * bs)))
+        exact = np.exp(np.float64(pref) - 0.5 * (x.astype(np.float64) - np.float64(mu)) ** 2 / np.float64(var))
+        d = r = f32(0.0)
+        ex = False
+        for c in range(nb + 1):
+            if c % block == 0:
+                d = f32(np.exp(t[c]))
+                dl = f32(nbs * f32(f32(2.0) * y[c] + bs))
+                r = f32(np.exp(dl))
+                ex = not ((d >= np.finfo(f32).tiny or dl <= 0) and dl <= 80)
+            else:
+                d = f32(np.exp(t[c])) if ex else f32(d * r)
+                r = f32(r * q)
+            if exact[c] > 1e-30:
+                e = abs(float(d) / exact[c] - 1.0)
+                worst = max(worst, e)
+                if t[c] > pref - 20:
+                    worst_peak = max(worst_peak, e)
+    assert worst < 2e-4 and worst_peak < 1e-4, (worst, worst_peak)
