@@ -280,6 +280,11 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
     if (x.active && (uint32_t)lane * LS >= *x.active) return;
+    // Tables of the E-step's log / reciprocal of bin probabilities, one entry per thread.
+    __shared__ float sRc[HW_LOGTAB_N], sLc[HW_LOGTAB_N];
+    static_assert(HW_LOGTAB_N == LS, "one table entry per thread");
+    hw_logtab_entry(z, &sRc[z], &sLc[z]);
+    __syncthreads();
     const hwpa_lane_hist& lh = hists[lane];
     const hwpa_baseliner_stats& bs = stats[lane];
     hwpa_lane_model& lm = models[lane];
@@ -537,8 +542,10 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
                 float f = 0.0f;   // n_b / binp_b, +0 for an empty bin
                 if (!__all(cnt == 0u)) {
                     const float nj = (float)cnt;
-                    wps = hw_fmaf(nj, hw_logf_normal(binp), wps);
-                    f = nj / binp;
+                    float lg, rcp;
+                    hw_log_rcp_tab(binp, sRc, sLc, &lg, &rcp);
+                    wps = hw_fmaf(nj, lg, wps);
+                    f = nj * rcp;
                 }
                 gather(dPrev, h.Start(b), fPrev + f);
                 bpPrev = bpNext;

@@ -190,6 +190,45 @@ HWPA_HD float hw_logf(float x)
     return hw_fmaf(ef, ln2hi, hw_fmaf(ef, ln2lo, lm));
 }
 
+/* Table-driven log and reciprocal of a positive normal finite x (the EM's bin
+ * probabilities).  x = 2^e m with m in [1, 2); k = the top 6 bits of m's fraction,
+ * c_k = 1 + (k + 1/2) / 64, rc_k = 1 / c_k rounded, lc_k = -hw_logf(rc_k) (the
+ * table, hw_logtab_entry); r = m rc_k - 1 in one fused step, |r| < 2^-7 + 2^-24:
+ *   log x = e ln2 + lc_k + log1p(r),  log1p(r) = r - r^2/2 + r^3/3 - r^4/4
+ *           (truncation below |r|^5 / 5 < 1e-11),
+ *   1 / x = 2^-e rc_k (1 - r + r^2 - r^3 + r^4)   (truncation below |r|^5).
+ * A few ulp, no division. */
+#define HW_LOGTAB_N 64
+HWPA_HD void hw_logtab_entry(int k, float* rc, float* lc)
+{
+    const float c = 1.0f + ((float)k + 0.5f) / 64.0f;
+    *rc = 1.0f / c;
+    *lc = -hw_logf(*rc);
+}
+
+HWPA_HD void hw_log_rcp_tab(float x, const float* rcTab, const float* lcTab, float* logOut, float* rcpOut)
+{
+    const uint32_t u = hw_f2u(x);
+    const int e = (int)((u >> 23) & 0xffu) - 127;
+    const uint32_t k = (u >> 17) & (uint32_t)(HW_LOGTAB_N - 1);
+    const float m = hw_u2f((u & 0x007fffffu) | 0x3f800000u);
+    const float rc = rcTab[k];
+    const float r = hw_fmaf(m, rc, -1.0f);
+    float p = hw_fmaf(-0.25f, r, 1.0f / 3.0f);
+    p = hw_fmaf(p, r, -0.5f);
+    p = hw_fmaf(p, r, 1.0f);
+    const float ef = (float)e;
+    const float ln2hi = 0.693145751953125f;
+    const float ln2lo = 1.428606765330187045e-06f;
+    *logOut = hw_fmaf(ef, ln2hi, hw_fmaf(ef, ln2lo, lcTab[k] + p * r));
+    float q = r - 1.0f;
+    q = hw_fmaf(q, r, 1.0f);
+    q = hw_fmaf(q, r, -1.0f);
+    q = hw_fmaf(q, r, 1.0f);
+    *rcpOut = hw_ldexpf(rc * q, -e);
+}
+
+
 HWPA_HD float hw_log2f(float x)
 {
     if (hw_isnan(x)) return x;

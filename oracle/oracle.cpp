@@ -34,6 +34,20 @@ constexpr int NS = HWPA_NUM_STATES;
 // Boundaries per exact restart of the E-step density recurrence (same as the device).
 constexpr int kRecurBlock = 10;
 
+// The E-step's log / reciprocal tables (hw_logtab_entry), as the device builds them.
+struct LogTabs {
+    float rc[HW_LOGTAB_N], lc[HW_LOGTAB_N];
+    LogTabs()
+    {
+        for (int k = 0; k < HW_LOGTAB_N; ++k) hw_logtab_entry(k, &rc[k], &lc[k]);
+    }
+};
+static const LogTabs& LogTab()
+{
+    static const LogTabs t;
+    return t;
+}
+
 int g_threads = 1;
 
 template <typename F>
@@ -1048,8 +1062,10 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
                 const float binp = hw_maxf(halfBin * (bpPrev + bpNext), e.binProbMin);
                 probSum += binp;
                 const float nj = (float)h.cnt[b];
-                wps = hw_fmaf(nj, hw_logf_normal(binp), wps);   // binp >= binProbMin >= FLT_MIN, finite
-                const float f = nj / binp;
+                float lg, rcp;   // binp >= binProbMin >= FLT_MIN and finite
+                hw_log_rcp_tab(binp, LogTab().rc, LogTab().lc, &lg, &rcp);
+                wps = hw_fmaf(nj, lg, wps);
+                const float f = nj * rcp;
                 gather(prevD, h.start[b], fPrev + f);
                 bpPrev = bpNext;
                 for (int m = 0; m < 5; ++m) prevD[m] = cur[m];
@@ -2980,7 +2996,8 @@ int64_t orc_baz_encode_batch(int32_t internal, const hwpa_pulse* pulses, const u
 extern "C" {
 
 // Element-wise evaluation of the shared math routines of include/hwpa_math.h.
-// kind: 0 exp, 1 log, 2 log2, 3 exp2, 4 pow(a,b), 5 f->half->f, 6 round_i16, 7 floor_i32
+// kind: 0 exp, 1 log, 2 log2, 3 exp2, 4 pow(a,b), 5 f->half->f, 6 round_i16, 7 floor_i32,
+// 8 log of a normal float, 9 / 10 table-driven log / reciprocal of a normal float
 void orc_math(int32_t kind, const float* a, const float* b, float* out, int32_t n)
 {
     for (int32_t i = 0; i < n; ++i) {
@@ -2993,6 +3010,13 @@ void orc_math(int32_t kind, const float* a, const float* b, float* out, int32_t 
         case 5: out[i] = hw_h2f(hw_f2h(a[i])); break;
         case 6: out[i] = (float)hw_round_i16(a[i]); break;
         case 8: out[i] = hw_logf_normal(a[i]); break;
+        case 9:
+        case 10: {
+            float lg, rcp;
+            hw_log_rcp_tab(a[i], LogTab().rc, LogTab().lc, &lg, &rcp);
+            out[i] = kind == 9 ? lg : rcp;
+            break;
+        }
         default: out[i] = (float)hw_floor_i32(a[i]); break;
         }
     }

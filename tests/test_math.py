@@ -117,3 +117,22 @@ def test_log_normal_entry_matches_log(O):
                                                                                   0x3fb504f3, 0x3fb504f4])])
     x = bits.view(np.float32)
     assert np.array_equal(_eval(O, 8, x).view(np.uint32), _eval(O, 1, x).view(np.uint32))
+
+
+def test_table_log_and_reciprocal_accuracy(O):
+    """hw_log_rcp_tab (the EM's log and reciprocal of bin probabilities, positive
+    normal floats): within 2 ulp of the float64 log where |log x| >= 0.5, within 1e-7
+    absolute below that, and within 2 ulp of 1/x, over every binade and every table
+    interval (x = 2^e (1 + j / 2^23) at a stride of the fraction)."""
+    lo = np.float32(1.1754944e-38).view(np.uint32)
+    hi = np.float32(1.7e38).view(np.uint32)
+    bits = np.concatenate([np.arange(lo, hi, 7919, dtype=np.uint32),
+                           np.uint32([lo, hi, 0x3f800000, 0x3f7fffff, 0x3f800001, 0x3fffffff])])
+    x = bits.view(np.float32)
+    lg = _eval(O, 9, x).astype(np.float64)
+    rcp = _eval(O, 10, x)
+    ref = np.log(x.astype(np.float64))
+    big = np.abs(ref) >= 0.5
+    assert _ulps(lg[big].astype(np.float32), ref[big]).max() <= 2
+    assert np.abs(lg[~big] - ref[~big]).max() <= 1e-7
+    assert _ulps(rcp, 1.0 / x.astype(np.float64)).max() <= 2
