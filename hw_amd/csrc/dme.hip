@@ -270,9 +270,11 @@ struct EmExtra {
 constexpr float kExpZeroBelow = -105.0f;
 // Boundaries per exact restart of the density recurrence of the E-step.
 constexpr int kRecurBlock = 10;
+// Bins between a count's load and its use in the E-step.
+constexpr int kCntAhead = 2;
 
 template <int MODE>
-__global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __restrict__ hists,
+__global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist* __restrict__ hists,
                                                         const hwpa_baseliner_stats* __restrict__ stats,
                                                         hwpa_lane_model* models, int32_t* iters, DmeDevCfg d,
                                                         float evolve, EmExtra x)
@@ -526,13 +528,19 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
 #pragma unroll
         for (int m = 0; m < 5; ++m) dPrev[m] = dd[m];
         float fPrev = 0.0f;
-        uint32_t cnt = h.Cnt(0);
+        // Bin counts kCntAhead bins ahead, so their loads are in flight over that many bins.
+        uint32_t cq[kCntAhead];
+#pragma unroll
+        for (int j = 0; j < kCntAhead; ++j) cq[j] = h.Cnt(j);
         static_assert(NB % kRecurBlock == 0, "whole blocks of boundaries");
         for (int b0 = 0; b0 < NB; b0 += kRecurBlock) {
 #pragma unroll
             for (int j = 0; j < kRecurBlock; ++j) {
                 const int b = b0 + j;
-                const uint32_t cntNext = h.Cnt(b + 1);   // in flight while this boundary is computed
+                const uint32_t cnt = cq[0];
+#pragma unroll
+                for (int q = 0; q + 1 < kCntAhead; ++q) cq[q] = cq[q + 1];
+                cq[kCntAhead - 1] = h.Cnt(b + kCntAhead);
                 if (j == kRecurBlock - 1) restart(b + 1);
                 else advance(b + 1);
                 const float bpNext = Sum5Tree(dd);
@@ -552,7 +560,6 @@ __global__ __launch_bounds__(64) void DmeEstimateKernel(const hwpa_lane_hist* __
 #pragma unroll
                 for (int m = 0; m < 5; ++m) dPrev[m] = dd[m];
                 fPrev = f;
-                cnt = cntNext;
             }
         }
         gather(dPrev, h.Start(NB), fPrev);

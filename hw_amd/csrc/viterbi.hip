@@ -128,6 +128,8 @@ __device__ __forceinline__ uint32_t Recur(const TransVals& T, const float* sc, f
 {
     float acc[NS];
     uint32_t word = 0;
+    // maxVal - penalty, the penalty +inf outside the ROI for every state but baseline
+    const float pen = isRoi ? 0.0f : hw_u2f(0x7f800000u);
     // rows with candidates {B, TU, GU, CU, AU, TD, GD, CD, AD}
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -142,7 +144,7 @@ __device__ __forceinline__ uint32_t Recur(const TransVals& T, const float* sc, f
             mx = c ? v : mx;
             code = c ? (uint32_t)(1 + p) : code;
         }
-        acc[r] = (r == 0 || isRoi) ? mx : (mx - hw_u2f(0x7f800000u));
+        acc[r] = (r == 0) ? mx : (mx - pen);
         word |= code << (4 * q);
     }
     // full rows 1..4 and down rows 9..12: candidates {full_i, up_i}
@@ -155,7 +157,7 @@ __device__ __forceinline__ uint32_t Recur(const TransVals& T, const float* sc, f
         const float b = (s + T.fd[q][1]) + ll[5 + i];
         const bool c = b >= a;
         const float mx = c ? b : a;
-        acc[r] = isRoi ? mx : (mx - hw_u2f(0x7f800000u));
+        acc[r] = mx - pen;
         word |= (c ? 1u : 0u) << (20 + q);
     }
 #pragma unroll
