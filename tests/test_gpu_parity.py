@@ -279,6 +279,29 @@ def test_dme_estimate_bit_exact(H, O, T, gpu, seed):
     _eq_struct(g_models, o_models, "models")
 
 
+def test_dme_estimate_recurrence_edge_paths_bit_exact(H, O, T, gpu):
+    """Histograms whose boundaries start far below every component (densities below
+    FLT_MIN and rising at the first restarts of the E-step's density recurrence, so
+    those blocks take exp at every boundary) and bins 8 or 16 times wider (ratios
+    between boundaries beyond e^80 for the narrow components), against the oracle."""
+    L, F = 4, 512
+    cfg, h, s, m = _em_case(H, O, T, L, F, 3)
+    h = h.copy()
+    bs = h["bin_size"].copy()
+    h["low_bound"][:, 0::2] -= 100.0 * bs[:, 0::2]
+    h["bin_size"][1, 1::4] *= 8.0
+    h["bin_size"][2, 1::4] *= 16.0
+    h["low_bound"][3, 1::4] -= 250.0 * bs[3, 1::4]
+    mfi, efi = (512, 1024), (1024, 1536)
+    o_models, diag = O.dme_estimate(cfg, h, s, m, mfi, efi, want_diag=True)
+    d_models = _dev(H, m)
+    d_iters = H.dev_bytes(L * 4)
+    H.dme_estimate(cfg, _dev(H, h), _dev(H, s), d_models, mfi, efi, L, d_iters)
+    iters = H.to_numpy(d_iters, np.int32, (L,))
+    assert list(iters) == [diag[l].iterations_run for l in range(L)]
+    _eq_struct(H.to_numpy(d_models, T.LANE_MODEL, (L,)), o_models, "models")
+
+
 @pytest.fixture(params=[0, 1])
 def label_variant(request, H):
     """Both autotuner variants of the labeler (16 or 8 frames per batch)."""
