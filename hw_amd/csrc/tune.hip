@@ -25,7 +25,7 @@ int32_t TuneVariants(int kernel)
     switch (kernel) {
     case kTuneHistogram: return 16;
     case kTuneBaseline: return 6;
-    case kTuneLabel: return 2;
+    case kTuneLabel: return 4;
     case kTunePulse: return 5;
     default: return 0;
     }

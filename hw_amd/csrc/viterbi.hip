@@ -39,7 +39,7 @@ struct LabelerDev {
     float roiUpper, roiLower;
     uint32_t lanes, maxFrames;
     LabLaneState* state = nullptr;
-    uint32_t* tb = nullptr;  // [lane][maxFrames + 16][LS]
+    uint32_t* tb = nullptr;  // [lane][maxFrames + 16 + NS][LS]: predecessor words, then the anchor row
 };
 
 struct ScorerZ {
@@ -178,8 +178,9 @@ __device__ __forceinline__ int Pred(uint32_t w, int state)
 }
 
 // PB = frames per prefetched batch (8 or 16; an ROI word holds 16 frames).
-template <int PB>
-__global__ __launch_bounds__(64) void LabelKernel(const int16_t* __restrict__ traces, int frames,
+// MINW = waves per SIMD the register allocation must allow (1: unconstrained).
+template <int PB, int MINW>
+__global__ __launch_bounds__(64, MINW) void LabelKernel(const int16_t* __restrict__ traces, int frames,
                                                   const hwpa_lane_model* __restrict__ models, LabLaneState* state,
                                                   uint32_t* __restrict__ tbAll, int tbFrames,
                                                   int16_t* __restrict__ labels, int16_t* __restrict__ latentOut,
@@ -274,7 +275,10 @@ __global__ __launch_bounds__(64) void LabelKernel(const int16_t* __restrict__ tr
 #pragma unroll
     for (int s = 0; s < NS; ++s) ll[s] = (lb == s) ? 0.0f : hw_u2f(0xff800000u);
     ScorerZ sc;
-    float anchorLL[NS];
+    // Log-likelihoods at the chunk's last frame, read back once the anchor state is known:
+    // kept in the last NS rows of the lane's traceback buffer (13 registers fewer through
+    // the recursion, no LDS).
+    uint32_t* anchorLL = tb + (size_t)(tbFrames - NS) * LS;
     auto step = [&](int f, int raw) {
         const float d = (float)raw;
         const bool isRoi = ((roiWords[(f >> 4) * LS + z] >> (2 * (f & 15))) & 1u) != 0;
@@ -283,7 +287,7 @@ __global__ __launch_bounds__(64) void LabelKernel(const int16_t* __restrict__ tr
         tb[(size_t)f * LS] = Recur(T, s, ll, isRoi);
         if (f == frames - 1) {
 #pragma unroll
-            for (int r = 0; r < NS; ++r) anchorLL[r] = ll[r];
+            for (int r = 0; r < NS; ++r) anchorLL[(size_t)r * LS] = hw_f2u(ll[r]);
         }
     };
     // The latent frames are scored with the previous chunk's model, the chunk
@@ -343,9 +347,7 @@ __global__ __launch_bounds__(64) void LabelKernel(const int16_t* __restrict__ tr
         maxProb = cond ? maxProb : prob[i];
         anchorState = cond ? anchorState : i;
     }
-    float sv = anchorLL[0];
-#pragma unroll
-    for (int i = 1; i < NS; ++i) sv = (anchorState == i) ? anchorLL[i] : sv;
+    const float sv = hw_u2f(anchorLL[(size_t)anchorState * LS]);
     score[(size_t)lane * LS + z] = sv;
 
     // Traceback, PB predecessor words per batch with the next batch in flight.
@@ -530,7 +532,7 @@ int32_t LabelerCreate(const hwpa_config& cfg, uint32_t lanes, uint32_t max_frame
     l->maxFrames = max_frames;
     int32_t rc = DeviceAlloc((void**)&l->state, sizeof(LabLaneState) * lanes, "labeler state");
     if (rc == HWPA_OK)
-        rc = DeviceAlloc((void**)&l->tb, (size_t)lanes * (max_frames + HWPA_VITERBI_LOOKBACK) * LS * sizeof(uint32_t),
+        rc = DeviceAlloc((void**)&l->tb, (size_t)lanes * (max_frames + HWPA_VITERBI_LOOKBACK + NS) * LS * sizeof(uint32_t),
                          "labeler traceback");
     if (rc == HWPA_OK) {
         hipLaunchKernelGGL(LabelInitKernel, dim3(lanes), dim3(LS), 0, nullptr, l->state);
@@ -562,15 +564,18 @@ int32_t LabelerRun(LabelerDev* l, const int16_t* traces, uint32_t frames, const 
         return Fail(HWPA_ERR_INVALID_ARG, "labeler_run: frames outside (24, max_frames]");
     const int nroi = (int)frames + HWPA_VITERBI_LOOKBACK + HWPA_ROI_STITCH;
     const size_t lds = (size_t)((nroi + 15) / 16) * LS * sizeof(uint32_t);
-    // Variants: frames per prefetched batch.
-    if (TuneGet(kTuneLabel, l->lanes, frames) == 1)
-        hipLaunchKernelGGL(LabelKernel<8>, dim3(l->lanes), dim3(LS), lds, s, traces, (int)frames, models, l->state,
-                           l->tb, (int)(l->maxFrames + HWPA_VITERBI_LOOKBACK), labels, latent_out, score, l->T,
-                           l->roiUpper, l->roiLower);
-    else
-        hipLaunchKernelGGL(LabelKernel<16>, dim3(l->lanes), dim3(LS), lds, s, traces, (int)frames, models, l->state,
-                           l->tb, (int)(l->maxFrames + HWPA_VITERBI_LOOKBACK), labels, latent_out, score, l->T,
-                           l->roiUpper, l->roiLower);
+    // Variants: frames per prefetched batch x registers held to 4 waves per SIMD.
+#define HWPA_LABEL_LAUNCH(PB, W)                                                                                   \
+    hipLaunchKernelGGL((LabelKernel<PB, W>), dim3(l->lanes), dim3(LS), lds, s, traces, (int)frames, models,        \
+                       l->state, l->tb, (int)(l->maxFrames + HWPA_VITERBI_LOOKBACK + NS), labels, latent_out,      \
+                       score, l->T, l->roiUpper, l->roiLower)
+    switch (TuneGet(kTuneLabel, l->lanes, frames)) {
+    case 1: HWPA_LABEL_LAUNCH(8, 1); break;
+    case 2: HWPA_LABEL_LAUNCH(8, 4); break;
+    case 3: HWPA_LABEL_LAUNCH(16, 4); break;
+    default: HWPA_LABEL_LAUNCH(16, 1); break;
+    }
+#undef HWPA_LABEL_LAUNCH
     return CheckLaunch("LabelKernel");
 }
 

@@ -88,7 +88,7 @@ def test_baseliner_lane_tile_variants(H, O, T, gpu, variant, F):
 def test_autotune_picks_a_timed_variant(H, gpu):
     cfg = H.bench_config(512)
     r = H.autotune(cfg, 64, 512, repeats=2)
-    counts = {"histogram": 16, "baseline": 6, "labels": 2, "pulses": 5}
+    counts = {"histogram": 16, "baseline": 6, "labels": 4, "pulses": 5}
     for i, (name, n) in enumerate(counts.items()):
         assert len(r[name]) == H.tune_variants(i) == n
         assert all(t > 0 for t in r[name])
@@ -302,9 +302,10 @@ def test_dme_estimate_recurrence_edge_paths_bit_exact(H, O, T, gpu):
     _eq_struct(H.to_numpy(d_models, T.LANE_MODEL, (L,)), o_models, "models")
 
 
-@pytest.fixture(params=[0, 1])
+@pytest.fixture(params=[0, 1, 2, 3])
 def label_variant(request, H):
-    """Both autotuner variants of the labeler (16 or 8 frames per batch)."""
+    """Every autotuner variant of the labeler (16 or 8 frames per batch, registers
+    unconstrained or held to 4 waves per SIMD)."""
     for F in (512, 200):
         H.tune_set(H.TUNE_LABEL, 4, F, request.param)
     yield request.param
