@@ -60,9 +60,10 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
                                                   uint32_t* __restrict__ overflows, hwpa_stat_acc* __restrict__ blStats,
                                                   PulseConsts k)
 {
-    constexpr int QCAP = QUEUE ? 12 : 1;
-    static_assert(!QUEUE || QCAP >= 2 * UNROLL - UNROLL / 2, "queue holds the fill limit plus one batch");
-    constexpr int QFILL = QCAP - UNROLL;  // flush when any queue is above this after a batch
+    // A queue holds QFILL entries plus one batch; it is flushed when any ZMW's holds more
+    // than QFILL after a batch, so it never overflows.
+    constexpr int QFILL = 4;
+    constexpr int QCAP = QUEUE ? UNROLL + QFILL : 1;
     __shared__ uint32_t queue[QCAP * 6 * LS];
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
@@ -272,6 +273,8 @@ int32_t PulserRun(PulserDev* p, const int16_t* labels, const int16_t* latent, co
     case 2: HWPA_PULSE_LAUNCH(16, true, false); break;
     case 3: HWPA_PULSE_LAUNCH(8, false, true); break;
     case 4: HWPA_PULSE_LAUNCH(8, true, true); break;
+    case 5: HWPA_PULSE_LAUNCH(4, true, true); break;
+    case 6: HWPA_PULSE_LAUNCH(4, false, true); break;
     default: HWPA_PULSE_LAUNCH(8, false, false); break;
     }
 #undef HWPA_PULSE_LAUNCH
