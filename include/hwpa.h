@@ -142,11 +142,14 @@ int32_t hwpa_batch_autotune(hwpa_batch* b, int32_t repeats);
 
 /* ---- kernel-shape autotuner (no reference counterpart) -------------------
  * Tunable kernels run in variants that give the same bits with different
- * shapes: kernel 0 = histogram (8, 16 or 32 frames per prefetched batch x
- * 128, 160 or 192 bins kept in LDS: 9 variants), kernel 1 = baseline filter (1, 2 or 4 lanes per
+ * shapes: kernel 0 = histogram (variants 0-8: one wave per lane, 8, 16 or 32
+ * frames per prefetched batch x 128, 160 or 192 bins kept in LDS; 9-15: 2, 4 or
+ * 8 waves per lane sharing its LDS counters), kernel 1 = baseline filter (1, 2 or 4 lanes per
  * workgroup, then the same with two groups of raw frames in flight), kernel 2 = Viterbi labeler (16 or 8 frames per prefetched
- * batch), kernel 3 = pulse accumulator (8 frames, 8 frames with the next
- * batch in flight, 16 with the next in flight).  The launchers look up the variant for their (lanes, frames);
+ * batch, then the same with registers held to 4 waves per SIMD), kernel 3 = pulse accumulator (8 frames,
+ * 8 frames with the next batch in flight, 16 with the next in flight, then 8, 8 with
+ * prefetch, 4 with prefetch and 4 frames writing records through an LDS queue).
+ * The launchers look up the variant for their (lanes, frames);
  * unset shapes use variant 0.  hwpa_autotune times every variant on
  * synthetic traces of the shape (`repeats` timed runs each, HIP events) and
  * keeps the fastest; ms (optional, ms_len entries) receives the mean times
