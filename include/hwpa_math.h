@@ -164,18 +164,6 @@ HWPA_HD float hw_log_reduce(float x, int* e_out)
     return hw_log_reduce_bits(u, e0, e_out);
 }
 
-/* hw_logf for finite x >= FLT_MIN: the same operations without the special
- * cases, so the same bits. */
-HWPA_HD float hw_logf_normal(float x)
-{
-    int e;
-    const float lm = hw_log_reduce_bits(hw_f2u(x), 0, &e);
-    const float ef = (float)e;
-    const float ln2hi = 0.693145751953125f;
-    const float ln2lo = 1.428606765330187045e-06f;
-    return hw_fmaf(ef, ln2hi, hw_fmaf(ef, ln2lo, lm));
-}
-
 HWPA_HD float hw_logf(float x)
 {
     if (hw_isnan(x)) return x;

@@ -2997,7 +2997,7 @@ extern "C" {
 
 // Element-wise evaluation of the shared math routines of include/hwpa_math.h.
 // kind: 0 exp, 1 log, 2 log2, 3 exp2, 4 pow(a,b), 5 f->half->f, 6 round_i16, 7 floor_i32,
-// 8 log of a normal float, 9 / 10 table-driven log / reciprocal of a normal float
+// 9 / 10 table-driven log / reciprocal of a normal float
 void orc_math(int32_t kind, const float* a, const float* b, float* out, int32_t n)
 {
     for (int32_t i = 0; i < n; ++i) {
@@ -3009,7 +3009,6 @@ void orc_math(int32_t kind, const float* a, const float* b, float* out, int32_t 
         case 4: out[i] = hw_powf(a[i], b[i]); break;
         case 5: out[i] = hw_h2f(hw_f2h(a[i])); break;
         case 6: out[i] = (float)hw_round_i16(a[i]); break;
-        case 8: out[i] = hw_logf_normal(a[i]); break;
         case 9:
         case 10: {
             float lg, rcp;

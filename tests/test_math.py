@@ -107,18 +107,6 @@ def test_round_and_floor_conversions(O):
     assert list(f) == [-1, 0, 299, -2147483648.0]
 
 
-def test_log_normal_entry_matches_log(O):
-    """hw_logf_normal (the EM's log of bin probabilities, which are >= FLT_MIN
-    and finite) gives hw_logf's bits on every positive normal float: all
-    binades at a stride, plus both ends."""
-    lo = np.float32(1.1754944e-38).view(np.uint32)
-    hi = np.float32(3.4028235e38).view(np.uint32)
-    bits = np.concatenate([np.arange(lo, hi, 9973, dtype=np.uint32), np.uint32([lo, hi, 0x3f800000, 0x3f3504f3,
-                                                                                  0x3fb504f3, 0x3fb504f4])])
-    x = bits.view(np.float32)
-    assert np.array_equal(_eval(O, 8, x).view(np.uint32), _eval(O, 1, x).view(np.uint32))
-
-
 def test_table_log_and_reciprocal_accuracy(O):
     """hw_log_rcp_tab (the EM's log and reciprocal of bin probabilities, positive
     normal floats): within 2 ulp of the float64 log where |log x| >= 0.5, within 1e-7
