@@ -1,4 +1,4 @@
-"""Training driver of the learned EM initialiser (no reference counterpart).
+"""Training loop of the learned EM initialiser (no reference counterpart).
 
 The plain pipeline generates the training pairs: an analyzer with capture
 armed writes, at every plain EM estimate, each ZMW's network input (its
