@@ -340,6 +340,49 @@ def test_labeler_bit_exact(H, O, T, gpu, label_variant, F):
         assert np.array_equal(H.to_numpy(d_score, np.float32, (L, 64)).view(np.uint32), o_score.view(np.uint32))
 
 
+@pytest.mark.parametrize("variant", [0, 3, 4, 5, 6])
+@pytest.mark.parametrize("max_calls", [0, 480])
+def test_pulses_dense_boundaries_bit_exact(H, O, T, gpu, variant, max_calls):
+    """A pulse boundary at every frame (up states cycling through the analogs, with a
+    baseline frame every 7th): about one record per frame and ZMW, so the queued variants
+    fill their LDS queues to capacity before flushes; with the default capacity every ZMW
+    overflows, with 480 slots every record is kept."""
+    L, F, C = 2, 512, 2
+    H.tune_set(H.TUNE_PULSE, L, F, variant)
+    try:
+        cfg = H.bench_config(F)
+        if max_calls:
+            cfg.max_calls_per_zmw = max_calls
+        cfg, chunks = _pipeline_inputs(H, O, T, L, F, C, seed=45, cfg=cfg)
+        agg = O.Aggregator(L)
+        op, gp = O.Pulser(cfg, L), H.Pulser(cfg, L)
+        n, mc = L * 64, cfg.max_calls_per_zmw
+        d_p = H.dev_bytes(n * mc * T.PULSE.itemsize)
+        d_c, d_o = H.dev_bytes(n * 4), H.dev_bytes(n * 4)
+        d_bl = H.dev_bytes(L * T.STAT_ACC.itemsize)
+        f = np.arange(F)[None, :, None] + np.arange(64)[None, None, :]
+        lab = np.where(f % 7 == 6, 0, 5 + f % 4).astype(np.int16) * np.ones((L, 1, 1), np.int16)
+        lat = np.zeros((L, 24, 64), np.int16)
+        for c, (bl, st) in enumerate(chunks):
+            agg.add(st)
+            models = O.dme_init_models(cfg, agg.get(T.BASELINER_STATS), T.LANE_MODEL)
+            ff = c * F - 24
+            o_p, o_c, o_o, o_bl = op.run(lab, lat, bl, ff, models, T.PULSE, T.STAT_ACC)
+            gp.run(_dev(H, lab), _dev(H, lat), _dev(H, bl), F, ff, _dev(H, models), d_p, d_c, d_o, d_bl)
+            assert np.array_equal(H.to_numpy(d_c, np.uint32, (n,)), o_c)
+            assert np.array_equal(H.to_numpy(d_o, np.uint32, (n,)), o_o)
+            assert (o_c + o_o).min() >= F * 6 // 7 - 2
+            assert (o_o.min() > 0) if not max_calls else (o_o.max() == 0)
+            g_p = H.to_numpy(d_p, T.PULSE, (n, mc))
+            for z in range(n):
+                k = o_c[z]
+                assert np.array_equal(np.ascontiguousarray(g_p[z, :k]).view(np.uint8),
+                                      np.ascontiguousarray(o_p[z, :k]).view(np.uint8)), (c, z)
+            _eq_struct(H.to_numpy(d_bl, T.STAT_ACC, (L,)), o_bl, "pulse baseline stats")
+    finally:
+        H.tune_set(H.TUNE_PULSE, L, F, 0)
+
+
 @pytest.mark.parametrize("variant,max_calls", [(0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (0, 5), (3, 5), (4, 5), (5, 5)])
 def test_pulses_bit_exact(H, O, T, gpu, variant, max_calls):
     """Pulse records, counts, overflow counts and baseline stats of every pulse-kernel
