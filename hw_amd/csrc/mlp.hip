@@ -475,8 +475,12 @@ static int32_t ColSum(const void* d, bool bf16, uint32_t rows, uint32_t cols, fl
 // GemmKernel, so outputs equal the three-GEMM pass bit for bit -- but the hidden
 // activations stay in LDS (written transposed into row-major tiles, read back as the
 // next layer's A operand) instead of HBM; h1 / h2 are written out only when given
-// (training keeps them for the backward pass).  Weights come from L2.
-constexpr int kFusedRows = 32, kFusedWaves = 4, kHsStride = MH + 8;
+// (training keeps them for the backward pass).  The weights of each 16-wide k-step of
+// layers 1 and 2 (all 256 output columns: 8 KB) are staged once per workgroup in LDS,
+// double-buffered with one barrier per step, so the 8 waves (256 rows) of a workgroup
+// share one read of them instead of each wave fetching all 192 KB from L2.
+constexpr int kFusedRows = 32, kFusedWaves = 8, kHsStride = MH + 8;
+constexpr int kSlab = MH * 16;   // bf16 weights of one k-step: [256 columns][16 k]
 
 __device__ __forceinline__ bf16x8 LoadRow8(const uint16_t* __restrict__ p)
 {
@@ -489,13 +493,21 @@ __device__ __forceinline__ bf16x8 LoadRow8(const uint16_t* __restrict__ p)
 // 16-wide k-step): it sums exactly those bin pairs, the two halves' partial totals give the
 // row total, and the scaled bf16 inputs stay in registers.
 template <bool HIST>
-__global__ __launch_bounds__(256) void MlpFusedForwardKernel(const uint16_t* __restrict__ x,
+__global__ __launch_bounds__(64 * kFusedWaves) void MlpFusedForwardKernel(const uint16_t* __restrict__ x,
                                                              const hwpa_lane_hist* __restrict__ hists, uint32_t rows,
                                                              const uint16_t* __restrict__ Wb,
                                                              const float* __restrict__ P, float* __restrict__ y,
                                                              uint16_t* __restrict__ h1out, uint16_t* __restrict__ h2out)
 {
     __shared__ __align__(16) uint16_t hs[kFusedWaves][kFusedRows][kHsStride];
+    __shared__ __align__(16) uint16_t ws[2][kSlab];
+    static_assert(64 * kFusedWaves == 2 * MH, "one 16-byte half column of a slab per thread");
+    // Thread t stages half t & 1 of column t >> 1 of a k-step's weights (16 bytes).
+    const int sn = threadIdx.x >> 1, sh = (threadIdx.x & 1) * 8;
+    auto slabLoad = [&](int off, int kdim, int k0) {
+        return *reinterpret_cast<const uint4*>(Wb + off + (size_t)sn * kdim + k0 + sh);
+    };
+    auto slabStore = [&](int buf, uint4 v) { *reinterpret_cast<uint4*>(&ws[buf][sn * 16 + sh]) = v; };
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int r = l & 31, kq = (l >> 5) * 8;
     const int64_t m0 = ((int64_t)blockIdx.x * kFusedWaves + wave) * kFusedRows;
@@ -537,8 +549,13 @@ __global__ __launch_bounds__(256) void MlpFusedForwardKernel(const uint16_t* __r
 #pragma unroll
     for (int j = 0; j < MH / 32; ++j) acc[j] = f32x16{};
     const uint16_t* xr = HIST ? nullptr : x + (m0 + r) * MI;
+    slabStore(0, slabLoad(OFF_W1, MI, 0));
+    __syncthreads();
 #pragma unroll
     for (int k0 = 0; k0 < MI; k0 += 16) {
+        const int buf = (k0 / 16) & 1;
+        uint4 next{};
+        if (k0 + 16 < MI) next = slabLoad(OFF_W1, MI, k0 + 16);
         bf16x8 a;
         if (HIST)
             a = __builtin_bit_cast(bf16x8, make_uint4(xin[k0 / 16][0], xin[k0 / 16][1], xin[k0 / 16][2],
@@ -547,9 +564,11 @@ __global__ __launch_bounds__(256) void MlpFusedForwardKernel(const uint16_t* __r
             a = rowValid ? LoadRow8(xr + k0 + kq) : zero8;
         bf16x8 b[MH / 32];
 #pragma unroll
-        for (int j = 0; j < MH / 32; ++j) b[j] = LoadRow8(Wb + OFF_W1 + (size_t)(j * 32 + r) * MI + k0 + kq);
+        for (int j = 0; j < MH / 32; ++j) b[j] = LoadRow8(&ws[buf][(j * 32 + r) * 16 + kq]);
 #pragma unroll
         for (int j = 0; j < MH / 32; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
+        if (k0 + 16 < MI) slabStore(buf ^ 1, next);
+        __syncthreads();
     }
 #pragma unroll
     for (int j = 0; j < MH / 32; ++j) {
@@ -569,15 +588,22 @@ __global__ __launch_bounds__(256) void MlpFusedForwardKernel(const uint16_t* __r
     // ---- layer 2: [32 x MH] x [MH x MH]
 #pragma unroll
     for (int j = 0; j < MH / 32; ++j) acc[j] = f32x16{};
+    // Layer 1's last barrier has passed: every wave is done with both slab buffers.
+    slabStore(0, slabLoad(OFF_W2, MH, 0));
+    __syncthreads();
     for (int k0 = 0; k0 < MH; k0 += 16) {
+        const int buf = (k0 / 16) & 1;
+        uint4 next{};
+        if (k0 + 16 < MH) next = slabLoad(OFF_W2, MH, k0 + 16);
         const bf16x8 a = LoadRow8(&h[r][k0 + kq]);
         bf16x8 b[MH / 32];
 #pragma unroll
-        for (int j = 0; j < MH / 32; ++j) b[j] = LoadRow8(Wb + OFF_W2 + (size_t)(j * 32 + r) * MH + k0 + kq);
+        for (int j = 0; j < MH / 32; ++j) b[j] = LoadRow8(&ws[buf][(j * 32 + r) * 16 + kq]);
 #pragma unroll
         for (int j = 0; j < MH / 32; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
+        if (k0 + 16 < MH) slabStore(buf ^ 1, next);
+        __syncthreads();
     }
-    __syncthreads();
 #pragma unroll
     for (int j = 0; j < MH / 32; ++j) {
         const int n = j * 32 + r;
@@ -617,7 +643,8 @@ static int32_t Forward(const hwpa_initializer* m, const uint16_t* x, uint32_t ro
     if (rows == 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: no rows");
     if (((uintptr_t)x & 15) != 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: input not 16-byte aligned");
     const uint32_t per = (uint32_t)(kFusedRows * kFusedWaves);
-    hipLaunchKernelGGL(MlpFusedForwardKernel<false>, dim3((rows + per - 1) / per), dim3(256), 0, s, x, nullptr, rows,
+    hipLaunchKernelGGL(MlpFusedForwardKernel<false>, dim3((rows + per - 1) / per), dim3(64 * kFusedWaves), 0, s, x,
+                       nullptr, rows,
                        m->Pb, m->P, y, h1, h2);
     return CheckLaunch("MlpFusedForwardKernel");
 }
@@ -629,7 +656,8 @@ static int32_t ForwardFromHist(const hwpa_initializer* m, const hwpa_lane_hist* 
     if (lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: no lanes");
     const uint32_t rows = lanes * (uint32_t)LS;
     const uint32_t per = (uint32_t)(kFusedRows * kFusedWaves);
-    hipLaunchKernelGGL(MlpFusedForwardKernel<true>, dim3((rows + per - 1) / per), dim3(256), 0, s, nullptr, hist, rows,
+    hipLaunchKernelGGL(MlpFusedForwardKernel<true>, dim3((rows + per - 1) / per), dim3(64 * kFusedWaves), 0, s, nullptr,
+                       hist, rows,
                        m->Pb, m->P, y, nullptr, nullptr);
     return CheckLaunch("MlpFusedForwardKernel");
 }
