@@ -521,16 +521,30 @@ __global__ __launch_bounds__(64 * kFusedWaves) void MlpFusedForwardKernel(const 
     // ---- inputs from the histogram (HIST)
     uint32_t xin[MI / 16][4];
     if (HIST) {
+        // The wave's 32 ZMWs x 2 MI bins of counters go through its activation buffer: 16-byte
+        // loads of 8 ZMWs of one bin (a wave instruction moves 16 bins), then each lane reads
+        // its ZMW's bins from LDS.  Rows come in whole half-lanes, so a wave is valid or not.
+        static_assert(2 * MI * kFusedRows <= kFusedRows * kHsStride, "counters fit the activation buffer");
+        uint16_t* hc = &h[0][0];   // [bin][32 ZMWs]
+        if (m0 < (int64_t)rows) {
+            const uint16_t* lb = &hists[m0 / LS].bin_count[0][m0 % LS];
+#pragma unroll 4
+            for (int it = 0; it < 2 * MI / 16; ++it) {
+                const int bin = 16 * it + (l >> 2), piece = (l & 3) * 8;
+                *reinterpret_cast<uint4*>(hc + bin * kFusedRows + piece) =
+                    *reinterpret_cast<const uint4*>(lb + (size_t)bin * LS + piece);
+            }
+        }
+        __syncthreads();
         uint32_t pair[MI / 16][8];
         uint32_t part = 0;
-        const int64_t row = rowValid ? m0 + r : 0;
-        const uint16_t* bc = &hists[row / LS].bin_count[0][row % LS];
 #pragma unroll
         for (int s = 0; s < MI / 16; ++s)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int i = 16 * s + kq + j;
-                const uint32_t v = rowValid ? (uint32_t)bc[(2 * i) * LS] + (uint32_t)bc[(2 * i + 1) * LS] : 0u;
+                const uint32_t v =
+                    rowValid ? (uint32_t)hc[(2 * i) * kFusedRows + r] + (uint32_t)hc[(2 * i + 1) * kFusedRows + r] : 0u;
                 pair[s][j] = v;
                 part += v;
             }
