@@ -440,8 +440,8 @@ uint32_t BaselinerStartupLatency(const hwpa_config& cfg)
 {
     // MultiScaleBaseliner::StartupLatency (MultiScaleBaseliner.h:56-99)
     const hwpa_filter_params fp = FilterParams(cfg.filter_type);
-    const float meanAlpha = hw_powf(0.5f, 1.0f / cfg.mean_ema_scale_strides);
-    const float sigmaAlpha = hw_exp2f(-1.0f / cfg.sigma_ema_scale_strides);
+    const float meanAlpha = std::pow(0.5f, 1.0f / cfg.mean_ema_scale_strides);
+    const float sigmaAlpha = std::exp2(-1.0f / cfg.sigma_ema_scale_strides);
     const float mess = -1.0f / std::log2(meanAlpha);
     const float sess = -1.0f / std::log2(sigmaAlpha);
     return (uint32_t)((float)fp.latent_size + std::max(mess, sess) * (float)fp.aggregate_stride);
@@ -454,10 +454,11 @@ int32_t BaselinerCreate(const hwpa_config& cfg, uint32_t lanes, BaselinerDev** o
     b->lanes = lanes;
     // LaneBaseliner ctor (HostMultiScaleBaseliner.h:58-66), MultiScaleBaseliner::Configure
     // (MultiScaleBaseliner.cpp:53-75)
-    b->k.cMeanBias = b->fp.mean_bias * hw_exp2f(cfg.mean_bias_adjust);
-    b->k.cSigmaBias = b->fp.sigma_bias * hw_exp2f(cfg.sigma_bias_adjust);
-    b->k.meanEmaAlpha = hw_powf(0.5f, 1.0f / cfg.mean_ema_scale_strides);
-    b->k.sigmaEmaAlpha = hw_exp2f(-1.0f / cfg.sigma_ema_scale_strides);
+    // Host constants from libm's float functions, as the reference computes them.
+    b->k.cMeanBias = b->fp.mean_bias * std::exp2(cfg.mean_bias_adjust);
+    b->k.cSigmaBias = b->fp.sigma_bias * std::exp2(cfg.sigma_bias_adjust);
+    b->k.meanEmaAlpha = std::pow(0.5f, 1.0f / cfg.mean_ema_scale_strides);
+    b->k.sigmaEmaAlpha = std::exp2(-1.0f / cfg.sigma_ema_scale_strides);
     b->k.jumpTol = (cfg.jump_suppression > 0.0f) ? 1.0f / cfg.jump_suppression : INFINITY;
     b->k.scaler = cfg.movie.photoelectron_sensitivity;
     b->k.pedestal = cfg.movie.pedestal;

@@ -53,7 +53,7 @@ static DmeDevCfg MakeDevCfg(const hwpa_config& c)
     d.successThresh = c.success_confidence_thresh;
     d.hl0 = c.confidence_half_life[0];
     d.hl1 = c.confidence_half_life[1];
-    d.log2pi = hw_logf(2.0f * 3.1415926536f);
+    d.log2pi = std::log(2.0f * 3.1415926536f);   // libm, scalar (DmeEmHost.cpp:380)
     d.iterLimit = (int32_t)c.em_iteration_limit;
     d.iterToLimit = c.iterate_to_limit;
     d.updateMethod = (int32_t)c.model_update_method;
@@ -135,7 +135,7 @@ __device__ void UpdateConf(const DmeDevCfg& d, Model& t, const Model& o)
         const float bw = a * t.bl.w + b * o.bl.w;
         t.bl.w = bw;
         t.bl.mean = a * t.bl.mean + b * o.bl.mean;
-        const float bv = hw_powf(t.bl.var, a) * hw_powf(o.bl.var, b);
+        const float bv = hw_dpowf(t.bl.var, a) * hw_dpowf(o.bl.var, b);
         t.bl.var = bv;
         const float aw = 0.25f * (1.0f - bw);
 #pragma unroll
@@ -153,7 +153,7 @@ __device__ void UpdateConf(const DmeDevCfg& d, Model& t, const Model& o)
             } else {
                 cv2 = d.cv[i] * d.cv[i];
             }
-            const float am = hw_powf(t.an[i].mean, a) * hw_powf(o.an[i].mean, b);
+            const float am = hw_dpowf(t.an[i].mean, a) * hw_dpowf(o.an[i].mean, b);
             t.an[i].mean = am;
             t.an[i].var = Covar(d, cv2, am, bv);
         }
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             varinv[k] = 1.0f / var[k];
-            pref[k] = (hw_logf(rho[k]) - 0.5f * hw_logf(var[k])) - 0.5f * d.log2pi;
+            pref[k] = (hw_cephes_logf(rho[k]) - 0.5f * hw_cephes_logf(var[k])) - 0.5f * d.log2pi;
             nhv[k] = -0.5f * varinv[k];
         }
         // Every use of a responsibility is boundaryProb * tau_i = exp(t_i)
@@ -570,7 +570,7 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
             mom1[m] *= halfBin;
         }
         const float logLikePrev = logLike;
-        logLike = wps - njSum * hw_logf(probSum);
+        logLike = wps - njSum * hw_cephes_logf(probSum);
         const float ds = s - sExpect;
         logLike -= 0.5f * (ds * ds) * sExpectWeight;
         const float delta = logLike - logLikePrev;
@@ -650,10 +650,10 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
         cf[0] = converged ? 1.0f : 0.0f;
         cf[1] = hw_satlin(0.05f, 0.15f, work.bl.w);
         float x = (work.bl.mean * work.bl.mean) / work.bl.var;
-        cf[2] = hw_expf(-x / (2.0f * (1.0f * 1.0f)));
-        x = hw_log2f(work.bl.var / det.bl.var);
+        cf[2] = hw_cephes_expf(-x / (2.0f * (1.0f * 1.0f)));
+        x = hw_dlog2f(work.bl.var / det.bl.var);
         const float tol = 1.5f / (0.5f + det.conf);
-        cf[3] = hw_expf(-x * x / (2.0f * (tol * tol)));
+        cf[3] = hw_cephes_expf(-x * x / (2.0f * (tol * tol)));
         x = 1.0f;
         if (d.mixThresh1 > 0.0f) {
             x *= hw_satlin(d.mixThresh0, d.mixThresh1, work.an[0].w);
@@ -853,7 +853,7 @@ static float EvolveFactor(const DmeDevCfg& d, const int32_t model_fi[2], const i
     const float thl = 0.5f * (float)(tModel + tEst);
     const float hl = d.hl0 + (d.hl1 - d.hl0) * hw_satlin(56160.0f, 168480.0f, thl);
     const float tDiff = (float)(tEst - tModel);
-    return hw_exp2f(-hw_fabsf(tDiff) / hl);
+    return hw_dexp2f(-hw_fabsf(tDiff) / hl);
 }
 
 int32_t DmeEstimate(const hwpa_config& cfg, const hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,

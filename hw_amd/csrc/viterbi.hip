@@ -62,7 +62,7 @@ __device__ void ScorerSetup(ScorerZ& s, const hwpa_lane_model& m, int z)
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
         const float aMean = hw_h2f(m.analogs[a].means[z]);
-        s.logPMean[a] = hw_logf(aMean - bMean);
+        s.logPMean[a] = hw_cephes_logf(aMean - bMean);
         const float aVar = hw_h2f(m.analogs[a].vars[z]);
         const float aSigma = hw_sqrtf(aVar);
         s.x1[a] = aMean - aSigma * sqrtHalfPi;
@@ -76,15 +76,15 @@ __device__ void ScorerSetup(ScorerZ& s, const hwpa_lane_model& m, int z)
         eVar += bVar + aVar;
         eVar *= 0.5f;
         s.dimMean = eMean;
-        s.dimNorm = -0.5f * (log2pi + hw_logf(eVar));
+        s.dimNorm = -0.5f * (log2pi + hw_cephes_logf(eVar));
         s.dimScale = 0.5f / eVar;
     }
     const float normConst = log2pi / 2.0f;
-    s.bgFixed = -0.5f * hw_logf(bVar) - normConst;
+    s.bgFixed = -0.5f * hw_cephes_logf(bVar) - normConst;
     s.bgMean = bMean;
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
-        s.ffFixed[a] = -0.5f * hw_logf(hw_h2f(m.analogs[a].vars[z])) - normConst;
+        s.ffFixed[a] = -0.5f * hw_cephes_logf(hw_h2f(m.analogs[a].vars[z])) - normConst;
         s.ffMean[a] = hw_h2f(m.analogs[a].means[z]);
     }
 }
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(64, MINW) void LabelKernel(const int16_t* __restric
         float sum = 0.0f;
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
-            prob[i] = hw_expf(ll[i] - mx);
+            prob[i] = hw_cephes_expf(ll[i] - mx);
             sum += prob[i];
         }
         const float inv = 1 / sum;

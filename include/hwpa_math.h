@@ -14,6 +14,10 @@
  *   log(x)  = e ln2 + 2 atanh(s),  s = (m-1)/(m+1),  m in [sqrt(1/2), sqrt(2)),
  *             atanh series to s^13 (truncation < 2e-10).
  * Accuracy is a few ulp; tests/test_math.py bounds the error against libm.
+ * Where the reference's own transcendental bits are cheap to reproduce, the
+ * kernels use its routines instead: hw_cephes_expf / hw_cephes_logf restate
+ * the Cephes exp / log of its SSE host build, and hw_dexp2f / hw_dlog2f /
+ * hw_dpowf give the libm values it calls for exp2, log2 and pow.
  *
  * Conversions follow the reference's SSE LaneArray semantics:
  *   float -> int32 : round-to-nearest-even, 0x80000000 when out of range or NaN
@@ -25,6 +29,9 @@
 #define HWPA_MATH_H
 
 #include <stdint.h>
+#if !defined(__HIPCC__)
+#include <math.h>
+#endif
 
 #if defined(__HIPCC__)
 #define HWPA_HD __host__ __device__ inline
@@ -236,6 +243,112 @@ HWPA_HD float hw_exp2f(float x)
     const float v = hw_ldexpf(hw_exp_poly(r), (int)kf);
     return hw_isnan(x) ? x : v;
 }
+
+/* The reference's LaneArray exp and log as its x86 host build evaluates them:
+ * m512f exp/log (common/simd/m512f_SSE.h:444-466) call _mm_exp_ps/_mm_log_ps
+ * (xcompile.h:148-151), i.e. sse_mathfun's exp_ps/log_ps, the Cephes library's
+ * single-precision expf/logf (S. L. Moshier) on SSE registers.  Restated here
+ * per element with the same basic operations in the same order, no fused steps,
+ * so host and GPU reproduce its bits:
+ *   exp(x): x clamped to +-88.3762626647949, n = floor(x log2(e) + 1/2) by
+ *           truncation and correction, r = (x - n C1) - n C2 with
+ *           C1 + C2 = ln 2, p(r) = 1 + r + r^2 (degree-5 minimax polynomial),
+ *           times 2^n built from its exponent bits.
+ *   log(x): x raised to FLT_MIN, x = m 2^e with m in [1/2, 1); if m < sqrt(1/2)
+ *           then r = 2m - 1, e -= 1, else r = m - 1; log = r + r^3 P(r)
+ *           - r^2/2 + e (q1 + q2) with q1 + q2 = ln 2; x <= 0 gives NaN
+ *           (all bits set), x == 0 gives -inf.
+ * A NaN argument of exp returns exp(88.376...) (+inf), of log returns
+ * log(FLT_MIN), as the SSE min/max rule makes them.  Coefficients are the
+ * Cephes ones, rounded from double as the reference's constant tables are. */
+HWPA_HD float hw_cephes_expf(float x)
+{
+    x = hw_minf(x, (float)88.3762626647949);
+    x = hw_maxf(x, (float)-88.3762626647949);
+    float fx = x * (float)1.44269504088896341;
+    fx = fx + 0.5f;
+    float t = (float)(int32_t)fx;
+    if (t > fx) t = t - 1.0f;
+    fx = t;
+    float p = fx * (float)0.693359375;
+    const float q = fx * (float)-2.12194440e-4;
+    x = x - p;
+    x = x - q;
+    const float z = x * x;
+    float y = (float)1.9875691500E-4;
+    y = y * x;
+    y = y + (float)1.3981999507E-3;
+    y = y * x;
+    y = y + (float)8.3334519073E-3;
+    y = y * x;
+    y = y + (float)4.1665795894E-2;
+    y = y * x;
+    y = y + (float)1.6666665459E-1;
+    y = y * x;
+    y = y + (float)5.0000001201E-1;
+    y = y * z;
+    y = y + x;
+    y = y + 1.0f;
+    p = hw_u2f((uint32_t)((int32_t)fx + 127) << 23);
+    return y * p;
+}
+
+HWPA_HD float hw_cephes_logf(float x)
+{
+    const bool invalid = x <= 0.0f;
+    const bool zero = x == 0.0f;
+    x = hw_maxf(x, hw_u2f(0x00800000u));
+    const uint32_t u = hw_f2u(x);
+    float e = (float)((int32_t)(u >> 23) - 127);
+    x = hw_u2f((u & ~0x7f800000u) | 0x3f000000u);
+    e = e + 1.0f;
+    const bool small = x < (float)0.707106781186547524;
+    const float t = small ? x : 0.0f;
+    x = x - 1.0f;
+    e = e - (small ? 1.0f : 0.0f);
+    x = x + t;
+    const float z = x * x;
+    float y = (float)7.0376836292E-2;
+    y = y * x;
+    y = y + (float)-1.1514610310E-1;
+    y = y * x;
+    y = y + (float)1.1676998740E-1;
+    y = y * x;
+    y = y + (float)-1.2420140846E-1;
+    y = y * x;
+    y = y + (float)1.4249322787E-1;
+    y = y * x;
+    y = y + (float)-1.6668057665E-1;
+    y = y * x;
+    y = y + (float)2.0000714765E-1;
+    y = y * x;
+    y = y + (float)-2.4999993993E-1;
+    y = y * x;
+    y = y + (float)3.3333331174E-1;
+    y = y * x;
+    y = y * z;
+    float s = e * (float)-2.12194440e-4;
+    y = y + s;
+    s = z * 0.5f;
+    y = y - s;
+    s = e * (float)0.693359375;
+    x = x + y;
+    x = x + s;
+    if (invalid) x = hw_u2f(0xffffffffu);
+    if (zero) x = hw_u2f(0xff800000u);
+    return x;
+}
+
+/* The reference calls libm per element for LaneArray exp2, log2 and pow
+ * (common/simd/xcompile.h:180-197, m512f.h:34-41).  These evaluate the
+ * double-precision function and round once to float: the correctly rounded
+ * float result (the host and the GPU double functions are both within an ulp
+ * of the exact value, far inside the float rounding step), which is libm's
+ * float result except on the under 0.1% of arguments where glibc's exp2f /
+ * log2f / powf are off by one ulp. */
+HWPA_HD float hw_dexp2f(float x) { return (float)exp2((double)x); }
+HWPA_HD float hw_dlog2f(float x) { return (float)log2((double)x); }
+HWPA_HD float hw_dpowf(float x, float y) { return (float)pow((double)x, (double)y); }
 
 /* x^y for x > 0; exact for y == 0 and y == 1. */
 HWPA_HD float hw_powf(float x, float y)

@@ -41,7 +41,15 @@ class DmeDiag(ctypes.Structure):
                 ("status", ctypes.c_int32 * LANE), ("iterations_run", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("log_like", ctypes.c_float * LANE), ("conf_factors", (ctypes.c_float * LANE) * 8),
                 ("scale_factor", ctypes.c_float * LANE), ("confidence", ctypes.c_float * LANE),
-                ("est_fp32", ((ctypes.c_float * LANE) * 3) * 5), ("out_fp32", ((ctypes.c_float * LANE) * 3) * 5)]
+                ("est_fp32", ((ctypes.c_float * LANE) * 3) * 5), ("out_fp32", ((ctypes.c_float * LANE) * 3) * 5),
+                ("capture_iter", ctypes.c_int32 * LANE)]
+
+
+# DME forms (oracle.h): "reference" follows DmeEmHost's arithmetic (log-sum-exp E-step,
+# the reference's Cephes exp/log, libm exp2/log2/pow); "mirror" is the GPU kernels'
+# restated E-step and math routines, bit for bit; "reference_libm" is "reference"
+# with libm expf/logf in place of the Cephes routines (a sensitivity yardstick).
+FORMS = {"reference": 0, "mirror": 1, "reference_libm": 2}
 
 
 def threads(n):
@@ -142,13 +150,14 @@ def dme_init_models(cfg, stats, model_dtype):
     return out
 
 
-def dme_estimate(cfg, hist, stats, models, model_fi, est_fi, want_diag=False):
+def dme_estimate(cfg, hist, stats, models, model_fi, est_fi, want_diag=False, form="reference"):
     models = np.ascontiguousarray(models).copy()
     lanes = len(models)
     mfi = (ctypes.c_int32 * 2)(*model_fi)
     efi = (ctypes.c_int32 * 2)(*est_fi)
     diag = (DmeDiag * lanes)() if want_diag else None
-    lib.orc_dme_estimate(ctypes.byref(cfg), ctypes.c_int32(lanes), _p(np.ascontiguousarray(hist)),
+    lib.orc_dme_estimate(ctypes.c_int32(FORMS[form]), ctypes.byref(cfg), ctypes.c_int32(lanes),
+                         _p(np.ascontiguousarray(hist)),
                          _p(np.ascontiguousarray(stats)), _p(models), mfi, efi, diag)
     return (models, diag) if want_diag else models
 
@@ -209,12 +218,14 @@ class Result(ctypes.Structure):
 
 
 class Analyzer:
-    """DynamicEstimateBatchAnalyzer restated on the CPU."""
+    """BatchAnalyzer (the configured mode) restated on the CPU; em selects the
+    DME form ("reference" or "mirror", see FORMS)."""
 
-    def __init__(self, cfg, pool_id, lanes, frames):
+    def __init__(self, cfg, pool_id, lanes, frames, em="reference"):
         lib.orc_analyzer_create.restype = _vp
         self.h = _vp(lib.orc_analyzer_create(ctypes.byref(cfg), ctypes.c_uint32(pool_id), ctypes.c_int32(lanes),
                                              ctypes.c_int32(frames)))
+        lib.orc_analyzer_set_em_form(self.h, ctypes.c_int32(FORMS[em]))
         self.lanes, self.frames, self.pool_id = lanes, frames, pool_id
         self.max_calls = cfg.max_calls_per_zmw
 

@@ -666,11 +666,48 @@ DmeCfg MakeDmeCfg(const hwpa_config& c)
     d.iterLimit = c.em_iteration_limit;
     d.iterToLimit = c.iterate_to_limit != 0;
     d.updateMethod = c.model_update_method;
-    d.log2pi = hw_logf(2.0f * 3.1415926536f);
+    d.log2pi = std::log(2.0f * 3.1415926536f);   // libm, scalar (DmeEmHost.cpp:380)
     return d;
 }
 
+// The two oracle forms of the DME differ in their transcendentals and in the EM
+// iteration.  ReferenceForm evaluates every function as the reference's x86
+// host build does: LaneArray exp / log are sse_mathfun's Cephes routines
+// (hw_cephes_expf / hw_cephes_logf), LaneArray exp2 / log2 / pow call libm per
+// element (common/simd/xcompile.h:180-197, m512f.h:34-41), scalar std::log is
+// libm; its EM is the log-sum-exp iteration of DmeEmHost.cpp:374-565 with a
+// division per bin and no fused steps.  MirrorForm is the GPU kernels'
+// arithmetic (include/hwpa_math.h routines and the restated E-step of
+// EmIterationsMirror), bit for bit.
+struct ReferenceForm {
+    static constexpr bool kReference = true;
+    static float Exp(float x) { return hw_cephes_expf(x); }
+    static float Log(float x) { return hw_cephes_logf(x); }
+    static float Exp2(float x) { return std::exp2(x); }
+    static float Log2(float x) { return std::log2(x); }
+    static float Pow(float x, float y) { return std::pow(x, y); }
+    static float ScalarLog(float x) { return std::log(x); }
+};
+// ReferenceForm with libm's correctly rounded-to-within-an-ulp expf / logf in place
+// of the Cephes routines: a restatement that differs from ReferenceForm only in the
+// last bits of exp and log, the yardstick for how far any two faithful
+// implementations of DmeEmHost can sit apart.
+struct ReferenceLibmForm : ReferenceForm {
+    static float Exp(float x) { return std::exp(x); }
+    static float Log(float x) { return std::log(x); }
+};
+struct MirrorForm {
+    static constexpr bool kReference = false;
+    static float Exp(float x) { return hw_cephes_expf(x); }
+    static float Log(float x) { return hw_cephes_logf(x); }
+    static float Exp2(float x) { return hw_dexp2f(x); }
+    static float Log2(float x) { return hw_dlog2f(x); }
+    static float Pow(float x, float y) { return hw_dpowf(x, y); }
+    static float ScalarLog(float x) { return std::log(x); }
+};
+
 // DetectionModelHost::Update0/1/2 (DetectionModelHost.cpp:69-165)
+template <class M>
 void UpdateFraction(const DmeCfg& d, ModelZ& t, const ModelZ& o, float f)
 {
     const float a = 1.0f - f;
@@ -688,7 +725,7 @@ void UpdateFraction(const DmeCfg& d, ModelZ& t, const ModelZ& o, float f)
     const float bw = a * t.bl.w + b * o.bl.w;
     t.bl.w = bw;
     t.bl.mean = a * t.bl.mean + b * o.bl.mean;
-    const float bv = hw_powf(t.bl.var, a) * hw_powf(o.bl.var, b);
+    const float bv = M::Pow(t.bl.var, a) * M::Pow(o.bl.var, b);
     t.bl.var = bv;
     const float aw = 0.25f * (1.0f - bw);
     for (int i = 0; i < NA; ++i) {
@@ -707,18 +744,19 @@ void UpdateFraction(const DmeCfg& d, ModelZ& t, const ModelZ& o, float f)
         } else {
             cv2 = d.cv[i] * d.cv[i];
         }
-        const float am = hw_powf(t.an[i].mean, a) * hw_powf(o.an[i].mean, b);
+        const float am = M::Pow(t.an[i].mean, a) * M::Pow(o.an[i].mean, b);
         t.an[i].mean = am;
         t.an[i].var = ModelSignalCovar(d.k, cv2, am, bv);
     }
 }
 
 // DetectionModelHost::Update(const DetectionModelHost&) (DetectionModelHost.cpp:183-205)
+template <class M>
 void UpdateConf(const DmeCfg& d, ModelZ& t, const ModelZ& o)
 {
     const float confSum = t.conf + o.conf;
     const float fraction = (confSum > 0.0f) ? o.conf / confSum : 0.0f;
-    UpdateFraction(d, t, o, fraction);
+    UpdateFraction<M>(d, t, o, fraction);
     t.conf = confSum;
 }
 
@@ -815,7 +853,8 @@ struct EmZ {
     float binProbMin;
     int status;
     bool converged;
-    int iterCount;
+    int iterCount;      // first converged iteration
+    int captureIter;    // last iteration whose parameters became the estimate (-1: none)
     float mlLogLike, mlDelta;
     uint16_t numFrames;
     float njSum;
@@ -841,16 +880,17 @@ float PrelimScaleFactor(const DmeCfg& d, const ModelZ& m, const HistZ& h)
 }
 
 // DmeEmHost::ComputeConfidence (DmeEmHost.cpp:717-803)
+template <class M>
 void ComputeConfidence(const DmeCfg& d, bool converged, const ModelZ& ref, const ModelZ& est, float* cf)
 {
     cf[0] = converged ? 1.0f : 0.0f;
     cf[1] = hw_satlin(0.05f, 0.15f, est.bl.w);
     float x = (est.bl.mean * est.bl.mean) / est.bl.var;
-    x = hw_expf(-x / (2.0f * (1.0f * 1.0f)));
+    x = M::Exp(-x / (2.0f * (1.0f * 1.0f)));
     cf[2] = x;
-    x = hw_log2f(est.bl.var / ref.bl.var);
+    x = M::Log2(est.bl.var / ref.bl.var);
     const float tol = 1.5f / (0.5f + ref.conf);
-    x = hw_expf(-x * x / (2.0f * (tol * tol)));
+    x = M::Exp(-x * x / (2.0f * (tol * tol)));
     cf[3] = x;
     x = 1.0f;
     if (d.mixThresh1 > 0.0f) {
@@ -876,110 +916,16 @@ void ComputeConfidence(const DmeCfg& d, bool converged, const ModelZ& ref, const
     cf[7] = 1.0f;  // G-test disabled (GTestStatFactor < 0): pValue stays 1
 }
 
-// DmeEmHost::EstimateLaneDetModel (DmeEmHost.cpp:216-615) for one lane.
-void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseliner_stats& bs,
-                  hwpa_lane_model& lm, const int32_t* modelFi, const int32_t* estFi, orc_dme_diag* dx)
+// The EM iterations as the GPU kernel computes them (hw_amd/csrc/dme.hip
+// DmeEstimateKernel): one pass over the bin boundaries per iteration with the
+// component densities carried directly (no log-sum-exp), per-boundary sums in
+// fused multiply-adds, the density recurrence restarted every kRecurBlock
+// boundaries, the table log and reciprocal of the bin probability, and the
+// baseline second moment from moments about the iteration's starting mean.
+// Returns the number of iterations run.
+unsigned EmIterationsMirror(const DmeCfg& d, std::vector<EmZ>& ez, const std::vector<HistZ>& hz, const float* rpa,
+                            float rpaMin)
 {
-    static thread_local std::vector<EmZ> ez;
-    ez.resize(LS);
-    std::vector<HistZ> hz(LS);
-
-    const float rpa[NA] = {d.relAmp[0], d.relAmp[1], d.relAmp[2], d.relAmp[3]};
-    const float rpaSqSum = (rpa[0] * rpa[0] + rpa[2] * rpa[2]) + (rpa[1] * rpa[1] + rpa[3] * rpa[3]);
-    const float rpaMin = std::min(std::min(rpa[0], rpa[1]), std::min(rpa[2], rpa[3]));
-
-    // EvolveModel (DmeEmHost.cpp:806-834) + EvolveConfidence (DetectionModelHost.h:153-163)
-    const int32_t tModel = modelFi[0] + (int32_t)((uint32_t)(modelFi[1] - modelFi[0]) / 2u);
-    const int32_t tEst = estFi[0] + (int32_t)((uint32_t)(estFi[1] - estFi[0]) / 2u);
-    const float thl = 0.5f * (float)(tModel + tEst);
-    const float hl = d.hl0 + (d.hl1 - d.hl0) * hw_satlin(56160.0f, 168480.0f, thl);
-    const float tDiff = (float)(tEst - tModel);
-    const float evolve = hw_exp2f(-hw_fabsf(tDiff) / hl);
-
-    for (int z = 0; z < LS; ++z) {
-        EmZ& e = ez[z];
-        HistZ& h = hz[z];
-        h.binSize = lh.bin_size[z];
-        for (int b = 0; b <= NB; ++b) h.start[b] = lh.low_bound[z] + (float)b * h.binSize;
-        for (int b = 0; b < NB; ++b) h.cnt[b] = lh.bin_count[b][z];
-        h.cnt[NB] = 0;
-        h.lowOut = lh.outlier_low[z];
-        h.highOut = lh.outlier_high[z];
-
-        e.det = ModelFromHalf(lm, z);
-        e.det.conf *= evolve;
-
-        // PrelimEstimate (DmeEmHost.cpp:161-214)
-        ModelZ w = e.det;
-        {
-            const float nBl = bs.baseline.m0[z];
-            const float total = bs.full.basic.m0[z];
-            const float blWeight = hw_maxf(nBl / total, 0.01f);
-            const bool mask = nBl >= 3.0f;
-            StatZ blsa{bs.baseline.offset[z], bs.baseline.m0[z], bs.baseline.m1[z], bs.baseline.m2[z]};
-            float blVar = mask ? blsa.Variance() : w.bl.var;
-            blVar = hw_clampf(blVar, d.k.blVarMin, d.k.blVarMax);
-            const float blMean = mask ? blsa.Mean() : w.bl.mean;
-            const float scale = hw_sqrtf(blVar / w.bl.var);
-            for (int a = 0; a < NA; ++a) {
-                const float mean = w.an[a].mean * scale;
-                const float var = ModelSignalCovar(d.k, d.cv[a] * d.cv[a], mean, blVar);
-                w.an[a].mean = mean;
-                w.an[a].var = var;
-                w.an[a].w = 0.25f * (1.0f - blWeight);
-            }
-            w.bl.w = blWeight;
-            w.bl.mean = blMean;
-            w.bl.var = blVar;
-            float conf = 0.1f * hw_satlin(0.0f, 500.0f, nBl - 3.0f);
-            conf *= hw_satlin(d.k.blVarMax, 0.5f * (d.k.blVarNominal + d.k.blVarMax), blVar);
-            w.conf = conf;
-        }
-        UpdateConf(d, e.det, w);
-        e.work = e.det;
-
-        e.numFrames = h.Total();
-        e.scaleFactor = PrelimScaleFactor(d, e.work, h);
-        // ScaleModelSnr (DmeEmHost.cpp:837-852)
-        for (int a = 0; a < NA; ++a) {
-            e.work.an[a].mean = e.scaleFactor * e.work.an[a].mean;
-            e.work.an[a].var = ModelSignalCovar(d.k, d.cv[a] * d.cv[a], e.work.an[a].mean, e.work.bl.var);
-        }
-        const float rhoMin = 0.001f;
-        e.rho[0] = hw_maxf(e.work.bl.w, rhoMin);
-        e.mu[0] = e.work.bl.mean;
-        e.var[0] = e.work.bl.var;
-        for (int a = 0; a < NA; ++a) {
-            e.rho[a + 1] = hw_maxf(e.work.an[a].w, rhoMin);
-            e.mu[a + 1] = e.work.an[a].mean;
-            e.var[a + 1] = e.work.an[a].var;
-        }
-        e.varQuant = (h.binSize * h.binSize) / 12.0f;
-        for (int k = 0; k < 5; ++k) e.var[k] += e.varQuant;
-        e.var[0] = hw_minf(e.var[0], d.k.blVarMax);
-        const float rs = Sum5Tree(e.rho);
-        for (int k = 0; k < 5; ++k) e.rho[k] = e.rho[k] / rs;
-        for (int k = 0; k < 5; ++k) {
-            e.rhoEst[k] = e.rho[k];
-            e.muEst[k] = e.mu[k];
-            e.varEst[k] = e.var[k];
-        }
-        e.s = ((e.mu[1] * rpa[0] + e.mu[2] * rpa[1]) + (e.mu[3] * rpa[2] + e.mu[4] * rpa[3])) / rpaSqSum;
-        e.sExpect = e.s / e.scaleFactor;
-        e.sExpectWeight = e.det.conf * d.pulseAmpReg;
-        e.logLike = -FLT_MAX;
-        e.logLikePrev = e.logLike;
-        e.binProbMin = hw_maxf((float)e.numFrames * h.binSize, 1.0f) * FLT_MIN;
-        e.status = (e.numFrames < 160) ? HWPA_ZS_INSUF_DATA : HWPA_ZS_OK;
-        e.converged = false;
-        e.iterCount = 0;
-        e.mlLogLike = -INFINITY;
-        e.mlDelta = -INFINITY;
-        float njs = 0.0f;
-        for (int b = 0; b < NB; ++b) njs += (float)h.cnt[b];
-        e.njSum = njs;
-    }
-
     unsigned it = 0;
     std::vector<float> sci((size_t)LS * 5), smom1((size_t)LS * 5), svarinv((size_t)LS * 5);
     for (; it < d.iterLimit; ++it) {
@@ -991,7 +937,7 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
             float varinv[5], pref[5], nhv[5];
             for (int k = 0; k < 5; ++k) {
                 varinv[k] = 1.0f / e.var[k];
-                pref[k] = (hw_logf(e.rho[k]) - 0.5f * hw_logf(e.var[k])) - 0.5f * d.log2pi;
+                pref[k] = (hw_cephes_logf(e.rho[k]) - 0.5f * hw_cephes_logf(e.var[k])) - 0.5f * d.log2pi;
                 nhv[k] = -0.5f * varinv[k];
             }
             // Every use of a responsibility is boundaryProb * tau_i = exp(t_i)
@@ -1081,7 +1027,7 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
                 mom1[m] *= halfBin;
             }
             e.logLikePrev = e.logLike;
-            e.logLike = wps - e.njSum * hw_logf(probSum);
+            e.logLike = wps - e.njSum * hw_cephes_logf(probSum);
             const float ds = e.s - e.sExpect;
             e.logLike -= 0.5f * (ds * ds) * e.sExpectWeight;
             const float delta = e.logLike - e.logLikePrev;
@@ -1093,6 +1039,7 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
             if (conv) {
                 e.mlLogLike = e.logLike;
                 e.mlDelta = delta;
+                e.captureIter = (int)it;
                 for (int k = 0; k < 5; ++k) {
                     e.rhoEst[k] = e.rho[k];
                     e.muEst[k] = e.mu[k];
@@ -1150,6 +1097,269 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
                 E.var[a + 1] = ModelSignalCovar(d.k, d.cv[a] * d.cv[a], E.mu[a + 1], E.var[0]);
         }
     }
+    return it;
+}
+
+// The EM iterations of DmeEmHost::EstimateLaneDetModel as the reference writes
+// them (DmeEmHost.cpp:374-565), per ZMW with lane-wide exit: the E-step takes
+// the log density tau_i of every mode at every bin boundary, the log-sum-exp
+// llb over the modes (Eigen's fixed-size-5 reductions, (a0+a1)+(a2+(a3+a4)),
+// for the max and the sum), tau_i := exp(tau_i - llb) and boundaryProb =
+// exp(llb); each bin then divides its count by its trapezoid probability and
+// gathers the responsibilities.  The M-step's baseline variance takes a second
+// pass over the bins with the new baseline mean.  Transcendentals of M (the
+// reference's Cephes exp / log), true divisions, no fused steps.
+template <class M>
+unsigned EmIterationsRef(const DmeCfg& d, std::vector<EmZ>& ez, const std::vector<HistZ>& hz, const float* rpa,
+                         float rpaMin)
+{
+    constexpr int NE = NB + 1;
+    static thread_local std::vector<float> tauBuf, bpBuf, binProbBuf;
+    tauBuf.resize((size_t)LS * 5 * NE);
+    bpBuf.resize((size_t)LS * NE);
+    binProbBuf.resize((size_t)LS * NB);
+    // static const float log_2pi = std::log(2.0f * pi_f) (DmeEmHost.cpp:380; libm, scalar)
+    const float log2pi = M::ScalarLog(2.0f * 3.1415926536f);
+    auto max5 = [](const float* v) { return hw_maxf(hw_maxf(v[0], v[1]), hw_maxf(v[2], hw_maxf(v[3], v[4]))); };
+    float ci[LS][5], mom1[LS][5], varinv[LS][5];
+    unsigned it = 0;
+    for (; it < d.iterLimit; ++it) {
+        bool allDone = true;
+        for (int z = 0; z < LS; ++z) {
+            EmZ& e = ez[z];
+            const HistZ& h = hz[z];
+            float* tau = &tauBuf[(size_t)z * 5 * NE];      // tau[i * NE + b]
+            float* bp = &bpBuf[(size_t)z * NE];
+            float* binProb = &binProbBuf[(size_t)z * NB];
+            float pref[5];
+            for (int i = 0; i < 5; ++i) {
+                varinv[z][i] = 1.0f / e.var[i];
+                pref[i] = (M::Log(e.rho[i]) - 0.5f * M::Log(e.var[i])) - 0.5f * log2pi;
+            }
+            float probSum = 0.0f, weightedProbSum = 0.0f;
+            for (int i = 0; i < 5; ++i) ci[z][i] = mom1[z][i] = 0.0f;
+            auto cornerComp = [&](int b) {
+                const float x = h.start[b];
+                float t[5];
+                for (int i = 0; i < 5; ++i) {
+                    const float y = x - e.mu[i];
+                    t[i] = pref[i] - 0.5f * y * y * varinv[z][i];
+                }
+                const float tauMax = max5(t);
+                float ex[5];
+                for (int i = 0; i < 5; ++i) ex[i] = M::Exp(t[i] - tauMax);
+                const float llb = tauMax + M::Log(Sum5Tree(ex));
+                for (int i = 0; i < 5; ++i) tau[i * NE + b] = M::Exp(t[i] - llb);
+                bp[b] = M::Exp(llb);
+            };
+            auto centerComp = [&](int b) {
+                binProb[b] = hw_maxf(0.5f * h.binSize * (bp[b] + bp[b + 1]), e.binProbMin);
+                probSum += binProb[b];
+                const float nj = (float)h.cnt[b];
+                weightedProbSum += nj * M::Log(binProb[b]);
+                const float factor = nj / binProb[b];
+                const float x0 = h.start[b];
+                const float x1 = h.start[b + 1];
+                for (int m = 0; m < 5; ++m) {
+                    const float tmp1 = bp[b] * tau[m * NE + b];
+                    const float tmp2 = bp[b + 1] * tau[m * NE + b + 1];
+                    ci[z][m] += (tmp1 + tmp2) * factor;
+                    mom1[z][m] += (tmp1 * x0 + tmp2 * x1) * factor;
+                }
+            };
+            cornerComp(0);
+            for (int b = 0; b < NB; ++b) {
+                cornerComp(b + 1);
+                centerComp(b);
+            }
+            const float hb = 0.5f * h.binSize;
+            for (int m = 0; m < 5; ++m) {
+                ci[z][m] *= hb;
+                mom1[z][m] *= hb;
+            }
+            e.logLikePrev = e.logLike;
+            e.logLike = weightedProbSum - e.njSum * M::Log(probSum);
+            const float ds = e.s - e.sExpect;
+            e.logLike -= 0.5f * (ds * ds) * e.sExpectWeight;
+            const float delta = e.logLike - e.logLikePrev;
+            const float convTol = 1e-4f * hw_fabsf(e.logLike);
+            bool conv = (delta >= 0.0f) && (delta < convTol);
+            conv = conv && (e.logLike >= e.mlLogLike);
+            // MaxLikelihoodDiagnostics::Converged (MaxLikelihoodDiagnostics.h:90-101)
+            if (conv && !e.converged) e.iterCount = (int)it;
+            if (conv) {
+                e.mlLogLike = e.logLike;
+                e.mlDelta = delta;
+                e.captureIter = (int)it;
+                for (int k = 0; k < 5; ++k) {
+                    e.rhoEst[k] = e.rho[k];
+                    e.muEst[k] = e.mu[k];
+                    e.varEst[k] = e.var[k];
+                }
+            }
+            e.converged = e.converged || conv;
+            if (!(e.converged || e.status != HWPA_ZS_OK)) allDone = false;
+        }
+        // Lane-wide early exit (DmeEmHost.cpp:485)
+        if (!d.iterToLimit && allDone) break;
+
+        // M-step (DmeEmHost.cpp:494-564)
+        for (int z = 0; z < LS; ++z) {
+            EmZ& E = ez[z];
+            const HistZ& H = hz[z];
+            const float* tau = &tauBuf[(size_t)z * 5 * NE];
+            const float* bp = &bpBuf[(size_t)z * NE];
+            const float* binProb = &binProbBuf[(size_t)z * NB];
+            float c[5];
+            for (int m = 0; m < 5; ++m) c[m] = hw_maxf(ci[z][m], 0.1f);
+            for (int m = 0; m < 5; ++m) E.rho[m] = c[m] / E.njSum;
+            const float rs = Sum5Tree(E.rho);
+            for (int m = 0; m < 5; ++m) E.rho[m] = E.rho[m] / rs;
+            E.mu[0] = mom1[z][0] / c[0];
+            float numer = 0.0f, denom = 0.0f;
+            for (int a = 0; a < NA; ++a) {
+                const int i = a + 1;
+                numer += rpa[a] * varinv[z][i] * mom1[z][i];
+                denom += (rpa[a] * rpa[a]) * varinv[z][i] * c[i];
+            }
+            numer += E.sExpect * E.sExpectWeight;
+            denom += E.sExpectWeight;
+            E.s = numer / denom;
+            const float minPulseMean = hw_maxf(1.0f, d.k.signalScaler);
+            const float sMin = hw_maxf(0.5f * hw_sqrtf(E.var[0]), minPulseMean) / rpaMin;
+            if (E.s < sMin) {
+                E.s = sMin;
+                E.status |= HWPA_ZS_VLOW_SIGNAL;
+            }
+            for (int a = 0; a < NA; ++a) E.mu[a + 1] = E.s * rpa[a];
+            float mom2 = 0.0f;
+            for (int j = 0; j < NB; ++j) {
+                const int k = j + 1;
+                const float x0 = H.start[j] - E.mu[0];
+                const float x1 = H.start[k] - E.mu[0];
+                float tmp = bp[j] * x0 * x0 * tau[0 * NE + j];
+                tmp += bp[k] * x1 * x1 * tau[0 * NE + k];
+                tmp *= (float)H.cnt[j] / binProb[j];
+                mom2 += tmp;
+            }
+            mom2 *= 0.5f * H.binSize;
+            E.var[0] = mom2 / c[0] + E.varQuant;
+            E.var[0] = hw_clampf(E.var[0], d.k.blVarMin, d.k.blVarMax);
+            for (int a = 0; a < NA; ++a)
+                E.var[a + 1] = ModelSignalCovar(d.k, d.cv[a] * d.cv[a], E.mu[a + 1], E.var[0]);
+        }
+    }
+    return it;
+}
+
+// DmeEmHost::EstimateLaneDetModel (DmeEmHost.cpp:216-615) for one lane.
+template <class M>
+void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseliner_stats& bs,
+                  hwpa_lane_model& lm, const int32_t* modelFi, const int32_t* estFi, orc_dme_diag* dx)
+{
+    static thread_local std::vector<EmZ> ez;
+    ez.resize(LS);
+    std::vector<HistZ> hz(LS);
+
+    const float rpa[NA] = {d.relAmp[0], d.relAmp[1], d.relAmp[2], d.relAmp[3]};
+    const float rpaSqSum = (rpa[0] * rpa[0] + rpa[2] * rpa[2]) + (rpa[1] * rpa[1] + rpa[3] * rpa[3]);
+    const float rpaMin = std::min(std::min(rpa[0], rpa[1]), std::min(rpa[2], rpa[3]));
+
+    // EvolveModel (DmeEmHost.cpp:806-834) + EvolveConfidence (DetectionModelHost.h:153-163)
+    const int32_t tModel = modelFi[0] + (int32_t)((uint32_t)(modelFi[1] - modelFi[0]) / 2u);
+    const int32_t tEst = estFi[0] + (int32_t)((uint32_t)(estFi[1] - estFi[0]) / 2u);
+    const float thl = 0.5f * (float)(tModel + tEst);
+    const float hl = d.hl0 + (d.hl1 - d.hl0) * hw_satlin(56160.0f, 168480.0f, thl);
+    const float tDiff = (float)(tEst - tModel);
+    const float evolve = M::Exp2(-hw_fabsf(tDiff) / hl);
+
+    for (int z = 0; z < LS; ++z) {
+        EmZ& e = ez[z];
+        HistZ& h = hz[z];
+        h.binSize = lh.bin_size[z];
+        for (int b = 0; b <= NB; ++b) h.start[b] = lh.low_bound[z] + (float)b * h.binSize;
+        for (int b = 0; b < NB; ++b) h.cnt[b] = lh.bin_count[b][z];
+        h.cnt[NB] = 0;
+        h.lowOut = lh.outlier_low[z];
+        h.highOut = lh.outlier_high[z];
+
+        e.det = ModelFromHalf(lm, z);
+        e.det.conf *= evolve;
+
+        // PrelimEstimate (DmeEmHost.cpp:161-214)
+        ModelZ w = e.det;
+        {
+            const float nBl = bs.baseline.m0[z];
+            const float total = bs.full.basic.m0[z];
+            const float blWeight = hw_maxf(nBl / total, 0.01f);
+            const bool mask = nBl >= 3.0f;
+            StatZ blsa{bs.baseline.offset[z], bs.baseline.m0[z], bs.baseline.m1[z], bs.baseline.m2[z]};
+            float blVar = mask ? blsa.Variance() : w.bl.var;
+            blVar = hw_clampf(blVar, d.k.blVarMin, d.k.blVarMax);
+            const float blMean = mask ? blsa.Mean() : w.bl.mean;
+            const float scale = hw_sqrtf(blVar / w.bl.var);
+            for (int a = 0; a < NA; ++a) {
+                const float mean = w.an[a].mean * scale;
+                const float var = ModelSignalCovar(d.k, d.cv[a] * d.cv[a], mean, blVar);
+                w.an[a].mean = mean;
+                w.an[a].var = var;
+                w.an[a].w = 0.25f * (1.0f - blWeight);
+            }
+            w.bl.w = blWeight;
+            w.bl.mean = blMean;
+            w.bl.var = blVar;
+            float conf = 0.1f * hw_satlin(0.0f, 500.0f, nBl - 3.0f);
+            conf *= hw_satlin(d.k.blVarMax, 0.5f * (d.k.blVarNominal + d.k.blVarMax), blVar);
+            w.conf = conf;
+        }
+        UpdateConf<M>(d, e.det, w);
+        e.work = e.det;
+
+        e.numFrames = h.Total();
+        e.scaleFactor = PrelimScaleFactor(d, e.work, h);
+        // ScaleModelSnr (DmeEmHost.cpp:837-852)
+        for (int a = 0; a < NA; ++a) {
+            e.work.an[a].mean = e.scaleFactor * e.work.an[a].mean;
+            e.work.an[a].var = ModelSignalCovar(d.k, d.cv[a] * d.cv[a], e.work.an[a].mean, e.work.bl.var);
+        }
+        const float rhoMin = 0.001f;
+        e.rho[0] = hw_maxf(e.work.bl.w, rhoMin);
+        e.mu[0] = e.work.bl.mean;
+        e.var[0] = e.work.bl.var;
+        for (int a = 0; a < NA; ++a) {
+            e.rho[a + 1] = hw_maxf(e.work.an[a].w, rhoMin);
+            e.mu[a + 1] = e.work.an[a].mean;
+            e.var[a + 1] = e.work.an[a].var;
+        }
+        e.varQuant = (h.binSize * h.binSize) / 12.0f;
+        for (int k = 0; k < 5; ++k) e.var[k] += e.varQuant;
+        e.var[0] = hw_minf(e.var[0], d.k.blVarMax);
+        const float rs = Sum5Tree(e.rho);
+        for (int k = 0; k < 5; ++k) e.rho[k] = e.rho[k] / rs;
+        for (int k = 0; k < 5; ++k) {
+            e.rhoEst[k] = e.rho[k];
+            e.muEst[k] = e.mu[k];
+            e.varEst[k] = e.var[k];
+        }
+        e.s = ((e.mu[1] * rpa[0] + e.mu[2] * rpa[1]) + (e.mu[3] * rpa[2] + e.mu[4] * rpa[3])) / rpaSqSum;
+        e.sExpect = e.s / e.scaleFactor;
+        e.sExpectWeight = e.det.conf * d.pulseAmpReg;
+        e.logLike = -FLT_MAX;
+        e.logLikePrev = e.logLike;
+        e.binProbMin = hw_maxf((float)e.numFrames * h.binSize, 1.0f) * FLT_MIN;
+        e.status = (e.numFrames < 160) ? HWPA_ZS_INSUF_DATA : HWPA_ZS_OK;
+        e.converged = false;
+        e.iterCount = 0;
+        e.captureIter = -1;
+        e.mlLogLike = -INFINITY;
+        e.mlDelta = -INFINITY;
+        float njs = 0.0f;
+        for (int b = 0; b < NB; ++b) njs += (float)h.cnt[b];
+        e.njSum = njs;
+    }
+
+    const unsigned it = M::kReference ? EmIterationsRef<M>(d, ez, hz, rpa, rpaMin)
+                                      : EmIterationsMirror(d, ez, hz, rpa, rpaMin);
     for (int z = 0; z < LS; ++z) {
         EmZ& e = ez[z];
         if (!e.converged) e.status |= HWPA_ZS_NO_CONVERGE;
@@ -1162,7 +1372,7 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
             e.work.an[a].var = e.varEst[a + 1];
         }
         float cf[8];
-        ComputeConfidence(d, e.converged, e.det, e.work, cf);
+        ComputeConfidence<M>(d, e.converged, e.det, e.work, cf);
         float conf = 1.0f;
         for (int k = 0; k < 8; ++k) conf *= cf[k];
         conf = hw_minf(hw_maxf(0.0f, conf), 1.0f);
@@ -1171,6 +1381,7 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
         if (dx) {
             dx->converged[z] = e.converged;
             dx->iter_count[z] = e.iterCount;
+            dx->capture_iter[z] = e.captureIter;
             dx->status[z] = e.status;
             dx->log_like[z] = e.mlLogLike;
             for (int k = 0; k < 8; ++k) dx->conf_factors[k][z] = cf[k];
@@ -1183,7 +1394,7 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
                 dx->est_fp32[m][2][z] = modes[m]->var;
             }
         }
-        UpdateConf(d, e.det, e.work);
+        UpdateConf<M>(d, e.det, e.work);
         if (dx) {
             const ModeZ* modes[5] = {&e.det.bl, &e.det.an[0], &e.det.an[1], &e.det.an[2], &e.det.an[3]};
             for (int m = 0; m < 5; ++m) {
@@ -1220,10 +1431,10 @@ void* orc_baseliner_create(const hwpa_config* cfg, int32_t lanes)
     c->fp = FilterParams(cfg->filter_type);
     // LaneBaseliner ctor (HostMultiScaleBaseliner.h:58-66) and MultiScaleBaseliner::Configure
     // (MultiScaleBaseliner.cpp:53-75)
-    c->cMeanBias = c->fp.mean_bias * hw_exp2f(cfg->mean_bias_adjust);
-    c->cSigmaBias = c->fp.sigma_bias * hw_exp2f(cfg->sigma_bias_adjust);
-    c->meanEmaAlpha = hw_powf(0.5f, 1.0f / cfg->mean_ema_scale_strides);
-    c->sigmaEmaAlpha = hw_exp2f(-1.0f / cfg->sigma_ema_scale_strides);
+    c->cMeanBias = c->fp.mean_bias * std::exp2(cfg->mean_bias_adjust);
+    c->cSigmaBias = c->fp.sigma_bias * std::exp2(cfg->sigma_bias_adjust);
+    c->meanEmaAlpha = std::pow(0.5f, 1.0f / cfg->mean_ema_scale_strides);
+    c->sigmaEmaAlpha = std::exp2(-1.0f / cfg->sigma_ema_scale_strides);
     c->jumpTol = (cfg->jump_suppression > 0.0f) ? 1.0f / cfg->jump_suppression : INFINITY;
     c->scaler = cfg->movie.photoelectron_sensitivity;
     c->pedestal = cfg->movie.pedestal;
@@ -1377,13 +1588,17 @@ void orc_dme_init_models(const hwpa_config* cfg, int32_t lanes, const hwpa_basel
     for (int l = 0; l < lanes; ++l) InitLaneModel(d, stats[l], out[l]);
 }
 
-void orc_dme_estimate(const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,
+void orc_dme_estimate(int32_t form, const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,
                       const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
                       const int32_t model_fi[2], const int32_t est_fi[2], orc_dme_diag* diag)
 {
     const DmeCfg d = MakeDmeCfg(*cfg);
     ParLanes(lanes, [&](int l) {
-        EstimateLane(d, hist[l], stats[l], models[l], model_fi, est_fi, diag ? diag + l : nullptr);
+        orc_dme_diag* dx = diag ? diag + l : nullptr;
+        if (form == ORC_FORM_MIRROR) EstimateLane<MirrorForm>(d, hist[l], stats[l], models[l], model_fi, est_fi, dx);
+        else if (form == ORC_FORM_REFERENCE_LIBM)
+            EstimateLane<ReferenceLibmForm>(d, hist[l], stats[l], models[l], model_fi, est_fi, dx);
+        else EstimateLane<ReferenceForm>(d, hist[l], stats[l], models[l], model_fi, est_fi, dx);
     });
 }
 
@@ -1493,7 +1708,7 @@ struct Scorer {
         bInvVar = 1.0f / bVar;
         for (int a = 0; a < NA; ++a) {
             const float aMean = hw_h2f(m.analogs[a].means[z]);
-            logPMean[a] = hw_logf(aMean - bMean);
+            logPMean[a] = hw_cephes_logf(aMean - bMean);
             const float aVar = hw_h2f(m.analogs[a].vars[z]);
             const float aSigma = hw_sqrtf(aVar);
             x1[a] = aMean - aSigma * sqrtHalfPi;
@@ -1508,14 +1723,14 @@ struct Scorer {
             eVar *= 0.5f;
             dimMean = eMean;
             dimVar = eVar;
-            dimNorm = -0.5f * (log2pi + hw_logf(eVar));
+            dimNorm = -0.5f * (log2pi + hw_cephes_logf(eVar));
             dimScale = 0.5f / eVar;
         }
         const float normConst = log2pi / 2.0f;
-        bgFixed = -0.5f * hw_logf(bVar) - normConst;
+        bgFixed = -0.5f * hw_cephes_logf(bVar) - normConst;
         bgMean = bMean;
         for (int a = 0; a < NA; ++a) {
-            ffFixed[a] = -0.5f * hw_logf(hw_h2f(m.analogs[a].vars[z])) - normConst;
+            ffFixed[a] = -0.5f * hw_cephes_logf(hw_h2f(m.analogs[a].vars[z])) - normConst;
             ffMean[a] = hw_h2f(m.analogs[a].means[z]);
         }
     }
@@ -1650,7 +1865,7 @@ void LabelLane(const LabelerCtx& c, LaneLabeler& L, const int16_t* tr, int frame
         for (int i = 1; i < NS; ++i) mx = hw_maxf(ll[i], mx);
         float sum = 0.0f;
         for (int i = 0; i < NS; ++i) {
-            prob[i] = hw_expf(ll[i] - mx);
+            prob[i] = hw_cephes_expf(ll[i] - mx);
             sum += prob[i];
         }
         const float inv = 1 / sum;
@@ -2175,6 +2390,7 @@ struct OrcAnalyzer {
     int lanes, frames;
     int32_t nextFrame = 0;
     uint32_t baselinerLatency, dmeDelay;
+    int32_t emForm = ORC_FORM_REFERENCE;
     bool fullEstimation = false;
     bool modelInitialized = false;
     int status = ST_HIST_INIT;
@@ -2373,8 +2589,8 @@ void* orc_analyzer_create(const hwpa_config* cfg, uint32_t pool_id, int32_t lane
     a->frames = frames;
     const hwpa_filter_params fp = FilterParams(cfg->filter_type);
     // MultiScaleBaseliner::StartupLatency (MultiScaleBaseliner.h:56-99)
-    const float meanAlpha = hw_powf(0.5f, 1.0f / cfg->mean_ema_scale_strides);
-    const float sigmaAlpha = hw_exp2f(-1.0f / cfg->sigma_ema_scale_strides);
+    const float meanAlpha = std::pow(0.5f, 1.0f / cfg->mean_ema_scale_strides);
+    const float sigmaAlpha = std::exp2(-1.0f / cfg->sigma_ema_scale_strides);
     const float mess = -1.0f / std::log2(meanAlpha);
     const float sess = -1.0f / std::log2(sigmaAlpha);
     a->baselinerLatency = (uint32_t)((float)fp.latent_size + std::max(mess, sess) * (float)fp.aggregate_stride);
@@ -2459,7 +2675,8 @@ static int32_t OrcDmeAddBatch(OrcAnalyzer* a, int32_t first_frame, bool* ran)
             orc_agg_get(a->agg, a->aggStats.data());
             const int32_t mfi[2] = {a->modelFi.lo, a->modelFi.hi};
             const int32_t efi[2] = {a->histFi.lo, a->histFi.hi};
-            orc_dme_estimate(&a->cfg, L, a->hists.data(), a->aggStats.data(), a->models.data(), mfi, efi, nullptr);
+            orc_dme_estimate(a->emForm, &a->cfg, L, a->hists.data(), a->aggStats.data(), a->models.data(), mfi, efi,
+                             nullptr);
             a->modelFi = a->histFi;
         }
         orc_agg_get(a->agg, a->aggStats.data());
@@ -2565,6 +2782,8 @@ int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id, i
 {
     return orc_analyzer_process_encoded(h, traces, HWPA_ENCODING_INT16, pool_id, first_frame, res);
 }
+
+void orc_analyzer_set_em_form(void* h, int32_t form) { static_cast<OrcAnalyzer*>(h)->emForm = form; }
 
 void orc_analyzer_set_activity_tree(void* h, const hwpa_activity_tree* t)
 {
@@ -2997,11 +3216,21 @@ extern "C" {
 
 // Element-wise evaluation of the shared math routines of include/hwpa_math.h.
 // kind: 0 exp, 1 log, 2 log2, 3 exp2, 4 pow(a,b), 5 f->half->f, 6 round_i16, 7 floor_i32,
-// 9 / 10 table-driven log / reciprocal of a normal float
+// 9 / 10 table-driven log / reciprocal of a normal float, 11 / 12 Cephes exp / log
+// (the reference's LaneArray exp / log), 13 / 14 / 15 libm exp2f / log2f / powf,
+// 16 / 17 / 18 the kernels' double-rounded exp2 / log2 / pow
 void orc_math(int32_t kind, const float* a, const float* b, float* out, int32_t n)
 {
     for (int32_t i = 0; i < n; ++i) {
         switch (kind) {
+        case 11: out[i] = hw_cephes_expf(a[i]); break;
+        case 12: out[i] = hw_cephes_logf(a[i]); break;
+        case 13: out[i] = std::exp2(a[i]); break;
+        case 14: out[i] = std::log2(a[i]); break;
+        case 15: out[i] = std::pow(a[i], b[i]); break;
+        case 16: out[i] = hw_dexp2f(a[i]); break;
+        case 17: out[i] = hw_dlog2f(a[i]); break;
+        case 18: out[i] = hw_dpowf(a[i], b[i]); break;
         case 0: out[i] = hw_expf(a[i]); break;
         case 1: out[i] = hw_logf(a[i]); break;
         case 2: out[i] = hw_log2f(a[i]); break;

@@ -29,7 +29,17 @@ typedef struct orc_dme_diag {
     float confidence[HWPA_LANE_SIZE];       /* estimate confidence before blending */
     float est_fp32[5][3][HWPA_LANE_SIZE];   /* [mode][weight,mean,var] of the estimate */
     float out_fp32[5][3][HWPA_LANE_SIZE];   /* blended model before half export */
+    int32_t capture_iter[HWPA_LANE_SIZE];   /* last iteration whose parameters became the estimate, -1 if none */
 } orc_dme_diag;
+
+/* Forms of the DME: ORC_FORM_REFERENCE follows DmeEmHost's arithmetic (log-sum-exp
+ * E-step, the reference's Cephes exp/log and libm exp2/log2/pow); ORC_FORM_MIRROR
+ * is the GPU kernels' restated E-step and math routines, bit for bit. */
+#define ORC_FORM_REFERENCE 0
+#define ORC_FORM_MIRROR 1
+/* ORC_FORM_REFERENCE with libm expf/logf in place of the Cephes routines (a
+ * sensitivity yardstick: the same algorithm, different last bits of exp/log). */
+#define ORC_FORM_REFERENCE_LIBM 2
 
 typedef struct orc_result {
     hwpa_pulse* pulses;              /* [L*64*max_calls] */
@@ -71,7 +81,7 @@ void orc_hist_set(void* h, const hwpa_lane_hist* in);
 
 void orc_dme_init_models(const hwpa_config* cfg, int32_t lanes,
                          const hwpa_baseliner_stats* stats, hwpa_lane_model* out);
-void orc_dme_estimate(const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,
+void orc_dme_estimate(int32_t form, const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,
                       const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
                       const int32_t model_fi[2], const int32_t est_fi[2], orc_dme_diag* diag);
 
@@ -104,6 +114,7 @@ int32_t orc_analyzer_process_encoded(void* h, const void* traces, int32_t encodi
 int32_t orc_analyzer_process(void* h, const int16_t* traces, uint32_t pool_id,
                              int32_t first_frame, orc_result* res);
 void orc_analyzer_set_activity_tree(void* h, const hwpa_activity_tree* t);
+void orc_analyzer_set_em_form(void* h, int32_t form);   /* ORC_FORM_*; default ORC_FORM_REFERENCE */
 
 void orc_tracegen(const hwpa_tracegen_params* p, int32_t lanes, int32_t lane_offset,
                   int32_t frames, uint32_t first_frame, hwpa_tracegen_state* state,

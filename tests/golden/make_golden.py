@@ -27,7 +27,7 @@ def run_case():
     cfg = H.bench_config(F)
     p = H.tracegen_default(seed=CASE["seed"])
     st = np.zeros(L, dtype=T.TRACEGEN_STATE)
-    an = O.Analyzer(cfg, CASE["pool_id"], L, F)
+    an = O.Analyzer(cfg, CASE["pool_id"], L, F, em="mirror")
     dts = (T.PULSE, T.STAT_ACC, T.BASELINER_STATS, T.LANE_MODEL)
     out = []
     for c in range(C):

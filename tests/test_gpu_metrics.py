@@ -133,7 +133,7 @@ def test_analyzer_metrics_end_to_end(H, O, T, gpu):
     cfg = H.bench_config(F)
     cfg.frames_per_hf_metric_block = 1024
     chunks = gen_traces(O, T, H, L, F, C, seed=91)
-    oa = O.Analyzer(cfg, 0, L, F)
+    oa = O.Analyzer(cfg, 0, L, F, em="mirror")
     ga = H.BatchAnalyzer(0, L, F, cfg)
     tree = _tree(H)
     oa.set_activity_tree(tree[0])

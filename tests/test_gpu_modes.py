@@ -44,7 +44,7 @@ def test_analyzer_modes_end_to_end(H, O, T, gpu, mode, encoding):
     if encoding:
         cfg.movie.pedestal = 64
         chunks = [_u8(tr) for tr in chunks]
-    oa = O.Analyzer(cfg, 0, L, F)
+    oa = O.Analyzer(cfg, 0, L, F, em="mirror")
     ga = H.BatchAnalyzer(0, L, F, cfg)
     seen = [0, 0, 0, 0]
     for c, tr in enumerate(chunks):

@@ -269,7 +269,7 @@ def test_dme_estimate_bit_exact(H, O, T, gpu, seed):
     L, F = 4, 512
     cfg, h, s, m = _em_case(H, O, T, L, F, seed)
     mfi, efi = (512, 1024), (1024, 1536)
-    o_models, diag = O.dme_estimate(cfg, h, s, m, mfi, efi, want_diag=True)
+    o_models, diag = O.dme_estimate(cfg, h, s, m, mfi, efi, want_diag=True, form="mirror")
     d_models = _dev(H, m)
     d_iters = H.dev_bytes(L * 4)
     H.dme_estimate(cfg, _dev(H, h), _dev(H, s), d_models, mfi, efi, L, d_iters)
@@ -293,7 +293,7 @@ def test_dme_estimate_recurrence_edge_paths_bit_exact(H, O, T, gpu):
     h["bin_size"][2, 1::4] *= 16.0
     h["low_bound"][3, 1::4] -= 250.0 * bs[3, 1::4]
     mfi, efi = (512, 1024), (1024, 1536)
-    o_models, diag = O.dme_estimate(cfg, h, s, m, mfi, efi, want_diag=True)
+    o_models, diag = O.dme_estimate(cfg, h, s, m, mfi, efi, want_diag=True, form="mirror")
     d_models = _dev(H, m)
     d_iters = H.dev_bytes(L * 4)
     H.dme_estimate(cfg, _dev(H, h), _dev(H, s), d_models, mfi, efi, L, d_iters)
@@ -431,7 +431,7 @@ def test_analyzer_end_to_end(H, O, T, gpu):
     L, F, C = 8, 512, 7
     cfg = H.bench_config(F)
     chunks = gen_traces(O, T, H, L, F, C, seed=77)
-    oa = O.Analyzer(cfg, 0, L, F)
+    oa = O.Analyzer(cfg, 0, L, F, em="mirror")
     ga = H.BatchAnalyzer(0, L, F, cfg)
     dts = (T.PULSE, T.STAT_ACC, T.BASELINER_STATS, T.LANE_MODEL)
     for c, tr in enumerate(chunks):

@@ -124,3 +124,73 @@ def test_table_log_and_reciprocal_accuracy(O):
     assert _ulps(lg[big].astype(np.float32), ref[big]).max() <= 2
     assert np.abs(lg[~big] - ref[~big]).max() <= 1e-7
     assert _ulps(rcp, 1.0 / x.astype(np.float64)).max() <= 2
+
+
+# ---------------------------------------------------------------------------
+# The reference's own transcendental routines (Cephes exp / log of its SSE host
+# build, common/simd/sse_mathfun.h via m512f_SSE.h:444-466 and xcompile.h:148-151)
+# and the correctly rounded exp2 / log2 / pow standing in for its libm calls.
+# ---------------------------------------------------------------------------
+
+def test_cephes_exp_accuracy_and_sse_limits(O):
+    """Cephes expf is within one ulp of exp over its range (a wrong coefficient,
+    a dropped C2 reduction term or a misplaced floor correction all exceed it);
+    outside the clamp at +-88.3762626647949 the SSE routine saturates: 0 below,
+    a finite value above, and NaN (which the SSE min/max rule turns into the
+    upper clamp) gives the same value as the clamp."""
+    x = np.concatenate([np.linspace(-87, 88, 400001, dtype=np.float32),
+                        np.float32([0.0, -0.0, 1e-30, -1e-30, 0.5, -0.5, 1.0, -1.0])])
+    got = _eval(O, 11, x)
+    assert _ulps(got, np.exp(x.astype(np.float64))).max() <= 1.0
+    assert _eval(O, 11, np.float32([0.0]))[0] == 1.0
+    hi = np.float32(88.3762626647949)
+    sp = _eval(O, 11, np.float32([100.0, hi, np.nan, -100.0, -88.3762626647949, np.inf, -np.inf]))
+    assert sp[0] == sp[1] == sp[2] and np.isfinite(sp[0]) and sp[0] > 1e38
+    assert sp[3] == 0.0 and sp[4] == 0.0 and sp[6] == 0.0 and sp[5] == sp[0]
+    # just above the lower clamp the result is a subnormal 2^-126 multiple, not 0
+    assert 0.0 < _eval(O, 11, np.float32([-87.5]))[0] < np.float32(1.2e-38)
+
+
+def test_cephes_log_accuracy_and_sse_limits(O):
+    """Cephes logf is within one ulp of log where |log x| >= 0.01 and on every
+    float within 4000 ulp of 1 (the r = 2m - 1 branch below sqrt(1/2) included);
+    x <= 0 gives NaN, 0 gives -inf, subnormals and NaN are raised to FLT_MIN
+    first (the SSE max rule), and +inf returns log of the largest binade's
+    mantissa bits, as the bit manipulation does."""
+    x = np.exp(np.linspace(-87, 88, 400001)).astype(np.float32)
+    got = _eval(O, 12, x).astype(np.float64)
+    ref = np.log(x.astype(np.float64))
+    big = np.abs(ref) >= 0.01
+    assert _ulps(got[big].astype(np.float32), ref[big]).max() <= 1.0
+    near1 = np.float32(1) + np.arange(-4000, 4000, dtype=np.float32) * np.float32(2 ** -23)
+    nz = near1 != 1
+    assert _ulps(_eval(O, 12, near1)[nz], np.log(near1[nz].astype(np.float64))).max() <= 1.0
+    assert _eval(O, 12, np.float32([1.0]))[0] == 0.0
+    sp = _eval(O, 12, np.float32([0.0, -0.0, -1.0, np.nan, 1e-40, 1.1754944e-38, np.inf]))
+    assert sp[0] == -np.inf and sp[1] == -np.inf and np.isnan(sp[2])
+    assert sp[3] == sp[4] == sp[5] and abs(sp[5] - np.log(1.1754944e-38)) < 1e-5
+    assert np.isfinite(sp[6]) and abs(sp[6] - 88.7228) < 1e-3
+    # both sides of the sqrt(1/2) branch point agree with log to an ulp
+    m = np.float32([0.70710677, 0.7071067, 0.70710683, 1.4142135, 1.4142137])
+    assert _ulps(_eval(O, 12, m), np.log(m.astype(np.float64))).max() <= 1.0
+
+
+def test_double_rounded_exp2_log2_pow_are_correctly_rounded(O):
+    """hw_dexp2f / hw_dlog2f / hw_dpowf (the kernels' stand-ins for the libm
+    exp2f / log2f / powf the reference calls) equal the float64 result rounded
+    once, and agree with glibc's float functions except on a small fraction of
+    arguments where those are off by exactly one ulp."""
+    y = np.linspace(-120, 120, 200001).astype(np.float32)
+    x = np.exp(np.linspace(-80, 80, 200001)).astype(np.float32)
+    a = np.repeat(np.exp(np.linspace(-5, 11, 401)), 401).astype(np.float32)
+    b = np.tile(np.linspace(0, 1, 401), 401).astype(np.float32)
+    cases = [(16, 13, y, None, np.exp2(y.astype(np.float64))),
+             (17, 14, x, None, np.log2(x.astype(np.float64))),
+             (18, 15, a, b, np.power(a.astype(np.float64), b.astype(np.float64)))]
+    for mine, libm, u, v, ref in cases:
+        got = _eval(O, mine, u, v)
+        assert np.array_equal(got, ref.astype(np.float32)), mine
+        lm = _eval(O, libm, u, v)
+        diff = got != lm
+        assert diff.mean() < 2e-3, (mine, diff.mean())
+        assert _ulps(lm[diff], got[diff].astype(np.float64)).max() <= 1.0

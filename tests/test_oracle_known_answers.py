@@ -327,8 +327,9 @@ def test_viterbi_recursion_brute_force(H, O, T, seed):
 # EM -- statistical recovery (BT/TestDmeEm.cpp:160-244) and invariants.
 # ---------------------------------------------------------------------------
 
+@pytest.mark.parametrize("form", ["reference", "mirror"])
 @pytest.mark.parametrize("snr", [4.0, 6.0])
-def test_em_recovers_mixture(H, O, T, snr):
+def test_em_recovers_mixture(H, O, T, snr, form):
     cfg = H.default_config()
     L = 1
     rng = np.random.default_rng(int(snr))
@@ -358,7 +359,7 @@ def test_em_recovers_mixture(H, O, T, snr):
         stats["full"]["basic"]["m0"][0, z] = nframes
     cfg.movie.ref_snr = snr
     models = O.dme_init_models(cfg, stats, T.LANE_MODEL)
-    out, diag = O.dme_estimate(cfg, hist, stats, models, (0, 4000), (0, 4000), want_diag=True)
+    out, diag = O.dme_estimate(cfg, hist, stats, models, (0, 4000), (0, 4000), want_diag=True, form=form)
     est = np.array(diag[0].est_fp32)          # [mode][w, mean, var][zmw]
     assert np.mean(np.array(diag[0].converged)) > 0.9
     np.testing.assert_allclose(est[1:, 1, :].mean(axis=1), means[1:], rtol=0.05)
@@ -372,11 +373,12 @@ def test_em_recovers_mixture(H, O, T, snr):
     assert np.all(est[0, 2, :] >= 1.0 / 12.0)
 
 
-def test_em_invariants_on_pipeline_data(H, O, T):
+@pytest.mark.parametrize("form", ["reference", "mirror"])
+def test_em_invariants_on_pipeline_data(H, O, T, form):
     from test_gpu_parity import _em_case
     L, F = 2, 512
     cfg, h, s, m = _em_case(H, O, T, L, F, seed=5)
-    out, diag = O.dme_estimate(cfg, h, s, m, (512, 1024), (1024, 1536), want_diag=True)
+    out, diag = O.dme_estimate(cfg, h, s, m, (512, 1024), (1024, 1536), want_diag=True, form=form)
     blmin, blmax = 1.0 / 12.0, 36 * 4.0
     for l in range(L):
         est = np.array(diag[l].est_fp32)
