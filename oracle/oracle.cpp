@@ -916,6 +916,30 @@ void ComputeConfidence(const DmeCfg& d, bool converged, const ModelZ& ref, const
     cf[7] = 1.0f;  // G-test disabled (GTestStatFactor < 0): pValue stays 1
 }
 
+// EvolveModel (DmeEmHost.cpp:806-834) + EvolveConfidence (DetectionModelHost.h:190-202):
+// the confidence multiplier 2^(-|tEst - tModel| / hl) between interval centres.
+template <class M>
+float EvolveFactor(const DmeCfg& d, const int32_t* modelFi, const int32_t* estFi)
+{
+    const int32_t tModel = modelFi[0] + (int32_t)((uint32_t)(modelFi[1] - modelFi[0]) / 2u);
+    const int32_t tEst = estFi[0] + (int32_t)((uint32_t)(estFi[1] - estFi[0]) / 2u);
+    const float thl = 0.5f * (float)(tModel + tEst);
+    const float hl = d.hl0 + (d.hl1 - d.hl0) * hw_satlin(56160.0f, 168480.0f, thl);
+    const float tDiff = (float)(tEst - tModel);
+    return M::Exp2(-hw_fabsf(tDiff) / hl);
+}
+
+// UHistogramSimd(const LaneHistogram&) for one ZMW (UHistogramSimd.cpp:100-118).
+void MakeHistZ(const hwpa_lane_hist& lh, int z, HistZ& h)
+{
+    h.binSize = lh.bin_size[z];
+    for (int b = 0; b <= NB; ++b) h.start[b] = lh.low_bound[z] + (float)b * h.binSize;
+    for (int b = 0; b < NB; ++b) h.cnt[b] = lh.bin_count[b][z];
+    h.cnt[NB] = 0;
+    h.lowOut = lh.outlier_low[z];
+    h.highOut = lh.outlier_high[z];
+}
+
 // The EM iterations as the GPU kernel computes them (hw_amd/csrc/dme.hip
 // DmeEstimateKernel): one pass over the bin boundaries per iteration with the
 // component densities carried directly (no log-sum-exp), per-boundary sums in
@@ -1265,23 +1289,12 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
     const float rpaSqSum = (rpa[0] * rpa[0] + rpa[2] * rpa[2]) + (rpa[1] * rpa[1] + rpa[3] * rpa[3]);
     const float rpaMin = std::min(std::min(rpa[0], rpa[1]), std::min(rpa[2], rpa[3]));
 
-    // EvolveModel (DmeEmHost.cpp:806-834) + EvolveConfidence (DetectionModelHost.h:153-163)
-    const int32_t tModel = modelFi[0] + (int32_t)((uint32_t)(modelFi[1] - modelFi[0]) / 2u);
-    const int32_t tEst = estFi[0] + (int32_t)((uint32_t)(estFi[1] - estFi[0]) / 2u);
-    const float thl = 0.5f * (float)(tModel + tEst);
-    const float hl = d.hl0 + (d.hl1 - d.hl0) * hw_satlin(56160.0f, 168480.0f, thl);
-    const float tDiff = (float)(tEst - tModel);
-    const float evolve = M::Exp2(-hw_fabsf(tDiff) / hl);
+    const float evolve = EvolveFactor<M>(d, modelFi, estFi);
 
     for (int z = 0; z < LS; ++z) {
         EmZ& e = ez[z];
         HistZ& h = hz[z];
-        h.binSize = lh.bin_size[z];
-        for (int b = 0; b <= NB; ++b) h.start[b] = lh.low_bound[z] + (float)b * h.binSize;
-        for (int b = 0; b < NB; ++b) h.cnt[b] = lh.bin_count[b][z];
-        h.cnt[NB] = 0;
-        h.lowOut = lh.outlier_low[z];
-        h.highOut = lh.outlier_high[z];
+        MakeHistZ(lh, z, h);
 
         e.det = ModelFromHalf(lm, z);
         e.det.conf *= evolve;
@@ -1767,6 +1780,45 @@ struct Scorer {
     }
 };
 
+// RoiFilterDefault::SmoothedVal, Roi::ForwardRecursion and Roi::BackwardRecursion
+// (RoiHost.h:75-248, FrameLabelerHost.cpp:283-309) for one ZMW: seq(k), k in
+// [0, nroi + 2), is the sigma-normalised trace and `before` the frame preceding it.
+// Forward, frame k's smoothed value is the largest of x_k, (x_k-1 + x_k)/sqrt 2,
+// (x_k-1 + x_k + x_k+1)/sqrt 3 and (x_k-1 + .. + x_k+2)/2; a frame is marked strong
+// when that value reaches the upper threshold, or the lower one while the frame
+// before was strong (hysteresis, the state carried in `boundary`), and weak when it
+// reaches the lower threshold.  Backward, a frame is in the ROI when it is strong,
+// or weak with the next frame in the ROI.  out[k * stride] receives 1 (ROI) or 0.
+template <class Seq>
+void RoiZ(float upper, float lower, float before, uint16_t boundary, Seq seq, int nroi, uint16_t* out,
+          size_t stride)
+{
+    float prev[3] = {before, seq(0), seq(1)};
+    uint16_t prevRoi = boundary;
+    const float s2 = hw_sqrtf(2.0f), s3 = hw_sqrtf(3.0f);
+    for (int k = 0; k < nroi; ++k) {
+        const float nv = seq(k + 2);
+        float mval = prev[1];
+        mval = hw_maxf(mval, (prev[0] + prev[1]) / s2);
+        mval = hw_maxf(mval, (prev[0] + prev[1] + prev[2]) / s3);
+        mval = hw_maxf(mval, (prev[0] + prev[1] + prev[2] + nv) / 2.0f);
+        const float thresh = (prevRoi & 1u) ? lower : upper;
+        const uint16_t v = (uint16_t)((mval >= thresh ? 1u : 0u) | (mval >= lower ? 2u : 0u));
+        out[(size_t)k * stride] = v;
+        prevRoi = v;
+        prev[0] = prev[1];
+        prev[1] = prev[2];
+        prev[2] = nv;
+    }
+    uint16_t pr = 0;
+    for (int k = nroi - 1; k >= 0; --k) {
+        const uint16_t r = out[(size_t)k * stride];
+        const bool b = (r & 1u) || ((r & 2u) && pr != 0);
+        pr = b ? 1 : 0;
+        out[(size_t)k * stride] = b ? 1 : 0;
+    }
+}
+
 struct LaneLabeler {
     hwpa_lane_model oldModel;
     int16_t labelBoundary[LS];
@@ -1801,30 +1853,7 @@ void LabelLane(const LabelerCtx& c, LaneLabeler& L, const int16_t* tr, int frame
         auto seq = [&](int k) -> float {  // concatenated prevLat ++ traces, sigma-normalised
             return (k < NL) ? (float)L.prevLat[k][z] * inv1 : (float)tr[(size_t)(k - NL) * LS + z] * inv2;
         };
-        float prev[3] = {(float)L.latentTrc[z] * inv1, seq(0), seq(1)};
-        uint16_t prevRoi = L.roiBoundary[z];
-        const float s2 = hw_sqrtf(2.0f), s3 = hw_sqrtf(3.0f);
-        for (int k = 0; k < nroi; ++k) {
-            const float nv = seq(k + 2);
-            float mval = prev[1];
-            mval = hw_maxf(mval, (prev[0] + prev[1]) / s2);
-            mval = hw_maxf(mval, (prev[0] + prev[1] + prev[2]) / s3);
-            mval = hw_maxf(mval, (prev[0] + prev[1] + prev[2] + nv) / 2.0f);
-            const float thresh = (prevRoi & 1u) ? c.roiLower : c.roiUpper;
-            const uint16_t v = (uint16_t)((mval >= thresh ? 1u : 0u) | (mval >= c.roiLower ? 2u : 0u));
-            roi[(size_t)k * LS + z] = v;
-            prevRoi = v;
-            prev[0] = prev[1];
-            prev[1] = prev[2];
-            prev[2] = nv;
-        }
-        uint16_t pr = 0;
-        for (int k = nroi - 1; k >= 0; --k) {
-            const uint16_t r = roi[(size_t)k * LS + z];
-            const bool b = (r & 1u) || ((r & 2u) && pr != 0);
-            pr = b ? 1 : 0;
-            roi[(size_t)k * LS + z] = b ? 1 : 0;
-        }
+        RoiZ(c.roiUpper, c.roiLower, (float)L.latentTrc[z] * inv1, L.roiBoundary[z], seq, nroi, &roi[z], LS);
 
         // Viterbi recursion (FrameLabelerHost.cpp:213-264)
         float ll[NS];
@@ -3271,6 +3300,96 @@ void orc_hgw(int32_t width, int32_t stride, int32_t is_max, int32_t chunks, int3
 // state scores: scores[f][13], roi[f], initial log-likelihoods ll0[13].
 // Writes the traceback-decoded best path ending in argmax of the final
 // log-likelihood (later state wins ties) and the final log-likelihoods.
+// ---- single-function hooks for the known-answer tests -----------------------------
+// Models as 16 floats: baseline {weight, mean, var}, analogs[4] {weight, mean, var}, confidence.
+static_assert(sizeof(ModelZ) == 16 * sizeof(float), "ModelZ is 16 packed floats");
+
+// DetectionModelHost::Update(other, fraction) (fraction >= 0) or Update(other) (fraction < 0).
+void orc_dme_update(int32_t form, const hwpa_config* cfg, const float* t16, const float* o16, float fraction,
+                    float* out16)
+{
+    const DmeCfg d = MakeDmeCfg(*cfg);
+    ModelZ t, o;
+    std::memcpy(&t, t16, sizeof t);
+    std::memcpy(&o, o16, sizeof o);
+    if (form == ORC_FORM_MIRROR) {
+        if (fraction < 0.0f) UpdateConf<MirrorForm>(d, t, o);
+        else UpdateFraction<MirrorForm>(d, t, o, fraction);
+    } else {
+        if (fraction < 0.0f) UpdateConf<ReferenceForm>(d, t, o);
+        else UpdateFraction<ReferenceForm>(d, t, o, fraction);
+    }
+    std::memcpy(out16, &t, sizeof t);
+}
+
+// DmeEmHost::ComputeConfidence: the 8 factors for one ZMW.
+void orc_dme_confidence(int32_t form, const hwpa_config* cfg, int32_t converged, const float* ref16,
+                        const float* est16, float* cf8)
+{
+    const DmeCfg d = MakeDmeCfg(*cfg);
+    ModelZ r, e;
+    std::memcpy(&r, ref16, sizeof r);
+    std::memcpy(&e, est16, sizeof e);
+    if (form == ORC_FORM_MIRROR) ComputeConfidence<MirrorForm>(d, converged != 0, r, e, cf8);
+    else ComputeConfidence<ReferenceForm>(d, converged != 0, r, e, cf8);
+}
+
+// EvolveModel's confidence multiplier between two frame intervals [lo, hi).
+float orc_dme_evolve(int32_t form, const hwpa_config* cfg, const int32_t model_fi[2], const int32_t est_fi[2])
+{
+    const DmeCfg d = MakeDmeCfg(*cfg);
+    return form == ORC_FORM_MIRROR ? EvolveFactor<MirrorForm>(d, model_fi, est_fi)
+                                   : EvolveFactor<ReferenceForm>(d, model_fi, est_fi);
+}
+
+// UHistogramSimd::Fractile / CumulativeCount per ZMW of one lane histogram.
+void orc_hist_fractile(const hwpa_lane_hist* lh, const float* frac, float* out)
+{
+    HistZ h;
+    for (int z = 0; z < LS; ++z) {
+        MakeHistZ(*lh, z, h);
+        out[z] = h.Fractile(frac[z]);
+    }
+}
+void orc_hist_cumulative(const hwpa_lane_hist* lh, const float* x, float* out)
+{
+    HistZ h;
+    for (int z = 0; z < LS; ++z) {
+        MakeHistZ(*lh, z, h);
+        out[z] = h.CumulativeCount(x[z]);
+    }
+}
+
+// DmeEmHost::PrelimScaleFactor per ZMW: models16 [64][16].
+void orc_prelim_scale_factor(const hwpa_config* cfg, const hwpa_lane_hist* lh, const float* models16, float* out)
+{
+    const DmeCfg d = MakeDmeCfg(*cfg);
+    HistZ h;
+    for (int z = 0; z < LS; ++z) {
+        MakeHistZ(*lh, z, h);
+        ModelZ m;
+        std::memcpy(&m, models16 + 16 * z, sizeof m);
+        out[z] = PrelimScaleFactor(d, m, h);
+    }
+}
+
+// BlockStateSubframeScorer::StateScores of ZMW z for n data values: out [n][13].
+void orc_scorer_scores(const hwpa_lane_model* m, int32_t z, const float* x, int32_t n, float* out)
+{
+    Scorer sc;
+    sc.Setup(*m, z);
+    for (int32_t i = 0; i < n; ++i)
+        for (int s = 0; s < NS; ++s) out[(size_t)i * NS + s] = sc.Score(x[i], s);
+}
+
+// The labeler's ROI of one ZMW: seq [nroi + 2] sigma-normalised frames after `before`.
+void orc_roi(float upper, float lower, float before, uint16_t boundary, const float* seq, int32_t nroi, uint8_t* out)
+{
+    std::vector<uint16_t> r((size_t)nroi);
+    RoiZ(upper, lower, before, boundary, [&](int k) { return seq[k]; }, nroi, r.data(), 1);
+    for (int32_t k = 0; k < nroi; ++k) out[k] = (uint8_t)r[k];
+}
+
 void orc_viterbi_core(const hwpa_config* cfg, int32_t frames, const float* scores, const uint8_t* roi,
                       const float* ll0, int16_t* path, float* llOut)
 {
