@@ -335,7 +335,7 @@ def dme_estimate_learned(cfg, init, d_hist, d_stats, d_models, model_fi, est_fi,
     return n.value
 
 
-MLP_IN, MLP_HIDDEN, MLP_OUT = 128, 256, 11
+MLP_IN, MLP_HIDDEN, MLP_OUT = 128, 256, 12
 MLP_PARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_OUT * MLP_HIDDEN + MLP_OUT
 EM_DIAG_ROWS = 18
 
@@ -357,13 +357,16 @@ class Initializer:
         check(lib.hwpa_initializer_set_params(self.h, p.ctypes.data_as(_vp), ctypes.c_size_t(p.size)))
 
     def set_output_map(self, offset, scale, weight):
+        """weight: 8 loss weights (6 regression outputs, mixture cross-entropy,
+        confidence logistic loss); 7 values keep the confidence weight at 1."""
+        weight = list(weight) + [1.0] * (8 - len(weight))
         off = (ctypes.c_float * 6)(*[float(v) for v in offset])
         sc = (ctypes.c_float * 6)(*[float(v) for v in scale])
-        w = (ctypes.c_float * 7)(*[float(v) for v in weight])
+        w = (ctypes.c_float * 8)(*[float(v) for v in weight])
         check(lib.hwpa_initializer_set_output_map(self.h, off, sc, w))
 
     def output_map(self):
-        out = (ctypes.c_float * 19)()
+        out = (ctypes.c_float * 20)()
         check(lib.hwpa_initializer_get_output_map(self.h, out))
         return np.array(out, dtype=np.float32)
 

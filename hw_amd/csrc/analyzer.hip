@@ -205,7 +205,11 @@ static int32_t DmeAddBatch(hwpa_analyzer* a, int32_t first_frame)
             a->fullEstimation = true;
         }
         a->plan |= kPlanReset;
-        if ((rc = HistReset(a->cfg, a->dHist, a->dAgg, L, s)) != HWPA_OK) return rc;
+        // The next chunk's add always comes before the counters are read again, so
+        // their zeros are left to it (chunks of up to 16384 frames).
+        rc = (F <= 16384u) ? HistResetDeferred(a->cfg, a->dHist, a->dEdge, a->dAgg, L, s)
+                           : HistReset(a->cfg, a->dHist, a->dAgg, L, s);
+        if (rc != HWPA_OK) return rc;
         a->histFi = Interval{};
         if ((rc = StatsReset(a->dAgg, L, s)) != HWPA_OK) return rc;
         a->aggFi = Interval{};

@@ -200,10 +200,16 @@ struct HistView {
     __device__ uint32_t Cnt(int b) const { return (b < NB) ? (uint32_t)h->bin_count[b][z] : 0u; }
 };
 
-__device__ uint32_t InRange(const HistView& h)
+__device__ uint32_t InRange(const HistView& h, int* lastNonzero = nullptr)
 {
     uint32_t s = 0;
-    for (int i = 0; i < NB; ++i) s = (s + h.Cnt(i)) & 0xffffu;
+    int last = -1;
+    for (int i = 0; i < NB; ++i) {
+        const uint32_t c = h.Cnt(i);
+        s = (s + c) & 0xffffu;
+        last = c ? i : last;
+    }
+    if (lastNonzero) *lastNonzero = last;
     return s;
 }
 
@@ -259,6 +265,7 @@ struct EmExtra {
     float acceptConf;         // EM_LEARNED: minimum confidence to accept the capped EM
     float convTol;            // EM_LEARNED: relative log-likelihood convergence tolerance
     int32_t iterLimit;        // EM_LEARNED: iteration cap
+    float gateLogit;          // EM_LEARNED: network confidence logit below which the capped EM is skipped
 };
 
 // DmeEmHost::EstimateLaneDetModel (DmeEmHost.cpp:216-615).  EM_LEARNED replaces
@@ -293,7 +300,8 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
 
     HistView h{&lh, z, lh.low_bound[z], lh.bin_size[z]};
     const uint32_t lowOut = lh.outlier_low[z], highOut = lh.outlier_high[z];
-    const uint32_t inRange = InRange(h);
+    int lastNz;
+    const uint32_t inRange = InRange(h, &lastNz);
     const uint32_t numFrames = (inRange + ((lowOut + highOut) & 0xffffu)) & 0xffffu;
 
     const float rpa[NA] = {d.relAmp[0], d.relAmp[1], d.relAmp[2], d.relAmp[3]};
@@ -393,6 +401,7 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
     for (int b = 0; b < NB; ++b) njSum += (float)h.Cnt(b);
     const float halfBin = 0.5f * h.bs;
     int iterLimit = d.iterLimit;
+    bool gated = false;       // EM_LEARNED: the network's confidence sends this ZMW to the plain EM
     if (MODE == EM_LEARNED) {
         // Network start: output j < 6 stands for off[j] + scale[j] * y[j] in units
         // of u = 2 binSize from the lower bound (means) and log(var / u^2);
@@ -430,7 +439,11 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
             varEst[k] = var[k];
         }
         s = ((mu[1] * rpa[0] + mu[2] * rpa[1]) + (mu[3] * rpa[2] + mu[4] * rpa[3])) / rpaSqSum;
-        iterLimit = x.iterLimit;
+        // Confidence gate: a ZMW below it takes no capped iteration of its own (it still
+        // runs in step with its lane, but counts as done and is rejected); a lane whose
+        // 64 ZMWs are all below it skips the capped EM.
+        gated = !(y[11] >= x.gateLogit);
+        iterLimit = __all(gated) ? 0 : x.iterLimit;
     }
 
     int it = 0;
@@ -533,7 +546,26 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
 #pragma unroll
         for (int j = 0; j < kCntAhead; ++j) cq[j] = h.Cnt(j);
         static_assert(NB % kRecurBlock == 0, "whole blocks of boundaries");
+        // The tail of the histogram is left out once, for every ZMW of the wave, the bins
+        // ahead are empty, the boundary lies beyond every mode's mean (so each density
+        // only falls from here on) and bs times the density sum is below 2^-27 of
+        // probSum, as is binProbMin: then every remaining bin probability is below half
+        // an ulp of probSum, so adding them would leave it unchanged, and every other sum
+        // of those boundaries is an exact zero (empty bins).  The result equals the full
+        // pass bit for bit.
+        float maxMu = mu[0];
+#pragma unroll
+        for (int k = 1; k < 5; ++k) maxMu = hw_maxf(maxMu, mu[k]);
+        int bEnd = NB;
         for (int b0 = 0; b0 < NB; b0 += kRecurBlock) {
+            if (b0 > 0) {
+                const float tol = hw_ldexpf(probSum, -27);
+                const bool tail = lastNz < b0 && h.Start(b0) > maxMu && h.bs * bpPrev < tol && binProbMin < tol;
+                if (__all(tail)) {
+                    bEnd = b0;
+                    break;
+                }
+            }
 #pragma unroll
             for (int j = 0; j < kRecurBlock; ++j) {
                 const int b = b0 + j;
@@ -562,7 +594,7 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
                 fPrev = f;
             }
         }
-        gather(dPrev, h.Start(NB), fPrev);
+        gather(dPrev, h.Start(bEnd), fPrev);
         const float q0 = ci[0];
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
@@ -588,7 +620,7 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
         }
         converged = converged || conv;
         // Lane-wide early exit (DmeEmHost.cpp:485)
-        const bool done = converged || (status != HWPA_ZS_OK);
+        const bool done = converged || (status != HWPA_ZS_OK) || gated;
         if (!d.iterToLimit && __all(done)) break;
 
         // ---- M-step
@@ -694,7 +726,7 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
         dg[17 * LS] = (float)status;
     }
     if (MODE == EM_LEARNED) {
-        const bool accept = converged && status == HWPA_ZS_OK && conf >= x.acceptConf;
+        const bool accept = !gated && converged && status == HWPA_ZS_OK && conf >= x.acceptConf;
         x.fallback[(size_t)lane * LS + z] = accept ? 0 : 1;
         if (!accept) return;
     }
@@ -929,6 +961,7 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
     x.acceptConf = cfg.learned_conf_threshold;
     x.convTol = cfg.learned_conv_tol;
     x.iterLimit = (int32_t)cfg.learned_em_iteration_limit;
+    x.gateLogit = MlpGateLogit(cfg.learned_gate_threshold);
     hipLaunchKernelGGL(DmeEstimateKernel<EM_LEARNED>, dim3(lanes), dim3(LS), 0, s, hist, stats, models, d_iters, d,
                        evolve, x);
     if ((rc = CheckLaunch("DmeEstimateKernel")) != HWPA_OK) return rc;

@@ -24,7 +24,8 @@ struct EdgeLane {
     int32_t i0;
     int32_t i1;
     int32_t needWrap;  // set by the fast pass for lanes left to the 16-bit wrapping pass
-    int32_t pad[29];
+    int32_t fresh;     // the counters are known to be zero (a reset that left them unwritten)
+    int32_t pad[28];
 };
 
 size_t EdgeStateBytes() { return sizeof(EdgeLane); }
@@ -40,6 +41,7 @@ __global__ __launch_bounds__(64) void EdgeInitKernel(EdgeLane* e)
         s.i0 = 0;
         s.i1 = -1;
         s.needWrap = 0;
+        s.fresh = 0;
     }
 }
 
@@ -55,8 +57,12 @@ struct HistConsts {
 };
 
 // TraceHistogramAccumHost::ResetImpl(const BaselinerMetrics&) + UHistogramSimd ctor.
-__global__ __launch_bounds__(64) void HistResetKernel(hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
-                                                      HistConsts k)
+// ZERO writes the 300 zero counters; otherwise the lane is marked fresh and the next
+// add starts its counters from zero without reading them (the reset's 38 KB of
+// zeros per lane are neither written here nor read back there).
+template <bool ZERO>
+__global__ __launch_bounds__(64) void HistResetKernel(hwpa_lane_hist* hist, EdgeLane* edge,
+                                                      const hwpa_baseliner_stats* stats, HistConsts k)
 {
     hwpa_lane_hist& h = hist[blockIdx.x];
     const hwpa_baseliner_stats& s = stats[blockIdx.x];
@@ -76,7 +82,23 @@ __global__ __launch_bounds__(64) void HistResetKernel(hwpa_lane_hist* hist, cons
     h.bin_size[z] = (up - lo) / (float)NB;
     h.outlier_low[z] = 0;
     h.outlier_high[z] = 0;
-    for (int b = 0; b < NB; ++b) h.bin_count[b][z] = 0;
+    if (ZERO) {
+        for (int b = 0; b < NB; ++b) h.bin_count[b][z] = 0;
+    } else if (z == 0) {
+        edge[blockIdx.x].fresh = 1;
+    }
+}
+
+// Zeroes bins [NL, NB) of a fresh lane before they are counted in place; the
+// caller's barrier orders these stores before the increments.
+template <int NL, int NT>
+__device__ __forceinline__ void ZeroHighBins(hwpa_lane_hist& h, int tid)
+{
+    if (NL < NB) {
+        uint4* rest = reinterpret_cast<uint4*>(&h.bin_count[NL][0]);
+        constexpr int NV = (NB - NL) * LS * 2 / 16;
+        for (int i = tid; i < NV; i += NT) rest[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
 }
 
 // Bin of a sample: floor((x - lo) / bs) with the reference's IEEE divide
@@ -247,8 +269,14 @@ __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLa
     const int col = (z >> 1) + (odd ? LS / 2 : 0);   // dword index within a bin pair (2 rows = 64 dwords)
     uint32_t mx = 0;
     constexpr int NJ = NL / 2;
+    const bool fresh = !WRAP && e.fresh != 0;
+    if (fresh) {
+        for (int j = 0; j < NJ; ++j) cnt[j * LS + z] = 0u;
+        ZeroHighBins<NL, LS>(h, z);
+        __threadfence_block();
+    }
 #pragma unroll 1
-    for (int j0 = 0; j0 < NJ; j0 += 16) {
+    for (int j0 = 0; j0 < (fresh ? 0 : NJ); j0 += 16) {
         uint32_t c[16];
 #pragma clang loop unroll(full)
         for (int k = 0; k < 16; ++k) {
@@ -265,7 +293,7 @@ __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLa
         }
     }
     // Bins [NL, NB): only their maximum matters; read as 16-byte vectors.
-    if (NL < NB) {
+    if (NL < NB && !fresh) {
         const uint4* rest = reinterpret_cast<const uint4*>(&h.bin_count[NL][0]);
         constexpr int NV = (NB - NL) * LS * 2 / 16;
 #pragma unroll 1
@@ -329,6 +357,7 @@ __global__ __launch_bounds__(64) void HistAddKernel(hwpa_lane_hist* hist, EdgeLa
         e.i0 = 0;
         e.i1 = (pushes == 0) ? -1 : (pushes == 1 ? 0 : 1);
         e.needWrap = 0;
+        e.fresh = 0;
     }
 }
 
@@ -359,8 +388,14 @@ __global__ __launch_bounds__(64 * W) void HistAddWavesKernel(hwpa_lane_hist* his
     const int col = (z >> 1) + (odd ? LS / 2 : 0);
     uint32_t mx = 0;
     constexpr int NJ = NL / 2;
+    const bool fresh = e.fresh != 0;
+    if (fresh) {
+        for (int j = w; j < NJ; j += W) cnt[j * LS + z] = 0u;
+        ZeroHighBins<NL, W * LS>(h, threadIdx.x);
+        __threadfence_block();
+    }
 #pragma unroll 1
-    for (int j0 = 16 * w; j0 < NJ; j0 += 16 * W) {
+    for (int j0 = 16 * w; j0 < (fresh ? 0 : NJ); j0 += 16 * W) {
         uint32_t c[16];
 #pragma clang loop unroll(full)
         for (int k = 0; k < 16; ++k) {
@@ -376,7 +411,7 @@ __global__ __launch_bounds__(64 * W) void HistAddWavesKernel(hwpa_lane_hist* his
             cnt[j * LS + z] = packed;   // a clamped duplicate writes the same value
         }
     }
-    if (NL < NB) {
+    if (NL < NB && !fresh) {
         const uint4* rest = reinterpret_cast<const uint4*>(&h.bin_count[NL][0]);
         constexpr int NV = (NB - NL) * LS * 2 / 16;
 #pragma unroll 1
@@ -457,18 +492,36 @@ __global__ __launch_bounds__(64 * W) void HistAddWavesKernel(hwpa_lane_hist* his
             e.needWrap = 0;
         }
     }
+    if (threadIdx.x == 0) e.fresh = 0;
+}
+
+static HistConsts MakeHistConsts(const hwpa_config& cfg)
+{
+    HistConsts k;
+    k.binSizeCoeff = cfg.bin_size_coeff;
+    k.binSizeMin = cfg.bin_size_low_bound_coeff * std::max(1.0f, cfg.movie.photoelectron_sensitivity);
+    k.fallbackSigma = cfg.fallback_baseline_sigma;
+    k.minFrameCount = (float)cfg.baseline_stat_min_frame_count;
+    return k;
+}
+
+// Reset that defers the zero counters to the lane's next add (which must come
+// before anything reads the counters, with at most 65535 frames).
+int32_t HistResetDeferred(const hwpa_config& cfg, hwpa_lane_hist* hist, void* edge, const hwpa_baseliner_stats* stats,
+                          uint32_t lanes, hipStream_t s)
+{
+    if (!hist || !edge || !stats || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "hist_reset: bad argument");
+    hipLaunchKernelGGL((HistResetKernel<false>), dim3(lanes), dim3(LS), 0, s, hist, (EdgeLane*)edge, stats,
+                       MakeHistConsts(cfg));
+    return CheckLaunch("HistResetKernel");
 }
 
 int32_t HistReset(const hwpa_config& cfg, hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats, uint32_t lanes,
                   hipStream_t s)
 {
     if (!hist || !stats || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "hist_reset: bad argument");
-    HistConsts k;
-    k.binSizeCoeff = cfg.bin_size_coeff;
-    k.binSizeMin = cfg.bin_size_low_bound_coeff * std::max(1.0f, cfg.movie.photoelectron_sensitivity);
-    k.fallbackSigma = cfg.fallback_baseline_sigma;
-    k.minFrameCount = (float)cfg.baseline_stat_min_frame_count;
-    hipLaunchKernelGGL(HistResetKernel, dim3(lanes), dim3(LS), 0, s, hist, stats, k);
+    hipLaunchKernelGGL((HistResetKernel<true>), dim3(lanes), dim3(LS), 0, s, hist, nullptr, stats,
+                       MakeHistConsts(cfg));
     return CheckLaunch("HistResetKernel");
 }
 

@@ -297,6 +297,7 @@ int32_t hwpa_config_default(hwpa_config* c)
     c->learned_conf_threshold = 0.0f;
     c->learned_em_iteration_limit = 3;
     c->learned_conv_tol = 2e-3f;
+    c->learned_gate_threshold = 0.5f;
     // Movie description of the synthetic workload: 100 fps, 1 e-/DN, four analogs
     // with amplitudes 180/120/80/50 over a 12-count baseline sigma.
     c->movie.frame_rate = 100.0f;
@@ -360,6 +361,7 @@ int32_t hwpa_config_load_json(hwpa_config* c, const char* json)
         {"pulseAccumConfig.XspWidthThresh", &c->xsp_width_thresh},
         {"learnedInit.ConfidenceThreshold", &c->learned_conf_threshold},
         {"learnedInit.ConvergenceTolerance", &c->learned_conv_tol},
+        {"learnedInit.GateThreshold", &c->learned_gate_threshold},
         {"staticDetModelConfig.baselineMean", &c->static_baseline_mean},
         {"staticDetModelConfig.baselineVariance", &c->static_baseline_variance},
         {"movie.frameRate", &c->movie.frame_rate},
@@ -479,6 +481,8 @@ int32_t hwpa_config_validate(const hwpa_config* c)
     if (!(MakeDmeConsts(*c).blVarMax < 6.5e4f)) return Fail(HWPA_ERR_BAD_CONFIG, "BaselineVarianceMax exceeds half range");
     if (c->learned_em_iteration_limit == 0 && c->learned_init) return Fail(HWPA_ERR_BAD_CONFIG, "learned EM iteration limit must be positive");
     if (!(c->learned_conv_tol > 0.0f)) return Fail(HWPA_ERR_BAD_CONFIG, "learned EM convergence tolerance must be positive");
+    if (!(c->learned_gate_threshold >= 0.0f && c->learned_gate_threshold < 1.0f))
+        return Fail(HWPA_ERR_BAD_CONFIG, "learned gate threshold must be in [0, 1)");
     return HWPA_OK;
 }
 
@@ -721,26 +725,42 @@ int32_t hwpa_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes)
     return HWPA_OK;
 }
 
-// Plain streaming copy, one 16-byte element per thread (the fastest of the
-// simple forms on MI355X: about 6.1 TB/s read + write over 2 GiB, against
-// 4.7 TB/s for a grid-stride loop): the reference point for achievable HBM
-// bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
-__global__ __launch_bounds__(256) void CopyKernel(uint4* __restrict__ d, const uint4* __restrict__ s, size_t n)
+// Streaming copies, the reference point for achievable HBM bandwidth and for
+// scaling the FETCH_SIZE / WRITE_SIZE counters: K 16-byte elements per thread, a
+// workgroup's K loads issued before its K stores and each of them coalesced over
+// the workgroup (element block * 256 K + k * 256 + tid); NT uses non-temporal
+// loads and stores.  hwpa_copy_bandwidth times every form and keeps the fastest.
+}  // extern "C"
+
+namespace {
+typedef unsigned int U32x4 __attribute__((ext_vector_type(4)));
+
+template <int K, bool NT>
+__global__ __launch_bounds__(256) void CopyKernel(U32x4* __restrict__ d, const U32x4* __restrict__ s, size_t n)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = s[i];
+    const size_t base = (size_t)blockIdx.x * 256 * K + threadIdx.x;
+    U32x4 v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const size_t i = base + (size_t)k * 256;
+        if (i < n) v[k] = NT ? __builtin_nontemporal_load(&s[i]) : s[i];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const size_t i = base + (size_t)k * 256;
+        if (i < n) {
+            if (NT) __builtin_nontemporal_store(v[k], &d[i]);
+            else d[i] = v[k];
+        }
+    }
 }
 
-int32_t hwpa_copy_bandwidth(void* d_dst, const void* d_src, size_t bytes, int32_t launches, float* ms_per_launch)
+template <int K, bool NT>
+static int32_t TimeCopy(void* d_dst, const void* d_src, size_t n, int32_t launches, float* ms_out)
 {
-    if (!d_dst || !d_src || !ms_per_launch || launches <= 0)
-        return Fail(HWPA_ERR_INVALID_ARG, "copy: null pointer or no launches");
-    if ((bytes % 16) != 0 || ((uintptr_t)d_dst % 16) != 0 || ((uintptr_t)d_src % 16) != 0)
-        return Fail(HWPA_ERR_INVALID_ARG, "copy: pointers and size must be 16-byte multiples");
-    const size_t n = bytes / 16;
-    if (n == 0) return Fail(HWPA_ERR_INVALID_ARG, "copy: empty");
-    if ((n + 255) / 256 > 0x7fffffffu) return Fail(HWPA_ERR_INVALID_ARG, "copy: too large");
-    const unsigned blocks = (unsigned)((n + 255) / 256);
+    const size_t per = (size_t)256 * K;
+    if ((n + per - 1) / per > 0x7fffffffu) return Fail(HWPA_ERR_INVALID_ARG, "copy: too large");
+    const unsigned blocks = (unsigned)((n + per - 1) / per);
     hipEvent_t e0, e1;
     HWPA_HIP(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -748,9 +768,14 @@ int32_t hwpa_copy_bandwidth(void* d_dst, const void* d_src, size_t bytes, int32_
         return Fail(HWPA_ERR_HIP, "hipEventCreate failed");
     }
     int32_t rc = HWPA_OK;
+    // one untimed launch so every form starts from the same cache state
+    hipLaunchKernelGGL((CopyKernel<K, NT>), dim3(blocks), dim3(256), 0, nullptr, (U32x4*)d_dst, (const U32x4*)d_src,
+                       n);
+    rc = CheckLaunch("CopyKernel");
     (void)hipEventRecord(e0, nullptr);
     for (int32_t i = 0; i < launches && rc == HWPA_OK; ++i) {
-        hipLaunchKernelGGL(CopyKernel, dim3(blocks), dim3(256), 0, nullptr, (uint4*)d_dst, (const uint4*)d_src, n);
+        hipLaunchKernelGGL((CopyKernel<K, NT>), dim3(blocks), dim3(256), 0, nullptr, (U32x4*)d_dst,
+                           (const U32x4*)d_src, n);
         rc = CheckLaunch("CopyKernel");
     }
     (void)hipEventRecord(e1, nullptr);
@@ -760,8 +785,33 @@ int32_t hwpa_copy_bandwidth(void* d_dst, const void* d_src, size_t bytes, int32_
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    *ms_per_launch = ms / (float)launches;
+    *ms_out = ms / (float)launches;
     return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t hwpa_copy_bandwidth(void* d_dst, const void* d_src, size_t bytes, int32_t launches, float* ms_per_launch)
+{
+    if (!d_dst || !d_src || !ms_per_launch || launches <= 0)
+        return Fail(HWPA_ERR_INVALID_ARG, "copy: null pointer or no launches");
+    if ((bytes % 16) != 0 || ((uintptr_t)d_dst % 16) != 0 || ((uintptr_t)d_src % 16) != 0)
+        return Fail(HWPA_ERR_INVALID_ARG, "copy: pointers and size must be 16-byte multiples");
+    const size_t n = bytes / 16;
+    if (n == 0) return Fail(HWPA_ERR_INVALID_ARG, "copy: empty");
+    float best = 0.0f, ms = 0.0f;
+    int32_t rc;
+    if ((rc = TimeCopy<1, false>(d_dst, d_src, n, launches, &ms)) != HWPA_OK) return rc;
+    best = ms;
+    if ((rc = TimeCopy<4, false>(d_dst, d_src, n, launches, &ms)) != HWPA_OK) return rc;
+    best = std::min(best, ms);
+    if ((rc = TimeCopy<4, true>(d_dst, d_src, n, launches, &ms)) != HWPA_OK) return rc;
+    best = std::min(best, ms);
+    if ((rc = TimeCopy<8, true>(d_dst, d_src, n, launches, &ms)) != HWPA_OK) return rc;
+    best = std::min(best, ms);
+    *ms_per_launch = best;
+    return HWPA_OK;
 }
 
 __global__ __launch_bounds__(256) void CountEqualKernel(const int16_t* __restrict__ a, const int16_t* __restrict__ b,

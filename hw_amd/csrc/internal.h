@@ -70,6 +70,10 @@ size_t EdgeStateBytes();
 int32_t EdgeStateInit(void* edge, uint32_t lanes, hipStream_t s);
 int32_t HistReset(const hwpa_config& cfg, hwpa_lane_hist* hist, const hwpa_baseliner_stats* stats,
                   uint32_t lanes, hipStream_t s);
+// HistReset that leaves the counters for the next add to start from zero (no
+// counters written or read back); that add must precede any read of the counters.
+int32_t HistResetDeferred(const hwpa_config& cfg, hwpa_lane_hist* hist, void* edge, const hwpa_baseliner_stats* stats,
+                          uint32_t lanes, hipStream_t s);
 int32_t HistAdd(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
                 const hwpa_lane_model* models, uint32_t lanes, hipStream_t s);
 
@@ -114,7 +118,10 @@ int32_t MlpForwardFromHist(const hwpa_initializer* m, const hwpa_lane_hist* hist
                            hipStream_t s);
 int32_t MlpForward(const hwpa_initializer* m, const uint16_t* x, uint32_t rows, float* y, uint16_t* h1, uint16_t* h2,
                    hipStream_t s);
-const float* MlpOutputMap(const hwpa_initializer* m);  // device {off[6], scale[6], w[7]}
+const float* MlpOutputMap(const hwpa_initializer* m);  // device {off[6], scale[6], w[8]}
+// Logit of the learned gate's probability threshold (host): the network's confidence
+// output is compared with it; a threshold of 0 lets every ZMW through.
+inline float MlpGateLogit(float p) { return hw_gate_logit(p); }
 int32_t MlpTargets(const hwpa_lane_hist* hist, const float* diag, uint32_t lanes, float minConf, float* t,
                    uint8_t* valid, hipStream_t s);
 

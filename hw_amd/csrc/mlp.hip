@@ -220,7 +220,9 @@ __global__ __launch_bounds__(64) void FeaturesKernel(const hwpa_lane_hist* __res
 // ---- training targets from the EM diagnostics ---------------------------------
 // Raw units: u = 2 * binSize (one re-binned bin), origin at the histogram's
 // lower bound:  r0 = (mu_bl - lo) / u,  r1 = log(var_bl / u^2),
-// r2..5 = (mu_a - lo) / u,  r6..10 = mixture weights (softmax targets).
+// r2..5 = (mu_a - lo) / u,  r6..10 = mixture weights (softmax targets),
+// r11 = 1 when the estimate is a valid target (converged, status OK, enough
+// confidence), else 0 (the confidence output's target).
 __global__ __launch_bounds__(64) void TargetsKernel(const hwpa_lane_hist* __restrict__ hists,
                                                     const float* __restrict__ diag, float minConf,
                                                     float* __restrict__ t, uint8_t* __restrict__ valid)
@@ -236,13 +238,16 @@ __global__ __launch_bounds__(64) void TargetsKernel(const hwpa_lane_hist* __rest
     for (int k = 0; k < 5; ++k) o[6 + k] = dg[k * LS];
     const bool ok = dg[16 * LS] > 0.0f && dg[17 * LS] == 0.0f && dg[15 * LS] >= minConf &&
                     hw_isfinite(o[0]) && hw_isfinite(o[1]) && var0 > 0.0f;
+    o[11] = ok ? 1.0f : 0.0f;
     valid[(size_t)lane * LS + z] = ok ? 1 : 0;
 }
 
 // ---- loss ----------------------------------------------------------------------
 // Network output j < 6 stands for the raw target off[j] + scale[j] * y[j]; the
-// loss is sum_j w[j] (y[j] - (r[j] - off[j]) / scale[j])^2 + w[6] * CE(softmax(y[6..10]), r[6..10]).
-// omap = {off[6], scale[6], w[7]}.
+// loss is sum_j w[j] (y[j] - (r[j] - off[j]) / scale[j])^2 + w[6] * CE(softmax(y[6..10]), r[6..10])
+// + w[7] * BCE(sigmoid(y[11]), r[11]), the last as softplus(y) - r y in its stable form
+// max(y, 0) - r y + log(1 + exp(-|y|)).  Samples marked invalid keep only the last term.
+// omap = {off[6], scale[6], w[8]}.
 __global__ __launch_bounds__(256) void LossKernel(const float* __restrict__ y, const float* __restrict__ t,
                                                   const uint8_t* __restrict__ valid, uint32_t batch,
                                                   const float* __restrict__ omap, float* __restrict__ dy,
@@ -250,19 +255,26 @@ __global__ __launch_bounds__(256) void LossKernel(const float* __restrict__ y, c
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= batch) return;
-    if (valid && valid[b] == 0) {
-        for (int j = 0; j < MO; ++j) {
-            dy[(size_t)b * MO + j] = 0.0f;
-            dyb[(size_t)b * MO + j] = 0;
-        }
-        lossPer[b] = 0.0f;
-        return;
-    }
     const float* yb = y + (size_t)b * MO;
     const float* tb = t + (size_t)b * MO;
     const float invB = 1.0f / (float)batch;
-    float loss = 0.0f;
     float g[MO];
+    // Confidence output: logistic loss against target 11.
+    const float wg = omap[19];
+    const float yc = yb[11], tc = tb[11];
+    const float ez = hw_expf(-hw_fabsf(yc));
+    float loss = wg * ((hw_maxf(yc, 0.0f) - tc * yc) + hw_logf(1.0f + ez));
+    const float sg = (yc >= 0.0f) ? 1.0f / (1.0f + ez) : ez / (1.0f + ez);
+    g[11] = (wg * (sg - tc)) * invB;
+    if (valid && valid[b] == 0) {
+        for (int j = 0; j < MO; ++j) {
+            const float gj = (j == 11) ? g[11] : 0.0f;
+            dy[(size_t)b * MO + j] = gj;
+            dyb[(size_t)b * MO + j] = hw_f2bf(gj);
+        }
+        lossPer[b] = loss;
+        return;
+    }
     for (int j = 0; j < 6; ++j) {
         const float tn = (tb[j] - omap[j]) / omap[6 + j];
         const float d = yb[j] - tn;
@@ -387,8 +399,8 @@ struct hwpa_initializer {
     float* lossPer = nullptr;
     float* colPart = nullptr;
     float* lossDev = nullptr;
-    float* omap = nullptr;      // device {off[6], scale[6], w[7]}
-    float omapHost[19] = {};
+    float* omap = nullptr;      // device {off[6], scale[6], w[8]}
+    float omapHost[20] = {};
     float* kPart = nullptr;     // split-K partial weight gradients, kMaxSplit x MH x MH
 };
 
@@ -470,7 +482,7 @@ static int32_t ColSum(const void* d, bool bf16, uint32_t rows, uint32_t cols, fl
     return CheckLaunch("ColSumFinalKernel");
 }
 
-// Fused forward pass: 32 rows per wave, 4 waves per workgroup.  Each layer is the
+// Fused forward pass: 32 rows per wave, kFusedWaves (8) waves per workgroup.  Each layer is the
 // 32x32x16 bf16 MFMA over 32-column tiles with k ascending -- the arithmetic of
 // GemmKernel, so outputs equal the three-GEMM pass bit for bit -- but the hidden
 // activations stay in LDS (written transposed into row-major tiles, read back as the
@@ -502,6 +514,8 @@ __global__ __launch_bounds__(64 * kFusedWaves) void MlpFusedForwardKernel(const 
     __shared__ __align__(16) uint16_t hs[kFusedWaves][kFusedRows][kHsStride];
     __shared__ __align__(16) uint16_t ws[2][kSlab];
     static_assert(64 * kFusedWaves == 2 * MH, "one 16-byte half column of a slab per thread");
+    // 132 KiB of activations + 16 KiB of weight slabs: one workgroup per CU's 160 KiB of LDS.
+    static_assert(sizeof(hs) + sizeof(ws) <= 160 * 1024, "LDS budget of a CU");
     // Thread t stages half t & 1 of column t >> 1 of a k-step's weights (16 bytes).
     const int sn = threadIdx.x >> 1, sh = (threadIdx.x & 1) * 8;
     auto slabLoad = [&](int off, int kdim, int k0) {
@@ -668,6 +682,9 @@ static int32_t ForwardFromHist(const hwpa_initializer* m, const hwpa_lane_hist* 
                                hipStream_t s)
 {
     if (lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "mlp forward: no lanes");
+    // Whole lanes, so every wave's 32 rows are all valid or all past the end: the
+    // HIST input staging reads a wave's half-lane of counters only when m0 < rows.
+    static_assert(LS % kFusedRows == 0, "a wave's rows lie in one lane");
     const uint32_t rows = lanes * (uint32_t)LS;
     const uint32_t per = (uint32_t)(kFusedRows * kFusedWaves);
     hipLaunchKernelGGL(MlpFusedForwardKernel<true>, dim3((rows + per - 1) / per), dim3(64 * kFusedWaves), 0, s, nullptr,
@@ -786,7 +803,7 @@ int32_t hwpa_initializer_create(uint64_t seed, hwpa_initializer** out)
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->Mo, NPARAM * 4, "mlp moments");
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->Ve, NPARAM * 4, "mlp moments");
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->lossDev, 16, "mlp loss");
-    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->omap, 19 * sizeof(float), "mlp output map");
+    if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->omap, 20 * sizeof(float), "mlp output map");
     if (rc == HWPA_OK) rc = DeviceAlloc((void**)&m->kPart, (size_t)kMaxSplit * MH * MH * 4, "mlp split-K partials");
     if (rc != HWPA_OK) {
         hwpa_initializer_destroy(m);
@@ -799,6 +816,7 @@ int32_t hwpa_initializer_create(uint64_t seed, hwpa_initializer** out)
         m->omapHost[12 + j] = 1.0f;
     }
     m->omapHost[18] = 1.0f;
+    m->omapHost[19] = 1.0f;
     if (hipMemcpy(m->omap, m->omapHost, sizeof(m->omapHost), hipMemcpyHostToDevice) != hipSuccess) {
         hwpa_initializer_destroy(m);
         return Fail(HWPA_ERR_HIP, "output map upload failed");
@@ -865,22 +883,24 @@ int32_t hwpa_initializer_set_output_map(hwpa_initializer* m, const float offset[
         if (!(scale[j] > 0.0f) || !hw_isfinite(offset[j]) || !(weight[j] >= 0.0f))
             return Fail(HWPA_ERR_INVALID_ARG, "output map: scales must be positive, weights non-negative");
     }
-    if (!(weight[6] >= 0.0f)) return Fail(HWPA_ERR_INVALID_ARG, "output map: weights must be non-negative");
+    if (!(weight[6] >= 0.0f) || !(weight[7] >= 0.0f))
+        return Fail(HWPA_ERR_INVALID_ARG, "output map: weights must be non-negative");
     for (int j = 0; j < 6; ++j) {
         m->omapHost[j] = offset[j];
         m->omapHost[6 + j] = scale[j];
         m->omapHost[12 + j] = weight[j];
     }
     m->omapHost[18] = weight[6];
+    m->omapHost[19] = weight[7];
     HWPA_HIP(hipDeviceSynchronize());
     HWPA_HIP(hipMemcpy(m->omap, m->omapHost, sizeof(m->omapHost), hipMemcpyHostToDevice));
     return HWPA_OK;
 }
 
-int32_t hwpa_initializer_get_output_map(const hwpa_initializer* m, float out[19])
+int32_t hwpa_initializer_get_output_map(const hwpa_initializer* m, float out[20])
 {
     if (!m || !out) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
-    for (int j = 0; j < 19; ++j) out[j] = m->omapHost[j];
+    for (int j = 0; j < 20; ++j) out[j] = m->omapHost[j];
     return HWPA_OK;
 }
 

@@ -53,14 +53,18 @@ __global__ __launch_bounds__(256) void RepackKernel(const PacketDev* __restrict_
 
 using namespace hwpa;
 
-// Descriptor slots used in rotation, so a call never rewrites the descriptors (host staging
-// or device copy) of one of the previous kRepackSlots - 1 calls still in flight.
+// Descriptor slots used in rotation.  Each slot records an event after its kernel; a
+// call that reuses the slot makes its stream wait for that event before the new
+// descriptors are copied in, so calls on different streams never overwrite
+// descriptors a kernel still reads.
 constexpr int kRepackSlots = 4;
 
 struct hwpa_repacker {
     uint32_t maxPackets = 0;
     PacketDev* d = nullptr;  // kRepackSlots x maxPackets
     std::vector<PacketDev> h;
+    hipEvent_t done[kRepackSlots] = {};
+    bool used[kRepackSlots] = {};
     int slot = 0;
 };
 
@@ -75,10 +79,12 @@ int32_t hwpa_repacker_create(uint32_t max_packets, hwpa_repacker** out)
     if (!r) return Fail(HWPA_ERR_OOM, "out of host memory");
     r->maxPackets = max_packets;
     r->h.resize((size_t)kRepackSlots * max_packets);
-    const int32_t rc =
-        DeviceAlloc((void**)&r->d, sizeof(PacketDev) * kRepackSlots * max_packets, "packet descriptors");
+    int32_t rc = DeviceAlloc((void**)&r->d, sizeof(PacketDev) * kRepackSlots * max_packets, "packet descriptors");
+    for (int i = 0; i < kRepackSlots && rc == HWPA_OK; ++i)
+        if (hipEventCreateWithFlags(&r->done[i], hipEventDisableTiming) != hipSuccess)
+            rc = Fail(HWPA_ERR_HIP, "repacker_create: hipEventCreate failed");
     if (rc != HWPA_OK) {
-        delete r;
+        hwpa_repacker_destroy(r);
         return rc;
     }
     *out = r;
@@ -88,6 +94,8 @@ int32_t hwpa_repacker_create(uint32_t max_packets, hwpa_repacker** out)
 int32_t hwpa_repacker_destroy(hwpa_repacker* r)
 {
     if (!r) return HWPA_OK;
+    for (int i = 0; i < kRepackSlots; ++i)
+        if (r->done[i]) (void)hipEventDestroy(r->done[i]);
     DeviceFree(r->d);
     delete r;
     return HWPA_OK;
@@ -122,11 +130,17 @@ int32_t hwpa_repack(hwpa_repacker* r, const hwpa_packet_desc* packets, uint32_t 
     const uint64_t gx = (maxPieces + 255) / 256;
     if (gx > 0x7fffffffull) return Fail(HWPA_ERR_INVALID_ARG, "repack: packet too large");
     hipStream_t s = AsStream(stream);
+    const int slot = r->slot;
+    if (r->used[slot]) HWPA_HIP(hipStreamWaitEvent(s, r->done[slot], 0));
     HWPA_HIP(hipMemcpyAsync(ds, hs, sizeof(PacketDev) * n, hipMemcpyHostToDevice, s));
-    r->slot = (r->slot + 1) % kRepackSlots;
+    r->slot = (slot + 1) % kRepackSlots;
     hipLaunchKernelGGL(RepackKernel, dim3((unsigned)gx, n), dim3(256), 0, s, ds, eb, static_cast<uint8_t*>(d_batch),
                        batch_first_zmw, batch_first_frame, batch_lanes, batch_frames);
-    return CheckLaunch("RepackKernel");
+    const int32_t rc = CheckLaunch("RepackKernel");
+    if (rc != HWPA_OK) return rc;
+    HWPA_HIP(hipEventRecord(r->done[slot], s));
+    r->used[slot] = true;
+    return HWPA_OK;
 }
 
 }  // extern "C"

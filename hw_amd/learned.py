@@ -44,9 +44,10 @@ class PairSet:
 
 
 def generate_pairs(lanes, chunks, seed, snr=(4.0, 20.0), frames=512, cfg=None, pairs=None, min_conf=0.1,
-                   gen_overrides=None, log=None):
+                   gen_overrides=None, log=None, limit=None):
     """Runs the plain pipeline on `chunks` chunks of seeded synthetic traces with
-    per-ZMW SNR ~ U[snr] and collects one pair per ZMW and estimate."""
+    per-ZMW SNR ~ U[snr] and collects one pair per ZMW and estimate (at most
+    `limit` pairs from this call)."""
     cfg = cfg if cfg is not None else bench_config(frames)
     kw = dict(seed=seed, snr_lo=float(snr[0]), snr_hi=float(snr[1]))
     kw.update(gen_overrides or {})
@@ -60,12 +61,15 @@ def generate_pairs(lanes, chunks, seed, snr=(4.0, 20.0), frames=512, cfg=None, p
     d_state = dev_bytes(lanes * T.TRACEGEN_STATE.itemsize)
     d_tr = dev_bytes(lanes * frames * T.LANE * 2)
     t0 = time.perf_counter()
+    start = pairs.n
     for c in range(chunks):
+        if limit is not None and pairs.n - start >= limit:
+            break
         tracegen(gen, lanes, frames, c * frames, d_state, c == 0, d_tr)
         an(d_tr, c * frames)
         if an.flags()[1]:
             an.sync()
-            pairs.append(cx, ct, cv, Z)
+            pairs.append(cx, ct, cv, Z if limit is None else min(Z, limit - (pairs.n - start)))
     sync()
     if log:
         log("generated %d pairs (%d lanes x %d chunks) in %.1f s" % (pairs.n, lanes, chunks, time.perf_counter() - t0))
@@ -91,13 +95,14 @@ def fit_output_map(init, pairs):
     tv = t[v][:, :6].astype(np.float64)
     off = tv.mean(axis=0)
     sc = np.maximum(tv.std(axis=0), 1e-3)
-    w = [LL_WEIGHTS[j] * sc[j] ** 2 / LL_FRAMES for j in range(6)] + [1.0]
+    w = [LL_WEIGHTS[j] * sc[j] ** 2 / LL_FRAMES for j in range(6)] + [1.0, 1.0]
     init.set_output_map(off, sc, w)
     return off, sc, w
 
 
-def train(init, pairs, epochs=4, batch=4096, lr=2e-3, lr_end=1e-4, log=None, fit_map=True):
-    """Adam over the pairs in order, cosine-decayed learning rate; returns the
+def train(init, pairs, epochs=4, batch=4096, lr=2e-3, lr_end=1e-4, log=None, fit_map=True, seed=5):
+    """Adam over the pairs in batches of consecutive pairs, the batches of each
+    epoch in a seeded random order, cosine-decayed learning rate; returns the
     loss of every 50th step (and the last)."""
     n_batches = pairs.n // batch
     if n_batches == 0:
@@ -107,9 +112,11 @@ def train(init, pairs, epochs=4, batch=4096, lr=2e-3, lr_end=1e-4, log=None, fit
     steps = epochs * n_batches
     d_loss = dev_bytes(4)
     losses = []
+    rng = np.random.default_rng(seed)
+    order = np.concatenate([rng.permutation(n_batches) for _ in range(epochs)])
     t0 = time.perf_counter()
     for s in range(steps):
-        b = s % n_batches
+        b = int(order[s])
         f = 0.5 * (1 + math.cos(math.pi * s / max(steps - 1, 1)))
         rate = lr_end + (lr - lr_end) * f
         init.train_step(pairs.x.view(b * batch * MLP_IN * 2, batch * MLP_IN * 2),

@@ -201,8 +201,8 @@ int32_t hwpa_baz_encode(hwpa_baz_encoder* e, const hwpa_pulse* d_pulses, const u
  * starting at (batch_first_zmw, batch_first_frame) into it, samples of HWPA_ENCODING_INT16
  * or HWPA_ENCODING_UINT8 kept as they are.  Packets need 16-byte aligned data and block
  * widths / start ZMWs of whole 16-byte pieces; the batch starts on a lane.  One launch per
- * call; calls on one repacker are ordered on one stream, and descriptor slots rotate over 4
- * calls, so at most 3 earlier calls may still be in flight. */
+ * call.  Descriptor slots rotate over 4 calls and each slot's reuse waits (on the new call's
+ * stream) for the kernel that last read it, so calls may go to different streams. */
 typedef struct hwpa_repacker hwpa_repacker;
 int32_t hwpa_repacker_create(uint32_t max_packets, hwpa_repacker** out);
 int32_t hwpa_repacker_destroy(hwpa_repacker* r);
@@ -283,12 +283,15 @@ int32_t hwpa_pulser_run(hwpa_pulser* p, const int16_t* d_labels, const int16_t* 
                         uint32_t* d_overflows, hwpa_stat_acc* d_bl_stats, void* stream);
 
 /* ---- learned initialiser of the EM (no reference counterpart) -----------
- * A 128 -> 256 -> 256 -> 11 perceptron with ReLU hidden layers maps each
+ * A 128 -> 256 -> 256 -> 12 perceptron with ReLU hidden layers maps each
  * ZMW's histogram (re-binned to 128 bins: bin i = bins 2i and 2i+1 of the
  * 300, scaled by 16 / their total) to a start model for the EM: baseline
  * mean, log variance and four analog means in units of u = 2 binSize from the
- * histogram's lower bound (through the output map below), and five
- * mixture-weight logits.  Matrices are bfloat16 for the matrix cores with fp32 accumulation;
+ * histogram's lower bound (through the output map below), five
+ * mixture-weight logits, and a confidence logit: the network's estimate that
+ * the ZMW's EM converges with status OK (trained on that outcome of the plain
+ * EM, a logistic loss).  ZMWs whose confidence is below
+ * cfg->learned_gate_threshold skip the capped EM and go straight to the plain EM.  Matrices are bfloat16 for the matrix cores with fp32 accumulation;
  * an fp32 master copy and Adam moments live on the device.  The start replaces
  * the EM's (DmeEmHost.cpp:236-249 onwards); the EM then runs at most
  * cfg->learned_em_iteration_limit iterations with the convergence test
@@ -297,7 +300,7 @@ int32_t hwpa_pulser_run(hwpa_pulser* p, const int16_t* d_labels, const int16_t* 
  * cfg->learned_conf_threshold, default 0) are re-estimated by the plain EM. */
 #define HWPA_MLP_IN 128
 #define HWPA_MLP_HIDDEN 256
-#define HWPA_MLP_OUT 11
+#define HWPA_MLP_OUT 12
 /* W1[256][128] b1[256] W2[256][256] b2[256] W3[11][256] b3[11] */
 #define HWPA_MLP_PARAMS (HWPA_MLP_HIDDEN * HWPA_MLP_IN + HWPA_MLP_HIDDEN + HWPA_MLP_HIDDEN * HWPA_MLP_HIDDEN + \
                          HWPA_MLP_HIDDEN + HWPA_MLP_OUT * HWPA_MLP_HIDDEN + HWPA_MLP_OUT)
@@ -312,18 +315,19 @@ int32_t hwpa_initializer_get_params(const hwpa_initializer* m, float* host_param
 int32_t hwpa_initializer_set_params(hwpa_initializer* m, const float* host_params, size_t n);
 /* Output map: network output j < 6 stands for offset[j] + scale[j] * y[j] in
  * the raw target units of hwpa_mlp_targets; weight[0..5] weigh their squared
- * errors in the loss, weight[6] the cross-entropy of the mixture weights.
+ * errors in the loss, weight[6] the cross-entropy of the mixture weights,
+ * weight[7] the logistic loss of the confidence output.
  * Default: offset 0, scale 1, weights 1.  get returns {offset, scale, weight}. */
 int32_t hwpa_initializer_set_output_map(hwpa_initializer* m, const float offset[6], const float scale[6],
-                                        const float weight[7]);
-int32_t hwpa_initializer_get_output_map(const hwpa_initializer* m, float out[19]);
+                                        const float weight[8]);
+int32_t hwpa_initializer_get_output_map(const hwpa_initializer* m, float out[20]);
 /* Network input from lane histograms: bf16 [lanes*64][128] (16-byte aligned). */
 int32_t hwpa_mlp_features(const hwpa_lane_hist* d_hist, uint32_t lanes, uint16_t* d_x, void* stream);
-/* Training targets from hwpa_dme_estimate_diag output: fp32 [lanes*64][11] in
+/* Training targets from hwpa_dme_estimate_diag output: fp32 [lanes*64][12] in
  * raw units, u = 2 binSize from the lower bound lo: (mu_bl - lo)/u,
- * log(var_bl/u^2), (mu_a - lo)/u for the four analogs, then the five mixture
- * weights; d_valid u8 [lanes*64] = converged, status OK, confidence >=
- * min_confidence. */
+ * log(var_bl/u^2), (mu_a - lo)/u for the four analogs, the five mixture
+ * weights, and 1 or 0 for the confidence output; d_valid u8 [lanes*64] = that
+ * last target: converged, status OK, confidence >= min_confidence. */
 int32_t hwpa_mlp_targets(const hwpa_lane_hist* d_hist, const float* d_diag, uint32_t lanes,
                          float min_confidence, float* d_t, uint8_t* d_valid, void* stream);
 /* Network outputs fp32 [rows][11]. */
@@ -333,9 +337,10 @@ int32_t hwpa_mlp_forward(hwpa_initializer* m, const uint16_t* d_x, uint32_t rows
 int32_t hwpa_mlp_forward_hist(hwpa_initializer* m, const hwpa_lane_hist* d_hist, uint32_t lanes, float* d_y,
                               void* stream);
 /* Mean loss over a batch (squared error on outputs 0-5, softmax cross-entropy
- * on 6-10) and its gradient, fp32 [HWPA_MLP_PARAMS] in parameter order.
- * Samples with d_valid[i] == 0 (d_valid may be NULL) contribute nothing; the
- * mean is still taken over the whole batch. */
+ * on 6-10, logistic loss of the confidence output 11 against target 11) and
+ * its gradient, fp32 [HWPA_MLP_PARAMS] in parameter order.  Samples with
+ * d_valid[i] == 0 (d_valid may be NULL) contribute only the confidence term;
+ * the mean is taken over the whole batch. */
 int32_t hwpa_mlp_gradients(hwpa_initializer* m, const uint16_t* d_x, const float* d_t, const uint8_t* d_valid,
                            uint32_t batch, float* d_grads, float* d_loss, void* stream);
 /* One Adam step (beta 0.9/0.999, eps 1e-8) on a batch; d_loss (optional)
@@ -371,10 +376,11 @@ int32_t hwpa_dev_free(void* d_ptr);
 int32_t hwpa_memcpy_htod(void* d_dst, const void* host_src, size_t bytes);
 int32_t hwpa_memcpy_dtoh(void* host_dst, const void* d_src, size_t bytes);
 int32_t hwpa_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
-/* Runs a plain 16-byte-per-thread copy kernel `launches` times on the null
- * stream and returns the mean duration per launch (HIP events).  No reference
- * counterpart: the achievable-bandwidth yardstick of the roofline figures.
- * bytes and both pointers must be multiples of 16. */
+/* Times streaming copy kernels (1, 4 and 8 16-byte elements per thread, plain
+ * and non-temporal), each `launches` times on the null stream after one
+ * untimed launch, and returns the fastest form's mean duration per launch (HIP
+ * events).  No reference counterpart: the achievable-bandwidth yardstick of the
+ * roofline figures.  bytes and both pointers must be multiples of 16. */
 int32_t hwpa_copy_bandwidth(void* d_dst, const void* d_src, size_t bytes, int32_t launches,
                             float* ms_per_launch);
 

@@ -350,6 +350,13 @@ HWPA_HD float hw_dexp2f(float x) { return (float)exp2((double)x); }
 HWPA_HD float hw_dlog2f(float x) { return (float)log2((double)x); }
 HWPA_HD float hw_dpowf(float x, float y) { return (float)pow((double)x, (double)y); }
 
+/* Logit of a probability threshold p in [0, 1), evaluated on the host in double
+ * (-inf for p = 0, so that every finite network output passes it). */
+static inline float hw_gate_logit(float p)
+{
+    return p > 0.0f ? (float)log((double)p / (1.0 - (double)p)) : -__builtin_inff();
+}
+
 /* x^y for x > 0; exact for y == 0 and y == 1. */
 HWPA_HD float hw_powf(float x, float y)
 {

@@ -276,6 +276,7 @@ typedef struct hwpa_config {
     float learned_conf_threshold;
     uint32_t learned_em_iteration_limit;
     float learned_conv_tol;      /* relative log-likelihood tolerance of the capped EM */
+    float learned_gate_threshold; /* network confidence below which the capped EM is skipped (0: never) */
     hwpa_movie_info movie;
 } hwpa_config;
 

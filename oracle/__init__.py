@@ -162,6 +162,21 @@ def dme_estimate(cfg, hist, stats, models, model_fi, est_fi, want_diag=False, fo
     return (models, diag) if want_diag else models
 
 
+def dme_estimate_learned(cfg, hist, stats, models, model_fi, est_fi, y, omap):
+    """Learned-start estimation (mirror arithmetic): y float32 [lanes*64][12] network
+    outputs, omap the 12 floats {offset[6], scale[6]}.  Returns (models, fallback
+    flags [lanes*64], fallback count)."""
+    models = np.ascontiguousarray(models).copy()
+    lanes = len(models)
+    y = np.ascontiguousarray(y, dtype=np.float32).reshape(lanes * LANE, -1)
+    omap = np.ascontiguousarray(omap, dtype=np.float32)[:12]
+    fb = np.zeros(lanes * LANE, dtype=np.uint8)
+    n = lib.orc_dme_estimate_learned(ctypes.byref(cfg), ctypes.c_int32(lanes), _p(np.ascontiguousarray(hist)),
+                                     _p(np.ascontiguousarray(stats)), _p(models), (ctypes.c_int32 * 2)(*model_fi),
+                                     (ctypes.c_int32 * 2)(*est_fi), _p(y), _p(omap), _p(fb))
+    return models, fb, int(n)
+
+
 class Labeler:
     def __init__(self, cfg, lanes):
         lib.orc_labeler_create.restype = _vp

@@ -1,6 +1,6 @@
 """CPU FP32 oracle of the learned EM initialiser -- TEST INFRASTRUCTURE ONLY.
 
-numpy restatement of the 128 -> 256 -> 256 -> 11 perceptron of
+numpy restatement of the 128 -> 256 -> 256 -> 12 perceptron of
 hw_amd/csrc/mlp.hip (no reference counterpart): the histogram features, the
 forward pass, the loss and the backward pass, with the bfloat16 rounding
 points of the GPU path (operands of every matrix product rounded to bf16,
@@ -10,7 +10,7 @@ CPU tests pin against central finite differences.  Only tests/ may use it.
 """
 import numpy as np
 
-IN, HID, OUT = 128, 256, 11
+IN, HID, OUT = 128, 256, 12
 SIZES = [("W1", (HID, IN)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)), ("W3", (OUT, HID)), ("b3", (OUT,))]
 NPARAM = sum(int(np.prod(s)) for _, s in SIZES)
 
@@ -72,29 +72,40 @@ def forward(p, x, round_bf16=True, dtype=np.float32):
     return {"x": x, "W1": W1, "W2": W2, "W3": W3, "h1": h1, "h2": h2, "y": y}
 
 
-IDENTITY_MAP = np.concatenate([np.zeros(6), np.ones(6), np.ones(7)]).astype(np.float32)
+IDENTITY_MAP = np.concatenate([np.zeros(6), np.ones(6), np.ones(8)]).astype(np.float32)
 
 
-def loss_terms(y, t, omap=IDENTITY_MAP):
+def gate_terms(y, t, omap=IDENTITY_MAP):
+    """Logistic loss of the confidence output 11 against target 11, weight omap[19]."""
+    yc, tc = y[:, 11], t[:, 11]
+    return omap[19] * (np.maximum(yc, 0) - tc * yc + np.log1p(np.exp(-np.abs(yc))))
+
+
+def loss_terms(y, t, omap=IDENTITY_MAP, valid=None):
     """Per-sample loss: weighted squared error of outputs 0-5 against the
-    targets mapped through (t - offset) / scale, plus the weighted softmax
-    cross-entropy of outputs 6-10.  omap = {offset[6], scale[6], weight[7]}."""
+    targets mapped through (t - offset) / scale, the weighted softmax
+    cross-entropy of outputs 6-10, and the confidence output's logistic loss;
+    a sample with valid == 0 keeps only the last.  omap = {offset[6], scale[6],
+    weight[8]}."""
     off, sc, w = omap[:6], omap[6:12], omap[12:]
     d = y[:, :6] - (t[:, :6] - off) / sc
-    logits = y[:, 6:]
+    logits = y[:, 6:11]
     mx = logits.max(axis=1, keepdims=True)
     lse = mx + np.log(np.exp(logits - mx).sum(axis=1, keepdims=True))
-    return (w[:6] * d * d).sum(axis=1) - w[6] * (t[:, 6:] * (logits - lse)).sum(axis=1)
+    main = (w[:6] * d * d).sum(axis=1) - w[6] * (t[:, 6:11] * (logits - lse)).sum(axis=1)
+    v = np.ones(y.shape[0], dtype=y.dtype) if valid is None else np.asarray(valid, dtype=y.dtype)
+    return main * v + gate_terms(y, t, omap)
 
 
-def loss(p, x, t, round_bf16=True, dtype=np.float32, omap=IDENTITY_MAP):
+def loss(p, x, t, round_bf16=True, dtype=np.float32, omap=IDENTITY_MAP, valid=None):
     f = forward(p, x, round_bf16, dtype)
-    return loss_terms(f["y"], np.asarray(t, dtype=dtype), np.asarray(omap, dtype=dtype)).mean()
+    return loss_terms(f["y"], np.asarray(t, dtype=dtype), np.asarray(omap, dtype=dtype), valid).mean()
 
 
 def gradients(p, x, t, round_bf16=True, dtype=np.float32, omap=IDENTITY_MAP, valid=None):
     """(mean loss, flat gradient) following the GPU's rounding points; samples
-    with valid == 0 contribute nothing (the mean stays over the batch)."""
+    with valid == 0 contribute only the confidence term (the mean stays over
+    the batch)."""
     t = np.asarray(t, dtype=dtype)
     omap = np.asarray(omap, dtype=dtype)
     off, sc, w = omap[:6], omap[6:12], omap[12:]
@@ -102,13 +113,14 @@ def gradients(p, x, t, round_bf16=True, dtype=np.float32, omap=IDENTITY_MAP, val
     y = f["y"]
     B = y.shape[0]
     v = np.ones(B, dtype=dtype) if valid is None else np.asarray(valid, dtype=dtype)
-    L = (loss_terms(y, t, omap) * v).mean()
+    L = loss_terms(y, t, omap, v).mean()
     dy = np.empty_like(y)
     dy[:, :6] = 2 * w[:6] * (y[:, :6] - (t[:, :6] - off) / sc) / B
-    logits = y[:, 6:]
+    logits = y[:, 6:11]
     e = np.exp(logits - logits.max(axis=1, keepdims=True))
-    dy[:, 6:] = w[6] * (e / e.sum(axis=1, keepdims=True) - t[:, 6:]) / B
-    dy *= v[:, None]
+    dy[:, 6:11] = w[6] * (e / e.sum(axis=1, keepdims=True) - t[:, 6:11]) / B
+    dy[:, :11] *= v[:, None]
+    dy[:, 11] = w[7] * (1.0 / (1.0 + np.exp(-y[:, 11])) - t[:, 11]) / B
     dyb = _r(dy, round_bf16)
     g = {"W3": dyb.T @ f["h2"], "b3": dy.sum(axis=0)}
     dh2 = _r((dyb @ f["W3"]) * (f["h2"] > 0), round_bf16)
@@ -120,21 +132,27 @@ def gradients(p, x, t, round_bf16=True, dtype=np.float32, omap=IDENTITY_MAP, val
     return L, pack(g)
 
 
-def targets(low_bound, bin_size, diag):
+def targets(low_bound, bin_size, diag, min_conf=0.0):
     """Training targets from the EM's raw estimate, in raw units u = 2 binSize
-    from the lower bound.  diag: [18, Z] rows (rho[5], mu[5], var[5], conf,
-    converged, status); returns float32 [Z, 11]."""
+    from the lower bound, and the validity flag as the confidence target (the
+    estimate converged with status OK, confidence >= min_conf, finite targets).
+    diag: [18, Z] rows (rho[5], mu[5], var[5], conf, converged, status);
+    returns (float32 [Z, 12], bool [Z])."""
     lo = np.asarray(low_bound, dtype=np.float32)
     u = np.float32(2.0) * np.asarray(bin_size, dtype=np.float32)
     d = np.asarray(diag, dtype=np.float32)
     t = np.empty((lo.shape[0], OUT), dtype=np.float32)
-    t[:, 0] = (d[5] - lo) / u
-    t[:, 1] = np.log(d[10] / (u * u))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t[:, 0] = (d[5] - lo) / u
+        t[:, 1] = np.log(d[10] / (u * u))
     for a in range(4):
         t[:, 2 + a] = (d[6 + a] - lo) / u
     for k in range(5):
         t[:, 6 + k] = d[k]
-    return t
+    ok = (d[16] > 0) & (d[17] == 0) & (d[15] >= np.float32(min_conf)) & np.isfinite(t[:, 0]) & \
+        np.isfinite(t[:, 1]) & (d[10] > 0)
+    t[:, 11] = ok.astype(np.float32)
+    return t, ok
 
 
 def decode(y, low_bound, bin_size, omap=IDENTITY_MAP):
@@ -144,7 +162,7 @@ def decode(y, low_bound, bin_size, omap=IDENTITY_MAP):
     r = omap[:6] + omap[6:12] * y[:, :6]
     u = 2.0 * np.asarray(bin_size, dtype=np.float64)
     lo = np.asarray(low_bound, dtype=np.float64)
-    e = np.exp(y[:, 6:] - y[:, 6:].max(axis=1, keepdims=True))
+    e = np.exp(y[:, 6:11] - y[:, 6:11].max(axis=1, keepdims=True))
     rho = e / e.sum(axis=1, keepdims=True)
     mu = np.empty((y.shape[0], 5))
     mu[:, 0] = lo + u * r[:, 0]

@@ -13,6 +13,7 @@
 // Parity pins for each part are listed in the test that covers it.
 
 #include "oracle.h"
+#include "../include/hwpa.h"
 #include "../include/hwpa_math.h"
 #include "../include/hwpa_tracegen.h"
 
@@ -948,11 +949,11 @@ void MakeHistZ(const hwpa_lane_hist& lh, int z, HistZ& h)
 // baseline second moment from moments about the iteration's starting mean.
 // Returns the number of iterations run.
 unsigned EmIterationsMirror(const DmeCfg& d, std::vector<EmZ>& ez, const std::vector<HistZ>& hz, const float* rpa,
-                            float rpaMin)
+                            float rpaMin, unsigned iterLimit, float convTolCoeff, const bool* gated = nullptr)
 {
     unsigned it = 0;
     std::vector<float> sci((size_t)LS * 5), smom1((size_t)LS * 5), svarinv((size_t)LS * 5);
-    for (; it < d.iterLimit; ++it) {
+    for (; it < iterLimit; ++it) {
         // E-step for every ZMW of the lane (DmeEmHost.cpp:374-483)
         bool allDone = true;
         for (int z = 0; z < LS; ++z) {
@@ -1055,7 +1056,7 @@ unsigned EmIterationsMirror(const DmeCfg& d, std::vector<EmZ>& ez, const std::ve
             const float ds = e.s - e.sExpect;
             e.logLike -= 0.5f * (ds * ds) * e.sExpectWeight;
             const float delta = e.logLike - e.logLikePrev;
-            const float convTol = 1e-4f * hw_fabsf(e.logLike);
+            const float convTol = convTolCoeff * hw_fabsf(e.logLike);
             bool conv = (delta >= 0.0f) && (delta < convTol);
             conv = conv && (e.logLike >= e.mlLogLike);
             // MaxLikelihoodDiagnostics::Converged (MaxLikelihoodDiagnostics.h:90-101)
@@ -1071,7 +1072,7 @@ unsigned EmIterationsMirror(const DmeCfg& d, std::vector<EmZ>& ez, const std::ve
                 }
             }
             e.converged = e.converged || conv;
-            if (!(e.converged || e.status != HWPA_ZS_OK)) allDone = false;
+            if (!(e.converged || e.status != HWPA_ZS_OK || (gated && gated[z]))) allDone = false;
             for (int m = 0; m < 5; ++m) {
                 sci[(size_t)z * 5 + m] = ci[m];
                 smom1[(size_t)z * 5 + m] = mom1[m];
@@ -1276,10 +1277,23 @@ unsigned EmIterationsRef(const DmeCfg& d, std::vector<EmZ>& ez, const std::vecto
     return it;
 }
 
+// The learned initialiser's start for one lane: network outputs y [64][11] through
+// the output map, the capped EM's limits and the acceptance threshold; `fallback`
+// [64] receives 1 for a ZMW whose capped estimate was rejected (its model is then
+// left as it was, for the plain EM).
+struct LearnedStart {
+    const float* y;
+    const float* omap;
+    unsigned iterLimit;
+    float convTol, acceptConf, gateLogit;
+    uint8_t* fallback;
+};
+
 // DmeEmHost::EstimateLaneDetModel (DmeEmHost.cpp:216-615) for one lane.
 template <class M>
 void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseliner_stats& bs,
-                  hwpa_lane_model& lm, const int32_t* modelFi, const int32_t* estFi, orc_dme_diag* dx)
+                  hwpa_lane_model& lm, const int32_t* modelFi, const int32_t* estFi, orc_dme_diag* dx,
+                  const LearnedStart* ls = nullptr)
 {
     static thread_local std::vector<EmZ> ez;
     ez.resize(LS);
@@ -1369,10 +1383,55 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
         float njs = 0.0f;
         for (int b = 0; b < NB; ++b) njs += (float)h.cnt[b];
         e.njSum = njs;
+        if (ls) {
+            // The network's start (the device's EM_LEARNED start, hw_amd/csrc/dme.hip): output
+            // j < 6 is off[j] + scale[j] y[j] in units of u = 2 binSize from the lower bound
+            // (means) and log(var / u^2); outputs 6..10 are softmax mixture logits.
+            const float* y = ls->y + (size_t)z * HWPA_MLP_OUT;
+            const float u = 2.0f * h.binSize;
+            float r[6];
+            for (int j = 0; j < 6; ++j) r[j] = ls->omap[j] + ls->omap[6 + j] * y[j];
+            float mx = y[6];
+            for (int k = 1; k < 5; ++k) mx = hw_maxf(mx, y[6 + k]);
+            float ex[5], se = 0.0f;
+            for (int k = 0; k < 5; ++k) {
+                ex[k] = hw_expf(y[6 + k] - mx);
+                se += ex[k];
+            }
+            for (int k = 0; k < 5; ++k) e.rho[k] = hw_maxf(ex[k] / se, rhoMin);
+            const float rs2 = Sum5Tree(e.rho);
+            for (int k = 0; k < 5; ++k) e.rho[k] = e.rho[k] / rs2;
+            e.mu[0] = lh.low_bound[z] + u * r[0];
+            e.var[0] = hw_clampf((u * u) * hw_expf(r[1]), d.k.blVarMin, d.k.blVarMax);
+            for (int a = 0; a < NA; ++a) {
+                e.mu[a + 1] = lh.low_bound[z] + u * r[2 + a];
+                e.var[a + 1] = ModelSignalCovar(d.k, d.cv[a] * d.cv[a], e.mu[a + 1], e.var[0]);
+            }
+            for (int k = 0; k < 5; ++k) {
+                e.rhoEst[k] = e.rho[k];
+                e.muEst[k] = e.mu[k];
+                e.varEst[k] = e.var[k];
+            }
+            e.s = ((e.mu[1] * rpa[0] + e.mu[2] * rpa[1]) + (e.mu[3] * rpa[2] + e.mu[4] * rpa[3])) / rpaSqSum;
+        }
     }
 
+    // The learned start's confidence gate (the device's EM_LEARNED): a ZMW whose confidence
+    // output is below the threshold takes no capped iteration of its own and is rejected; a
+    // lane with every ZMW below it skips the capped EM.
+    bool gated[LS] = {};
+    unsigned iterLimit = d.iterLimit;
+    if (ls) {
+        bool all = true;
+        for (int z = 0; z < LS; ++z) {
+            gated[z] = !(ls->y[(size_t)z * HWPA_MLP_OUT + 11] >= ls->gateLogit);
+            all = all && gated[z];
+        }
+        iterLimit = all ? 0u : ls->iterLimit;
+    }
     const unsigned it = M::kReference ? EmIterationsRef<M>(d, ez, hz, rpa, rpaMin)
-                                      : EmIterationsMirror(d, ez, hz, rpa, rpaMin);
+                                      : EmIterationsMirror(d, ez, hz, rpa, rpaMin, iterLimit, ls ? ls->convTol : 1e-4f,
+                                                           ls ? gated : nullptr);
     for (int z = 0; z < LS; ++z) {
         EmZ& e = ez[z];
         if (!e.converged) e.status |= HWPA_ZS_NO_CONVERGE;
@@ -1406,6 +1465,13 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
                 dx->est_fp32[m][1][z] = modes[m]->mean;
                 dx->est_fp32[m][2][z] = modes[m]->var;
             }
+        }
+        if (ls) {
+            // Accept converged, status-OK estimates with enough confidence; the model of a
+            // rejected ZMW stays as it was for the plain EM.
+            const bool accept = !gated[z] && e.converged && e.status == HWPA_ZS_OK && conf >= ls->acceptConf;
+            ls->fallback[z] = accept ? 0 : 1;
+            if (!accept) continue;
         }
         UpdateConf<M>(d, e.det, e.work);
         if (dx) {
@@ -1599,6 +1665,79 @@ void orc_dme_init_models(const hwpa_config* cfg, int32_t lanes, const hwpa_basel
 {
     const DmeCfg d = MakeDmeCfg(*cfg);
     for (int l = 0; l < lanes; ++l) InitLaneModel(d, stats[l], out[l]);
+}
+
+// Learned-start estimation as the device runs it (hwpa_dme_estimate_learned): the
+// capped EM from the network outputs y [lanes * 64][11] with the output map omap
+// ({offset[6], scale[6]}), then the rejected ZMWs packed in lane order into compact
+// lanes (the last one filled with copies of its first ZMW), re-estimated by the
+// plain EM and written back.  Mirror arithmetic only (the path has no reference
+// counterpart).  fallback [lanes * 64] (optional) receives the rejected ZMWs;
+// returns their number.
+int32_t orc_dme_estimate_learned(const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,
+                                 const hwpa_baseliner_stats* stats, hwpa_lane_model* models, const int32_t model_fi[2],
+                                 const int32_t est_fi[2], const float* y, const float* omap, uint8_t* fallback)
+{
+    const DmeCfg d = MakeDmeCfg(*cfg);
+    std::vector<uint8_t> fb((size_t)lanes * LS, 0);
+    ParLanes(lanes, [&](int l) {
+        LearnedStart ls{y + (size_t)l * LS * HWPA_MLP_OUT, omap, cfg->learned_em_iteration_limit, cfg->learned_conv_tol,
+                        cfg->learned_conf_threshold, hw_gate_logit(cfg->learned_gate_threshold),
+                        &fb[(size_t)l * LS]};
+        EstimateLane<MirrorForm>(d, hist[l], stats[l], models[l], model_fi, est_fi, nullptr, &ls);
+    });
+    std::vector<uint32_t> map;
+    for (int l = 0; l < lanes; ++l)
+        for (int z = 0; z < LS; ++z)
+            if (fb[(size_t)l * LS + z]) map.push_back((uint32_t)(l * LS + z));
+    const uint32_t total = (uint32_t)map.size();
+    if (fallback) std::memcpy(fallback, fb.data(), fb.size());
+    if (total == 0) return 0;
+    const int cl = (int)((total + LS - 1) / LS);
+    std::vector<hwpa_lane_hist> ch(cl);
+    std::vector<hwpa_baseliner_stats> cs(cl);
+    std::vector<hwpa_lane_model> cm(cl);
+    std::memset(ch.data(), 0, sizeof(hwpa_lane_hist) * cl);
+    std::memset(cs.data(), 0, sizeof(hwpa_baseliner_stats) * cl);
+    std::memset(cm.data(), 0, sizeof(hwpa_lane_model) * cl);
+    auto copyModel = [](const hwpa_lane_model& m, int sz, hwpa_lane_model& dm, int dz) {
+        for (int a = 0; a < NA; ++a) {
+            dm.analogs[a].means[dz] = m.analogs[a].means[sz];
+            dm.analogs[a].vars[dz] = m.analogs[a].vars[sz];
+            dm.analogs[a].weights[dz] = m.analogs[a].weights[sz];
+        }
+        dm.baseline.means[dz] = m.baseline.means[sz];
+        dm.baseline.vars[dz] = m.baseline.vars[sz];
+        dm.baseline.weights[dz] = m.baseline.weights[sz];
+        dm.confidence[dz] = m.confidence[sz];
+    };
+    auto copyZ = [&](const hwpa_lane_hist& h, const hwpa_baseliner_stats& st, const hwpa_lane_model& m, int sz,
+                     hwpa_lane_hist& dh, hwpa_baseliner_stats& dst, hwpa_lane_model& dm, int dz) {
+        dh.low_bound[dz] = h.low_bound[sz];
+        dh.bin_size[dz] = h.bin_size[sz];
+        dh.outlier_low[dz] = h.outlier_low[sz];
+        dh.outlier_high[dz] = h.outlier_high[sz];
+        for (int b = 0; b < NB; ++b) dh.bin_count[b][dz] = h.bin_count[b][sz];
+        dst.baseline.offset[dz] = st.baseline.offset[sz];
+        dst.baseline.m0[dz] = st.baseline.m0[sz];
+        dst.baseline.m1[dz] = st.baseline.m1[sz];
+        dst.baseline.m2[dz] = st.baseline.m2[sz];
+        dst.full.basic.m0[dz] = st.full.basic.m0[sz];
+        copyModel(m, sz, dm, dz);
+    };
+    for (uint32_t p = 0; p < total; ++p) {
+        const uint32_t src = map[p];
+        copyZ(hist[src / LS], stats[src / LS], models[src / LS], (int)(src % LS), ch[p / LS], cs[p / LS], cm[p / LS],
+              (int)(p % LS));
+    }
+    for (uint32_t p = total; p < (uint32_t)cl * LS; ++p)
+        copyZ(ch[cl - 1], cs[cl - 1], cm[cl - 1], 0, ch[cl - 1], cs[cl - 1], cm[cl - 1], (int)(p % LS));
+    ParLanes(cl, [&](int l) { EstimateLane<MirrorForm>(d, ch[l], cs[l], cm[l], model_fi, est_fi, nullptr); });
+    for (uint32_t p = 0; p < total; ++p) {
+        const uint32_t dst = map[p];
+        copyModel(cm[p / LS], (int)(p % LS), models[dst / LS], (int)(dst % LS));
+    }
+    return (int32_t)total;
 }
 
 void orc_dme_estimate(int32_t form, const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,

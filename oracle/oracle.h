@@ -81,6 +81,13 @@ void orc_hist_set(void* h, const hwpa_lane_hist* in);
 
 void orc_dme_init_models(const hwpa_config* cfg, int32_t lanes,
                          const hwpa_baseliner_stats* stats, hwpa_lane_model* out);
+/* Learned-start estimation (hwpa_dme_estimate_learned's arithmetic, mirror form):
+ * network outputs y [lanes*64][12] (the last the confidence logit, gated by
+ * cfg->learned_gate_threshold), output map omap {offset[6], scale[6]};
+ * fallback [lanes*64] optional; returns the number of ZMWs re-estimated by the plain EM. */
+int32_t orc_dme_estimate_learned(const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,
+                                 const hwpa_baseliner_stats* stats, hwpa_lane_model* models, const int32_t model_fi[2],
+                                 const int32_t est_fi[2], const float* y, const float* omap, uint8_t* fallback);
 void orc_dme_estimate(int32_t form, const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,
                       const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
                       const int32_t model_fi[2], const int32_t est_fi[2], orc_dme_diag* diag);

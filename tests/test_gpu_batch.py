@@ -105,3 +105,47 @@ def test_batch_learned_start_in_graph(H, T, gpu):
     ref = _alone(H, pools, chunks, init=init)
     got, _ = _batched(H, pools, chunks, True, init=init)
     _same(ref, got)
+
+
+def test_batch_pools_match_oracle(H, O, T, gpu):
+    """Every pool of the captured batch against the oracle's analyzer of that pool,
+    bit for bit (mirror EM), every step: 256-, 512- and 1024-frame pools at the
+    bench cadence, and a pool at the reference's estimation interval (1000
+    pre-accumulation + 4000 estimation frames) with pool id 10, whose stagger is
+    (10 mod 8) x 512 frames, run long enough for its first estimates."""
+    steps = 18
+    pools = _pools(H)
+    res = [[] for _ in pools]
+    b = H.Batch(pools)
+    b.set_graph(True)
+    gens = [(H.tracegen_default(seed=700 + i), H.dev_bytes(L * T.TRACEGEN_STATE.itemsize))
+            for i, (_, L, _, _) in enumerate(pools)]
+    bufs = [H.dev_bytes(L * F * 64 * 2) for (_, L, F, _) in pools]
+    oas = [O.Analyzer(cfg, pid, L, F, em="mirror") for (pid, L, F, cfg) in pools]
+    dts = (T.PULSE, T.STAT_ACC, T.BASELINER_STATS, T.LANE_MODEL)
+    estimates = [0] * len(pools)
+    for c in range(steps):
+        for i, (_, L, F, _) in enumerate(pools):
+            H.tracegen(gens[i][0], L, F, c * F, gens[i][1], c == 0, bufs[i])
+        H.sync()
+        b.process(bufs, [c * p[2] for p in pools])
+        b.sync()
+        for i, (pid, L, F, cfg) in enumerate(pools):
+            tr = H.to_numpy(bufs[i], np.int16, (L, F, 64))
+            r = oas[i](tr, c * F, dts, want=("pulses", "labels", "models", "baselined"))
+            g = _snapshot(b.pool(i))
+            assert g["flags"][:3] == r["flags"][:3], (pid, c)
+            assert np.array_equal(g["baselined"], r["baselined"]), (pid, c)
+            assert np.array_equal(g["counts"], r["counts"]), (pid, c)
+            if r["flags"][2]:
+                assert g["models"] == r["models"].tobytes(), (pid, c)
+                assert np.array_equal(g["labels"], r["labels"]), (pid, c)
+            p = b.pool(i).pulses()[0]
+            for z in np.nonzero(g["counts"])[0]:
+                k = g["counts"][z]
+                assert np.array_equal(np.ascontiguousarray(p[z, :k]).view(np.uint8),
+                                      np.ascontiguousarray(r["pulses"][z, :k]).view(np.uint8)), (pid, c, z)
+            estimates[i] += r["flags"][1]
+    # the bench-cadence pools estimate every chunk after start-up; the reference-cadence
+    # pool (start-up 2 chunks + stagger 2 + 2 pre-accumulation + 8 estimation) from step 13
+    assert all(n >= 10 for n in estimates[:3]) and estimates[3] >= 1, estimates

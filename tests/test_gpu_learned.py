@@ -42,7 +42,7 @@ def _run(H, T, cfg, traces, init=None, capture=False):
     Z = LANES * 64
     cap = None
     if capture:
-        cap = (H.dev_bytes(Z * 128 * 2), H.dev_bytes(Z * 11 * 4), H.dev_bytes(Z))
+        cap = (H.dev_bytes(Z * 128 * 2), H.dev_bytes(Z * H.MLP_OUT * 4), H.dev_bytes(Z))
         an.capture_training(*cap, 0.0)
     out = []
     for c, tr in enumerate(traces):
@@ -115,3 +115,29 @@ def test_learned_start_gains_over_ablations(H, T, trained):
     assert acc_learned > acc_untrained + 0.1
     # measured 0.993-0.996 with the default gate (cap 3, convergence tolerance 2e-3)
     assert min(agree) > 0.99
+
+
+def test_learned_estimate_bit_exact_with_oracle(H, O, T, trained):
+    """The learned-start estimation (network outputs -> start model, EM capped at
+    learned_em_iteration_limit with its own convergence tolerance, acceptance,
+    compaction of the rejected ZMWs in lane order, plain EM on the compact lanes,
+    scatter) against the oracle's restatement of it, bit for bit: the network's
+    outputs come from the GPU and go to both, so the EM arithmetic and the
+    acceptance / fallback bookkeeping are what is compared."""
+    from test_gpu_parity import _dev, _em_case
+    L = 48
+    cfg, h, s, m = _em_case(H, O, T, L, F, seed=404, tg=dict(snr_lo=4.0, snr_hi=20.0))
+    cfg.learned_init = 1
+    mfi, efi = (512, 1024), (1024, 1536)
+    d_hist = _dev(H, h)
+    d_y = H.dev_bytes(L * 64 * H.MLP_OUT * 4)
+    trained.forward_hist(d_hist, L, d_y)
+    y = H.to_numpy(d_y, np.float32, (L * 64, H.MLP_OUT))
+    d_models = _dev(H, m)
+    n_gpu = H.dme_estimate_learned(cfg, trained, d_hist, _dev(H, s), d_models, mfi, efi, L)
+    g_models = H.to_numpy(d_models, T.LANE_MODEL, (L,))
+    o_models, fb, n_orc = O.dme_estimate_learned(cfg, h, s, m, mfi, efi, y, trained.output_map())
+    print("\nlearned start: %d of %d ZMWs fell back" % (n_orc, L * 64))
+    assert n_gpu == n_orc
+    assert g_models.tobytes() == o_models.tobytes()
+    assert 0 < n_orc < L * 64                # both the accepted and the fallback path are exercised

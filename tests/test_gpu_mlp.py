@@ -103,8 +103,8 @@ def _realistic_batch(rng, B):
         x[i] = 12 * np.exp(-0.5 * ((k - centers[i]) / 2.5) ** 2) / (2.5 * np.sqrt(2 * np.pi))
         x[i] += rng.uniform(0, 0.3, M.IN) * (k < 60)
     x = M.bf16(x)
-    t = np.concatenate([rng.normal(0.5, 0.3, size=(B, 6)), rng.dirichlet(np.ones(5) * 2, size=B)],
-                       axis=1).astype(np.float32)
+    t = np.concatenate([rng.normal(0.5, 0.3, size=(B, 6)), rng.dirichlet(np.ones(5) * 2, size=B),
+                        (rng.uniform(size=(B, 1)) > 0.1)], axis=1).astype(np.float32)
     return x, t
 
 
@@ -113,11 +113,11 @@ def test_gradients_match_oracle(H, gpu, batch):
     rng = np.random.default_rng(batch)
     init = H.Initializer(seed=9)
     p = init.params()
-    omap = np.concatenate([rng.normal(0, 1, 6), rng.uniform(0.5, 2, 6), rng.uniform(0.2, 3, 7)]).astype(np.float32)
+    omap = np.concatenate([rng.normal(0, 1, 6), rng.uniform(0.5, 2, 6), rng.uniform(0.2, 3, 8)]).astype(np.float32)
     init.set_output_map(omap[:6], omap[6:12], omap[12:])
     assert np.array_equal(init.output_map(), omap)
     x, t = _realistic_batch(rng, batch)
-    valid = (rng.uniform(size=batch) > 0.1).astype(np.uint8)
+    valid = t[:, 11].astype(np.uint8)
     d_x, d_t, d_v = _upload_bf16(H, x), H.dev_from_numpy(t), H.dev_from_numpy(valid)
     d_g = H.dev_bytes(M.NPARAM * 4)
     d_l = H.dev_bytes(4)
@@ -125,7 +125,7 @@ def test_gradients_match_oracle(H, gpu, batch):
     H.sync()
     g = H.to_numpy(d_g, np.float32, (M.NPARAM,))
     loss = H.to_numpy(d_l, np.float32, (1,))[0]
-    # oracle: invalid samples contribute nothing, the mean stays over the batch
+    # oracle: invalid samples contribute only the confidence term, the mean stays over the batch
     L_ref, g_ref = M.gradients(p, x, t, omap=omap, valid=valid)
     assert abs(loss - L_ref) <= 1e-4 * abs(L_ref)
     o = 0
@@ -143,8 +143,9 @@ def test_training_lowers_loss(H, gpu):
     B = 512
     x, _ = _realistic_batch(rng, 4 * B)
     teacher = M.forward(H.Initializer(seed=77).params(), x)["y"]
-    t = np.concatenate([teacher[:, :6] * 0.1, np.exp(teacher[:, 6:]) / np.exp(teacher[:, 6:]).sum(1, keepdims=True)],
-                       axis=1).astype(np.float32)
+    t = np.concatenate([teacher[:, :6] * 0.1,
+                        np.exp(teacher[:, 6:11]) / np.exp(teacher[:, 6:11]).sum(1, keepdims=True),
+                        (teacher[:, 11:12] > 0)], axis=1).astype(np.float32)
     init = H.Initializer(seed=3)
     d_x, d_t = _upload_bf16(H, x), H.dev_from_numpy(t)
     d_l = H.dev_bytes(4)
@@ -158,6 +159,38 @@ def test_training_lowers_loss(H, gpu):
             losses.append(float(H.to_numpy(d_l, np.float32, (1,))[0]))
     assert np.all(np.isfinite(losses))
     # the cross-entropy term cannot go below the entropy of the target weights
-    floor = float(np.mean(-(t[:, 6:] * np.log(t[:, 6:])).sum(axis=1)))
+    floor = float(np.mean(-(t[:, 6:11] * np.log(t[:, 6:11])).sum(axis=1)))
     excess = [v - floor for v in losses]
     assert excess[-1] < 0.1 * excess[0], (floor, losses)
+
+
+def test_targets_match_oracle(H, T, gpu):
+    """hwpa_mlp_targets against oracle/mlp.py targets on hand-made EM results:
+    the raw-unit means and weights bit for bit, log(var / u^2) within 3 ulp (the
+    library's log against numpy's), and the validity flag / confidence target
+    exactly (converged, status OK, confidence >= min_confidence)."""
+    rng = np.random.default_rng(12)
+    L = 3
+    Z = L * 64
+    hist = np.zeros(L, dtype=T.LANE_HIST)
+    hist["low_bound"] = rng.uniform(-60, -30, (L, 64)).astype(np.float32)
+    hist["bin_size"] = rng.uniform(1.5, 3.0, (L, 64)).astype(np.float32)
+    diag = np.zeros((L, H.EM_DIAG_ROWS, 64), dtype=np.float32)
+    diag[:, 0:5] = rng.dirichlet(np.ones(5), size=(L, 64)).transpose(0, 2, 1)
+    diag[:, 5] = rng.normal(0, 2, (L, 64))
+    diag[:, 6:10] = np.array([180, 120, 80, 50])[None, :, None] * rng.uniform(0.5, 1.5, (L, 4, 64))
+    diag[:, 10] = rng.uniform(100, 200, (L, 64))
+    diag[:, 15] = rng.uniform(0, 1, (L, 64))
+    diag[:, 16] = rng.uniform(size=(L, 64)) > 0.2
+    diag[:, 17] = np.where(rng.uniform(size=(L, 64)) > 0.9, 4, 0)
+    d_t, d_v = H.dev_bytes(Z * H.MLP_OUT * 4), H.dev_bytes(Z)
+    H.mlp_targets(H.dev_from_numpy(hist), H.dev_from_numpy(diag), L, 0.3, d_t, d_v)
+    g_t = H.to_numpy(d_t, np.float32, (Z, H.MLP_OUT))
+    g_v = H.to_numpy(d_v, np.uint8, (Z,))
+    flat = diag.transpose(1, 0, 2).reshape(H.EM_DIAG_ROWS, Z)
+    o_t, o_ok = M.targets(hist["low_bound"].reshape(Z), hist["bin_size"].reshape(Z), flat, min_conf=0.3)
+    assert np.array_equal(g_v.astype(bool), o_ok)
+    cols = [0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11]
+    assert np.array_equal(g_t[:, cols], o_t[:, cols])
+    spacing = np.spacing(np.abs(o_t[:, 1]))
+    assert np.all(np.abs(g_t[:, 1] - o_t[:, 1]) <= 3 * spacing)

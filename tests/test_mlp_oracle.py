@@ -38,12 +38,15 @@ def test_backward_matches_finite_differences(mapped):
     rng = np.random.default_rng(7)
     p = _rand_params(rng)
     x = rng.uniform(0, 2, size=(6, M.IN))
-    t = np.concatenate([rng.normal(0, 1, size=(6, 6)), rng.dirichlet(np.ones(5), size=6)], axis=1)
+    t = np.concatenate([rng.normal(0, 1, size=(6, 6)), rng.dirichlet(np.ones(5), size=6),
+                        np.array([[1], [0], [1], [1], [0], [1]])], axis=1)
     omap = M.IDENTITY_MAP.astype(np.float64)
+    valid = None
     if mapped:
-        omap = np.concatenate([rng.normal(0, 2, 6), rng.uniform(0.5, 3, 6), rng.uniform(0.1, 5, 7)])
-    L, g = M.gradients(p, x, t, round_bf16=False, dtype=np.float64, omap=omap)
-    assert np.isclose(L, M.loss(p, x, t, round_bf16=False, dtype=np.float64, omap=omap))
+        omap = np.concatenate([rng.normal(0, 2, 6), rng.uniform(0.5, 3, 6), rng.uniform(0.1, 5, 8)])
+        valid = t[:, 11].copy()          # invalid samples keep only the confidence term
+    L, g = M.gradients(p, x, t, round_bf16=False, dtype=np.float64, omap=omap, valid=valid)
+    assert np.isclose(L, M.loss(p, x, t, round_bf16=False, dtype=np.float64, omap=omap, valid=valid))
     idx = np.concatenate([rng.choice(np.arange(o, o + n), size=8, replace=False)
                           for o, n in _offsets()])
     eps = 1e-6
@@ -51,7 +54,8 @@ def test_backward_matches_finite_differences(mapped):
         pp, pm = p.copy(), p.copy()
         pp[i] += eps
         pm[i] -= eps
-        fd = (M.loss(pp, x, t, False, np.float64, omap) - M.loss(pm, x, t, False, np.float64, omap)) / (2 * eps)
+        fd = (M.loss(pp, x, t, False, np.float64, omap, valid) -
+              M.loss(pm, x, t, False, np.float64, omap, valid)) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
 
@@ -68,7 +72,8 @@ def test_rounded_gradients_close_to_exact():
     rng = np.random.default_rng(3)
     p = _rand_params(rng).astype(np.float32)
     x = M.bf16(rng.uniform(0, 1, size=(64, M.IN)).astype(np.float32))
-    t = np.concatenate([rng.normal(0, 1, size=(64, 6)), rng.dirichlet(np.ones(5), size=64)], axis=1)
+    t = np.concatenate([rng.normal(0, 1, size=(64, 6)), rng.dirichlet(np.ones(5), size=64),
+                        rng.integers(0, 2, size=(64, 1))], axis=1)
     _, g32 = M.gradients(p, x, t)
     _, g64 = M.gradients(p, x, t, round_bf16=False, dtype=np.float64)
     assert np.linalg.norm(g32 - g64) / np.linalg.norm(g64) < 2e-2
@@ -97,11 +102,16 @@ def test_targets_decode_round_trip():
     diag[5] = rng.normal(0, 2, Z)
     diag[6:10] = np.array([180, 120, 80, 50])[:, None] * rng.uniform(0.5, 1.5, Z)
     diag[10] = rng.uniform(100, 200, Z)
-    t = M.targets(lo, bs, diag)
-    omap = np.concatenate([rng.normal(0, 3, 6), rng.uniform(0.5, 4, 6), np.ones(7)]).astype(np.float32)
+    diag[15] = rng.uniform(0, 1, Z)                  # confidence
+    diag[16] = rng.uniform(size=Z) > 0.2             # converged
+    diag[17] = np.where(rng.uniform(size=Z) > 0.9, 4, 0)   # status bits
+    t, ok = M.targets(lo, bs, diag, min_conf=0.3)
+    assert np.array_equal(ok, (diag[16] > 0) & (diag[17] == 0) & (diag[15] >= np.float32(0.3)))
+    assert np.array_equal(t[:, 11], ok.astype(np.float32))
+    omap = np.concatenate([rng.normal(0, 3, 6), rng.uniform(0.5, 4, 6), np.ones(8)]).astype(np.float32)
     y = t.copy()
     y[:, :6] = (t[:, :6] - omap[:6]) / omap[6:12]   # outputs that map back onto the targets
-    y[:, 6:] = np.log(t[:, 6:])         # logits whose softmax is the target weights
+    y[:, 6:11] = np.log(t[:, 6:11])     # logits whose softmax is the target weights
     r, mu, v0 = M.decode(y, lo, bs, omap)
     assert np.allclose(r, rho.T, rtol=1e-5)
     assert np.allclose(mu[:, 0], diag[5], atol=1e-3)
@@ -110,4 +120,4 @@ def test_targets_decode_round_trip():
 
 
 def test_param_count_matches_library(H):
-    assert M.NPARAM == H.MLP_PARAMS == 101643
+    assert M.NPARAM == H.MLP_PARAMS == 101900
