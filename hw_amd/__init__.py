@@ -335,6 +335,23 @@ def dme_estimate_learned(cfg, init, d_hist, d_stats, d_models, model_fi, est_fi,
     return n.value
 
 
+NOALLOC_MAX_LANES = 256
+
+
+def dme_estimate_learned_noalloc(cfg, init, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters=None,
+                                 stream=None):
+    """As dme_estimate_learned, with the workspace in the library's fixed device
+    array (at most NOALLOC_MAX_LANES lanes; no device allocation)."""
+    mfi = (ctypes.c_int32 * 2)(*model_fi)
+    efi = (ctypes.c_int32 * 2)(*est_fi)
+    n = ctypes.c_uint32(0)
+    check(lib.hwpa_dme_estimate_learned_noalloc(ctypes.byref(cfg), init.h, ptr(d_hist), ptr(d_stats), ptr(d_models),
+                                                mfi, efi, ctypes.c_uint32(lanes),
+                                                ptr(d_iters) if d_iters is not None else None, ctypes.byref(n),
+                                                _vp(stream)))
+    return n.value
+
+
 MLP_IN, MLP_HIDDEN, MLP_OUT = 128, 256, 12
 MLP_PARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_OUT * MLP_HIDDEN + MLP_OUT
 EM_DIAG_ROWS = 18

@@ -14,6 +14,7 @@
 #include "internal.h"
 
 #include <cfloat>
+#include <mutex>
 
 namespace hwpa {
 
@@ -984,3 +985,106 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
 }
 
 }  // namespace hwpa
+
+// ---- learned-start estimation without allocation ---------------------------
+// The workspace of DmeEstimateLearned for at most HWPA_NOALLOC_MAX_LANES lanes,
+// in one device array.  Host code never passes `scratch` itself to a kernel
+// (its host-side name is not a device address): NoAllocScratch looks the
+// array's device address up once per device with hipGetSymbolAddress.
+// Every buffer carved from it is written before it is read within one call,
+// so nothing needs clearing between calls.
+
+namespace hwpa {
+
+constexpr size_t kNoAllocAlign = 256;
+constexpr size_t AlignUp(size_t n) { return (n + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign; }
+
+// Bytes of the learned-start workspace for `lanes` lanes, each buffer
+// starting on a 256-byte boundary (the order of CarveLearnedWs).
+constexpr size_t LearnedWsBytes(size_t lanes)
+{
+    return AlignUp(lanes * LS * HWPA_MLP_OUT * 4) + AlignUp(lanes * LS) + AlignUp(lanes * 4) + AlignUp(lanes * 4) +
+           AlignUp(16) + AlignUp(lanes * LS * 4) + AlignUp(lanes * sizeof(hwpa_lane_hist)) +
+           AlignUp(lanes * sizeof(hwpa_baseliner_stats)) + AlignUp(lanes * sizeof(hwpa_lane_model)) +
+           AlignUp(lanes * 4);
+}
+
+constexpr size_t kNoAllocBytes = LearnedWsBytes(HWPA_NOALLOC_MAX_LANES);
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+
+static int32_t NoAllocScratch(unsigned char** out)
+{
+    static std::mutex mu;
+    static void* addr[kNoAllocMaxDevices] = {};
+    int dev = -1;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return Fail(HWPA_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return Fail(HWPA_ERR_UNSUPPORTED, "device index beyond the workspace table");
+    std::lock_guard<std::mutex> g(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (r != hipSuccess || !p)
+            return Fail(HWPA_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(r));
+        addr[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(addr[dev]);
+    return HWPA_OK;
+}
+
+static LearnedWs CarveLearnedWs(unsigned char* base, uint32_t lanes)
+{
+    LearnedWs w{};
+    size_t at = 0;
+    auto take = [&](size_t n) {
+        unsigned char* p = base + at;
+        at += AlignUp(n);
+        return p;
+    };
+    const size_t L = lanes;
+    w.y = reinterpret_cast<float*>(take(L * LS * HWPA_MLP_OUT * 4));
+    w.fallback = take(L * LS);
+    w.cnt = reinterpret_cast<uint32_t*>(take(L * 4));
+    w.off = reinterpret_cast<uint32_t*>(take(L * 4));
+    w.total = reinterpret_cast<uint32_t*>(take(16));
+    w.map = reinterpret_cast<uint32_t*>(take(L * LS * 4));
+    w.cHist = reinterpret_cast<hwpa_lane_hist*>(take(L * sizeof(hwpa_lane_hist)));
+    w.cStats = reinterpret_cast<hwpa_baseliner_stats*>(take(L * sizeof(hwpa_baseliner_stats)));
+    w.cModels = reinterpret_cast<hwpa_lane_model*>(take(L * sizeof(hwpa_lane_model)));
+    w.cIters = reinterpret_cast<int32_t*>(take(L * 4));
+    w.lanes = lanes;
+    return w;
+}
+
+}  // namespace hwpa
+
+extern "C" int32_t hwpa_dme_estimate_learned_noalloc(const hwpa_config* cfg, hwpa_initializer* m,
+                                                     const hwpa_lane_hist* d_hist,
+                                                     const hwpa_baseliner_stats* d_stats, hwpa_lane_model* d_models,
+                                                     const int32_t model_fi[2], const int32_t est_fi[2],
+                                                     uint32_t lanes, int32_t* d_iters, uint32_t* fallback_count,
+                                                     void* stream)
+{
+    using namespace hwpa;
+    if (!cfg || !m || !model_fi || !est_fi || lanes == 0)
+        return Fail(HWPA_ERR_INVALID_ARG, "dme_estimate_learned_noalloc: null argument or no lanes");
+    if (lanes > HWPA_NOALLOC_MAX_LANES)
+        return Fail(HWPA_ERR_INVALID_ARG, "dme_estimate_learned_noalloc: " + std::to_string(lanes) +
+                                              " lanes exceed the fixed workspace of " +
+                                              std::to_string(HWPA_NOALLOC_MAX_LANES) + " lanes");
+    unsigned char* base = nullptr;
+    int32_t rc = NoAllocScratch(&base);
+    if (rc != HWPA_OK) return rc;
+    LearnedWs w = CarveLearnedWs(base, lanes);
+    hipStream_t s = AsStream(stream);
+    rc = DmeEstimateLearned(*cfg, m, w, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, s);
+    if (hipStreamSynchronize(s) != hipSuccess && rc == HWPA_OK) rc = Fail(HWPA_ERR_HIP, "learned EM failed");
+    if (rc == HWPA_OK && fallback_count) {
+        if (hipMemcpy(fallback_count, w.total, 4, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = Fail(HWPA_ERR_HIP, "fallback count copy failed");
+    }
+    return rc;
+}

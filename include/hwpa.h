@@ -353,6 +353,22 @@ int32_t hwpa_dme_estimate_learned(const hwpa_config* cfg, hwpa_initializer* m, c
                                   const hwpa_baseliner_stats* d_stats, hwpa_lane_model* d_models,
                                   const int32_t model_fi[2], const int32_t est_fi[2], uint32_t lanes,
                                   int32_t* d_iters, uint32_t* fallback_count, void* stream);
+/* As hwpa_dme_estimate_learned, but allocates nothing: its workspace (network
+ * outputs, fallback flags, per-lane counts and offsets, the compacted
+ * fallback lanes and their histograms, statistics and models) lives in a
+ * fixed device array of the library, sized for HWPA_NOALLOC_MAX_LANES lanes
+ * and held on each GPU that calls it until the process ends.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  More lanes than the maximum -> HWPA_ERR_INVALID_ARG.  Every
+ * call on one GPU uses the same array, so calls that overlap on one GPU (two
+ * streams or two threads at once) race.  Synchronous, like the variant above. */
+#define HWPA_NOALLOC_MAX_LANES 256
+int32_t hwpa_dme_estimate_learned_noalloc(const hwpa_config* cfg, hwpa_initializer* m,
+                                          const hwpa_lane_hist* d_hist, const hwpa_baseliner_stats* d_stats,
+                                          hwpa_lane_model* d_models, const int32_t model_fi[2],
+                                          const int32_t est_fi[2], uint32_t lanes, int32_t* d_iters,
+                                          uint32_t* fallback_count, void* stream);
 /* Attaches an initialiser to an analyzer (NULL detaches); with
  * cfg->learned_init set, its estimations use the learned start.  The
  * initialiser must outlive the analyzer or be detached first. */
