@@ -169,10 +169,15 @@ static int32_t DmeAddBatch(hwpa_analyzer* a, int32_t first_frame)
     if (a->status != kHistInit) {
         HWPA_STAGE_BEGIN(a, kStHistogram);
         a->plan |= kPlanHistAdd;
-        if ((rc = HistAdd(a->dHist, a->dEdge, a->dBaselined, F, a->dModels, L, s)) != HWPA_OK) return rc;
+        if ((rc = HistAdd(a->dHist, a->dEdge, a->dBaselined, F, a->dModels, L, s, false)) != HWPA_OK) return rc;
         a->histFi.Hull(first_frame, first_frame + (int32_t)F);
     }
     HWPA_STAGE_BEGIN(a, kStDme);
+    // The wrapping pass of the histogram (the lanes whose counts could pass 65535) opens this
+    // stage, so the histogram stage's events time its fast pass alone.
+    if (a->status != kHistInit) {
+        if ((rc = HistAddWrap(a->dHist, a->dEdge, a->dBaselined, F, a->dModels, L, s)) != HWPA_OK) return rc;
+    }
     if (a->status != kSequencing) {
         a->plan |= kPlanDmeInit;
         if ((rc = DmeInitModels(a->cfg, a->dAgg, a->dModels, L, s)) != HWPA_OK) return rc;

@@ -495,6 +495,275 @@ __global__ __launch_bounds__(64 * W) void HistAddWavesKernel(hwpa_lane_hist* his
     if (threadIdx.x == 0) e.fresh = 0;
 }
 
+// ---- all 300 bins in LDS ---------------------------------------------------------------
+// The lane's counters sit in LDS in the HBM layout ([bin][zmw] u16, ZMWs 2i and 2i+1 of a bin
+// sharing a 32-bit word), 38,400 bytes, so they move between LDS and HBM as 16-byte vectors
+// and a sample of ZMW z in bin b is one ds_add_u32 of 1 << (16 (z & 1)) to the word of
+// (b, z & ~1).  No carry crosses into the neighbour's half while every count stays below
+// 65536 (checked before the chunk, as above).
+//
+// Bin of a sample without a divide: the bin is floor(Q), Q = RN(a / bs), a = RN(x - lo).
+// With q = RN(a * RN(1 / bs)) (within 2^-23 |a / bs| of it) and |q| < 2048, let k = rint(q)
+// and r = fma(-k, bs, a): for bs > 0, r has the sign of a - k bs exactly (the exact value is
+// a multiple of the smaller of ulp(a) and ulp(bs), so it cannot round to 0 unless it is 0).
+//  * r >= 0: a / bs >= k, so Q >= k, and a / bs < k + 0.501, so Q < k + 1: the bin is k.
+//  * r < 0: a / bs < k, so Q <= k, and Q > k - 1: the bin is k - 1 unless Q rounded up to k
+//    (a / bs within half an ulp of k below it, at most |k| 2^-24) or, for k = 0, to -0
+//    (|a / bs| < 2^-149).  Both lie inside -r <= bs |k| 2^-22 + bs 2^-100; such samples,
+//    bs <= 0 or non-finite, |q| >= 2048 or NaN, and bins outside [0, NB) take the exact path
+//    (BinFast / the divide and the outlier counts), which a batch runs only for the samples
+//    that need it after the whole batch has been counted.
+
+struct ShortBinZ {
+    float lo, bs, inv, bsc, bst;   // inv = 1 / bs (NaN unless 0 < bs < inf), bsc = bs 2^-22, bst = bs 2^-100
+};
+
+__device__ __forceinline__ ShortBinZ MakeShortBinZ(float lo, float bs)
+{
+    ShortBinZ zb;
+    zb.lo = lo;
+    zb.bs = bs;
+    // A NaN reciprocal sends every sample of an unusable bin size to the exact path (|q| < 2048 fails).
+    zb.inv = (bs > 0.0f && bs < hw_u2f(0x7f800000u)) ? 1.0f / bs : hw_u2f(0x7fc00000u);
+    zb.bsc = bs * 2.384185791015625e-07f;
+    zb.bst = bs * 7.888609052210118e-31f;
+    return zb;
+}
+
+// The short path's bin of one sample: true (and the bin in b) unless it must take the exact path.
+// The conditions combine bitwise, so the test has no branches.
+__device__ __forceinline__ bool ShortBin(int cand, const ShortBinZ& zb, int& b)
+{
+    const float a = (float)cand - zb.lo;
+    const float q = a * zb.inv;
+    const float k = __builtin_rintf(q);
+    const float r = __builtin_fmaf(-k, zb.bs, a);
+    const float band = __builtin_fmaf(hw_fabsf(k), zb.bsc, zb.bst);
+    const bool below = r < 0.0f;
+    b = (int)k - (below ? 1 : 0);
+    const bool amb = below & (r >= -band);
+    return (hw_fabsf(q) < 2048.0f) & !amb & ((uint32_t)b < (uint32_t)NB);
+}
+
+template <int N>
+__device__ __forceinline__ void HistBatchFull(char* cntB, uint32_t zoff, uint32_t inc, const int (&fv)[N],
+                                              const ShortBinZ& zb, int thr, int& prev, bool& bl1, bool& bl2,
+                                              int& pushes, uint32_t& lowOut, uint32_t& highOut)
+{
+    const bool s1 = bl1, s2 = bl2;
+    const int sp = pushes, sprev = prev;
+    bool slow = false;
+#pragma clang loop unroll(full)
+    for (int u = 0; u < N; ++u) {
+        const bool isBl = fv[u] < thr;
+        const bool counted = (u >= 2 || pushes + u >= 2) && (isBl == bl2);
+        bl2 = bl1;
+        bl1 = isBl;
+        const int cand = (u == 0) ? prev : fv[u - 1];
+        int b;
+        const bool ok = ShortBin(cand, zb, b);
+        slow = slow || (counted && !ok);
+        const uint32_t bc = ok ? (uint32_t)b : 0u;   // a sample off the short path adds 0 to bin 0
+        atomicAdd(reinterpret_cast<uint32_t*>(cntB + bc * (uint32_t)(LS * 2) + zoff), (counted && ok) ? inc : 0u);
+    }
+    if (__any(slow)) {
+        bool b1 = s1, b2 = s2;
+        uint32_t nLow = 0, nHigh = 0;
+#pragma clang loop unroll(full)
+        for (int u = 0; u < N; ++u) {
+            const bool isBl = fv[u] < thr;
+            const bool counted = (u >= 2 || sp + u >= 2) && (isBl == b2);
+            b2 = b1;
+            b1 = isBl;
+            const int cand = (u == 0) ? sprev : fv[u - 1];
+            int sb;
+            if (counted && !ShortBin(cand, zb, sb)) {
+                bool ex;
+                int bin = BinFast((float)cand - zb.lo, zb.inv, ex);
+                if (ex) bin = hw_floor_i32(((float)cand - zb.lo) / zb.bs);
+                if (bin < 0) {
+                    ++nLow;
+                } else if (bin >= NB) {
+                    ++nHigh;
+                } else {
+                    atomicAdd(reinterpret_cast<uint32_t*>(cntB + (uint32_t)bin * (uint32_t)(LS * 2) + zoff), inc);
+                }
+            }
+        }
+        lowOut = (lowOut + nLow) & 0xffffu;
+        highOut = (highOut + nHigh) & 0xffffu;
+    }
+    prev = fv[N - 1];
+    pushes = (pushes + N < 2) ? pushes + N : 2;
+}
+
+// Frames in batches of B with PD batches in flight.  Slot j of the register ring holds batch
+// g0 + j; the loop body always covers PD whole batches, binning each slot and refilling it
+// in place with batch g0 + j + PD, so no register is copied while its load is in flight
+// (a copy would make the wave wait for every outstanding load) and each batch's loads have
+// PD - 1 batches of work to land.  The last full % PD batches, already loaded, follow.
+// Prime() issues the first PD batches; the kernel calls it before its other set-up loads so
+// their latencies overlap.
+template <int B, int PD>
+struct FrameRing {
+    const int16_t* tr;
+    int frames, full;
+    int fv[PD][B];
+
+    __device__ __forceinline__ void Load(int j, int g)
+    {
+        g = (g < full) ? g : full - 1;   // past the end: re-read the last batch (never binned)
+#pragma clang loop unroll(full)
+        for (int u = 0; u < B; ++u) fv[j][u] = tr[(size_t)(g * B + u) * LS];
+    }
+    __device__ __forceinline__ void Prime(const int16_t* t, int n)
+    {
+        tr = t;
+        frames = n;
+        full = (n > 0) ? n / B : 0;
+        if (full > 0) {
+#pragma clang loop unroll(full)
+            for (int j = 0; j < PD; ++j) Load(j, j);
+        }
+    }
+    __device__ __forceinline__ void Run(char* cntB, uint32_t zoff, uint32_t inc, const ShortBinZ& zb, int thr, int& prev,
+                                        bool& bl1, bool& bl2, int& pushes, uint32_t& lowOut, uint32_t& highOut)
+    {
+        if (full > 0) {
+            int g0 = 0;
+            for (; g0 + PD <= full; g0 += PD) {
+#pragma clang loop unroll(full)
+                for (int j = 0; j < PD; ++j) {
+                    HistBatchFull<B>(cntB, zoff, inc, fv[j], zb, thr, prev, bl1, bl2, pushes, lowOut, highOut);
+                    Load(j, g0 + j + PD);
+                }
+            }
+#pragma clang loop unroll(full)
+            for (int j = 0; j + 1 < PD; ++j) {
+                if (g0 + j < full)
+                    HistBatchFull<B>(cntB, zoff, inc, fv[j], zb, thr, prev, bl1, bl2, pushes, lowOut, highOut);
+            }
+        }
+        for (int t = full * B; t < frames; ++t) {
+            const int f1[1] = {tr[(size_t)t * LS]};
+            HistBatchFull<1>(cntB, zoff, inc, f1, zb, thr, prev, bl1, bl2, pushes, lowOut, highOut);
+        }
+    }
+};
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t PkMaxU16(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_elementwise_max(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+
+// W waves per lane, as HistAddWavesKernel, with every bin in LDS; PD batches of B frames in
+// flight per wave.
+template <int B, int W, int PD>
+__global__ __launch_bounds__(64 * W) void HistAddFullKernel(hwpa_lane_hist* hist, EdgeLane* edge,
+                                                            const int16_t* __restrict__ traces, int frames,
+                                                            const hwpa_lane_model* __restrict__ models)
+{
+    static_assert(W >= 2, "waves");
+    constexpr int NV = NB * LS * 2 / 16;   // 16-byte vectors of counters
+    __shared__ uint4 cnt[NV];
+    __shared__ uint32_t outl[2 * LS];
+    const int lane = blockIdx.x;
+    const int z = threadIdx.x % LS;
+    const int w = threadIdx.x / LS;
+    hwpa_lane_hist& h = hist[lane];
+    EdgeLane& e = edge[lane];
+
+    // Every load the set-up needs is issued here, the wave's first PD trace batches first.
+    const int len = ((frames + W - 1) / W + B - 1) / B * B;   // >= B >= 2: the launcher needs frames >= W B
+    const int f0 = w * len;
+    const int n = (frames - f0 < len) ? frames - f0 : len;   // <= 0 for waves past the end
+    const int16_t* tr = traces + (size_t)lane * frames * LS + z;
+    FrameRing<B, PD> ring;
+    ring.Prime(tr + (size_t)(n > 0 ? f0 : 0) * LS, n);
+    const int fs = (n > 0) ? f0 : 2;   // waves past the end read in-range frames they do not use
+    const int before1 = (w > 0) ? (int)tr[(size_t)(fs - 1) * LS] : 0;
+    const int before2 = (w > 0) ? (int)tr[(size_t)(fs - 2) * LS] : 0;
+    const int fresh = e.fresh;
+    const float lo = h.low_bound[z], bs = h.bin_size[z];
+    const hwpa_lane_model& m = models[lane];
+    const float bvar = hw_h2f(m.baseline.vars[z]);
+    const float bmean = hw_h2f(m.baseline.means[z]);
+
+    for (int i = threadIdx.x; i < 2 * LS; i += W * LS) outl[i] = 0;
+    uint4* hv = reinterpret_cast<uint4*>(&h.bin_count[0][0]);
+    uint32_t mx = 0;   // two 16-bit maxima
+    if (fresh != 0) {
+        for (int i = threadIdx.x; i < NV; i += W * LS) cnt[i] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        constexpr int K = 4;
+#pragma unroll 1
+        for (int i0 = threadIdx.x; i0 < NV; i0 += K * W * LS) {
+            uint4 c[K];
+#pragma clang loop unroll(full)
+            for (int k = 0; k < K; ++k) {
+                const int i = i0 + k * W * LS;
+                c[k] = hv[i < NV ? i : NV - 1];
+            }
+#pragma clang loop unroll(full)
+            for (int k = 0; k < K; ++k) {
+                const int i = i0 + k * W * LS;
+                if (i < NV) cnt[i] = c[k];
+                mx = PkMaxU16(mx, PkMaxU16(PkMaxU16(c[k].x, c[k].y), PkMaxU16(c[k].z, c[k].w)));
+            }
+        }
+    }
+    const uint32_t mxc = UMax(mx & 0xffffu, mx >> 16);
+    if (__syncthreads_or(mxc + (uint32_t)frames > 65535u ? 1 : 0)) {
+        if (threadIdx.x == 0) e.needWrap = 1;   // counters untouched in HBM; the wrapping pass takes the lane
+        __builtin_amdgcn_s_waitcnt(0x0F70);      // the primed trace loads land before the wave ends
+        return;
+    }
+    const ShortBinZ zb = MakeShortBinZ(lo, bs);
+    const int thr = hw_wrap16(hw_round_i32(2.0f * hw_sqrtf(bvar) + bmean));
+    const uint32_t zoff = (uint32_t)(z & ~1) * 2u;
+    const uint32_t inc = 1u << (16u * (uint32_t)(z & 1));
+    char* cntB = reinterpret_cast<char*>(cnt);
+
+    int prev, pushes;
+    bool bl1, bl2;
+    if (w == 0) {
+        prev = e.prev[z];
+        const int i0 = e.i0, i1 = e.i1;
+        pushes = (i1 < 0) ? 0 : ((2 + i1 - i0) & 1) + 1;
+        bl1 = (i1 >= 0) && e.hist[i1 & 1][z] != 0;
+        bl2 = (pushes == 2) && e.hist[i0 & 1][z] != 0;
+    } else {
+        prev = before1;
+        bl1 = before1 < thr;
+        bl2 = before2 < thr;
+        pushes = 2;
+    }
+    uint32_t lowP = 0, highP = 0;
+    ring.Run(cntB, zoff, inc, zb, thr, prev, bl1, bl2, pushes, lowP, highP);
+    atomicAdd(&outl[z], lowP);
+    atomicAdd(&outl[LS + z], highP);
+    __syncthreads();
+
+    for (int i = threadIdx.x; i < NV; i += W * LS) hv[i] = cnt[i];
+    if (w == 0) {
+        h.outlier_low[z] = (uint16_t)((uint32_t)h.outlier_low[z] + outl[z]);
+        h.outlier_high[z] = (uint16_t)((uint32_t)h.outlier_high[z] + outl[LS + z]);
+    }
+    if (w == (frames - 1) / len) {
+        e.prev[z] = (int16_t)prev;
+        e.hist[0][z] = (pushes == 2 ? bl2 : bl1) ? 1 : 0;
+        e.hist[1][z] = bl1 ? 1 : 0;
+        if (z == 0) {
+            e.i0 = 0;
+            e.i1 = (pushes == 0) ? -1 : (pushes == 1 ? 0 : 1);
+            e.needWrap = 0;
+        }
+    }
+    if (threadIdx.x == 0) e.fresh = 0;
+}
+
 static HistConsts MakeHistConsts(const hwpa_config& cfg)
 {
     HistConsts k;
@@ -525,66 +794,96 @@ int32_t HistReset(const hwpa_config& cfg, hwpa_lane_hist* hist, const hwpa_basel
     return CheckLaunch("HistResetKernel");
 }
 
+// The 16-bit wrapping pass over the lanes a fast pass left to it (edge state needWrap).
+static int32_t HistWrapLaunch(hwpa_lane_hist* hist, EdgeLane* edge, const int16_t* traces, uint32_t frames,
+                              const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
+{
+    hipLaunchKernelGGL((HistAddKernel<16, true, NB>), dim3(lanes), dim3(LS), 0, s, hist, edge, traces, (int)frames,
+                       models);
+    return CheckLaunch("HistAddKernel");
+}
+
 template <int B, int NL>
 static int32_t HistAddLaunch(hwpa_lane_hist* hist, EdgeLane* edge, const int16_t* traces, uint32_t frames,
-                             const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
+                             const hwpa_lane_model* models, uint32_t lanes, hipStream_t s, bool wrap)
 {
     hipLaunchKernelGGL((HistAddKernel<B, false, NL>), dim3(lanes), dim3(LS), 0, s, hist, edge, traces, (int)frames,
                        models);
     const int32_t rc = CheckLaunch("HistAddKernel");
-    if (rc != HWPA_OK) return rc;
-    hipLaunchKernelGGL((HistAddKernel<16, true, NB>), dim3(lanes), dim3(LS), 0, s, hist, edge, traces, (int)frames,
-                       models);
-    return CheckLaunch("HistAddKernel");
+    if (rc != HWPA_OK || !wrap) return rc;
+    return HistWrapLaunch(hist, edge, traces, frames, models, lanes, s);
 }
 
 template <int B, int NL, int W>
 static int32_t HistAddWavesLaunch(hwpa_lane_hist* hist, EdgeLane* edge, const int16_t* traces, uint32_t frames,
-                                  const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
+                                  const hwpa_lane_model* models, uint32_t lanes, hipStream_t s, bool wrap)
 {
-    if (frames < (uint32_t)(W * B)) return HistAddLaunch<B, NL>(hist, edge, traces, frames, models, lanes, s);
+    if (frames < (uint32_t)(W * B)) return HistAddLaunch<B, NL>(hist, edge, traces, frames, models, lanes, s, wrap);
     hipLaunchKernelGGL((HistAddWavesKernel<B, NL, W>), dim3(lanes), dim3(W * LS), 0, s, hist, edge, traces,
                        (int)frames, models);
     const int32_t rc = CheckLaunch("HistAddWavesKernel");
-    if (rc != HWPA_OK) return rc;
-    hipLaunchKernelGGL((HistAddKernel<16, true, NB>), dim3(lanes), dim3(LS), 0, s, hist, edge, traces, (int)frames,
+    if (rc != HWPA_OK || !wrap) return rc;
+    return HistWrapLaunch(hist, edge, traces, frames, models, lanes, s);
+}
+
+template <int B, int W, int PD>
+static int32_t HistAddFullLaunch(hwpa_lane_hist* hist, EdgeLane* edge, const int16_t* traces, uint32_t frames,
+                                 const hwpa_lane_model* models, uint32_t lanes, hipStream_t s, bool wrap)
+{
+    if (frames < (uint32_t)(W * B)) return HistAddLaunch<B, 192>(hist, edge, traces, frames, models, lanes, s, wrap);
+    hipLaunchKernelGGL((HistAddFullKernel<B, W, PD>), dim3(lanes), dim3(W * LS), 0, s, hist, edge, traces, (int)frames,
                        models);
-    return CheckLaunch("HistAddKernel");
+    const int32_t rc = CheckLaunch("HistAddFullKernel");
+    if (rc != HWPA_OK || !wrap) return rc;
+    return HistWrapLaunch(hist, edge, traces, frames, models, lanes, s);
 }
 
 // Histogram variants: frames per prefetched batch x bins kept in LDS (x waves per lane).
 int32_t HistAddVariant(int32_t variant, hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
-                       const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
+                       const hwpa_lane_model* models, uint32_t lanes, hipStream_t s, bool wrap)
 {
     if (!hist || !edge || !traces || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "hist_add: bad argument");
     if (reinterpret_cast<uintptr_t>(hist) % 16 != 0)
         return Fail(HWPA_ERR_INVALID_ARG, "hist_add: histograms must be 16-byte aligned");
     EdgeLane* e = (EdgeLane*)edge;
     switch (variant) {
-    case 0: return HistAddLaunch<16, 160>(hist, e, traces, frames, models, lanes, s);
-    case 1: return HistAddLaunch<16, 128>(hist, e, traces, frames, models, lanes, s);
-    case 2: return HistAddLaunch<16, 192>(hist, e, traces, frames, models, lanes, s);
-    case 3: return HistAddLaunch<8, 160>(hist, e, traces, frames, models, lanes, s);
-    case 4: return HistAddLaunch<8, 128>(hist, e, traces, frames, models, lanes, s);
-    case 5: return HistAddLaunch<8, 192>(hist, e, traces, frames, models, lanes, s);
-    case 6: return HistAddLaunch<32, 160>(hist, e, traces, frames, models, lanes, s);
-    case 7: return HistAddLaunch<32, 128>(hist, e, traces, frames, models, lanes, s);
-    case 8: return HistAddLaunch<32, 192>(hist, e, traces, frames, models, lanes, s);
-    case 9: return HistAddWavesLaunch<8, 160, 4>(hist, e, traces, frames, models, lanes, s);
-    case 10: return HistAddWavesLaunch<8, 128, 4>(hist, e, traces, frames, models, lanes, s);
-    case 11: return HistAddWavesLaunch<16, 160, 4>(hist, e, traces, frames, models, lanes, s);
-    case 12: return HistAddWavesLaunch<8, 160, 2>(hist, e, traces, frames, models, lanes, s);
-    case 13: return HistAddWavesLaunch<8, 192, 4>(hist, e, traces, frames, models, lanes, s);
-    case 14: return HistAddWavesLaunch<8, 192, 8>(hist, e, traces, frames, models, lanes, s);
-    case 15: return HistAddWavesLaunch<8, 160, 8>(hist, e, traces, frames, models, lanes, s);
+    case 0: return HistAddLaunch<16, 160>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 1: return HistAddLaunch<16, 128>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 2: return HistAddLaunch<16, 192>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 3: return HistAddLaunch<8, 160>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 4: return HistAddLaunch<8, 128>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 5: return HistAddLaunch<8, 192>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 6: return HistAddLaunch<32, 160>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 7: return HistAddLaunch<32, 128>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 8: return HistAddLaunch<32, 192>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 9: return HistAddWavesLaunch<8, 160, 4>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 10: return HistAddWavesLaunch<8, 128, 4>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 11: return HistAddWavesLaunch<16, 160, 4>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 12: return HistAddWavesLaunch<8, 160, 2>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 13: return HistAddWavesLaunch<8, 192, 4>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 14: return HistAddWavesLaunch<8, 192, 8>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 15: return HistAddWavesLaunch<8, 160, 8>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 16: return HistAddFullLaunch<8, 4, 4>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 17: return HistAddFullLaunch<16, 4, 3>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 18: return HistAddFullLaunch<8, 8, 1>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 19: return HistAddFullLaunch<8, 8, 2>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 20: return HistAddFullLaunch<16, 8, 1>(hist, e, traces, frames, models, lanes, s, wrap);
+    case 21: return HistAddFullLaunch<4, 8, 4>(hist, e, traces, frames, models, lanes, s, wrap);
     default: return Fail(HWPA_ERR_INVALID_ARG, "hist_add: unknown variant");
     }
 }
 
 int32_t HistAdd(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
-                const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
+                const hwpa_lane_model* models, uint32_t lanes, hipStream_t s, bool wrap)
 {
-    return HistAddVariant(TuneGet(kTuneHistogram, lanes, frames), hist, edge, traces, frames, models, lanes, s);
+    return HistAddVariant(TuneGet(kTuneHistogram, lanes, frames), hist, edge, traces, frames, models, lanes, s, wrap);
+}
+
+int32_t HistAddWrap(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
+                    const hwpa_lane_model* models, uint32_t lanes, hipStream_t s)
+{
+    if (!hist || !edge || !traces || !models || lanes == 0) return Fail(HWPA_ERR_INVALID_ARG, "hist_add: bad argument");
+    return HistWrapLaunch(hist, (EdgeLane*)edge, traces, frames, models, lanes, s);
 }
 
 }  // namespace hwpa

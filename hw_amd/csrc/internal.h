@@ -74,8 +74,13 @@ int32_t HistReset(const hwpa_config& cfg, hwpa_lane_hist* hist, const hwpa_basel
 // counters written or read back); that add must precede any read of the counters.
 int32_t HistResetDeferred(const hwpa_config& cfg, hwpa_lane_hist* hist, void* edge, const hwpa_baseliner_stats* stats,
                           uint32_t lanes, hipStream_t s);
+// Fast pass over the chunk; with wrap, also the 16-bit wrapping pass over the lanes it left
+// (lanes whose counts could pass 65535).  Without it, HistAddWrap must run before the
+// traces change or the counters are read.
 int32_t HistAdd(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
-                const hwpa_lane_model* models, uint32_t lanes, hipStream_t s);
+                const hwpa_lane_model* models, uint32_t lanes, hipStream_t s, bool wrap = true);
+int32_t HistAddWrap(hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
+                    const hwpa_lane_model* models, uint32_t lanes, hipStream_t s);
 
 // Kernel shape variants chosen per (lanes, frames) by the autotuner; every
 // variant computes the same bits.  Variant 0 is the default.
@@ -84,7 +89,7 @@ int32_t TuneVariants(int kernel);
 int32_t TuneGet(int kernel, uint32_t lanes, uint32_t frames);
 void TuneSet(int kernel, uint32_t lanes, uint32_t frames, int32_t variant);
 int32_t HistAddVariant(int32_t variant, hwpa_lane_hist* hist, void* edge, const int16_t* traces, uint32_t frames,
-                       const hwpa_lane_model* models, uint32_t lanes, hipStream_t s);
+                       const hwpa_lane_model* models, uint32_t lanes, hipStream_t s, bool wrap = true);
 
 int32_t DmeInitModels(const hwpa_config& cfg, const hwpa_baseliner_stats* stats, hwpa_lane_model* models,
                       uint32_t lanes, hipStream_t s);

@@ -144,7 +144,8 @@ int32_t hwpa_batch_autotune(hwpa_batch* b, int32_t repeats);
  * Tunable kernels run in variants that give the same bits with different
  * shapes: kernel 0 = histogram (variants 0-8: one wave per lane, 8, 16 or 32
  * frames per prefetched batch x 128, 160 or 192 bins kept in LDS; 9-15: 2, 4 or
- * 8 waves per lane sharing its LDS counters), kernel 1 = baseline filter (1, 2 or 4 lanes per
+ * 8 waves per lane sharing its LDS counters; 16-21: all 300 bins in LDS, 8 or 16
+ * frames per batch x 4, 8 or 2 waves per lane), kernel 1 = baseline filter (1, 2 or 4 lanes per
  * workgroup, then the same with two groups of raw frames in flight), kernel 2 = Viterbi labeler (16 or 8 frames per prefetched
  * batch, then the same with registers held to 4 waves per SIMD), kernel 3 = pulse accumulator (8 frames,
  * 8 frames with the next batch in flight, 16 with the next in flight, then 8, 8 with

@@ -88,7 +88,7 @@ def test_baseliner_lane_tile_variants(H, O, T, gpu, variant, F):
 def test_autotune_picks_a_timed_variant(H, gpu):
     cfg = H.bench_config(512)
     r = H.autotune(cfg, 64, 512, repeats=2)
-    counts = {"histogram": 16, "baseline": 6, "labels": 4, "pulses": 7}
+    counts = {"histogram": 22, "baseline": 6, "labels": 4, "pulses": 7}
     for i, (name, n) in enumerate(counts.items()):
         assert len(r[name]) == H.tune_variants(i) == n
         assert all(t > 0 for t in r[name])
@@ -137,7 +137,7 @@ def test_stats_aggregate_bit_exact(H, O, T, gpu):
             H.stats_reset(d_agg, L)
 
 
-@pytest.fixture(params=range(16))
+@pytest.fixture(params=range(22))
 def hist_variant(request, H):
     """Every autotuner variant of the histogram kernel (shape 6 lanes x 512)."""
     H.tune_set(H.TUNE_HISTOGRAM, 6, 512, request.param)
@@ -170,7 +170,7 @@ def test_histogram_bit_exact(H, O, T, gpu, hist_variant):
         _eq_struct(H.to_numpy(d_hist, T.LANE_HIST, (L,)), hist.get(T.LANE_HIST), "hist[%d]" % c)
 
 
-@pytest.mark.parametrize("variant", [0, 9, 11, 12, 14])
+@pytest.mark.parametrize("variant", [0, 9, 11, 12, 14, 16, 17, 19, 21])
 @pytest.mark.parametrize("frames", [144, 48, 208])
 def test_histogram_uneven_frames(H, O, T, gpu, variant, frames):
     """Chunk lengths that do not split evenly over a lane's waves (144 frames in 4 waves of
@@ -425,10 +425,21 @@ def test_pulses_bit_exact(H, O, T, gpu, variant, max_calls):
     H.tune_set(H.TUNE_PULSE, L, F, 0)
 
 
-def test_analyzer_end_to_end(H, O, T, gpu):
+@pytest.mark.parametrize("hist_v", [0, 16, 19])
+def test_analyzer_end_to_end(H, O, T, gpu, hist_v):
     """Whole analyzer (bench configuration: one estimate per chunk) against
-    the CPU analyzer, every chunk, every output."""
+    the CPU analyzer, every chunk, every output; with the one-wave histogram and
+    with the all-bins-in-LDS forms, whose counters start from the reset that
+    leaves the zeros to them."""
     L, F, C = 8, 512, 7
+    H.tune_set(H.TUNE_HISTOGRAM, L, F, hist_v)
+    try:
+        _analyzer_end_to_end(H, O, T, L, F, C)
+    finally:
+        H.tune_set(H.TUNE_HISTOGRAM, L, F, 0)
+
+
+def _analyzer_end_to_end(H, O, T, L, F, C):
     cfg = H.bench_config(F)
     chunks = gen_traces(O, T, H, L, F, C, seed=77)
     oa = O.Analyzer(cfg, 0, L, F, em="mirror")
