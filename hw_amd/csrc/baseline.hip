@@ -177,18 +177,19 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
     BlLaneState& st = state[lane];
     uint32_t* wl = win + (size_t)lane * T::winPerLane * LS + z;
 
-    uint32_t h0a[W0], h0b[W0], h1a[W1], h1b[W1];
+    uint32_t h0a[W0], h0b[W0];
 #pragma unroll
     for (int i = 0; i < W0; ++i) {
         h0a[i] = wl[i * LS];
         h0b[i] = wl[(W0 + i) * LS];
     }
+    // Stage-1 windows wait in LDS between groups (each thread its own column): they are used
+    // once per group, and their 2 W1 registers are free for the rest of it.
+    __shared__ uint32_t sH1[LPB][2 * W1][LS];
+    uint32_t(*h1)[LS] = sH1[threadIdx.x >> 6];
     if constexpr (NST == 3) {
 #pragma unroll
-        for (int i = 0; i < W1; ++i) {
-            h1a[i] = wl[(2 * W0 + i) * LS];
-            h1b[i] = wl[(2 * W0 + W1 + i) * LS];
-        }
+        for (int i = 0; i < 2 * W1; ++i) h1[i][z] = wl[(2 * W0 + i) * LS];
     }
     uint32_t* ringA = wl + T::rowsHist * LS;
     uint32_t* ringB = ringA + WF * LS;
@@ -201,11 +202,14 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
     bool latL = st.lat_l[z] != 0, latH1 = st.lat_h1[z] != 0;
 
     // per-chunk statistics (offsets are 0)
-    float cm0 = 0.f, cm1 = 0.f, cm2 = 0.f, lagM2 = 0.f;
-    float fbuf[4] = {0.f, 0.f, 0.f, 0.f}, bbuf[4] = {0.f, 0.f, 0.f, 0.f};
+    // (the sample and group counts are the chunk's frame and group counts; the first four
+    // samples go straight to the output record)
+    hwpa_baseliner_stats& o = stats[lane];
+    float cm1 = 0.f, cm2 = 0.f, lagM2 = 0.f;
+    float bbuf[4] = {0.f, 0.f, 0.f, 0.f};
     int tmin = 32767, tmax = -32768;
     float bm0 = 0.f, bm1 = 0.f, bm2 = 0.f, rawBl = 0.f;
-    float gm0 = 0.f, gm1 = 0.f, gm2 = 0.f;
+    float gm1 = 0.f, gm2 = 0.f;
 
     const E* rl = raw + (size_t)lane * frames * LS + z;
     int16_t* ol = out + (size_t)lane * frames * LS + z;
@@ -236,25 +240,46 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
 #pragma unroll
         for (int j = 0; j < S; ++j) nv[j] = rl[(size_t)(gn * S + j) * LS];
 
-        // Stage 0 on (x, ~x): erode pair, then dilate pair -> (lower, ~upper).
-        uint32_t a0[N0], e0[N0], d0[N0];
+        // Stage 0 on (x, ~x): erode pair, then dilate pair -> (lower, ~upper), in two halves of
+        // the group (the same window extrema, with about half the live registers).
+        constexpr int H0 = N0 / 2;
+        static_assert(N0 % 2 == 0 && (NST == 2 || H0 % T::s[1] == 0), "stage-1 inputs fall in whole halves");
+        uint32_t a1[N1];
+        uint32_t fin = 0;
 #pragma unroll
-        for (int i = 0; i < N0; ++i) {
-            const int x = rv[i * T::s[0]];
-            a0[i] = Pack(x, ~x);
+        for (int hf = 0; hf < 2; ++hf) {
+            uint32_t a0[H0], e0[H0], d0[H0];
+#pragma unroll
+            for (int i = 0; i < H0; ++i) {
+                const int x = rv[(hf * H0 + i) * T::s[0]];
+                a0[i] = Pack(x, ~x);
+            }
+            Slide<W0, H0, false>(h0a, a0, e0);
+            Slide<W0, H0, true>(h0b, e0, d0);
+            if constexpr (NST == 3) {
+#pragma unroll
+                for (int i = 0; i < H0 / T::s[1]; ++i) a1[hf * (H0 / T::s[1]) + i] = d0[i * T::s[1]] ^ HI_NOT;
+            } else {
+                if (hf == 0) fin = d0[0] ^ HI_NOT;
+            }
         }
-        Slide<W0, N0, false>(h0a, a0, e0);
-        Slide<W0, N0, true>(h0b, e0, d0);
-        uint32_t fin;
         if constexpr (NST == 3) {
-            uint32_t a1[N1], e1[N1], d1[N1];
+            uint32_t h1a[W1], h1b[W1], e1[N1], d1[N1];
 #pragma unroll
-            for (int i = 0; i < N1; ++i) a1[i] = d0[i * T::s[1]] ^ HI_NOT;
+            for (int i = 0; i < W1; ++i) {
+                h1a[i] = h1[i][z];
+                h1b[i] = h1[W1 + i][z];
+            }
             Slide<W1, N1, false>(h1a, a1, e1);
             Slide<W1, N1, true>(h1b, e1, d1);
+#pragma unroll
+            for (int i = 0; i < W1; ++i) {
+                h1[i][z] = h1a[i];
+                h1[W1 + i][z] = h1b[i];
+            }
+            // Keeps the windows in LDS across groups (no promotion back to registers).
+            asm volatile("" ::: "memory");
             fin = d1[0];
-        } else {
-            fin = d0[0] ^ HI_NOT;
         }
         // Final stage: erode pair then dilate pair.
         const uint32_t fe = RingResult<false>(ctr, frontA, sA, lastA);
@@ -296,10 +321,9 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
             latH1 = maskHp1;
             // BaselinerStatAccumulator::AddSample on the latent frame
             const float bs = (float)latData;
-            if (g == 0 && j < 4) fbuf[j] = bs;
+            if (g == 0 && j < 4) o.full.fbuf[j][z] = bs;
             lagM2 += bbuf[j & 3] * bs;
             bbuf[j & 3] = bs;
-            cm0 += 1.0f;
             cm1 += bs;
             cm2 += bs * bs;
             tmin = latData < tmin ? latData : tmin;
@@ -313,7 +337,6 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
             latData = bls;
         }
         const float bgv = (float)hw_round_i16(subBl);
-        gm0 += 1.0f;
         gm1 += bgv;
         gm2 += bgv * bgv;
 #pragma unroll
@@ -331,10 +354,7 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
     }
     if constexpr (NST == 3) {
 #pragma unroll
-        for (int i = 0; i < W1; ++i) {
-            wl[(2 * W0 + i) * LS] = h1a[i];
-            wl[(2 * W0 + W1 + i) * LS] = h1b[i];
-        }
+        for (int i = 0; i < 2 * W1; ++i) wl[(2 * W0 + i) * LS] = h1[i][z];
     }
     st.hold[0][z] = sA;
     st.hold[1][z] = sB;
@@ -349,15 +369,13 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
     st.lat_l[z] = latL ? 1 : 0;
     st.lat_h1[z] = latH1 ? 1 : 0;
 
-    hwpa_baseliner_stats& o = stats[lane];
     o.full.basic.offset[z] = 0.f;
-    o.full.basic.m0[z] = cm0;
+    o.full.basic.m0[z] = (float)frames;   // exact: frames < 2^24
     o.full.basic.m1[z] = cm1;
     o.full.basic.m2[z] = cm2;
     o.full.lag_m2[z] = lagM2;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        o.full.fbuf[q][z] = fbuf[q];
         o.full.bbuf[q][z] = bbuf[q];
     }
     o.full.bidx[0][z] = (uint8_t)(frames < 4 ? frames : 4);
@@ -370,7 +388,7 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
     o.baseline.m2[z] = bm2;
     o.raw_baseline_sum[z] = rawBl;
     o.background.offset[z] = 0.f;
-    o.background.m0[z] = gm0;
+    o.background.m0[z] = (float)groups;
     o.background.m1[z] = gm1;
     o.background.m2[z] = gm2;
 }

@@ -614,7 +614,7 @@ struct FrameRing {
     {
         g = (g < full) ? g : full - 1;   // past the end: re-read the last batch (never binned)
 #pragma clang loop unroll(full)
-        for (int u = 0; u < B; ++u) fv[j][u] = tr[(size_t)(g * B + u) * LS];
+        for (int u = 0; u < B; ++u) fv[j][u] = __builtin_nontemporal_load(&tr[(size_t)(g * B + u) * LS]);
     }
     __device__ __forceinline__ void Prime(const int16_t* t, int n)
     {
@@ -746,7 +746,12 @@ __global__ __launch_bounds__(64 * W) void HistAddFullKernel(hwpa_lane_hist* hist
     atomicAdd(&outl[LS + z], highP);
     __syncthreads();
 
-    for (int i = threadIdx.x; i < NV; i += W * LS) hv[i] = cnt[i];
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    for (int i = threadIdx.x; i < NV; i += W * LS) {
+        const uint4 c = cnt[i];
+        const u32x4 v = {c.x, c.y, c.z, c.w};
+        __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(&hv[i]));
+    }
     if (w == 0) {
         h.outlier_low[z] = (uint16_t)((uint32_t)h.outlier_low[z] + outl[z]);
         h.outlier_high[z] = (uint16_t)((uint32_t)h.outlier_high[z] + outl[LS + z]);
