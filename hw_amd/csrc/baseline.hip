@@ -135,23 +135,29 @@ __device__ __forceinline__ uint32_t RingResult(int ctr, uint32_t front, uint32_t
     return POp<MX>(s, front);
 }
 
-template <int W, bool MX>
-__device__ __forceinline__ void RingRefill(uint32_t* ring, uint32_t last)
+// The rings now hold one complete block each: replace each by its suffix extrema (erode
+// pair ringA, dilate pair ringB), working down from the top in batches of 12 loads per ring;
+// both rings' batches are in flight together.
+template <int W>
+__device__ __forceinline__ void RingRefill2(uint32_t* ringA, uint32_t lastA, uint32_t* ringB, uint32_t lastB)
 {
-    // The ring now holds one complete block: replace it by its suffix
-    // extrema, working down from the top in batches of 12 loads.
     constexpr int B = 12;
-    uint32_t acc = last;
+    uint32_t accA = lastA, accB = lastB;
 #pragma unroll 1
     for (int top = W - 2; top >= 0; top -= B) {
-        uint32_t v[B];
+        uint32_t va[B], vb[B];
 #pragma unroll
-        for (int q = 0; q < B; ++q) v[q] = (top - q >= 0) ? ring[(top - q) * LS] : 0u;
+        for (int q = 0; q < B; ++q) {
+            va[q] = (top - q >= 0) ? ringA[(top - q) * LS] : 0u;
+            vb[q] = (top - q >= 0) ? ringB[(top - q) * LS] : 0u;
+        }
 #pragma unroll
         for (int q = 0; q < B; ++q) {
             if (top - q >= 0) {
-                acc = POp<MX>(acc, v[q]);
-                ring[(top - q) * LS] = acc;
+                accA = PMin(accA, va[q]);
+                ringA[(top - q) * LS] = accA;
+                accB = PMax(accB, vb[q]);
+                ringB[(top - q) * LS] = accB;
             }
         }
     }
@@ -223,14 +229,13 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
     if (PD == 2) {
         const int g1 = (groups > 1) ? 1 : 0;
 #pragma unroll
-        for (int j = 0; j < S; ++j) pv[j] = rl[(size_t)(g1 * S + j) * LS];
+        for (int j = 0; j < S; ++j) pv[j] = (rl + (size_t)g1 * S * LS)[j * LS];
     }
 
     for (int g = 0; g < groups; ++g) {
         // Final-stage ring: refill on wrap, then read the fronts early.
         if (ctr == 0) {
-            RingRefill<WF, false>(ringA, lastA);
-            RingRefill<WF, true>(ringB, lastB);
+            RingRefill2<WF>(ringA, lastA, ringB, lastB);
         }
         const uint32_t frontA = ringA[ctr * LS];
         const uint32_t frontB = ringB[ctr * LS];
@@ -238,7 +243,7 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
         int nv[S];
         const int gn = (g + PD < groups) ? g + PD : groups - 1;
 #pragma unroll
-        for (int j = 0; j < S; ++j) nv[j] = rl[(size_t)(gn * S + j) * LS];
+        for (int j = 0; j < S; ++j) nv[j] = (rl + (size_t)gn * S * LS)[j * LS];   // group base + immediate offsets
 
         // Stage 0 on (x, ~x): erode pair, then dilate pair -> (lower, ~upper), in two halves of
         // the group (the same window extrema, with about half the live registers).
@@ -313,7 +318,7 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
         for (int j = 0; j < S; ++j) {
             const float rawScaled = (float)hw_wrap16(rv[j] - k.pedestal) * k.scaler;
             const int bls = hw_round_i16(rawScaled - subBl);
-            ol[(size_t)(g * S + j) * LS] = (int16_t)bls;
+            (ol + (size_t)g * S * LS)[j * LS] = (int16_t)bls;
             const int rr = hw_round_i16(rawScaled);
             const bool maskHp1 = (float)bls < thr;
             const bool m = latH1 && latL && maskHp1;

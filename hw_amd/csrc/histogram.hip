@@ -614,7 +614,7 @@ struct FrameRing {
     {
         g = (g < full) ? g : full - 1;   // past the end: re-read the last batch (never binned)
 #pragma clang loop unroll(full)
-        for (int u = 0; u < B; ++u) fv[j][u] = __builtin_nontemporal_load(&tr[(size_t)(g * B + u) * LS]);
+        for (int u = 0; u < B; ++u) fv[j][u] = __builtin_nontemporal_load(&(tr + (size_t)g * B * LS)[u * LS]);
     }
     __device__ __forceinline__ void Prime(const int16_t* t, int n)
     {
