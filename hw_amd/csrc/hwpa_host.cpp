@@ -297,7 +297,7 @@ int32_t hwpa_config_default(hwpa_config* c)
     c->learned_conf_threshold = 0.0f;
     c->learned_em_iteration_limit = 3;
     c->learned_conv_tol = 2e-3f;
-    c->learned_gate_threshold = 0.5f;
+    c->learned_gate_threshold = 0.9f;
     // Movie description of the synthetic workload: 100 fps, 1 e-/DN, four analogs
     // with amplitudes 180/120/80/50 over a 12-count baseline sigma.
     c->movie.frame_rate = 100.0f;
