@@ -502,52 +502,14 @@ __global__ __launch_bounds__(64 * W) void HistAddWavesKernel(hwpa_lane_hist* his
 // (b, z & ~1).  No carry crosses into the neighbour's half while every count stays below
 // 65536 (checked before the chunk, as above).
 //
-// Bin of a sample without a divide: the bin is floor(Q), Q = RN(a / bs), a = RN(x - lo).
-// With q = RN(a * RN(1 / bs)) (within 2^-23 |a / bs| of it) and |q| < 2048, let k = rint(q)
-// and r = fma(-k, bs, a): for bs > 0, r has the sign of a - k bs exactly (the exact value is
-// a multiple of the smaller of ulp(a) and ulp(bs), so it cannot round to 0 unless it is 0).
-//  * r >= 0: a / bs >= k, so Q >= k, and a / bs < k + 0.501, so Q < k + 1: the bin is k.
-//  * r < 0: a / bs < k, so Q <= k, and Q > k - 1: the bin is k - 1 unless Q rounded up to k
-//    (a / bs within half an ulp of k below it, at most |k| 2^-24) or, for k = 0, to -0
-//    (|a / bs| < 2^-149).  Both lie inside -r <= bs |k| 2^-22 + bs 2^-100; such samples,
-//    bs <= 0 or non-finite, |q| >= 2048 or NaN, and bins outside [0, NB) take the exact path
-//    (BinFast / the divide and the outlier counts), which a batch runs only for the samples
-//    that need it after the whole batch has been counted.
-
-struct ShortBinZ {
-    float lo, bs, inv, bsc, bst;   // inv = 1 / bs (NaN unless 0 < bs < inf), bsc = bs 2^-22, bst = bs 2^-100
-};
-
-__device__ __forceinline__ ShortBinZ MakeShortBinZ(float lo, float bs)
-{
-    ShortBinZ zb;
-    zb.lo = lo;
-    zb.bs = bs;
-    // A NaN reciprocal sends every sample of an unusable bin size to the exact path (|q| < 2048 fails).
-    zb.inv = (bs > 0.0f && bs < hw_u2f(0x7f800000u)) ? 1.0f / bs : hw_u2f(0x7fc00000u);
-    zb.bsc = bs * 2.384185791015625e-07f;
-    zb.bst = bs * 7.888609052210118e-31f;
-    return zb;
-}
-
-// The short path's bin of one sample: true (and the bin in b) unless it must take the exact path.
-// The conditions combine bitwise, so the test has no branches.
-__device__ __forceinline__ bool ShortBin(int cand, const ShortBinZ& zb, int& b)
-{
-    const float a = (float)cand - zb.lo;
-    const float q = a * zb.inv;
-    const float k = __builtin_rintf(q);
-    const float r = __builtin_fmaf(-k, zb.bs, a);
-    const float band = __builtin_fmaf(hw_fabsf(k), zb.bsc, zb.bst);
-    const bool below = r < 0.0f;
-    b = (int)k - (below ? 1 : 0);
-    const bool amb = below & (r >= -band);
-    return (hw_fabsf(q) < 2048.0f) & !amb & ((uint32_t)b < (uint32_t)NB);
-}
+// Bin of a sample without a divide: hw_shortbin (include/hwpa_math.h), which decides every
+// sample except those within a rounding band of a bin edge, outliers and unusable bounds;
+// such samples take the exact path (BinFast / the divide and the outlier counts), which a
+// batch runs only for the samples that need it after the whole batch has been counted.
 
 template <int N>
 __device__ __forceinline__ void HistBatchFull(char* cntB, uint32_t zoff, uint32_t inc, const int (&fv)[N],
-                                              const ShortBinZ& zb, int thr, int& prev, bool& bl1, bool& bl2,
+                                              const hw_shortbin_z& zb, int thr, int& prev, bool& bl1, bool& bl2,
                                               int& pushes, uint32_t& lowOut, uint32_t& highOut)
 {
     const bool s1 = bl1, s2 = bl2;
@@ -561,7 +523,7 @@ __device__ __forceinline__ void HistBatchFull(char* cntB, uint32_t zoff, uint32_
         bl1 = isBl;
         const int cand = (u == 0) ? prev : fv[u - 1];
         int b;
-        const bool ok = ShortBin(cand, zb, b);
+        const bool ok = hw_shortbin(cand, zb, NB, &b);
         slow = slow || (counted && !ok);
         const uint32_t bc = ok ? (uint32_t)b : 0u;   // a sample off the short path adds 0 to bin 0
         atomicAdd(reinterpret_cast<uint32_t*>(cntB + bc * (uint32_t)(LS * 2) + zoff), (counted && ok) ? inc : 0u);
@@ -577,7 +539,7 @@ __device__ __forceinline__ void HistBatchFull(char* cntB, uint32_t zoff, uint32_
             b1 = isBl;
             const int cand = (u == 0) ? sprev : fv[u - 1];
             int sb;
-            if (counted && !ShortBin(cand, zb, sb)) {
+            if (counted && !hw_shortbin(cand, zb, NB, &sb)) {
                 bool ex;
                 int bin = BinFast((float)cand - zb.lo, zb.inv, ex);
                 if (ex) bin = hw_floor_i32(((float)cand - zb.lo) / zb.bs);
@@ -626,7 +588,7 @@ struct FrameRing {
             for (int j = 0; j < PD; ++j) Load(j, j);
         }
     }
-    __device__ __forceinline__ void Run(char* cntB, uint32_t zoff, uint32_t inc, const ShortBinZ& zb, int thr, int& prev,
+    __device__ __forceinline__ void Run(char* cntB, uint32_t zoff, uint32_t inc, const hw_shortbin_z& zb, int thr, int& prev,
                                         bool& bl1, bool& bl2, int& pushes, uint32_t& lowOut, uint32_t& highOut)
     {
         if (full > 0) {
@@ -720,7 +682,7 @@ __global__ __launch_bounds__(64 * W) void HistAddFullKernel(hwpa_lane_hist* hist
         __builtin_amdgcn_s_waitcnt(0x0F70);      // the primed trace loads land before the wave ends
         return;
     }
-    const ShortBinZ zb = MakeShortBinZ(lo, bs);
+    const hw_shortbin_z zb = hw_shortbin_make(lo, bs);
     const int thr = hw_wrap16(hw_round_i32(2.0f * hw_sqrtf(bvar) + bmean));
     const uint32_t zoff = (uint32_t)(z & ~1) * 2u;
     const uint32_t inc = 1u << (16u * (uint32_t)(z & 1));

@@ -423,4 +423,49 @@ HWPA_HD float hw_bf2f(uint16_t b) { return hw_u2f((uint32_t)b << 16); }
 /* Round a float through binary16 and back. */
 HWPA_HD float hw_half_round(float f) { return hw_h2f(hw_f2h(f)); }
 
+/* Histogram bin without a divide (TraceHistogramAccumHost::AddBlock's
+ * floorCastInt((x - lo) / binSize), dataTypes/UHistogramSimd.h:199-229, for the
+ * samples where it can be decided exactly).  The bin is floor(Q), Q = RN(a / bs),
+ * a = RN(x - lo).  With q = RN(a * RN(1 / bs)), within 2^-23 |a / bs| of a / bs,
+ * |q| < 2048, k = rint(q) and r = fma(-k, bs, a): for bs > 0, r has the sign of
+ * a - k bs exactly (the exact value is a multiple of the smaller of ulp(a) and
+ * ulp(bs), so it cannot round to 0 unless it is 0), and
+ *   r >= 0: a / bs >= k, so Q >= k, and a / bs < k + 0.501, so Q < k + 1: bin k;
+ *   r < 0:  a / bs < k, so Q <= k, and Q > k - 1: bin k - 1, unless Q rounded up
+ *           to k (a / bs within half an ulp of k below it, at most |k| 2^-24) or,
+ *           for k = 0, to -0 (|a / bs| < 2^-149).
+ * Both exceptions lie inside -r <= bs |k| 2^-22 + bs 2^-100.  hw_shortbin returns
+ * false for them, for bs <= 0 or non-finite (a NaN reciprocal fails |q| < 2048),
+ * for |q| >= 2048 or NaN and for bins outside [0, nbins): those samples need the
+ * divide.  The conditions combine bitwise, so the test has no branches. */
+typedef struct hw_shortbin_z {
+    float lo, bs, inv, bsc, bst;   /* inv = 1 / bs or NaN, bsc = bs 2^-22, bst = bs 2^-100 */
+} hw_shortbin_z;
+
+HWPA_HD hw_shortbin_z hw_shortbin_make(float lo, float bs)
+{
+    hw_shortbin_z zb;
+    zb.lo = lo;
+    zb.bs = bs;
+    zb.inv = (bs > 0.0f && bs < hw_u2f(0x7f800000u)) ? 1.0f / bs : hw_u2f(0x7fc00000u);
+    zb.bsc = bs * 2.384185791015625e-07f;
+    zb.bst = bs * 7.888609052210118e-31f;
+    return zb;
+}
+
+HWPA_HD bool hw_shortbin(int32_t x, const hw_shortbin_z& zb, int32_t nbins, int32_t* bin)
+{
+    const float a = (float)x - zb.lo;
+    const float q = a * zb.inv;
+    const float k = hw_rintf(q);
+    const float r = hw_fmaf(-k, zb.bs, a);
+    const float band = hw_fmaf(hw_fabsf(k), zb.bsc, zb.bst);
+    const bool inq = hw_fabsf(q) < 2048.0f;
+    const bool below = r < 0.0f;
+    const int32_t b = (int32_t)(inq ? k : 0.0f) - (below ? 1 : 0);
+    *bin = b;
+    const bool amb = below & (r >= -band);
+    return inq & !amb & ((uint32_t)b < (uint32_t)nbins);
+}
+
 #endif /* HWPA_MATH_H */

@@ -584,6 +584,27 @@ void HistAddLane(HistCtx& c, int l, const int16_t* tr, int frames, const hwpa_la
     }
 }
 
+// Checker of hw_shortbin against the divide of AddBlock (see oracle.h).
+static void ShortbinSweep(const float* lo, const float* bs, int32_t n, uint64_t out[3])
+{
+    uint64_t inRange = 0, decided = 0, wrong = 0;
+    for (int32_t c = 0; c < n; ++c) {
+        const hw_shortbin_z zb = hw_shortbin_make(lo[c], bs[c]);
+        for (int32_t x = -32768; x <= 32767; ++x) {
+            const int exact = hw_floor_i32(((float)x - lo[c]) / bs[c]);
+            int32_t b;
+            const bool ok = hw_shortbin(x, zb, NB, &b);
+            const bool in = exact >= 0 && exact < NB;
+            inRange += in ? 1u : 0u;
+            decided += (in && ok) ? 1u : 0u;
+            wrong += (ok && (!in || b != exact)) ? 1u : 0u;
+        }
+    }
+    out[0] = inRange;
+    out[1] = decided;
+    out[2] = wrong;
+}
+
 void HistGetLane(const HistCtx& c, int l, hwpa_lane_hist& o)
 {
     const LaneHistState& H = c.h[l];
@@ -1636,6 +1657,11 @@ void orc_hist_add(void* h, const int16_t* traces, int32_t frames, const hwpa_lan
 
 // Loads bounds, counts and outliers of each lane (the LaneHistogram view of
 // UHistogramSimd, dataTypes/UHistogramSimd.cpp:100-118); edge state is kept.
+void orc_shortbin_sweep(const float* lo, const float* bs, int32_t n, uint64_t out[3])
+{
+    ShortbinSweep(lo, bs, n, out);
+}
+
 void orc_hist_set(void* h, const hwpa_lane_hist* in)
 {
     auto* c = static_cast<HistCtx*>(h);

@@ -78,6 +78,12 @@ int32_t orc_hist_reset(void* h, const hwpa_baseliner_stats* stats);
 void orc_hist_add(void* h, const int16_t* traces, int32_t frames, const hwpa_lane_model* models);
 void orc_hist_get(void* h, hwpa_lane_hist* out);
 void orc_hist_set(void* h, const hwpa_lane_hist* in);
+/* Every int16 sample against n (low bound, bin size) pairs: hw_shortbin, the
+ * histogram kernels' divide-free bin, beside the reference's
+ * floor((x - lo) / binSize) with the IEEE divide.  out[0] = samples whose bin is
+ * in [0, 300), out[1] = those hw_shortbin decides, out[2] = samples it decides
+ * wrongly or decides although their bin is outside [0, 300). */
+void orc_shortbin_sweep(const float* lo, const float* bs, int32_t n, uint64_t out[3]);
 
 void orc_dme_init_models(const hwpa_config* cfg, int32_t lanes,
                          const hwpa_baseliner_stats* stats, hwpa_lane_model* out);
