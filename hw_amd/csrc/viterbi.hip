@@ -22,7 +22,8 @@
 namespace hwpa {
 
 struct TransVals {
-    float bu[5][9];   // rows B, TU, GU, CU, AU: candidates {B, TU..AU, TD..AD}
+    float bu[5][5];   // rows B, TU, GU, CU, AU: candidates {B, TU, GU, CU, AU}
+    float bd[5];      // the same rows: the one value shared by candidates TD, GD, CD, AD
     float fd[8][2];   // rows T, G, C, A, TD, GD, CD, AD: candidates {full_i, up_i}
 };
 
@@ -42,9 +43,11 @@ struct LabelerDev {
     uint32_t* tb = nullptr;  // [lane][maxFrames + 16 + NS][LS]: predecessor words, then the anchor row
 };
 
+// nhB = -1/(2 var_B), nhP = -1/(2 var_a): the reference's -0.5f * y * invVar * y
+// equals (y * (-0.5f * invVar)) * y bit for bit (scaling by -0.5 is exact here).
 struct ScorerZ {
-    float bgMean, bInvVar, bgFixed, x0;
-    float ffMean[NA], pInvVar[NA], ffFixed[NA], logPMean[NA], x1[NA];
+    float bgMean, nhB, bgFixed, x0;
+    float ffMean[NA], nhP[NA], ffFixed[NA], logPMean[NA], x1[NA];
     float dimMean, dimNorm, dimScale;
 };
 
@@ -58,7 +61,7 @@ __device__ void ScorerSetup(ScorerZ& s, const hwpa_lane_model& m, int z)
     const float bVar = hw_h2f(m.baseline.vars[z]);
     const float bSigma = hw_sqrtf(bVar);
     s.x0 = bMean + bSigma * sqrtHalfPi;
-    s.bInvVar = 1.0f / bVar;
+    s.nhB = -0.5f * (1.0f / bVar);
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
         const float aMean = hw_h2f(m.analogs[a].means[z]);
@@ -66,7 +69,7 @@ __device__ void ScorerSetup(ScorerZ& s, const hwpa_lane_model& m, int z)
         const float aVar = hw_h2f(m.analogs[a].vars[z]);
         const float aSigma = hw_sqrtf(aVar);
         s.x1[a] = aMean - aSigma * sqrtHalfPi;
-        s.pInvVar[a] = 1.0f / aVar;
+        s.nhP[a] = -0.5f * (1.0f / aVar);
     }
     {
         const float aMean = hw_h2f(m.analogs[3].means[z]);
@@ -94,21 +97,20 @@ __device__ __forceinline__ void Scores(const ScorerZ& s, float d, float* sc)
 {
     {
         const float y = d - s.bgMean;
-        sc[0] = -0.5f * y * s.bInvVar * y + s.bgFixed;
+        sc[0] = (y * s.nhB) * y + s.bgFixed;
     }
     float ff[NA];
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
         const float y = d - s.ffMean[i];
-        ff[i] = -0.5f * y * s.pInvVar[i] * y + s.ffFixed[i];
+        ff[i] = (y * s.nhP[i]) * y + s.ffFixed[i];
         sc[1 + i] = ff[i];
     }
     const bool lowExt = d < s.x0;
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
         const float cap = lowExt ? s.x0 : s.x1[i];
-        float fixedTerm = lowExt ? s.bInvVar : s.pInvVar[i];
-        fixedTerm = -0.5f * fixedTerm;
+        const float fixedTerm = lowExt ? s.nhB : s.nhP[i];
         const bool ext = lowExt || (d > s.x1[i]);
         const float dc = d - cap;
         float v = (ext ? fixedTerm * (dc * dc) : 0.0f) - s.logPMean[i];
@@ -123,29 +125,44 @@ __device__ __forceinline__ void Scores(const ScorerZ& s, float d, float* sc)
     }
 }
 
-// One Viterbi recursion step (FrameLabelerHost.cpp:213-264).
-__device__ __forceinline__ uint32_t Recur(const TransVals& T, const float* sc, float* ll, bool isRoi)
+// One Viterbi recursion step (FrameLabelerHost.cpp:213-264): (score + transition) +
+// previous log-likelihood per candidate, the later candidate winning ties.  In rows B
+// and TU..AU the four down candidates share one transition value, so score + value is
+// formed once per row (the same rounded sum the reference forms four times).
+// pen = 0 inside the ROI, +inf outside (every state but the baseline is excluded).
+__device__ __forceinline__ uint32_t Recur(const TransVals& T, const float* sc, float* ll, float pen)
 {
     float acc[NS];
     uint32_t word = 0;
-    // maxVal - penalty, the penalty +inf outside the ROI for every state but baseline
-    const float pen = isRoi ? 0.0f : hw_u2f(0x7f800000u);
-    // rows with candidates {B, TU, GU, CU, AU, TD, GD, CD, AD}
+    // The five rows' maximum chains advance together, one candidate at a time, so
+    // each compare has independent work behind it before its select.
+    float mx[5], sr[5];
+    uint32_t code[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        sr[q] = sc[(q == 0) ? 0 : 4 + q];
+        mx[q] = (sr[q] + T.bu[q][0]) + ll[0];
+        code[q] = 0;
+    }
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        if (p == 4) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) sr[q] = sr[q] + T.bd[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const float v = (p < 4) ? (sr[q] + T.bu[q][1 + p]) + ll[5 + p] : sr[q] + ll[5 + p];
+            const bool c = v >= mx[q];
+            mx[q] = c ? v : mx[q];
+            code[q] = c ? (uint32_t)(1 + p) : code[q];
+        }
+    }
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
         const int r = (q == 0) ? 0 : 4 + q;
-        const float s = sc[r];
-        float mx = (s + T.bu[q][0]) + ll[0];
-        uint32_t code = 0;
-#pragma unroll
-        for (int p = 0; p < 8; ++p) {
-            const float v = (s + T.bu[q][1 + p]) + ll[5 + p];
-            const bool c = v >= mx;
-            mx = c ? v : mx;
-            code = c ? (uint32_t)(1 + p) : code;
-        }
-        acc[r] = (r == 0) ? mx : (mx - pen);
-        word |= code << (4 * q);
+        acc[r] = (r == 0) ? mx[q] : (mx[q] - pen);
+        word |= code[q] << (4 * q);
     }
     // full rows 1..4 and down rows 9..12: candidates {full_i, up_i}
 #pragma unroll
@@ -156,8 +173,8 @@ __device__ __forceinline__ uint32_t Recur(const TransVals& T, const float* sc, f
         const float a = (s + T.fd[q][0]) + ll[1 + i];
         const float b = (s + T.fd[q][1]) + ll[5 + i];
         const bool c = b >= a;
-        const float mx = c ? b : a;
-        acc[r] = mx - pen;
+        const float m = c ? b : a;
+        acc[r] = m - pen;
         word |= (c ? 1u : 0u) << (20 + q);
     }
 #pragma unroll
@@ -177,7 +194,14 @@ __device__ __forceinline__ int Pred(uint32_t w, int state)
     return ((w >> (20 + q)) & 1u) ? 5 + i : 1 + i;
 }
 
-// PB = frames per prefetched batch (8 or 16; an ROI word holds 16 frames).
+// The ROI penalty of the frame whose final ROI bit is bit `pos` of `w`: 0 or +inf.
+__device__ __forceinline__ float RoiPen(uint32_t w, uint32_t pos)
+{
+    const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int32_t)w, pos, 1);   // 0 or all ones
+    return hw_u2f(~m & 0x7f800000u);
+}
+
+// PB = frames per prefetched batch (8 or 16) in the recursion and the traceback.
 // MINW = waves per SIMD the register allocation must allow (1: unconstrained).
 template <int PB, int MINW>
 __global__ __launch_bounds__(64, MINW) void LabelKernel(const int16_t* __restrict__ traces, int frames,
@@ -187,6 +211,7 @@ __global__ __launch_bounds__(64, MINW) void LabelKernel(const int16_t* __restric
                                                   float* __restrict__ score, TransVals T, float roiUpper,
                                                   float roiLower)
 {
+    static_assert(PB % 8 == 0, "recursion batches cover whole 8-frame groups of an ROI word");
     extern __shared__ uint32_t roiWords[];  // [(nroi + 15) / 16][LS], 2 bits per frame
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
@@ -205,68 +230,63 @@ __global__ __launch_bounds__(64, MINW) void LabelKernel(const int16_t* __restric
     const float inv2 = 1.0f / hw_sqrtf(hw_h2f(model.baseline.vars[z]));
     // Frame k of the latent ++ chunk sequence (k < NL: the carried latent frames).
     auto rawAt = [&](int k) -> int { return (k < NL) ? (int)st.prevLat[k][z] : (int)tr[(size_t)(k - NL) * LS]; };
-    // Frames are read in batches of PB with the next batch in flight; indices
-    // past `limit` re-read the last frame (their values are never used).
-    auto loadBatch = [&](int base, int limit, int (&dst)[PB]) {
-#pragma unroll
-        for (int u = 0; u < PB; ++u) {
-            const int k = (base + u < limit) ? base + u : limit - 1;
-            dst[u] = rawAt(k);
-        }
-    };
 
-    // ROI forward pass (RoiHost.h:137-193) over k < nroi, reading frame k + 2.
+    // ROI forward pass (RoiHost.h:137-193) over k < nroi, reading frame k + 2, one
+    // 16-frame word at a time with the next word's frames in flight; indices past
+    // the sequence re-read its last frame (their values are never used).
     {
+        const float kS2 = 1.41421353816986083984375f;   // sqrt(2.0f)
+        const float kS3 = 1.73205077648162841796875f;   // sqrt(3.0f)
+        const float kR2 = 1.0f / kS2, kR3 = 1.0f / kS3;
         float p0 = (float)st.latentTrc[z] * inv1, p1 = (float)rawAt(0) * inv1, p2 = (float)rawAt(1) * inv1;
         uint32_t prevRoi = st.roiBoundary[z];
-        const float s2 = hw_sqrtf(2.0f), s3 = hw_sqrtf(3.0f);
         const int limit = nroi + 2;
-        int cur[PB], nxt[PB];
-        loadBatch(2, limit, cur);
-        uint32_t word = 0;
-        for (int k0 = 0; k0 < nroi; k0 += PB) {
-            loadBatch(k0 + PB + 2, limit, nxt);
+        auto load16 = [&](int base, int (&dst)[16]) {
 #pragma unroll
-            for (int u = 0; u < PB; ++u) {
-                const int k = k0 + u;
-                if (k < nroi) {
-                    const float nv = (float)cur[u] * ((k + 2 < NL) ? inv1 : inv2);
-                    float mval = p1;
-                    mval = hw_maxf(mval, (p0 + p1) / s2);
-                    mval = hw_maxf(mval, (p0 + p1 + p2) / s3);
-                    mval = hw_maxf(mval, (p0 + p1 + p2 + nv) / 2.0f);
-                    const float thresh = (prevRoi & 1u) ? roiLower : roiUpper;
-                    const uint32_t v = (mval >= thresh ? 1u : 0u) | (mval >= roiLower ? 2u : 0u);
-                    word |= v << (2 * (k & 15));
-                    if ((k & 15) == 15 || k == nroi - 1) {
-                        roiWords[(k >> 4) * LS + z] = word;
-                        word = 0;
-                    }
-                    prevRoi = v;
-                    p0 = p1;
-                    p1 = p2;
-                    p2 = nv;
-                }
-            }
+            for (int u = 0; u < 16; ++u) dst[u] = rawAt(min(base + u, limit - 1));
+        };
+        auto frame = [&](int k, int raw) -> uint32_t {
+            const float nv = (float)raw * ((k + 2 < NL) ? inv1 : inv2);
+            const float a2 = p0 + p1;
+            const float a3 = a2 + p2;
+            const float a4 = a3 + nv;
+            float mval = hw_maxf(p1, hw_div_by_const(a2, kS2, kR2));
+            mval = hw_maxf(mval, hw_div_by_const(a3, kS3, kR3));
+            mval = hw_maxf(mval, a4 * 0.5f);
+            const float thresh = (prevRoi & 1u) ? roiLower : roiUpper;
+            const uint32_t v = (mval >= thresh ? 1u : 0u) | (mval >= roiLower ? 2u : 0u);
+            prevRoi = v;
+            p0 = p1;
+            p1 = p2;
+            p2 = nv;
+            return v;
+        };
+        int cur[16], nxt[16];
+        load16(2, cur);
+        const int full = nroi / 16;
+        for (int wi = 0; wi < full; ++wi) {
+            load16(16 * wi + 18, nxt);
+            uint32_t word = 0;
 #pragma unroll
-            for (int u = 0; u < PB; ++u) cur[u] = nxt[u];
+            for (int u = 0; u < 16; ++u) word |= frame(16 * wi + u, cur[u]) << (2 * u);
+            roiWords[wi * LS + z] = word;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
+        }
+        if (full < nwords) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (16 * full + u < nroi) word |= frame(16 * full + u, cur[u]) << (2 * u);
+            roiWords[full * LS + z] = word;
         }
     }
-    // ROI backward pass (RoiHost.h:195-243): final ROI bit into bit 0 of each field
+    // ROI backward pass (RoiHost.h:195-243), a word at a time: the final ROI bit of
+    // each frame into bit 0 of its field (fields past the last frame stay zero).
     {
-        uint32_t pr = 0;
-        for (int wi = nwords - 1; wi >= 0; --wi) {
-            uint32_t word = roiWords[wi * LS + z];
-            for (int k = 15; k >= 0; --k) {
-                const int idx = wi * 16 + k;
-                if (idx >= nroi) continue;
-                const uint32_t r = (word >> (2 * k)) & 3u;
-                const bool b = (r & 1u) || ((r & 2u) && pr != 0);
-                pr = b ? 1u : 0u;
-                word = (word & ~(3u << (2 * k))) | (pr << (2 * k));
-            }
-            roiWords[wi * LS + z] = word;
-        }
+        uint32_t carry = 0;
+        for (int wi = nwords - 1; wi >= 0; --wi)
+            roiWords[wi * LS + z] = hw_roi_backward_word(roiWords[wi * LS + z], &carry);
     }
 
     // Forward recursion over latent, chunk and stitch frames.
@@ -275,37 +295,60 @@ __global__ __launch_bounds__(64, MINW) void LabelKernel(const int16_t* __restric
 #pragma unroll
     for (int s = 0; s < NS; ++s) ll[s] = (lb == s) ? 0.0f : hw_u2f(0xff800000u);
     ScorerZ sc;
-    // Log-likelihoods at the chunk's last frame, read back once the anchor state is known:
-    // kept in the last NS rows of the lane's traceback buffer (13 registers fewer through
-    // the recursion, no LDS).
-    uint32_t* anchorLL = tb + (size_t)(tbFrames - NS) * LS;
-    auto step = [&](int f, int raw) {
-        const float d = (float)raw;
-        const bool isRoi = ((roiWords[(f >> 4) * LS + z] >> (2 * (f & 15))) & 1u) != 0;
+    auto step = [&](int f, int raw, uint32_t rw) {
         float s[NS];
-        Scores(sc, d, s);
-        tb[(size_t)f * LS] = Recur(T, s, ll, isRoi);
-        if (f == frames - 1) {
-#pragma unroll
-            for (int r = 0; r < NS; ++r) anchorLL[(size_t)r * LS] = hw_f2u(ll[r]);
-        }
+        Scores(sc, (float)raw, s);
+        tb[(size_t)f * LS] = Recur(T, s, ll, RoiPen(rw, 2u * (uint32_t)(f & 15)));
     };
     // The latent frames are scored with the previous chunk's model, the chunk
     // and stitch frames with the current one.
     ScorerSetup(sc, st.oldModel, z);
-    for (int f = 0; f < NL; ++f) step(f, (int)st.prevLat[f][z]);
+    for (int f = 0; f < NL; ++f) step(f, (int)st.prevLat[f][z], roiWords[(f >> 4) * LS + z]);
+    // Log-likelihoods at the chunk's last frame, read back once the anchor state is known:
+    // kept in the last NS rows of the lane's traceback buffer (13 registers fewer through
+    // the stitch frames, no LDS).
+    uint32_t* anchorLL = tb + (size_t)(tbFrames - NS) * LS;
+    auto storeAnchor = [&]() {
+#pragma unroll
+        for (int r = 0; r < NS; ++r) anchorLL[(size_t)r * LS] = hw_f2u(ll[r]);
+    };
     ScorerSetup(sc, model, z);
     {
-        int cur[PB], nxt[PB];
-        loadBatch(NL, nlab, cur);
-        for (int f0 = NL; f0 < nlab; f0 += PB) {
-            loadBatch(f0 + PB, nlab, nxt);
+        // Chunk frames f in [NL, frames) in batches of PB (f0 a multiple of 8, so each
+        // group of 8 frames reads one ROI word) with the next batch in flight, then
+        // the remainder; chunk sample j = f - NL.
+        const int last = nlab - NL - 1;   // last chunk sample the recursion reads
+        auto loadBatch = [&](int j0, int (&dst)[PB]) {
 #pragma unroll
-            for (int u = 0; u < PB; ++u) {
-                if (f0 + u < nlab) step(f0 + u, cur[u]);
+            for (int u = 0; u < PB; ++u) dst[u] = (int)tr[(size_t)min(j0 + u, last) * LS];
+        };
+        const int nb = (frames - NL) / PB;
+        int cur[PB], nxt[PB];
+        loadBatch(0, cur);
+        for (int b = 0; b < nb; ++b) {
+            const int f0 = NL + b * PB;
+            loadBatch((b + 1) * PB, nxt);
+#pragma unroll
+            for (int g = 0; g < PB / 8; ++g) {
+                const int fg = f0 + 8 * g;
+                const uint32_t rw = roiWords[(fg >> 4) * LS + z];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) step(fg + u, cur[8 * g + u], rw);
             }
 #pragma unroll
             for (int u = 0; u < PB; ++u) cur[u] = nxt[u];
+        }
+        // The remainder of the chunk frames, the anchor, then the stitch frames f in
+        // [frames, nlab) (chunk samples frames - NL .. last), one frame at a time with
+        // the next sample in flight.
+        const int f1 = NL + nb * PB;
+        int raw = (int)tr[(size_t)(f1 - NL) * LS];
+#pragma unroll 1
+        for (int f = f1; f < nlab; ++f) {
+            const int nextRaw = (int)tr[(size_t)min(f + 1 - NL, last) * LS];
+            if (f == frames) storeAnchor();
+            step(f, raw, roiWords[(f >> 4) * LS + z]);
+            raw = nextRaw;
         }
     }
 
@@ -350,31 +393,33 @@ __global__ __launch_bounds__(64, MINW) void LabelKernel(const int16_t* __restric
     const float sv = hw_u2f(anchorLL[(size_t)anchorState * LS]);
     score[(size_t)lane * LS + z] = sv;
 
-    // Traceback, PB predecessor words per batch with the next batch in flight.
+    // Traceback from the anchor, PB predecessor words per batch with the next batch in
+    // flight, then the remainder.
     int16_t* lo = labels + (size_t)lane * frames * LS + z;
     int stt = anchorState;
     {
         auto loadTb = [&](int top, uint32_t (&dst)[PB]) {
 #pragma unroll
-            for (int u = 0; u < PB; ++u) {
-                const int f = (top - u >= 0) ? top - u : 0;
-                dst[u] = tb[(size_t)f * LS];
-            }
+            for (int u = 0; u < PB; ++u) dst[u] = tb[(size_t)max(top - u, 0) * LS];
         };
+        const int nb = frames / PB;
         uint32_t cur[PB], nxt[PB];
         loadTb(frames - 1, cur);
-        for (int top = frames - 1; top >= 0; top -= PB) {
+        for (int b = 0; b < nb; ++b) {
+            const int top = frames - 1 - b * PB;
             loadTb(top - PB, nxt);
 #pragma unroll
             for (int u = 0; u < PB; ++u) {
-                const int f = top - u;
-                if (f >= 0) {
-                    lo[(size_t)f * LS] = (int16_t)stt;
-                    stt = Pred(cur[u], stt);
-                }
+                lo[(size_t)(top - u) * LS] = (int16_t)stt;
+                stt = Pred(cur[u], stt);
             }
 #pragma unroll
             for (int u = 0; u < PB; ++u) cur[u] = nxt[u];
+        }
+#pragma unroll 1
+        for (int f = frames - 1 - nb * PB; f >= 0; --f) {
+            lo[(size_t)f * LS] = (int16_t)stt;
+            stt = Pred(tb[(size_t)f * LS], stt);
         }
     }
 
@@ -501,14 +546,20 @@ void BuildTransition(const hwpa_config& cfg, hwpa_transition& T)
         for (int j = 0; j < NS; ++j) T.logp[i][j] = (data[i][j] > 0.0f) ? std::log(data[i][j]) : 0.0f;
 }
 
-static TransVals PackTransition(const hwpa_transition& T)
+// Packs the transition matrix into the kernel's candidate order.  Fails when the
+// four down candidates of a row B / TU..AU do not share one value (the matrix the
+// reference builds always has that form: SubframeScorer.cpp:173-200).
+static bool PackTransition(const hwpa_transition& T, TransVals& v)
 {
-    TransVals v{};
+    v = TransVals{};
     const int buRows[5] = {0, 5, 6, 7, 8};
     for (int q = 0; q < 5; ++q) {
         const int r = buRows[q];
         v.bu[q][0] = T.logp[r][0];
-        for (int p = 0; p < 8; ++p) v.bu[q][1 + p] = T.logp[r][5 + p];
+        for (int p = 0; p < 4; ++p) v.bu[q][1 + p] = T.logp[r][5 + p];
+        v.bd[q] = T.logp[r][9];
+        for (int p = 10; p < 13; ++p)
+            if (hw_f2u(T.logp[r][p]) != hw_f2u(v.bd[q])) return false;
     }
     for (int q = 0; q < 8; ++q) {
         const int i = q & 3;
@@ -516,16 +567,19 @@ static TransVals PackTransition(const hwpa_transition& T)
         v.fd[q][0] = T.logp[r][1 + i];
         v.fd[q][1] = T.logp[r][5 + i];
     }
-    return v;
+    return true;
 }
 
 int32_t LabelerCreate(const hwpa_config& cfg, uint32_t lanes, uint32_t max_frames, LabelerDev** out)
 {
     if (max_frames < (uint32_t)NL) return Fail(HWPA_ERR_INVALID_ARG, "labeler: max_frames must be >= 24");
-    auto* l = new LabelerDev;
     hwpa_transition T;
     BuildTransition(cfg, T);
-    l->T = PackTransition(T);
+    TransVals packed;
+    if (!PackTransition(T, packed))
+        return Fail(HWPA_ERR_INVALID_ARG, "labeler: transition rows do not share their down-state values");
+    auto* l = new LabelerDev;
+    l->T = packed;
     l->roiUpper = cfg.roi_upper_threshold;
     l->roiLower = cfg.roi_lower_threshold;
     l->lanes = lanes;

@@ -468,4 +468,54 @@ HWPA_HD bool hw_shortbin(int32_t x, const hw_shortbin_z& zb, int32_t nbins, int3
     return inq & !amb & ((uint32_t)b < (uint32_t)nbins);
 }
 
+/* x / d for a constant divisor d with rd = RN(1 / d), without a divide: the
+ * product q0 = RN(x rd), its exact residual x - q0 d (one fused multiply-add)
+ * and one correction step q0 + RN(residual) rd (Markstein's theorem for a
+ * correctly rounded reciprocal).  For d = RN(sqrt 2) and RN(sqrt 3) (the ROI
+ * filter's divisors) the result equals the IEEE quotient for every finite x with
+ * |x| >= 2^-100, for every zero up to its sign, and for infinities and NaN (the
+ * final select); below 2^-100 the residual is no longer exact.  The ROI sums it
+ * serves are 0 or at least 2^-31 in magnitude (int16 samples times 1/sigma with
+ * sigma^2 a finite half). */
+HWPA_HD float hw_div_by_const(float x, float d, float rd)
+{
+    const float q0 = x * rd;
+    const float e = hw_fmaf(-q0, d, x);
+    const float q1 = hw_fmaf(e, rd, q0);
+    return (hw_fabsf(q0) < hw_u2f(0x7f800000u)) ? q1 : q0;
+}
+
+/* The ROI backward recursion over one word of 16 two-bit fields (field k in
+ * bits 2k, 2k+1: the forward pass's roi and mid bits), from field 15 down to
+ * field 0: final_k = roi_k | (mid_k & final_{k+1}), with final_16 = *carry.
+ * Bit-reversed, the recurrence runs towards higher bits and is the carry chain
+ * of one addition: roi generates, mid propagates, and the bit between two
+ * fields passes the carry on.  Returns final_k in bit 2k (odd bits zero);
+ * *carry becomes final_0.  Fields holding 0 give 0 and pass on no carry. */
+HWPA_HD uint32_t hw_bitrev32(uint32_t v)
+{
+#if defined(__clang__)
+    return __builtin_bitreverse32(v);
+#else
+    v = ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
+    v = ((v >> 2) & 0x33333333u) | ((v & 0x33333333u) << 2);
+    v = ((v >> 4) & 0x0F0F0F0Fu) | ((v & 0x0F0F0F0Fu) << 4);
+    v = ((v >> 8) & 0x00FF00FFu) | ((v & 0x00FF00FFu) << 8);
+    return (v >> 16) | (v << 16);
+#endif
+}
+
+HWPA_HD uint32_t hw_roi_backward_word(uint32_t w, uint32_t* carry)
+{
+    const uint32_t rv = hw_bitrev32(w);               /* roi_k at 31-2k, mid_k at 30-2k */
+    const uint32_t g = rv & 0xAAAAAAAAu;
+    const uint32_t p = (rv << 1) & 0xAAAAAAAAu;
+    const uint32_t a = g | p | 0x55555555u;
+    const uint32_t s = a + g + *carry;                /* the carry enters at bit 0 */
+    const uint32_t cin = s ^ a ^ g;                   /* carry into every bit */
+    const uint32_t out = hw_bitrev32(g | (p & cin));
+    *carry = out & 1u;
+    return out;
+}
+
 #endif /* HWPA_MATH_H */

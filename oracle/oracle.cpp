@@ -1662,6 +1662,39 @@ void orc_shortbin_sweep(const float* lo, const float* bs, int32_t n, uint64_t ou
     ShortbinSweep(lo, bs, n, out);
 }
 
+void orc_div_by_const_sweep(float d, const int32_t* exps, int32_t n, uint64_t out[2])
+{
+    const float rd = 1.0f / d;
+    uint64_t checked = 0, wrong = 0;
+    auto check = [&](float x) {
+        const float q = x / d;
+        const float c = hw_div_by_const(x, d, rd);
+        const bool same = (q == c) || (hw_isnan(q) && hw_isnan(c));
+        const bool sameSign = hw_isnan(q) || q == 0.0f || (hw_f2u(q) == hw_f2u(c));
+        ++checked;
+        wrong += (same && sameSign) ? 0u : 1u;
+    };
+    for (int32_t i = 0; i < n; ++i)
+        for (uint32_t sign = 0; sign < 2; ++sign)
+            for (uint32_t m = 0; m < (1u << 23); ++m)
+                check(hw_u2f((sign << 31) | ((uint32_t)exps[i] << 23) | m));
+    const uint32_t specials[] = {0x00000000u, 0x80000000u, 0x7f800000u, 0xff800000u, 0x7fc00000u, 0x7f7fffffu,
+                                 0xff7fffffu};
+    for (uint32_t u : specials) check(hw_u2f(u));
+    out[0] = checked;
+    out[1] = wrong;
+}
+
+void orc_roi_backward_words(const uint32_t* words, const uint32_t* carries, int32_t n, uint32_t* out_words,
+                            uint32_t* out_carries)
+{
+    for (int32_t i = 0; i < n; ++i) {
+        uint32_t c = carries[i];
+        out_words[i] = hw_roi_backward_word(words[i], &c);
+        out_carries[i] = c;
+    }
+}
+
 void orc_hist_set(void* h, const hwpa_lane_hist* in)
 {
     auto* c = static_cast<HistCtx*>(h);

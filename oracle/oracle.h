@@ -84,6 +84,15 @@ void orc_hist_set(void* h, const hwpa_lane_hist* in);
  * in [0, 300), out[1] = those hw_shortbin decides, out[2] = samples it decides
  * wrongly or decides although their bin is outside [0, 300). */
 void orc_shortbin_sweep(const float* lo, const float* bs, int32_t n, uint64_t out[3]);
+/* Every float with a biased exponent in exps[0..n) (both signs, all 2^23
+ * mantissas) and the special values: hw_div_by_const(x, d, 1/d), the labeler's
+ * divide-free quotient, against the IEEE quotient x / d, zeros compared up to
+ * sign.  out[0] = values checked, out[1] = values where they differ. */
+void orc_div_by_const_sweep(float d, const int32_t* exps, int32_t n, uint64_t out[2]);
+/* hw_roi_backward_word, the labeler's word-parallel ROI backward pass, over n
+ * (word, carry) pairs; the checker is the oracle's per-frame loop (RoiZ). */
+void orc_roi_backward_words(const uint32_t* words, const uint32_t* carries, int32_t n, uint32_t* out_words,
+                            uint32_t* out_carries);
 
 void orc_dme_init_models(const hwpa_config* cfg, int32_t lanes,
                          const hwpa_baseliner_stats* stats, hwpa_lane_model* out);
