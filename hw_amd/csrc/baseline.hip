@@ -225,11 +225,11 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
     static_assert(PD == 1 || PD == 2, "prefetch depth");
     int rv[S], pv[S];
 #pragma unroll
-    for (int j = 0; j < S; ++j) rv[j] = rl[(size_t)j * LS];
+    for (int j = 0; j < S; ++j) rv[j] = __builtin_nontemporal_load(&rl[(size_t)j * LS]);
     if (PD == 2) {
         const int g1 = (groups > 1) ? 1 : 0;
 #pragma unroll
-        for (int j = 0; j < S; ++j) pv[j] = (rl + (size_t)g1 * S * LS)[j * LS];
+        for (int j = 0; j < S; ++j) pv[j] = __builtin_nontemporal_load(&(rl + (size_t)g1 * S * LS)[j * LS]);
     }
 
     for (int g = 0; g < groups; ++g) {
@@ -243,7 +243,8 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
         int nv[S];
         const int gn = (g + PD < groups) ? g + PD : groups - 1;
 #pragma unroll
-        for (int j = 0; j < S; ++j) nv[j] = (rl + (size_t)gn * S * LS)[j * LS];   // group base + immediate offsets
+        for (int j = 0; j < S; ++j)   // group base + immediate offsets; the raw trace is read once
+            nv[j] = __builtin_nontemporal_load(&(rl + (size_t)gn * S * LS)[j * LS]);
 
         // Stage 0 on (x, ~x): erode pair, then dilate pair -> (lower, ~upper), in two halves of
         // the group (the same window extrema, with about half the live registers).
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
         for (int j = 0; j < S; ++j) {
             const float rawScaled = (float)hw_wrap16(rv[j] - k.pedestal) * k.scaler;
             const int bls = hw_round_i16(rawScaled - subBl);
-            (ol + (size_t)g * S * LS)[j * LS] = (int16_t)bls;
+            (ol + (size_t)g * S * LS)[j * LS] = (int16_t)bls;   // cached: the histogram reads it next
             const int rr = hw_round_i16(rawScaled);
             const bool maskHp1 = (float)bls < thr;
             const bool m = latH1 && latL && maskHp1;

@@ -319,7 +319,9 @@ __global__ __launch_bounds__(256) void MeanKernel(const float* __restrict__ v, u
 }
 
 // Column sums of a [rows][cols] matrix (bf16 or fp32), two fixed-order passes.
-constexpr int COLSUM_ROWS = 256;
+// Rows per partial sum of the bias-gradient column sums: batch / 32 workgroups, each
+// thread with its 32 loads in flight; the partials are then added in slice order.
+constexpr int COLSUM_ROWS = 32;
 template <typename T>
 __device__ __forceinline__ float LoadVal(const T* p, size_t i);
 template <>
@@ -335,8 +337,13 @@ __global__ __launch_bounds__(256) void ColSumPartKernel(const T* __restrict__ d,
     if (c >= cols) return;
     const uint32_t r0 = blockIdx.x * COLSUM_ROWS;
     const uint32_t r1 = min(rows, r0 + COLSUM_ROWS);
+    // All loads of the slice are issued before the (row-ordered) adds.
+    float v[COLSUM_ROWS];
+#pragma unroll
+    for (int i = 0; i < COLSUM_ROWS; ++i) v[i] = (r0 + i < r1) ? LoadVal<T>(d, (size_t)(r0 + i) * cols + c) : 0.0f;
     float s = 0.0f;
-    for (uint32_t r = r0; r < r1; ++r) s += LoadVal<T>(d, (size_t)r * cols + c);
+#pragma unroll
+    for (int i = 0; i < COLSUM_ROWS; ++i) s += v[i];
     part[(size_t)blockIdx.x * cols + c] = s;
 }
 
@@ -345,8 +352,15 @@ __global__ __launch_bounds__(256) void ColSumFinalKernel(const float* __restrict
 {
     const uint32_t c = blockIdx.x * 256 + threadIdx.x;
     if (c >= cols) return;
+    // Partials in slice order, 16 loads in flight at a time.
     float s = 0.0f;
-    for (uint32_t k = 0; k < chunks; ++k) s += part[(size_t)k * cols + c];
+    for (uint32_t k0 = 0; k0 < chunks; k0 += 16) {
+        float v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = (k0 + i < chunks) ? part[(size_t)(k0 + i) * cols + c] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) s += v[i];
+    }
     out[c] = s;
 }
 
