@@ -178,6 +178,31 @@ def dev_bytes(nbytes):
     return DevBuf(nbytes)
 
 
+class HostBuf:
+    """Page-locked host memory (hwpa_host_alloc); `array(dtype)` views it."""
+
+    def __init__(self, nbytes):
+        self.nbytes = max(int(nbytes), 1)
+        self.p = _vp()
+        check(lib.hwpa_host_alloc(ctypes.byref(self.p), ctypes.c_size_t(self.nbytes)))
+
+    def data_ptr(self):
+        return self.p.value
+
+    def array(self, dtype=np.int16):
+        n = self.nbytes // np.dtype(dtype).itemsize
+        return np.ctypeslib.as_array((ctypes.c_uint8 * (n * np.dtype(dtype).itemsize)).from_address(self.p.value)).view(dtype)
+
+    def __del__(self):
+        if getattr(self, "p", None) is not None and self.p.value:
+            lib.hwpa_host_free(self.p)
+            self.p = None
+
+
+def host_pinned(nbytes):
+    return HostBuf(nbytes)
+
+
 def dev_from_numpy(a):
     a = np.ascontiguousarray(a)
     return DevBuf(a.nbytes).upload(a)
@@ -572,7 +597,10 @@ class BatchAnalyzer:
     def __call__(self, traces, first_frame, pool_id=None, on_device=None, encoding=None):
         """traces: numpy int16 or uint8 [lanes, frames, 64] (host) or a device
         buffer (anything with data_ptr()) already in HBM; `encoding` (ENCODING_*)
-        is taken from a numpy array's dtype, and defaults to int16 for device buffers."""
+        is taken from a numpy array's dtype, and defaults to int16 for device buffers.
+        on_device=False marks a data_ptr() buffer as host memory (a pinned tensor):
+        the chunk is copied in, overlapping the previous chunk's work, and the call
+        returns once the copy is done."""
         pid = self.pool_id if pool_id is None else pool_id
         if isinstance(traces, np.ndarray):
             enc = T.ENCODING_UINT8 if traces.dtype == np.uint8 else T.ENCODING_INT16
@@ -584,7 +612,8 @@ class BatchAnalyzer:
                                                    ctypes.c_int32(first_frame))
         else:
             enc = T.ENCODING_INT16 if encoding is None else encoding
-            rc = lib.hwpa_analyzer_process_encoded(self.h, ptr(traces), ctypes.c_int32(enc), ctypes.c_int32(1),
+            dev = 1 if on_device is None or on_device else 0
+            rc = lib.hwpa_analyzer_process_encoded(self.h, ptr(traces), ctypes.c_int32(enc), ctypes.c_int32(dev),
                                                    ctypes.c_uint32(pid), ctypes.c_int32(first_frame))
         check(rc)
         return self

@@ -90,6 +90,14 @@ struct hwpa_analyzer {
     Interval aggFi, histFi, modelFi;
     int32_t flags[4] = {0, 0, 0, 0};
     hipStream_t stream = nullptr;
+    // Host-buffer input: chunks are copied on their own stream into two staging buffers in
+    // turn, so the copy of one chunk runs while the previous chunk is processed.
+    hipStream_t copyStream = nullptr;
+    int16_t* dTraces2 = nullptr;
+    hipEvent_t copied[2] = {}, bufFree[2] = {};
+    bool bufUsed[2] = {};
+    int copyBuf = 0;
+    int pendingCopy = -1;   // staging buffer whose copy the current call waits for before returning
     bool profiling = false;
     hipEvent_t ev[kNumStages + 1] = {};
     bool stageRan[kNumStages] = {};
@@ -130,6 +138,7 @@ struct hwpa_analyzer {
 static void FreeAnalyzer(hwpa_analyzer* a)
 {
     if (!a) return;
+    if (a->copyStream) (void)hipStreamSynchronize(a->copyStream);
     if (a->stream) (void)hipStreamSynchronize(a->stream);
     BaselinerDestroy(a->bl);
     LabelerDestroy(a->lab);
@@ -143,6 +152,12 @@ static void FreeAnalyzer(hwpa_analyzer* a)
     DeviceFree(a->dDiag);
     for (auto& e : a->ev)
         if (e) (void)hipEventDestroy(e);
+    DeviceFree(a->dTraces2);
+    for (int i = 0; i < 2; ++i) {
+        if (a->copied[i]) (void)hipEventDestroy(a->copied[i]);
+        if (a->bufFree[i]) (void)hipEventDestroy(a->bufFree[i]);
+    }
+    if (a->copyStream) (void)hipStreamDestroy(a->copyStream);
     if (a->stream) (void)hipStreamDestroy(a->stream);
     delete a;
 }
@@ -328,11 +343,31 @@ int32_t hwpa_analyzer_destroy(hwpa_analyzer* a)
 }
 
 
+// The host-input copy path, set up on the first chunk handed over in host memory: the copy
+// stream (default priority), the second staging buffer and the events that order them.
+static int32_t EnsureCopyPath(hwpa_analyzer* a)
+{
+    if (a->copyStream) return HWPA_OK;
+    const size_t bytes = (size_t)a->lanes * a->frames * LS * 2;
+    int32_t rc = DeviceAlloc((void**)&a->dTraces2, bytes, "traces (second staging buffer)");
+    if (rc != HWPA_OK) return rc;
+    for (int i = 0; i < 2; ++i) {
+        if (hipEventCreateWithFlags(&a->copied[i], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&a->bufFree[i], hipEventDisableTiming) != hipSuccess)
+            return Fail(HWPA_ERR_HIP, "hipEventCreate failed");
+    }
+    if (hipStreamCreateWithFlags(&a->copyStream, hipStreamNonBlocking) != hipSuccess) {
+        a->copyStream = nullptr;
+        return Fail(HWPA_ERR_HIP, "hipStreamCreate failed");
+    }
+    return HWPA_OK;
+}
+
 // BatchAnalyzer::operator()(const TraceBatchVariant&) (analyzer/BatchAnalyzer.cpp:149-170) and
 // the AnalyzeImpl of the configured mode: FixedModel (:172-205), SingleEstimate (:207-283),
 // DynamicEstimate (:288-373).
-static int32_t AnalyzerProcess(hwpa_analyzer* a, const void* traces, int32_t encoding, int32_t on_device,
-                               uint32_t pool_id, int32_t first_frame)
+static int32_t AnalyzerProcessImpl(hwpa_analyzer* a, const void* traces, int32_t encoding, int32_t on_device,
+                                   uint32_t pool_id, int32_t first_frame)
 {
     if (!a || !traces) return Fail(HWPA_ERR_INVALID_ARG, "analyzer_process: null argument");
     if (encoding != HWPA_ENCODING_INT16 && encoding != HWPA_ENCODING_UINT8)
@@ -351,13 +386,30 @@ static int32_t AnalyzerProcess(hwpa_analyzer* a, const void* traces, int32_t enc
     if (encoding == HWPA_ENCODING_UINT8) a->plan |= kPlanUint8;
 
     const void* src = traces;
+    int buf = -1;
     if (!on_device) {
-        HWPA_HIP(hipMemcpyAsync(a->dTraces, traces, traceBytes, hipMemcpyHostToDevice, s));
-        src = a->dTraces;
+        // Copy into the staging buffer the chunk before last used, once its baseline pass is
+        // done, on the copy stream, while the previous chunk may still be in flight on `s`;
+        // the chunk's stages are enqueued behind the copy, and the call returns once the copy
+        // is complete (the caller's buffer is free again: AnalyzerProcess).
+        if ((rc = EnsureCopyPath(a)) != HWPA_OK) return rc;
+        buf = a->copyBuf;
+        a->copyBuf ^= 1;
+        int16_t* dst = buf ? a->dTraces2 : a->dTraces;
+        if (a->bufUsed[buf]) HWPA_HIP(hipStreamWaitEvent(a->copyStream, a->bufFree[buf], 0));
+        HWPA_HIP(hipMemcpyAsync(dst, traces, traceBytes, hipMemcpyHostToDevice, a->copyStream));
+        HWPA_HIP(hipEventRecord(a->copied[buf], a->copyStream));
+        a->pendingCopy = buf;
+        HWPA_HIP(hipStreamWaitEvent(s, a->copied[buf], 0));
+        src = dst;
         a->plan |= kPlanCopyIn;
     }
     HWPA_STAGE_BEGIN(a, kStBaseline);
     if ((rc = BaselinerRun(a->bl, src, F, a->dBaselined, a->dStats, s, encoding)) != HWPA_OK) return rc;
+    if (buf >= 0) {
+        HWPA_HIP(hipEventRecord(a->bufFree[buf], s));
+        a->bufUsed[buf] = true;
+    }
     a->nextFrame = first_frame + (int32_t)F;
 
     auto finish = [&](int32_t r) -> int32_t {
@@ -441,6 +493,20 @@ static int32_t AnalyzerProcess(hwpa_analyzer* a, const void* traces, int32_t enc
         if ((rc = zeroCounts()) != HWPA_OK) return rc;
     }
     return finish(HWPA_OK);
+}
+
+// The host-input copy of this call (if any) is waited for on every return path, errors included.
+static int32_t AnalyzerProcess(hwpa_analyzer* a, const void* traces, int32_t encoding, int32_t on_device,
+                               uint32_t pool_id, int32_t first_frame)
+{
+    const int32_t rc = AnalyzerProcessImpl(a, traces, encoding, on_device, pool_id, first_frame);
+    if (a && a->pendingCopy >= 0) {
+        const int b = a->pendingCopy;
+        a->pendingCopy = -1;
+        if (hipEventSynchronize(a->copied[b]) != hipSuccess && rc == HWPA_OK)
+            return Fail(HWPA_ERR_HIP, "host chunk copy failed");
+    }
+    return rc;
 }
 
 int32_t hwpa_analyzer_process(hwpa_analyzer* a, const int16_t* traces, int32_t on_device, uint32_t pool_id,

@@ -711,6 +711,23 @@ int32_t hwpa_dev_free(void* d_ptr)
     return HWPA_OK;
 }
 
+int32_t hwpa_host_alloc(void** h_ptr, size_t bytes)
+{
+    if (!h_ptr) return Fail(HWPA_ERR_INVALID_ARG, "null argument");
+    *h_ptr = nullptr;
+    if (hipHostMalloc(h_ptr, bytes > 0 ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        *h_ptr = nullptr;
+        return Fail(HWPA_ERR_OOM, "hipHostMalloc failed (pinned host buffer)");
+    }
+    return HWPA_OK;
+}
+
+int32_t hwpa_host_free(void* h_ptr)
+{
+    if (h_ptr) HWPA_HIP(hipHostFree(h_ptr));
+    return HWPA_OK;
+}
+
 int32_t hwpa_memcpy_htod(void* d_dst, const void* host_src, size_t bytes)
 {
     if (!d_dst || !host_src) return Fail(HWPA_ERR_INVALID_ARG, "memcpy: null pointer");

@@ -390,6 +390,11 @@ int32_t hwpa_device_count(int32_t* count);
 int32_t hwpa_device_sync(void);
 int32_t hwpa_dev_alloc(void** d_ptr, size_t bytes);
 int32_t hwpa_dev_free(void* d_ptr);
+/* Page-locked host memory (hipHostMalloc): host chunks in it reach the device
+ * by DMA, so hwpa_analyzer_process with on_device = 0 overlaps their copy with
+ * the previous chunk's work. */
+int32_t hwpa_host_alloc(void** h_ptr, size_t bytes);
+int32_t hwpa_host_free(void* h_ptr);
 int32_t hwpa_memcpy_htod(void* d_dst, const void* host_src, size_t bytes);
 int32_t hwpa_memcpy_dtoh(void* host_dst, const void* d_src, size_t bytes);
 int32_t hwpa_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
