@@ -295,7 +295,7 @@ int32_t hwpa_config_default(hwpa_config* c)
     c->static_baseline_variance = 33.0f;
     c->learned_init = 0;
     c->learned_conf_threshold = 0.0f;
-    c->learned_em_iteration_limit = 3;
+    c->learned_em_iteration_limit = 5;
     c->learned_conv_tol = 2e-3f;
     c->learned_gate_threshold = 0.9f;
     // Movie description of the synthetic workload: 100 fps, 1 e-/DN, four analogs

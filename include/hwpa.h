@@ -295,14 +295,14 @@ int32_t hwpa_pulser_run(hwpa_pulser* p, const int16_t* d_labels, const int16_t* 
  * cfg->learned_gate_threshold skip the capped EM and go straight to the plain EM.  Matrices are bfloat16 for the matrix cores with fp32 accumulation;
  * an fp32 master copy and Adam moments live on the device.  The start replaces
  * the EM's (DmeEmHost.cpp:236-249 onwards); the EM then runs at most
- * cfg->learned_em_iteration_limit iterations with the convergence test
+ * cfg->learned_em_iteration_limit (default 5) iterations with the convergence test
  * delta LL < cfg->learned_conv_tol * |LL| (the plain EM uses 1e-4), and ZMWs
  * that did not converge (or whose model confidence is below
  * cfg->learned_conf_threshold, default 0) are re-estimated by the plain EM. */
 #define HWPA_MLP_IN 128
 #define HWPA_MLP_HIDDEN 256
 #define HWPA_MLP_OUT 12
-/* W1[256][128] b1[256] W2[256][256] b2[256] W3[11][256] b3[11] */
+/* W1[256][128] b1[256] W2[256][256] b2[256] W3[12][256] b3[12] */
 #define HWPA_MLP_PARAMS (HWPA_MLP_HIDDEN * HWPA_MLP_IN + HWPA_MLP_HIDDEN + HWPA_MLP_HIDDEN * HWPA_MLP_HIDDEN + \
                          HWPA_MLP_HIDDEN + HWPA_MLP_OUT * HWPA_MLP_HIDDEN + HWPA_MLP_OUT)
 #define HWPA_EM_DIAG_ROWS 18
