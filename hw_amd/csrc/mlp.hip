@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void GemmKernel(GemmArgs g, uint32_t nTiles, u
             g.Cf[c] = v + g.bias[n];
         } else if (EPI == EPI_BIAS_RELU_BF16) {
             const float z = v + g.bias[n];
-            g.Cb[c] = hw_f2bf(z > 0.0f ? z : 0.0f);
+            g.Cb[c] = hw_f2bf_nonnan(z > 0.0f ? z : 0.0f);
         } else {
             const uint16_t mk = g.mask[(int64_t)m * g.mk_m + (int64_t)n * g.mk_n];
             g.Cb[c] = PositiveBf16(mk) ? hw_f2bf(v) : (uint16_t)0;
@@ -499,12 +499,12 @@ static int32_t ColSum(const void* d, bool bf16, uint32_t rows, uint32_t cols, fl
 // Fused forward pass: 32 rows per wave, kFusedWaves (8) waves per workgroup.  Each layer is the
 // 32x32x16 bf16 MFMA over 32-column tiles with k ascending -- the arithmetic of
 // GemmKernel, so outputs equal the three-GEMM pass bit for bit -- but the hidden
-// activations stay in LDS (written transposed into row-major tiles, read back as the
-// next layer's A operand) instead of HBM; h1 / h2 are written out only when given
-// (training keeps them for the backward pass).  The weights of each 16-wide k-step of
-// layers 1 and 2 (all 256 output columns: 8 KB) are staged once per workgroup in LDS,
-// double-buffered with one barrier per step, so the 8 waves (256 rows) of a workgroup
-// share one read of them instead of each wave fetching all 192 KB from L2.
+// activations stay in LDS (row-major, read back as the next layer's operand) instead of
+// HBM; h1 / h2 are written out only when given (training keeps them for the backward
+// pass).  The weights of each 16-wide k-step of layers 1 and 2 (all 256 output columns:
+// 8 KB) are staged once per workgroup in LDS, double-buffered with one barrier per step, so
+// the 8 waves (256 rows) of a workgroup share one read of them instead of each wave
+// fetching all 192 KB from L2.
 constexpr int kFusedRows = 32, kFusedWaves = 8, kHsStride = MH + 8;
 constexpr int kSlab = MH * 16;   // bf16 weights of one k-step: [256 columns][16 k]
 
@@ -541,10 +541,37 @@ __global__ __launch_bounds__(64 * kFusedWaves) void MlpFusedForwardKernel(const 
     const int64_t m0 = ((int64_t)blockIdx.x * kFusedWaves + wave) * kFusedRows;
     const bool rowValid = m0 + r < (int64_t)rows;
     uint16_t (*h)[kHsStride] = hs[wave];
+    auto hp = [&](int row, int col) { return &h[row][col]; };
     const bf16x8 zero8 = __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
 
-    // Output row of accumulator element i of this lane (f32x16 layout of the 32x32 MFMA).
+    // Every product is (weights) x (activations): the weights are the A operand and a wave's 32
+    // rows the B operand, so accumulator element i of lane l is output feature outRow(i) of
+    // the tile (f32x16 layout of the 32x32 MFMA) for row r: each lane holds four runs of four
+    // consecutive features of its own row.
     auto outRow = [&](int i) { return (i >> 2) * 8 + (l >> 5) * 4 + (i & 3); };
+    // Bias, ReLU, bf16 and the row's activations back into LDS (and to hout when given): one
+    // 8-byte store per run of four features.
+    auto hiddenOut = [&](const f32x16 (&a)[MH / 32], int boff, uint16_t* hout) {
+#pragma unroll
+        for (int j = 0; j < MH / 32; ++j) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int f0 = j * 32 + g * 8 + (l >> 5) * 4;
+                const float4 bias = *reinterpret_cast<const float4*>(P + boff + f0);
+                const float bv[4] = {bias.x, bias.y, bias.z, bias.w};
+                uint32_t hv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float zv = a[j][4 * g + q] + bv[q];
+                    hv[q] = hw_f2bf_nonnan(zv > 0.0f ? zv : 0.0f);
+                }
+                const uint2 pk = make_uint2(hv[0] | (hv[1] << 16), hv[2] | (hv[3] << 16));
+                *reinterpret_cast<uint2*>(hp(r, f0)) = pk;
+                if (hout && rowValid) *reinterpret_cast<uint2*>(hout + (m0 + r) * MH + f0) = pk;
+            }
+        }
+    };
+    static_assert(OFF_B1 % 4 == 0 && OFF_B2 % 4 == 0 && (kHsStride * 2) % 8 == 0, "aligned runs of four");
 
     // ---- inputs from the histogram (HIST)
     uint32_t xin[MI / 16][4];
@@ -556,12 +583,15 @@ __global__ __launch_bounds__(64 * kFusedWaves) void MlpFusedForwardKernel(const 
         uint16_t* hc = &h[0][0];   // [bin][32 ZMWs]
         if (m0 < (int64_t)rows) {
             const uint16_t* lb = &hists[m0 / LS].bin_count[0][m0 % LS];
-#pragma unroll 4
-            for (int it = 0; it < 2 * MI / 16; ++it) {
-                const int bin = 16 * it + (l >> 2), piece = (l & 3) * 8;
-                *reinterpret_cast<uint4*>(hc + bin * kFusedRows + piece) =
-                    *reinterpret_cast<const uint4*>(lb + (size_t)bin * LS + piece);
-            }
+            constexpr int NIT = 2 * MI / 16;
+            const int piece = (l & 3) * 8;
+            uint4 v[NIT];   // every load in flight before the first store
+#pragma unroll
+            for (int it = 0; it < NIT; ++it)
+                v[it] = *reinterpret_cast<const uint4*>(lb + (size_t)(16 * it + (l >> 2)) * LS + piece);
+#pragma unroll
+            for (int it = 0; it < NIT; ++it)
+                *reinterpret_cast<uint4*>(hc + (16 * it + (l >> 2)) * kFusedRows + piece) = v[it];
         }
         __syncthreads();
         uint32_t pair[MI / 16][8];
@@ -582,8 +612,8 @@ __global__ __launch_bounds__(64 * kFusedWaves) void MlpFusedForwardKernel(const 
         for (int s = 0; s < MI / 16; ++s)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                xin[s][q] = (uint32_t)hw_f2bf((float)pair[s][2 * q] * scale) |
-                            ((uint32_t)hw_f2bf((float)pair[s][2 * q + 1] * scale) << 16);
+                xin[s][q] = (uint32_t)hw_f2bf_nonnan((float)pair[s][2 * q] * scale) |   // count x finite scale
+                            ((uint32_t)hw_f2bf_nonnan((float)pair[s][2 * q + 1] * scale) << 16);
     }
 
     // ---- layer 1: [32 x MI] x [MI x MH]
@@ -591,13 +621,16 @@ __global__ __launch_bounds__(64 * kFusedWaves) void MlpFusedForwardKernel(const 
 #pragma unroll
     for (int j = 0; j < MH / 32; ++j) acc[j] = f32x16{};
     const uint16_t* xr = HIST ? nullptr : x + (m0 + r) * MI;
+    // Weight slabs: step k + 1's slab is in flight during step k and stored after its
+    // products; step k + 2's load is issued at the start of step k.
+    uint4 next = slabLoad(OFF_W1, MI, 16);
     slabStore(0, slabLoad(OFF_W1, MI, 0));
     __syncthreads();
 #pragma unroll
     for (int k0 = 0; k0 < MI; k0 += 16) {
         const int buf = (k0 / 16) & 1;
-        uint4 next{};
-        if (k0 + 16 < MI) next = slabLoad(OFF_W1, MI, k0 + 16);
+        uint4 next2{};
+        if (k0 + 32 < MI) next2 = slabLoad(OFF_W1, MI, k0 + 32);
         bf16x8 a;
         if (HIST)
             a = __builtin_bit_cast(bf16x8, make_uint4(xin[k0 / 16][0], xin[k0 / 16][1], xin[k0 / 16][2],
@@ -608,72 +641,54 @@ __global__ __launch_bounds__(64 * kFusedWaves) void MlpFusedForwardKernel(const 
 #pragma unroll
         for (int j = 0; j < MH / 32; ++j) b[j] = LoadRow8(&ws[buf][(j * 32 + r) * 16 + kq]);
 #pragma unroll
-        for (int j = 0; j < MH / 32; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
+        for (int j = 0; j < MH / 32; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a, acc[j], 0, 0, 0);
         if (k0 + 16 < MI) slabStore(buf ^ 1, next);
+        next = next2;
         __syncthreads();
     }
-#pragma unroll
-    for (int j = 0; j < MH / 32; ++j) {
-        const int n = j * 32 + r;
-        const float bias = P[OFF_B1 + n];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const float zv = acc[j][i] + bias;
-            const uint16_t hv = hw_f2bf(zv > 0.0f ? zv : 0.0f);
-            const int mr = outRow(i);
-            h[mr][n] = hv;
-            if (h1out && m0 + mr < (int64_t)rows) h1out[(m0 + mr) * MH + n] = hv;
-        }
-    }
+    hiddenOut(acc, OFF_B1, h1out);
     __syncthreads();
 
     // ---- layer 2: [32 x MH] x [MH x MH]
 #pragma unroll
     for (int j = 0; j < MH / 32; ++j) acc[j] = f32x16{};
     // Layer 1's last barrier has passed: every wave is done with both slab buffers.
+    next = slabLoad(OFF_W2, MH, 16);
     slabStore(0, slabLoad(OFF_W2, MH, 0));
     __syncthreads();
+#pragma unroll
     for (int k0 = 0; k0 < MH; k0 += 16) {
         const int buf = (k0 / 16) & 1;
-        uint4 next{};
-        if (k0 + 16 < MH) next = slabLoad(OFF_W2, MH, k0 + 16);
-        const bf16x8 a = LoadRow8(&h[r][k0 + kq]);
+        uint4 next2{};
+        if (k0 + 32 < MH) next2 = slabLoad(OFF_W2, MH, k0 + 32);
+        const bf16x8 a = LoadRow8(hp(r, k0 + kq));
         bf16x8 b[MH / 32];
 #pragma unroll
         for (int j = 0; j < MH / 32; ++j) b[j] = LoadRow8(&ws[buf][(j * 32 + r) * 16 + kq]);
 #pragma unroll
-        for (int j = 0; j < MH / 32; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
+        for (int j = 0; j < MH / 32; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a, acc[j], 0, 0, 0);
         if (k0 + 16 < MH) slabStore(buf ^ 1, next);
+        next = next2;
         __syncthreads();
     }
-#pragma unroll
-    for (int j = 0; j < MH / 32; ++j) {
-        const int n = j * 32 + r;
-        const float bias = P[OFF_B2 + n];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const float zv = acc[j][i] + bias;
-            const uint16_t hv = hw_f2bf(zv > 0.0f ? zv : 0.0f);
-            const int mr = outRow(i);
-            h[mr][n] = hv;
-            if (h2out && m0 + mr < (int64_t)rows) h2out[(m0 + mr) * MH + n] = hv;
-        }
-    }
+    hiddenOut(acc, OFF_B2, h2out);
     __syncthreads();
 
     // ---- layer 3: [32 x MH] x [MH x MO], one 32-column tile of which MO columns are real
     f32x16 out = f32x16{};
-    for (int k0 = 0; k0 < MH; k0 += 16) {
-        const bf16x8 a = LoadRow8(&h[r][k0 + kq]);
-        const bf16x8 b = r < MO ? LoadRow8(Wb + OFF_W3 + (size_t)r * MH + k0 + kq) : zero8;
-        out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, out, 0, 0, 0);
+    bf16x8 b3[MH / 16];   // all of this lane's layer-3 weights in flight at once
+#pragma unroll
+    for (int k = 0; k < MH / 16; ++k) b3[k] = r < MO ? LoadRow8(Wb + OFF_W3 + (size_t)r * MH + 16 * k + kq) : zero8;
+#pragma unroll
+    for (int k = 0; k < MH / 16; ++k) {
+        const bf16x8 a = LoadRow8(hp(r, 16 * k + kq));
+        out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b3[k], a, out, 0, 0, 0);
     }
-    if (r < MO) {
-        const float bias = P[OFF_B3 + r];
+    if (rowValid) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            const int64_t m = m0 + outRow(i);
-            if (m < (int64_t)rows) y[m * MO + r] = out[i] + bias;
+            const int f = outRow(i);
+            if (f < MO) y[(m0 + r) * MO + f] = out[i] + P[OFF_B3 + f];
         }
     }
 }

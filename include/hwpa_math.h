@@ -418,6 +418,13 @@ HWPA_HD uint16_t hw_f2bf(float f)
     if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
     return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
+/* hw_f2bf for a value known not to be NaN (a ReLU output, (z > 0 ? z : 0), never is):
+ * the same round-to-nearest-even without the NaN case. */
+HWPA_HD uint16_t hw_f2bf_nonnan(float f)
+{
+    const uint32_t u = hw_f2u(f);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
 HWPA_HD float hw_bf2f(uint16_t b) { return hw_u2f((uint32_t)b << 16); }
 
 /* Round a float through binary16 and back. */
