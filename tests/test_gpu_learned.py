@@ -5,8 +5,8 @@ A network is trained on (histogram, converged EM) pairs that the plain
 pipeline generates (per-ZMW SNR ~ U[4, 20]), using the library's own
 forward/backward/Adam kernels.  On held-out traces the estimators see the
 same histograms:
-  * plain EM from the reference start, capped at 3 iterations (the method
-    without the network);
+  * plain EM from the reference start, capped at the learned start's
+    iteration limit (5 by default; the method without the network);
   * the learned start with an untrained network, same cap;
   * the learned start with the trained network, same cap.
 The trained start must be accepted (converged within the cap) far more often
@@ -26,11 +26,11 @@ LANES, F, CHUNKS = 96, 512, 8
 @pytest.fixture(scope="module")
 def trained(H, gpu):
     from hw_amd import learned as Lr
-    pairs = Lr.generate_pairs(256, 10, seed=101, snr=(4.0, 20.0), min_conf=0.0, log=print)
-    assert pairs.n >= 256 * 64 * 7
+    pairs = Lr.generate_pairs(4096, 10, seed=101, snr=(4.0, 20.0), min_conf=0.0, log=print)
+    assert pairs.n >= 4096 * 64 * 7
     assert pairs.valid_fraction() > 0.8
     init = H.Initializer(seed=1)
-    losses, _ = Lr.train(init, pairs, epochs=20, batch=2048, lr=2e-3, log=print)
+    losses, _ = Lr.train(init, pairs, epochs=10, batch=4096, lr=2e-3, log=print)
     assert losses[-1][1] < 0.5 * losses[0][1], losses
     return init
 
@@ -92,12 +92,12 @@ def test_learned_start_gains_over_ablations(H, T, trained):
     Z = LANES * 64
     plain = _run(H, T, H.bench_config(F), traces)
 
-    cfg3 = H.bench_config(F)
-    cfg3.em_iteration_limit = 3
-    capped = _run(H, T, cfg3, traces, capture=True)
-
     cfgL = H.bench_config(F)
     cfgL.learned_init = 1
+    cfg3 = H.bench_config(F)
+    cfg3.em_iteration_limit = cfgL.learned_em_iteration_limit
+    capped = _run(H, T, cfg3, traces, capture=True)
+
     learned = _run(H, T, cfgL, traces, init=trained)
     untrained = _run(H, T, cfgL, traces, init=H.Initializer(seed=1))
 
@@ -109,12 +109,14 @@ def test_learned_start_gains_over_ablations(H, T, trained):
     agree = [np.mean(plain[c]["labels"] == learned[c]["labels"]) for c in est
              if plain[c]["flags"][2] and learned[c]["flags"][2]]
     assert len(agree) >= 3
-    print("accepted within 3 iterations: plain start %.4f, untrained net %.4f, trained net %.4f; "
-          "label agreement with plain %s" % (acc_capped, acc_untrained, acc_learned, ["%.5f" % x for x in agree]))
+    print("accepted within %d iterations: plain start %.4f, untrained net %.4f, trained net %.4f; "
+          "label agreement with plain %s" % (cfgL.learned_em_iteration_limit, acc_capped, acc_untrained, acc_learned,
+                                             ["%.5f" % x for x in agree]))
     assert acc_learned > acc_capped + 0.1
     assert acc_learned > acc_untrained + 0.1
-    # measured 0.993-0.996 with the default gate (cap 3, convergence tolerance 2e-3)
-    assert min(agree) > 0.99
+    # measured 0.9937-0.9951 with the defaults (cap 5, gate 0.9, convergence tolerance 2e-3) and this
+    # training set (1.8 M pairs, 10 epochs); the full-size bench network reaches 0.9964
+    assert min(agree) > 0.993
 
 
 def test_learned_estimate_bit_exact_with_oracle(H, O, T, trained):
