@@ -52,7 +52,7 @@ struct PulseEnd {
 // during the next batch, the wave writes the queued records with one loop over the longest
 // queue.  The record's divisions and fixed-point conversions then run about once per
 // pulse of the busiest ZMW instead of at every frame where any ZMW of the wave ends one.
-template <int UNROLL, bool PREFETCH, bool QUEUE>
+template <int UNROLL, int PD, bool QUEUE>
 __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ labels, const int16_t* __restrict__ latent,
                                                   const int16_t* __restrict__ traces, int frames, int32_t firstFrame,
                                                   const hwpa_lane_model* __restrict__ models, SegLane* seg,
@@ -121,8 +121,7 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
         q = 0;
     };
 
-    // Labels and signal of frames [f0, f0 + UNROLL); with PREFETCH the next
-    // batch is loaded while one is accumulated.
+    // Labels and signal of frames [f0, f0 + UNROLL) (frames past the chunk read nothing).
     auto load = [&](int f0, int (&lv)[UNROLL], int (&sv)[UNROLL]) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
@@ -131,12 +130,7 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
             sv[u] = (rf < NL) ? (int)lt[(size_t)rf * LS] : ((rf < frames) ? (int)tr[(size_t)(rf - NL) * LS] : 0);
         }
     };
-    int lv[UNROLL], sv[UNROLL];
-    if (PREFETCH) load(0, lv, sv);
-    for (int f0 = 0; f0 < frames; f0 += UNROLL) {
-        int ln[UNROLL], sn[UNROLL];
-        if (PREFETCH) load(f0 + UNROLL, ln, sn);
-        else load(f0, lv, sv);
+    auto accumulate = [&](int f0, const int (&lv)[UNROLL], const int (&sv)[UNROLL]) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             const int rf = f0 + u;
@@ -184,11 +178,26 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
             }
         }
         if (QUEUE && __any(q > (uint32_t)QFILL)) flush();
-        if (PREFETCH) {
+    };
+    if (PD == 1) {
+        for (int f0 = 0; f0 < frames; f0 += UNROLL) {
+            int lv[UNROLL], sv[UNROLL];
+            load(f0, lv, sv);
+            accumulate(f0, lv, sv);
+        }
+    } else {
+        // A ring of PD batches, each refilled in place with the batch PD batches ahead as
+        // soon as it has been accumulated: PD - 1 batches of loads stay in flight, and no
+        // register holding an outstanding load is ever copied.
+        int lv[PD][UNROLL], sv[PD][UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                lv[u] = ln[u];
-                sv[u] = sn[u];
+        for (int p = 0; p < PD; ++p) load(p * UNROLL, lv[p], sv[p]);
+        for (int f0 = 0; f0 < frames; f0 += PD * UNROLL) {
+#pragma unroll
+            for (int p = 0; p < PD; ++p) {
+                const int fb = f0 + p * UNROLL;
+                accumulate(fb, lv[p], sv[p]);
+                load(fb + PD * UNROLL, lv[p], sv[p]);
             }
         }
     }
@@ -267,15 +276,21 @@ int32_t PulserRun(PulserDev* p, const int16_t* labels, const int16_t* latent, co
 #define HWPA_PULSE_LAUNCH(U, PF, Q)                                                                               \
     hipLaunchKernelGGL((PulseKernel<U, PF, Q>), dim3(p->lanes), dim3(LS), 0, s, labels, latent, traces, (int)frames, \
                        first_frame, models, p->seg, pulses, counts, overflows, bl_stats, p->k)
-    // Variants: frames per batch x next batch in flight x queued record writing.
+    // Variants: frames per batch x batches in the load ring x queued record writing.
     switch (TuneGet(kTunePulse, p->lanes, frames)) {
-    case 1: HWPA_PULSE_LAUNCH(8, true, false); break;
-    case 2: HWPA_PULSE_LAUNCH(16, true, false); break;
-    case 3: HWPA_PULSE_LAUNCH(8, false, true); break;
-    case 4: HWPA_PULSE_LAUNCH(8, true, true); break;
-    case 5: HWPA_PULSE_LAUNCH(4, true, true); break;
-    case 6: HWPA_PULSE_LAUNCH(4, false, true); break;
-    default: HWPA_PULSE_LAUNCH(8, false, false); break;
+    case 1: HWPA_PULSE_LAUNCH(8, 2, false); break;
+    case 2: HWPA_PULSE_LAUNCH(16, 2, false); break;
+    case 3: HWPA_PULSE_LAUNCH(8, 1, true); break;
+    case 4: HWPA_PULSE_LAUNCH(8, 2, true); break;
+    case 5: HWPA_PULSE_LAUNCH(4, 2, true); break;
+    case 6: HWPA_PULSE_LAUNCH(4, 1, true); break;
+    case 7: HWPA_PULSE_LAUNCH(4, 4, true); break;
+    case 8: HWPA_PULSE_LAUNCH(8, 3, true); break;
+    case 9: HWPA_PULSE_LAUNCH(4, 8, true); break;
+    case 10: HWPA_PULSE_LAUNCH(4, 12, true); break;
+    case 11: HWPA_PULSE_LAUNCH(8, 4, true); break;
+    case 12: HWPA_PULSE_LAUNCH(2, 16, true); break;
+    default: HWPA_PULSE_LAUNCH(8, 1, false); break;
     }
 #undef HWPA_PULSE_LAUNCH
     return CheckLaunch("PulseKernel");

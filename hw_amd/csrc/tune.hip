@@ -26,7 +26,7 @@ int32_t TuneVariants(int kernel)
     case kTuneHistogram: return 22;
     case kTuneBaseline: return 6;
     case kTuneLabel: return 4;
-    case kTunePulse: return 7;
+    case kTunePulse: return 13;
     default: return 0;
     }
 }

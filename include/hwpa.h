@@ -147,9 +147,9 @@ int32_t hwpa_batch_autotune(hwpa_batch* b, int32_t repeats);
  * 8 waves per lane sharing its LDS counters; 16-21: all 300 bins in LDS, 8 or 16
  * frames per batch x 4, 8 or 2 waves per lane), kernel 1 = baseline filter (1, 2 or 4 lanes per
  * workgroup, then the same with two groups of raw frames in flight), kernel 2 = Viterbi labeler (16 or 8 frames per prefetched
- * batch, then the same with registers held to 4 waves per SIMD), kernel 3 = pulse accumulator (8 frames,
- * 8 frames with the next batch in flight, 16 with the next in flight, then 8, 8 with
- * prefetch, 4 with prefetch and 4 frames writing records through an LDS queue).
+ * batch, then the same with registers held to 4 waves per SIMD), kernel 3 = pulse accumulator (frames
+ * per batch x batches in the load ring x record queue: 8x1, 8x2, 16x2 writing records directly; 8x1,
+ * 8x2, 4x2, 4x1, 4x4, 8x3, 4x8, 4x12, 8x4, 2x16 writing them through an LDS queue).
  * The launchers look up the variant for their (lanes, frames);
  * unset shapes use variant 0.  hwpa_autotune times every variant on
  * synthetic traces of the shape (`repeats` timed runs each, HIP events) and

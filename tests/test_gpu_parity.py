@@ -88,7 +88,7 @@ def test_baseliner_lane_tile_variants(H, O, T, gpu, variant, F):
 def test_autotune_picks_a_timed_variant(H, gpu):
     cfg = H.bench_config(512)
     r = H.autotune(cfg, 64, 512, repeats=2)
-    counts = {"histogram": 22, "baseline": 6, "labels": 4, "pulses": 7}
+    counts = {"histogram": 22, "baseline": 6, "labels": 4, "pulses": 13}
     for i, (name, n) in enumerate(counts.items()):
         assert len(r[name]) == H.tune_variants(i) == n
         assert all(t > 0 for t in r[name])
@@ -340,7 +340,7 @@ def test_labeler_bit_exact(H, O, T, gpu, label_variant, F):
         assert np.array_equal(H.to_numpy(d_score, np.float32, (L, 64)).view(np.uint32), o_score.view(np.uint32))
 
 
-@pytest.mark.parametrize("variant", [0, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12])
 @pytest.mark.parametrize("max_calls", [0, 480])
 def test_pulses_dense_boundaries_bit_exact(H, O, T, gpu, variant, max_calls):
     """A pulse boundary at every frame (up states cycling through the analogs, with a
@@ -383,7 +383,9 @@ def test_pulses_dense_boundaries_bit_exact(H, O, T, gpu, variant, max_calls):
         H.tune_set(H.TUNE_PULSE, L, F, 0)
 
 
-@pytest.mark.parametrize("variant,max_calls", [(0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (0, 5), (3, 5), (4, 5), (5, 5)])
+@pytest.mark.parametrize("variant,max_calls", [(0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (7, 0), (8, 0), (9, 0),
+                                              (10, 0), (11, 0), (12, 0), (0, 5), (3, 5), (4, 5), (5, 5), (7, 5), (9, 5),
+                                              (10, 5), (12, 5)])
 def test_pulses_bit_exact(H, O, T, gpu, variant, max_calls):
     """Pulse records, counts, overflow counts and baseline stats of every pulse-kernel
     variant; max_calls 5 makes most ZMWs overflow their record capacity."""
