@@ -3,13 +3,19 @@
 // UHistogramSimd binning, dataTypes/UHistogramSimd.{h,cpp}; EdgeFrameClassifier,
 // traceAnalysis/EdgeFrameClassifier.h).
 //
-// Mapping: one wave64 per lane, one thread per ZMW.  The counters of bins
-// [0, NL) -- where baseline and pulses of ordinary signal levels land; NL is
-// 128, 160 or 192 (16, 20 or 24 KB of LDS per lane) -- are staged in LDS for
-// the chunk; higher bins are counted with global atomics in place.  Each thread owns one
-// ZMW column, and the [bin pair][zmw] layout puts the 64 lanes of every
-// increment on distinct LDS banks whatever the bins.  Frames are read in
-// batches of 16 with the next batch in flight while one is binned.
+// Three forms, all bit-exact, chosen per shape by the autotuner:
+//  * HistAddKernel: one wave64 per lane, one thread per ZMW.  The counters of bins
+//    [0, NL) -- where baseline and pulses of ordinary signal levels land; NL is
+//    128, 160 or 192 (16, 20 or 24 KB of LDS per lane) -- are staged in LDS for
+//    the chunk; higher bins are counted with global atomics in place.  Each thread owns one
+//    ZMW column, and the [bin pair][zmw] layout puts the 64 lanes of every
+//    increment on distinct LDS banks whatever the bins.  Frames are read in
+//    batches with the next batch in flight while one is binned.
+//  * HistAddWavesKernel: 2, 4 or 8 waves per lane share its LDS counters, each wave
+//    taking a contiguous share of the chunk's frames.
+//  * HistAddFullKernel: every one of the 300 bins in LDS, the bin found without a
+//    divide (hw_shortbin) for all but the samples next to a bin edge, frames from a
+//    ring of batches refilled in place ("all 300 bins in LDS" below).
 
 #include "internal.h"
 
