@@ -71,3 +71,28 @@ def test_host_chunks_match_device_chunks(H, O, T, gpu):
         buf.fill(-12345)
     _same(_snapshot(pin), final, "pinned host input")
     _same(_snapshot(page), final, "pageable host input")
+
+
+def test_host_uint8_chunks_match_device_chunks(H, O, T, gpu):
+    """uint8 chunks (half the bytes per sample of the staging buffers) from pinned and
+    pageable host memory, back to back, against the same chunks from device memory."""
+    L, F, C = 4, 512, 7
+    cfg = H.bench_config(F)
+    cfg.movie.pedestal = 64
+    chunks = [np.clip(c.astype(np.int32) // 2 + 64, 0, 255).astype(np.uint8)
+              for c in gen_traces(O, T, H, L, F, C, seed=37)]
+    dev = [H.dev_from_numpy(c) for c in chunks]
+    ref = H.BatchAnalyzer(0, L, F, cfg)
+    pin = H.BatchAnalyzer(0, L, F, cfg)
+    page = H.BatchAnalyzer(0, L, F, cfg)
+    pinned = [H.host_pinned(c.nbytes) for c in chunks]
+    for c in range(C):
+        ref(dev[c], c * F, encoding=T.ENCODING_UINT8)
+        pinned[c].array(np.uint8)[:] = chunks[c].reshape(-1)
+        pin(pinned[c], c * F, on_device=False, encoding=T.ENCODING_UINT8)
+        pinned[c].array(np.uint8)[:] = 7
+        page(chunks[c].copy(), c * F)
+    final = _snapshot(ref)
+    _same(_snapshot(pin), final, "pinned uint8 host input")
+    _same(_snapshot(page), final, "pageable uint8 host input")
+    assert final["flags"][2]
