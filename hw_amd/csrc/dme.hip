@@ -267,6 +267,7 @@ struct EmExtra {
     float convTol;            // EM_LEARNED: relative log-likelihood convergence tolerance
     int32_t iterLimit;        // EM_LEARNED: iteration cap
     float gateLogit;          // EM_LEARNED: network confidence logit below which the capped EM is skipped
+    int32_t laneMinConf;      // EM_LEARNED: a lane with fewer ZMWs above the gate gates all of them
 };
 
 // DmeEmHost::EstimateLaneDetModel (DmeEmHost.cpp:216-615).  EM_LEARNED replaces
@@ -441,9 +442,11 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
         }
         s = ((mu[1] * rpa[0] + mu[2] * rpa[1]) + (mu[3] * rpa[2] + mu[4] * rpa[3])) / rpaSqSum;
         // Confidence gate: a ZMW below it takes no capped iteration of its own (it still
-        // runs in step with its lane, but counts as done and is rejected); a lane whose
-        // 64 ZMWs are all below it skips the capped EM.
+        // runs in step with its lane, but counts as done and is rejected); a lane with fewer
+        // than laneMinConf ZMWs above it sends all of them to the plain EM and skips the
+        // capped EM (its iterations would cost the whole lane for those few).
         gated = !(y[11] >= x.gateLogit);
+        if (__popcll(__ballot(!gated)) < (uint32_t)x.laneMinConf) gated = true;
         iterLimit = __all(gated) ? 0 : x.iterLimit;
     }
 
@@ -963,6 +966,7 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
     x.convTol = cfg.learned_conv_tol;
     x.iterLimit = (int32_t)cfg.learned_em_iteration_limit;
     x.gateLogit = MlpGateLogit(cfg.learned_gate_threshold);
+    x.laneMinConf = (int32_t)cfg.learned_lane_min_confident;
     hipLaunchKernelGGL(DmeEstimateKernel<EM_LEARNED>, dim3(lanes), dim3(LS), 0, s, hist, stats, models, d_iters, d,
                        evolve, x);
     if ((rc = CheckLaunch("DmeEstimateKernel")) != HWPA_OK) return rc;

@@ -298,6 +298,7 @@ int32_t hwpa_config_default(hwpa_config* c)
     c->learned_em_iteration_limit = 5;
     c->learned_conv_tol = 2e-3f;
     c->learned_gate_threshold = 0.9f;
+    c->learned_lane_min_confident = 16;
     // Movie description of the synthetic workload: 100 fps, 1 e-/DN, four analogs
     // with amplitudes 180/120/80/50 over a 12-count baseline sigma.
     c->movie.frame_rate = 100.0f;
@@ -379,6 +380,7 @@ int32_t hwpa_config_load_json(hwpa_config* c, const char* json)
         {"metricsConfig.sandwichTolerance", &c->sandwich_tolerance},
         {"metricsConfig.framesPerHFMetricBlock", &c->frames_per_hf_metric_block},
         {"learnedInit.EmIterationLimit", &c->learned_em_iteration_limit},
+        {"learnedInit.LaneMinConfident", &c->learned_lane_min_confident},
     };
     for (const auto& kv : jf.out) {
         const std::string& k = kv.first;
@@ -483,6 +485,7 @@ int32_t hwpa_config_validate(const hwpa_config* c)
     if (!(c->learned_conv_tol > 0.0f)) return Fail(HWPA_ERR_BAD_CONFIG, "learned EM convergence tolerance must be positive");
     if (!(c->learned_gate_threshold >= 0.0f && c->learned_gate_threshold < 1.0f))
         return Fail(HWPA_ERR_BAD_CONFIG, "learned gate threshold must be in [0, 1)");
+    if (c->learned_lane_min_confident > 64) return Fail(HWPA_ERR_BAD_CONFIG, "learned lane minimum must be at most 64");
     return HWPA_OK;
 }
 

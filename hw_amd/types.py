@@ -65,7 +65,8 @@ class Config(ctypes.Structure):
         ("static_baseline_variance", ctypes.c_float),
         ("learned_init", ctypes.c_int32),
         ("learned_conf_threshold", ctypes.c_float), ("learned_em_iteration_limit", ctypes.c_uint32),
-        ("learned_conv_tol", ctypes.c_float), ("learned_gate_threshold", ctypes.c_float), ("movie", MovieInfo)]
+        ("learned_conv_tol", ctypes.c_float), ("learned_gate_threshold", ctypes.c_float),
+        ("learned_lane_min_confident", ctypes.c_uint32), ("movie", MovieInfo)]
 
 
 class FilterParams(ctypes.Structure):

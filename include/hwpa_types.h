@@ -277,6 +277,7 @@ typedef struct hwpa_config {
     uint32_t learned_em_iteration_limit;
     float learned_conv_tol;      /* relative log-likelihood tolerance of the capped EM */
     float learned_gate_threshold; /* network confidence below which the capped EM is skipped (0: never) */
+    uint32_t learned_lane_min_confident; /* a lane with fewer ZMWs above the gate sends all 64 to the plain EM */
     hwpa_movie_info movie;
 } hwpa_config;
 

@@ -1307,6 +1307,7 @@ struct LearnedStart {
     const float* omap;
     unsigned iterLimit;
     float convTol, acceptConf, gateLogit;
+    unsigned laneMinConf;
     uint8_t* fallback;
 };
 
@@ -1439,16 +1440,19 @@ void EstimateLane(const DmeCfg& d, const hwpa_lane_hist& lh, const hwpa_baseline
 
     // The learned start's confidence gate (the device's EM_LEARNED): a ZMW whose confidence
     // output is below the threshold takes no capped iteration of its own and is rejected; a
-    // lane with every ZMW below it skips the capped EM.
+    // lane with fewer than laneMinConf ZMWs above it gates all of them; a lane with every ZMW
+    // gated skips the capped EM.
     bool gated[LS] = {};
     unsigned iterLimit = d.iterLimit;
     if (ls) {
-        bool all = true;
+        unsigned nConf = 0;
         for (int z = 0; z < LS; ++z) {
             gated[z] = !(ls->y[(size_t)z * HWPA_MLP_OUT + 11] >= ls->gateLogit);
-            all = all && gated[z];
+            nConf += gated[z] ? 0u : 1u;
         }
-        iterLimit = all ? 0u : ls->iterLimit;
+        if (nConf < ls->laneMinConf)
+            for (int z = 0; z < LS; ++z) gated[z] = true;
+        iterLimit = (nConf == 0 || nConf < ls->laneMinConf) ? 0u : ls->iterLimit;
     }
     const unsigned it = M::kReference ? EmIterationsRef<M>(d, ez, hz, rpa, rpaMin)
                                       : EmIterationsMirror(d, ez, hz, rpa, rpaMin, iterLimit, ls ? ls->convTol : 1e-4f,
@@ -1742,7 +1746,7 @@ int32_t orc_dme_estimate_learned(const hwpa_config* cfg, int32_t lanes, const hw
     ParLanes(lanes, [&](int l) {
         LearnedStart ls{y + (size_t)l * LS * HWPA_MLP_OUT, omap, cfg->learned_em_iteration_limit, cfg->learned_conv_tol,
                         cfg->learned_conf_threshold, hw_gate_logit(cfg->learned_gate_threshold),
-                        &fb[(size_t)l * LS]};
+                        cfg->learned_lane_min_confident, &fb[(size_t)l * LS]};
         EstimateLane<MirrorForm>(d, hist[l], stats[l], models[l], model_fi, est_fi, nullptr, &ls);
     });
     std::vector<uint32_t> map;
