@@ -679,7 +679,7 @@ int32_t hwpa_analyzer_fallback_count(const hwpa_analyzer* a, uint32_t* count)
     *count = 0;
     if (!a->ranEstimate || !a->ranLearned) return HWPA_OK;
     HWPA_HIP(hipStreamSynchronize(a->stream));
-    HWPA_HIP(hipMemcpy(count, a->lws.total, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HWPA_HIP(hipMemcpy(count, a->lws.total + 1, sizeof(uint32_t), hipMemcpyDeviceToHost));
     return HWPA_OK;
 }
 

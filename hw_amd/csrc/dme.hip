@@ -263,6 +263,7 @@ struct EmExtra {
     uint8_t* fallback;        // EM_LEARNED: 1 = start rejected, model left for the plain EM
     float* diag;              // optional raw estimate [lane][HWPA_EM_DIAG_ROWS][64]
     const uint32_t* active;   // optional: number of valid ZMWs; lanes at or beyond it exit
+    const uint8_t* laneSel;   // optional: only lanes whose 64 flags here are all set run
     float acceptConf;         // EM_LEARNED: minimum confidence to accept the capped EM
     float convTol;            // EM_LEARNED: relative log-likelihood convergence tolerance
     int32_t iterLimit;        // EM_LEARNED: iteration cap
@@ -291,6 +292,22 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
     const int lane = blockIdx.x;
     const int z = threadIdx.x;
     if (x.active && (uint32_t)lane * LS >= *x.active) return;
+    if (x.laneSel && !__all(x.laneSel[(size_t)lane * LS + z] != 0)) return;
+    bool gated = false;       // EM_LEARNED: the network's confidence sends this ZMW to the plain EM
+    if (MODE == EM_LEARNED) {
+        // Confidence gate: a ZMW below it takes no capped iteration of its own (it still
+        // runs in step with its lane, but counts as done and is rejected); a lane with fewer
+        // than laneMinConf ZMWs above it sends all of them to the plain EM (its iterations
+        // would cost the whole lane for those few).  A lane gated whole leaves its models
+        // as they are, as every rejected ZMW does, and stops here.
+        gated = !(x.init[((size_t)lane * LS + z) * HWPA_MLP_OUT + 11] >= x.gateLogit);
+        if (__popcll(__ballot(!gated)) < (uint32_t)x.laneMinConf) gated = true;
+        if (__all(gated)) {
+            x.fallback[(size_t)lane * LS + z] = 1;
+            if (z == 0 && iters) iters[lane] = 0;
+            return;
+        }
+    }
     // Tables of the E-step's log / reciprocal of bin probabilities, one entry per thread.
     __shared__ float sRc[HW_LOGTAB_N], sLc[HW_LOGTAB_N];
     static_assert(HW_LOGTAB_N == LS, "one table entry per thread");
@@ -403,7 +420,6 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
     for (int b = 0; b < NB; ++b) njSum += (float)h.Cnt(b);
     const float halfBin = 0.5f * h.bs;
     int iterLimit = d.iterLimit;
-    bool gated = false;       // EM_LEARNED: the network's confidence sends this ZMW to the plain EM
     if (MODE == EM_LEARNED) {
         // Network start: output j < 6 stands for off[j] + scale[j] * y[j] in units
         // of u = 2 binSize from the lower bound (means) and log(var / u^2);
@@ -441,13 +457,7 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
             varEst[k] = var[k];
         }
         s = ((mu[1] * rpa[0] + mu[2] * rpa[1]) + (mu[3] * rpa[2] + mu[4] * rpa[3])) / rpaSqSum;
-        // Confidence gate: a ZMW below it takes no capped iteration of its own (it still
-        // runs in step with its lane, but counts as done and is rejected); a lane with fewer
-        // than laneMinConf ZMWs above it sends all of them to the plain EM and skips the
-        // capped EM (its iterations would cost the whole lane for those few).
-        gated = !(y[11] >= x.gateLogit);
-        if (__popcll(__ballot(!gated)) < (uint32_t)x.laneMinConf) gated = true;
-        iterLimit = __all(gated) ? 0 : x.iterLimit;
+        iterLimit = x.iterLimit;
     }
 
     int it = 0;
@@ -740,7 +750,8 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
 }
 
 // ---- fallback compaction for the learned start ---------------------------------
-// ZMWs whose capped EM was rejected are packed densely (in lane order, so the
+// A lane whose 64 ZMWs were all rejected runs the plain EM in place.  The
+// rejected ZMWs of the other lanes are packed densely (in lane order, so the
 // grouping is deterministic) into compact lanes, run through the plain EM and
 // scattered back.
 
@@ -797,31 +808,41 @@ __global__ __launch_bounds__(64) void FallbackCountKernel(const uint8_t* __restr
     if (threadIdx.x == 0) cnt[blockIdx.x] = (uint32_t)__popcll(m);
 }
 
-// Exclusive scan of per-lane counts in one workgroup (fixed order).
+// Exclusive scan of per-lane counts in one workgroup (fixed order).  Lanes whose
+// 64 ZMWs all fall back run the plain EM in place and take no compact slots:
+// total[0] counts the compacted ZMWs, total[1] every fallback ZMW.
 __global__ __launch_bounds__(1024) void FallbackScanKernel(const uint32_t* __restrict__ cnt, uint32_t lanes,
                                                            uint32_t* __restrict__ off, uint32_t* __restrict__ total)
 {
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t part[1024], whole[1024];
     const uint32_t per = (lanes + 1023) / 1024;
     const uint32_t b0 = threadIdx.x * per, b1 = min(lanes, b0 + per);
-    uint32_t s = 0;
-    for (uint32_t i = b0; i < b1; ++i) s += cnt[i];
+    uint32_t s = 0, w = 0;
+    for (uint32_t i = b0; i < b1; ++i) {
+        const uint32_t c = cnt[i];
+        s += c == LS ? 0u : c;
+        w += c == LS ? c : 0u;
+    }
     part[threadIdx.x] = s;
+    whole[threadIdx.x] = w;
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t run = 0;
+        uint32_t run = 0, all = 0;
         for (int i = 0; i < 1024; ++i) {
             const uint32_t v = part[i];
             part[i] = run;
             run += v;
+            all += whole[i];
         }
-        *total = run;
+        total[0] = run;
+        total[1] = run + all;
     }
     __syncthreads();
     uint32_t run = part[threadIdx.x];
     for (uint32_t i = b0; i < b1; ++i) {
         off[i] = run;
-        run += cnt[i];
+        const uint32_t c = cnt[i];
+        run += c == LS ? 0u : c;
     }
 }
 
@@ -835,7 +856,7 @@ __global__ __launch_bounds__(64) void FallbackGatherKernel(const uint8_t* __rest
     const int lane = blockIdx.x, z = threadIdx.x;
     const bool f = fb[(size_t)lane * LS + z] != 0;
     const uint64_t m = __ballot(f);
-    if (!f) return;
+    if (!f || m == ~0ull) return;
     const uint32_t rank = (uint32_t)__popcll(m & ((1ull << z) - 1ull));
     const uint32_t pos = off[lane] + rank;
     const uint32_t cl = pos / LS;
@@ -979,6 +1000,11 @@ int32_t DmeEstimateLearned(const hwpa_config& cfg, hwpa_initializer* mlp, Learne
     if ((rc = CheckLaunch("FallbackGatherKernel")) != HWPA_OK) return rc;
     hipLaunchKernelGGL(FallbackPadKernel, dim3(1), dim3(LS), 0, s, w.total, w.cHist, w.cStats, w.cModels);
     if ((rc = CheckLaunch("FallbackPadKernel")) != HWPA_OK) return rc;
+    EmExtra q{};
+    q.laneSel = w.fallback;
+    hipLaunchKernelGGL(DmeEstimateKernel<EM_PLAIN>, dim3(lanes), dim3(LS), 0, s, hist, stats, models,
+                       static_cast<int32_t*>(nullptr), d, evolve, q);
+    if ((rc = CheckLaunch("DmeEstimateKernel")) != HWPA_OK) return rc;
     EmExtra p{};
     p.active = w.total;
     hipLaunchKernelGGL(DmeEstimateKernel<EM_PLAIN>, dim3(lanes), dim3(LS), 0, s, w.cHist, w.cStats, w.cModels, w.cIters,
@@ -1087,7 +1113,7 @@ extern "C" int32_t hwpa_dme_estimate_learned_noalloc(const hwpa_config* cfg, hwp
     rc = DmeEstimateLearned(*cfg, m, w, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, s);
     if (hipStreamSynchronize(s) != hipSuccess && rc == HWPA_OK) rc = Fail(HWPA_ERR_HIP, "learned EM failed");
     if (rc == HWPA_OK && fallback_count) {
-        if (hipMemcpy(fallback_count, w.total, 4, hipMemcpyDeviceToHost) != hipSuccess)
+        if (hipMemcpy(fallback_count, w.total + 1, 4, hipMemcpyDeviceToHost) != hipSuccess)
             rc = Fail(HWPA_ERR_HIP, "fallback count copy failed");
     }
     return rc;

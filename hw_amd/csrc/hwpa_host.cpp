@@ -618,7 +618,7 @@ int32_t hwpa_dme_estimate_learned(const hwpa_config* cfg, hwpa_initializer* m, c
     if (rc == HWPA_OK) rc = DmeEstimateLearned(*cfg, m, w, d_hist, d_stats, d_models, model_fi, est_fi, lanes, d_iters, s);
     if (hipStreamSynchronize(s) != hipSuccess && rc == HWPA_OK) rc = Fail(HWPA_ERR_HIP, "learned EM failed");
     if (rc == HWPA_OK && fallback_count) {
-        if (hipMemcpy(fallback_count, w.total, 4, hipMemcpyDeviceToHost) != hipSuccess)
+        if (hipMemcpy(fallback_count, w.total + 1, 4, hipMemcpyDeviceToHost) != hipSuccess)
             rc = Fail(HWPA_ERR_HIP, "fallback count copy failed");
     }
     LearnedWsFree(w);

@@ -104,8 +104,8 @@ struct LearnedWs {
     float* y = nullptr;               // [Z][11] network output
     uint8_t* fallback = nullptr;      // [Z]
     uint32_t* cnt = nullptr;          // [lanes] fallback ZMWs per lane
-    uint32_t* off = nullptr;          // [lanes] exclusive prefix
-    uint32_t* total = nullptr;        // number of fallback ZMWs
+    uint32_t* off = nullptr;          // [lanes] exclusive prefix over lanes with 1..63 fallback ZMWs
+    uint32_t* total = nullptr;        // [0] compacted fallback ZMWs, [1] all fallback ZMWs
     uint32_t* map = nullptr;          // compact slot -> lane * 64 + zmw
     hwpa_lane_hist* cHist = nullptr;  // compacted fallback inputs
     hwpa_baseliner_stats* cStats = nullptr;

@@ -1732,9 +1732,10 @@ void orc_dme_init_models(const hwpa_config* cfg, int32_t lanes, const hwpa_basel
 
 // Learned-start estimation as the device runs it (hwpa_dme_estimate_learned): the
 // capped EM from the network outputs y [lanes * 64][11] with the output map omap
-// ({offset[6], scale[6]}), then the rejected ZMWs packed in lane order into compact
-// lanes (the last one filled with copies of its first ZMW), re-estimated by the
-// plain EM and written back.  Mirror arithmetic only (the path has no reference
+// ({offset[6], scale[6]}); then a lane whose 64 ZMWs were all rejected is re-estimated
+// by the plain EM in place, and the rejected ZMWs of the other lanes are packed in lane
+// order into compact lanes (the last one filled with copies of its first ZMW),
+// re-estimated by the plain EM and written back.  Mirror arithmetic only (the path has no reference
 // counterpart).  fallback [lanes * 64] (optional) receives the rejected ZMWs;
 // returns their number.
 int32_t orc_dme_estimate_learned(const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,
@@ -1750,12 +1751,25 @@ int32_t orc_dme_estimate_learned(const hwpa_config* cfg, int32_t lanes, const hw
         EstimateLane<MirrorForm>(d, hist[l], stats[l], models[l], model_fi, est_fi, nullptr, &ls);
     });
     std::vector<uint32_t> map;
-    for (int l = 0; l < lanes; ++l)
+    std::vector<int> whole;
+    for (int l = 0; l < lanes; ++l) {
+        int n = 0;
+        for (int z = 0; z < LS; ++z) n += fb[(size_t)l * LS + z] ? 1 : 0;
+        if (n == LS) {
+            whole.push_back(l);
+            continue;
+        }
         for (int z = 0; z < LS; ++z)
             if (fb[(size_t)l * LS + z]) map.push_back((uint32_t)(l * LS + z));
+    }
     const uint32_t total = (uint32_t)map.size();
+    const int32_t all = (int32_t)(total + (uint32_t)whole.size() * LS);
     if (fallback) std::memcpy(fallback, fb.data(), fb.size());
-    if (total == 0) return 0;
+    ParLanes((int)whole.size(), [&](int i) {
+        const int l = whole[i];
+        EstimateLane<MirrorForm>(d, hist[l], stats[l], models[l], model_fi, est_fi, nullptr);
+    });
+    if (total == 0) return all;
     const int cl = (int)((total + LS - 1) / LS);
     std::vector<hwpa_lane_hist> ch(cl);
     std::vector<hwpa_baseliner_stats> cs(cl);
@@ -1800,7 +1814,7 @@ int32_t orc_dme_estimate_learned(const hwpa_config* cfg, int32_t lanes, const hw
         const uint32_t dst = map[p];
         copyModel(cm[p / LS], (int)(p % LS), models[dst / LS], (int)(dst % LS));
     }
-    return (int32_t)total;
+    return all;
 }
 
 void orc_dme_estimate(int32_t form, const hwpa_config* cfg, int32_t lanes, const hwpa_lane_hist* hist,

@@ -125,10 +125,18 @@ def test_learned_estimate_bit_exact_with_oracle(H, O, T, trained):
     compaction of the rejected ZMWs in lane order, plain EM on the compact lanes,
     scatter) against the oracle's restatement of it, bit for bit: the network's
     outputs come from the GPU and go to both, so the EM arithmetic and the
-    acceptance / fallback bookkeeping are what is compared."""
+    acceptance / fallback bookkeeping are what is compared.  Lanes 5 and 29 hold
+    100 frames per ZMW (insufficient data: every ZMW is rejected), so lanes that
+    fall back whole (plain EM in place) and lanes that fall back in part
+    (compaction) are both exercised."""
     from test_gpu_parity import _dev, _em_case
     L = 48
     cfg, h, s, m = _em_case(H, O, T, L, F, seed=404, tg=dict(snr_lo=4.0, snr_hi=20.0))
+    for l in (5, 29):
+        h["bin_count"][l] = 0
+        h["bin_count"][l, 10] = 100
+        h["outlier_low"][l] = 0
+        h["outlier_high"][l] = 0
     cfg.learned_init = 1
     mfi, efi = (512, 1024), (1024, 1536)
     d_hist = _dev(H, h)
@@ -143,3 +151,7 @@ def test_learned_estimate_bit_exact_with_oracle(H, O, T, trained):
     assert n_gpu == n_orc
     assert g_models.tobytes() == o_models.tobytes()
     assert 0 < n_orc < L * 64                # both the accepted and the fallback path are exercised
+    per_lane = np.asarray(fb).reshape(L, 64).astype(np.int32).sum(axis=1)
+    print("lanes falling back whole %d, in part %d" % ((per_lane == 64).sum(), ((per_lane > 0) & (per_lane < 64)).sum()))
+    assert per_lane[5] == 64 and per_lane[29] == 64
+    assert ((per_lane > 0) & (per_lane < 64)).any()
