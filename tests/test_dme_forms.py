@@ -127,3 +127,36 @@ def test_em_expected_counts_recover_model(H, O, T, form):
     # the baseline variance carries the quantisation term binSize^2 / 12
     np.testing.assert_allclose(est[0, 2], var[0] + bs * bs / 12.0, rtol=2e-2)
     np.testing.assert_allclose(est[1:, 1] / rel, est[1, 1] / rel[0], rtol=1e-6)
+
+
+def test_forms_over_several_seeds(H, O, T):
+    """Five seeded cases of 48 lanes (15,360 ZMWs), the mirror form (the GPU's bits) and
+    the libm yardstick each against the reference form.  Beyond capture flips, a few
+    ZMWs whose capture iteration matches still differ by a little over 1e-4: slowly
+    converging ZMWs carry the last-bit differences of the E-step through 13-19
+    iterations.  The yardstick, two faithful restatements apart, shows the same kind."""
+    nb = {"mirror": 0, "reference_libm": 0}
+    mx = {"mirror": 0.0, "reference_libm": 0.0}
+    within = flips = total = 0
+    for seed in (5, 9, 11, 21, 33):
+        cfg, h, s, m = _em_case(H, O, T, 48, 512, seed=seed)
+        _, e_ref, _, c_ref, it_ref = _run(O, cfg, h, s, m, "reference")
+        total += c_ref.size
+        for form in nb:
+            _, e, _, c, it = _run(O, cfg, h, s, m, form)
+            assert np.array_equal(it, it_ref)
+            rel = _zmw_rel(e, e_ref)
+            flip = c != c_ref
+            nb[form] += int(((rel > 1e-4) & ~flip).sum())
+            mx[form] = max(mx[form], float(rel[~flip].max()))
+            if form == "mirror":
+                within += int((rel <= 1e-4).sum())
+                flips += int(flip.sum())
+    print("\nmirror: %.3f%% within 1e-4, %.2f%% flips, %d same-capture above 1e-4 (max %.2e); yardstick %d (max %.2e)"
+          % (100.0 * within / total, 100.0 * flips / total, nb["mirror"], mx["mirror"], nb["reference_libm"],
+             mx["reference_libm"]))
+    # measured: 97.82% within, 2.32% flips, 3 same-capture ZMWs above 1e-4 (max 3.2e-4); yardstick 1 (1.5e-4)
+    assert within / total > 0.97
+    assert flips / total < 0.03
+    assert nb["mirror"] <= 0.0005 * total and mx["mirror"] < 5e-4
+    assert nb["reference_libm"] >= 1      # the floor exists between faithful restatements too
