@@ -25,7 +25,10 @@ struct SegLane {
 struct PulseConsts {
     float widthThresh, ampThresh;
     uint32_t maxCalls;
-    uint8_t analogMap[NA];
+    // base code of analog a in byte a.  One word, so a per-ZMW lookup is a shift of a
+    // scalar register: a per-lane index into a byte array of the kernel arguments is a
+    // memory load, whose wait (loads complete in order) would drain the load ring.
+    uint32_t analogMap;
 };
 
 struct PulserDev {
@@ -96,7 +99,8 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
         p.mid_signal = ToFixed(width < 3 ? 0.0f : StdMin(maxSignal, StdMax(0.0f, rawMid)));
         p.max_signal = ToFixed(StdMin(maxSignal, StdMax(0.0f, (float)e.smax)));
         p.signal_m2 = e.m2;
-        p.label = (l >= 1 && l <= NA) ? k.analogMap[l - 1] : (uint8_t)4;
+        const uint32_t code = (k.analogMap >> (8u * ((uint32_t)(l - 1) & 3u))) & 0xffu;
+        p.label = (l >= 1 && l <= NA) ? (uint8_t)code : (uint8_t)4;
         p.is_reject = keep ? 0 : 1;
         p.pad[0] = p.pad[1] = 0;
         if (n < k.maxCalls) out[n++] = p;
@@ -121,13 +125,17 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
         q = 0;
     };
 
-    // Labels and signal of frames [f0, f0 + UNROLL) (frames past the chunk read nothing).
+    // Labels and signal of frames [f0, f0 + UNROLL); frames past the chunk re-read its last
+    // frame (never used).  Every load is unconditional, from a selected address: a load under
+    // a branch leaves the compiler unable to count the loads issued after an earlier one, and
+    // it then waits for all of them (the ring's loads in flight included).
     auto load = [&](int f0, int (&lv)[UNROLL], int (&sv)[UNROLL]) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-            const int rf = f0 + u;
-            lv[u] = (rf < frames) ? (int)lb[(size_t)rf * LS] : 0;
-            sv[u] = (rf < NL) ? (int)lt[(size_t)rf * LS] : ((rf < frames) ? (int)tr[(size_t)(rf - NL) * LS] : 0);
+            const int rf = min(f0 + u, frames - 1);
+            lv[u] = (int)lb[(size_t)rf * LS];
+            const int16_t* sp = (rf < NL) ? lt + (size_t)rf * LS : tr + (size_t)(rf - NL) * LS;
+            sv[u] = (int)*sp;
         }
     };
     auto accumulate = [&](int f0, const int (&lv)[UNROLL], const int (&sv)[UNROLL]) {
@@ -179,7 +187,11 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
         }
         if (QUEUE && __any(q > (uint32_t)QFILL)) flush();
     };
+    // minMean is read before the frame loop (behind the first loads when there is a ring):
+    // a value still in flight when the record loop starts would make the compiler wait for
+    // every outstanding load before that loop.
     if (PD == 1) {
+        asm volatile("" ::"v"(minMean));
         for (int f0 = 0; f0 < frames; f0 += UNROLL) {
             int lv[UNROLL], sv[UNROLL];
             load(f0, lv, sv);
@@ -192,6 +204,7 @@ __global__ __launch_bounds__(64) void PulseKernel(const int16_t* __restrict__ la
         int lv[PD][UNROLL], sv[PD][UNROLL];
 #pragma unroll
         for (int p = 0; p < PD; ++p) load(p * UNROLL, lv[p], sv[p]);
+        asm volatile("" ::"v"(minMean));
         for (int f0 = 0; f0 < frames; f0 += PD * UNROLL) {
 #pragma unroll
             for (int p = 0; p < PD; ++p) {
@@ -239,9 +252,11 @@ int32_t PulserCreate(const hwpa_config& cfg, uint32_t lanes, PulserDev** out)
     const double x = cfg.xsp_amp_thresh;
     p->k.ampThresh = (float)(x / (1.0 - x));
     p->k.maxCalls = cfg.max_calls_per_zmw;
+    p->k.analogMap = 0;
     for (int a = 0; a < NA; ++a) {
         const uint8_t b = cfg.movie.analogs[a].base_label;
-        p->k.analogMap[a] = (b == 'A') ? 0 : (b == 'C') ? 1 : (b == 'G') ? 2 : (b == 'T') ? 3 : 4;
+        const uint32_t code = (b == 'A') ? 0 : (b == 'C') ? 1 : (b == 'G') ? 2 : (b == 'T') ? 3 : 4;
+        p->k.analogMap |= code << (8 * a);
     }
     int32_t rc = DeviceAlloc((void**)&p->seg, sizeof(SegLane) * lanes, "pulse segments");
     if (rc == HWPA_OK) {
