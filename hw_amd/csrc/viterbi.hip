@@ -194,6 +194,20 @@ __device__ __forceinline__ int Pred(uint32_t w, int state)
     return ((w >> (20 + q)) & 1u) ? 5 + i : 1 + i;
 }
 
+// Pred for a state that differs between ZMWs (the traceback): both field lookups and a
+// select, no divergent branch.  Shift amounts stay below 32 for every state.
+__device__ __forceinline__ int PredSel(uint32_t w, int state)
+{
+    const uint32_t qb = (state == 0) ? 0u : ((uint32_t)(state - 4) & 7u);
+    const int code = (int)((w >> (4u * qb)) & 15u);
+    const int pb = code == 0 ? 0 : code + 4;
+    const int i = (state <= 4 ? state - 1 : state - 9) & 3;
+    const uint32_t q = (uint32_t)((state <= 4) ? i : 4 + i);
+    const int pf = ((w >> (20u + q)) & 1u) ? 5 + i : 1 + i;
+    const bool bu = state == 0 || (state >= 5 && state <= 8);
+    return bu ? pb : pf;
+}
+
 // The ROI penalty of the frame whose final ROI bit is bit `pos` of `w`: 0 or +inf.
 __device__ __forceinline__ float RoiPen(uint32_t w, uint32_t pos)
 {
@@ -411,7 +425,7 @@ __global__ __launch_bounds__(64, MINW) void LabelKernel(const int16_t* __restric
 #pragma unroll
             for (int u = 0; u < PB; ++u) {
                 lo[(size_t)(top - u) * LS] = (int16_t)stt;
-                stt = Pred(cur[u], stt);
+                stt = PredSel(cur[u], stt);
             }
 #pragma unroll
             for (int u = 0; u < PB; ++u) cur[u] = nxt[u];
@@ -419,7 +433,7 @@ __global__ __launch_bounds__(64, MINW) void LabelKernel(const int16_t* __restric
 #pragma unroll 1
         for (int f = frames - 1 - nb * PB; f >= 0; --f) {
             lo[(size_t)f * LS] = (int16_t)stt;
-            stt = Pred(tb[(size_t)f * LS], stt);
+            stt = PredSel(tb[(size_t)f * LS], stt);
         }
     }
 
