@@ -513,8 +513,8 @@ __global__ __launch_bounds__(64 * W) void HistAddWavesKernel(hwpa_lane_hist* his
 // such samples take the exact path (BinFast / the divide and the outlier counts), which a
 // batch runs only for the samples that need it after the whole batch has been counted.
 
-template <int N>
-__device__ __forceinline__ void HistBatchFull(char* cntB, uint32_t zoff, uint32_t inc, const int (&fv)[N],
+template <int N, class V>
+__device__ __forceinline__ void HistBatchFull(char* cntB, uint32_t zoff, uint32_t inc, const V (&fv)[N],
                                               const hw_shortbin_z& zb, int thr, int& prev, bool& bl1, bool& bl2,
                                               int& pushes, uint32_t& lowOut, uint32_t& highOut)
 {
@@ -523,7 +523,7 @@ __device__ __forceinline__ void HistBatchFull(char* cntB, uint32_t zoff, uint32_
     bool slow = false;
 #pragma clang loop unroll(full)
     for (int u = 0; u < N; ++u) {
-        const bool isBl = fv[u] < thr;
+        const bool isBl = (int)fv[u] < thr;
         const bool counted = (u >= 2 || pushes + u >= 2) && (isBl == bl2);
         bl2 = bl1;
         bl1 = isBl;
@@ -539,7 +539,7 @@ __device__ __forceinline__ void HistBatchFull(char* cntB, uint32_t zoff, uint32_
         uint32_t nLow = 0, nHigh = 0;
 #pragma clang loop unroll(full)
         for (int u = 0; u < N; ++u) {
-            const bool isBl = fv[u] < thr;
+            const bool isBl = (int)fv[u] < thr;
             const bool counted = (u >= 2 || sp + u >= 2) && (isBl == b2);
             b2 = b1;
             b1 = isBl;
@@ -576,7 +576,9 @@ template <int B, int PD>
 struct FrameRing {
     const int16_t* tr;
     int frames, full;
-    int fv[PD][B];
+    // Raw samples: the loads of the set-up and of the loop are then the same instruction, so
+    // the ring keeps its registers across the loop (no copy that would wait for the loads).
+    int16_t fv[PD][B];
 
     __device__ __forceinline__ void Load(int j, int g)
     {
