@@ -163,6 +163,27 @@ __device__ __forceinline__ void RingRefill2(uint32_t* ringA, uint32_t lastA, uin
     }
 }
 
+// The raw samples of a group of S frames, two 16-bit values per register (int16 samples,
+// or uint8 zero-extended: both fit a short), loaded straight into the halves.
+typedef short Pair16 __attribute__((ext_vector_type(2)));
+template <int S>
+struct Group16 {
+    static_assert(S % 2 == 0, "whole pairs");
+    Pair16 p[S / 2];
+    __device__ __forceinline__ int At(int j) const { return (j & 1) ? (int)p[j >> 1].y : (int)p[j >> 1].x; }
+};
+
+// Loads a group's samples of one ZMW column (src: its first frame); the raw trace is read once.
+template <int S, class E>
+__device__ __forceinline__ void LoadGroup(Group16<S>& g, const E* src)
+{
+#pragma unroll
+    for (int k = 0; k < S / 2; ++k) {
+        g.p[k].x = (short)__builtin_nontemporal_load(&src[(size_t)(2 * k) * LS]);
+        g.p[k].y = (short)__builtin_nontemporal_load(&src[(size_t)(2 * k + 1) * LS]);
+    }
+}
+
 // E: raw sample type (int16_t, or uint8_t zero-extended as BatchData::Extract does,
 // dataTypes/BatchData.h:160-182); LPB lanes (waves) per workgroup -- the autotuned lane
 // tile; PD groups of raw frames in flight (1 or 2).
@@ -223,14 +244,9 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
     const float minSigma = .288675135f;
 
     static_assert(PD == 1 || PD == 2, "prefetch depth");
-    int rv[S], pv[S];
-#pragma unroll
-    for (int j = 0; j < S; ++j) rv[j] = __builtin_nontemporal_load(&rl[(size_t)j * LS]);
-    if (PD == 2) {
-        const int g1 = (groups > 1) ? 1 : 0;
-#pragma unroll
-        for (int j = 0; j < S; ++j) pv[j] = __builtin_nontemporal_load(&(rl + (size_t)g1 * S * LS)[j * LS]);
-    }
+    Group16<S> rv, pv;
+    LoadGroup(rv, rl);
+    if (PD == 2) LoadGroup(pv, rl + (size_t)((groups > 1) ? 1 : 0) * S * LS);
 
     for (int g = 0; g < groups; ++g) {
         // Final-stage ring: refill on wrap, then read the fronts early.
@@ -240,11 +256,9 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
         const uint32_t frontA = ringA[ctr * LS];
         const uint32_t frontB = ringB[ctr * LS];
         // Prefetch the raw frames of group g + PD (the last group re-read past the end).
-        int nv[S];
+        Group16<S> nv;
         const int gn = (g + PD < groups) ? g + PD : groups - 1;
-#pragma unroll
-        for (int j = 0; j < S; ++j)   // group base + immediate offsets; the raw trace is read once
-            nv[j] = __builtin_nontemporal_load(&(rl + (size_t)gn * S * LS)[j * LS]);
+        LoadGroup(nv, rl + (size_t)gn * S * LS);
 
         // Stage 0 on (x, ~x): erode pair, then dilate pair -> (lower, ~upper), in two halves of
         // the group (the same window extrema, with about half the live registers).
@@ -257,7 +271,7 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
             uint32_t a0[H0], e0[H0], d0[H0];
 #pragma unroll
             for (int i = 0; i < H0; ++i) {
-                const int x = rv[(hf * H0 + i) * T::s[0]];
+                const int x = rv.At((hf * H0 + i) * T::s[0]);
                 a0[i] = Pack(x, ~x);
             }
             Slide<W0, H0, false>(h0a, a0, e0);
@@ -317,7 +331,7 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
         const float thr = (4.5f * sigmaEma) * k.scaler;
 #pragma unroll
         for (int j = 0; j < S; ++j) {
-            const float rawScaled = (float)hw_wrap16(rv[j] - k.pedestal) * k.scaler;
+            const float rawScaled = (float)hw_wrap16(rv.At(j) - k.pedestal) * k.scaler;
             const int bls = hw_round_i16(rawScaled - subBl);
             (ol + (size_t)g * S * LS)[j * LS] = (int16_t)bls;   // cached: the histogram reads it next
             const int rr = hw_round_i16(rawScaled);
@@ -345,11 +359,8 @@ __global__ __launch_bounds__(64 * LPB) void BaselineKernel(const E* __restrict__
         const float bgv = (float)hw_round_i16(subBl);
         gm1 += bgv;
         gm2 += bgv * bgv;
-#pragma unroll
-        for (int j = 0; j < S; ++j) {
-            rv[j] = (PD == 2) ? pv[j] : nv[j];
-            pv[j] = nv[j];
-        }
+        rv = (PD == 2) ? pv : nv;
+        pv = nv;
     }
 
     // Persist carried state.
