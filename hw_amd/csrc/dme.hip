@@ -498,6 +498,7 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
 #pragma unroll
         for (int k = 0; k < 5; ++k) qq[k] = hw_expf((2.0f * nhv[k]) * (h.bs * h.bs));
         uint32_t exMask = 0;
+        bool anyEx = false;   // some ZMW of the wave takes exp at every boundary of this block
         auto tAt = [&](float x, int i) {
             const float y = x - mu[i];
             return hw_fmaf(y * y, nhv[i], pref[i]);
@@ -512,15 +513,22 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
             for (int i = 0; i < 5; ++i) {
                 const float y0 = x0 - mu[i];
                 const float t0 = hw_fmaf(y0 * y0, nhv[i], pref[i]);
-                // exp(t) is +0 below the threshold: waves skip the exp when all ZMWs are.
-                if (__all(t0 < kExpZeroBelow)) dd[i] = 0.0f;
-                else dd[i] = hw_expf(t0);
                 // t(x0 + bs) - t(x0) on the ideal grid (no cancellation of rounded t)
                 const float dl = nbs[i] * hw_fmaf(2.0f, y0, h.bs);
-                rr[i] = hw_expf(dl);
+                // exp(t) is +0 below the threshold: waves skip the exp when all ZMWs are, and
+                // then the ratio's too.  The density stays +0 through the block (+0 times any
+                // finite ratio >= 0); a rising one (dl > 0) takes exp at every boundary below.
+                if (__all(t0 < kExpZeroBelow)) {
+                    dd[i] = 0.0f;
+                    rr[i] = 0.0f;
+                } else {
+                    dd[i] = hw_expf(t0);
+                    rr[i] = hw_expf(dl);
+                }
                 const bool rec = (dd[i] >= FLT_MIN || dl <= 0.0f) && dl <= 80.0f;
                 exMask |= (rec ? 0u : 1u) << i;
             }
+            anyEx = __any(exMask != 0u);
         };
         auto advance = [&](int c) {
 #pragma unroll
@@ -528,7 +536,7 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
                 dd[i] = dd[i] * rr[i];
                 rr[i] = rr[i] * qq[i];
             }
-            if (__any(exMask != 0u)) {
+            if (anyEx) {
                 const float x = h.Start(c);
 #pragma unroll
                 for (int i = 0; i < 5; ++i)
