@@ -599,7 +599,7 @@ __global__ __launch_bounds__(64, 3) void DmeEstimateKernel(const hwpa_lane_hist*
                 else advance(b + 1);
                 const float bpNext = Sum5Tree(dd);
                 // binp >= binProbMin >= FLT_MIN and finite: the log needs no special cases
-                const float binp = hw_maxf(halfBin * (bpPrev + bpNext), binProbMin);
+                const float binp = hw_max_pos(halfBin * (bpPrev + bpNext), binProbMin);
                 probSum += binp;
                 float f = 0.0f;   // n_b / binp_b, +0 for an empty bin
                 if (!__all(cnt == 0u)) {

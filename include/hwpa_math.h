@@ -56,6 +56,10 @@ HWPA_HD bool hw_isfinite(float x) { return (hw_f2u(x) & 0x7f800000u) != 0x7f8000
 HWPA_HD float hw_minf(float a, float b) { return (a < b) ? a : b; }
 HWPA_HD float hw_maxf(float a, float b) { return (a > b) ? a : b; }
 HWPA_HD float hw_clampf(float v, float lo, float hi) { return hw_minf(hw_maxf(v, lo), hi); }
+/* hw_maxf(a, b) for b positive and not NaN, in one instruction on the GPU: the IEEE
+ * maximum returns b for a NaN a, as the comparison does, and the two operands are never
+ * zeros of opposite sign. */
+HWPA_HD float hw_max_pos(float a, float b) { return __builtin_fmaxf(a, b); }
 /* satlin (common/NumericUtil.h:70-76). */
 HWPA_HD float hw_satlin(float a, float b, float x)
 {
